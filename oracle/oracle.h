@@ -1,0 +1,64 @@
+// CPU oracle for the Mirorr block-matching registration hot path.
+//
+// TEST INFRASTRUCTURE ONLY. This library is a plain C++ restatement of the
+// reference algorithms (aehrc/Mirorr, src/mirorr/*) used as the checker for
+// the HIP implementation in mirorr_amd/. Only tests/, __graft_entry__.smoke()
+// and bench.py's cpu_baseline leg may load it. The product never links it.
+//
+// Every function cites the reference file:line it restates. Arithmetic types
+// follow the reference: float voxels, double accumulation, float score buffer.
+#pragma once
+#include <cstdint>
+#include <cstddef>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+// Geometry of a 3-D volume: x fastest, then y, then z (ITK raster order).
+typedef struct {
+  int dims[3];
+  double spacing[3];
+  double origin[3];
+  double direction[9];  // row-major 3x3 direction cosines
+} orc_geom;
+
+// ---- Block matcher (itkBlockMatcher.txx, itkBlockMatcherThreaded.txx) -----
+
+// NCC^2 of one block pair, masked (itkBlockMatcher.txx:46-101).
+double orc_ncc_pair(const float* base, const uint8_t* base_mask,
+                    const float* search, const uint8_t* search_mask,
+                    const int dims[3], const int base_idx[3],
+                    const int search_idx[3], int bw);
+
+// Unmasked variant (itkAbstractBlockMatcher.h:167-204).
+double orc_ncc_pair_unmasked(const float* base, const float* search,
+                             const int dims[3], const int base_idx[3],
+                             const int search_idx[3], int bw);
+
+// Block grid (itkBlockMatcher.txx:531-600): per-axis count and start.
+void orc_block_grid(const int dims[3], int bw, int ng, int nblocks[3],
+                    int start[3]);
+
+// Full GetOffsets. Raw per-block outputs are written in block raster order
+// (x fastest) whatever the thread count: block_idx/match_idx are n x 3 voxel
+// indices, best/second the float score buffer. Kept points (mm) come out in
+// the reference's thread-concatenation order (itkBlockMatcherThreaded.txx:
+// 73-89). nthreads <= 0 means single-threaded BlockMatcher::GetOffsets.
+// mask pointers may be NULL (reference default: all 255,
+// itkBlockMatcher.txx:507-529). Returns 0 on success, 1 if the region has no
+// block (the reference throws, itkBlockMatcher.txx:936-943).
+int orc_bm_get_offsets(const float* base, const uint8_t* base_mask,
+                       const float* search, const uint8_t* search_mask,
+                       const orc_geom* geom, int bw, int bg, int nw, int ng,
+                       int nthreads, int32_t* block_idx, int32_t* match_idx,
+                       float* best, float* second, double* base_mm,
+                       double* match_mm, int64_t* n_kept);
+
+// Number of candidates evaluated for the whole grid (voxel-evals / bw^3).
+int64_t orc_bm_candidate_count(const int dims[3], int bw, int bg, int nw,
+                               int ng);
+
+#ifdef __cplusplus
+}
+#endif
