@@ -1,0 +1,110 @@
+"""ctypes binding of the C ABI (include/mirorr_gpu.h).
+
+Loads mirorr_amd/lib/libmirorr_gpu.so, built by mirorr_amd/build.py. There is
+no fallback: if the library is missing, importing the device path raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libmirorr_gpu.so")
+
+OK, ERR_ARG, ERR_PLATFORM, ERR_ALGO, ERR_NOMEM, ERR_STATE = range(6)
+ROLE_BASE, ROLE_SEARCH = 0, 1
+
+
+class MirorrError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"[status {status}] {msg}")
+        self.status = status
+
+
+class PlatformError(MirorrError):
+    """Device failure (the reference's BlockMatcherGPUExceptionPlatform)."""
+
+
+class AlgoError(MirorrError):
+    """No block matches (the reference's BlockMatcherGPUExceptionAlgo)."""
+
+
+class Geom(C.Structure):
+    _fields_ = [("dims", C.c_int * 3), ("spacing", C.c_double * 3),
+                ("origin", C.c_double * 3), ("direction", C.c_double * 9)]
+
+    @classmethod
+    def make(cls, dims, spacing=(1.0, 1.0, 1.0), origin=(0.0, 0.0, 0.0), direction=None):
+        g = cls()
+        for i in range(3):
+            g.dims[i] = int(dims[i])
+            g.spacing[i] = float(spacing[i])
+            g.origin[i] = float(origin[i])
+        d = direction if direction is not None else (1, 0, 0, 0, 1, 0, 0, 0, 1)
+        for i in range(9):
+            g.direction[i] = float(d[i])
+        return g
+
+
+class BMParams(C.Structure):
+    _fields_ = [("block_width", C.c_int), ("block_gap", C.c_int), ("nhood_width", C.c_int),
+                ("nhood_gap", C.c_int), ("portion_kept", C.c_double),
+                ("variance_select", C.c_int), ("verbosity", C.c_int)]
+
+
+_P = C.c_void_p
+_lib = None
+
+# name: (restype, argtypes)
+_SIGS = {
+    "mirorr_ctx_create": (C.c_int, [C.c_int, C.c_size_t, C.POINTER(_P)]),
+    "mirorr_ctx_destroy": (None, [_P]),
+    "mirorr_last_error": (C.c_char_p, [_P]),
+    "mirorr_ctx_synchronize": (C.c_int, [_P]),
+    "mirorr_ctx_stream": (_P, [_P]),
+    "mirorr_bm_default_params": (None, [C.POINTER(BMParams)]),
+    "mirorr_bm_create": (C.c_int, [_P, C.POINTER(BMParams), C.POINTER(_P)]),
+    "mirorr_bm_destroy": (None, [_P]),
+    "mirorr_bm_set_params": (C.c_int, [_P, C.POINTER(BMParams)]),
+    "mirorr_bm_set_volume": (C.c_int, [_P, C.c_int, _P, _P, C.POINTER(Geom)]),
+    "mirorr_bm_set_volume_device": (C.c_int, [_P, C.c_int, _P, _P, C.POINTER(Geom)]),
+    "mirorr_bm_swap": (C.c_int, [_P]),
+    "mirorr_bm_get_offsets": (C.c_int, [_P, _P, _P, C.c_size_t, C.POINTER(C.c_size_t), _P, _P,
+                                        _P]),
+    "mirorr_bm_block_count": (C.c_int, [_P, C.POINTER(C.c_size_t), _P]),
+    "mirorr_bm_get_raw": (C.c_int, [_P, _P, _P, _P, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "mirorr_bm_match_async": (C.c_int, [_P]),
+    "mirorr_bm_voxel_evals": (C.c_int, [_P, C.POINTER(C.c_int64)]),
+}
+
+
+def declared_symbols():
+    return sorted(_SIGS)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"HIP library not built: {LIB_PATH}. Run `python -m mirorr_amd.build` "
+                "(there is no CPU fallback).")
+        L = C.CDLL(LIB_PATH)
+        for name, (rt, at) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = rt
+            f.argtypes = at
+        _lib = L
+    return _lib
+
+
+def check(status: int, ctx=None):
+    if status == OK:
+        return
+    msg = lib().mirorr_last_error(ctx).decode() if ctx else "error"
+    cls = {ERR_PLATFORM: PlatformError, ERR_ALGO: AlgoError}.get(status, MirorrError)
+    raise cls(status, msg)
+
+
+def ptr(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)

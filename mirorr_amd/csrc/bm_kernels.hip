@@ -1,0 +1,950 @@
+// Exhaustive NCC block search for gfx950 (MI355X).
+//
+// Replaces the reference OpenCL kernel optKmatch (kernels/optKmatchMask.cl:67-332)
+// and the CPU matcher it mirrors (itkBlockMatcher.txx:46-101,631-848). The
+// score of block b at candidate c is r^2 = Sxy^2 / (Sxx * Syy) with means over
+// all BW^3 voxels and sums over voxels where both masks are on; -2 when a sum of
+// squares is zero (itkBlockMatcher.txx:97-100).
+//
+// Design (not a translation of the OpenCL work-item-per-voxel scheme):
+//  * A search image is summarised once per match by window statistics
+//    (mean and masked centred sum of squares of every BW^3 window, in double).
+//  * A base image is summarised once per match by centred blocks x - mean.
+//  * Fast path (BW=4, BG=1): one workgroup owns a tile of 1 x TY x TZ blocks
+//    stacked in y/z, whose union search window is staged in LDS once, shifted
+//    by a tile reference value. A lane owns KY x NW candidates (one z offset,
+//    KY y offsets, all NW x offsets) and accumulates Sum x~ * y' with one FMA
+//    per voxel-eval, the centred block held in 64 VGPRs and rows of the window
+//    read as 16-byte LDS loads shared by 4*KY*NW FMAs. Sxy, Syy then follow
+//    from the window statistics. Tiles whose masks are not all on go to a
+//    queue served by the masked kernel (exact masked sums, 6 ops/voxel-eval).
+//  * Top-2 per block uses the float score buffer semantics of the reference
+//    with ties on the best broken by the earliest candidate in scan order.
+#include "bm_kernels.hpp"
+
+#include <climits>
+#include <cmath>
+
+namespace mirorr {
+namespace {
+
+constexpr int kThreads = 256;
+
+__device__ __forceinline__ int64_t lin3(int nx, int ny, int x, int y, int z) {
+  return (int64_t)x + (int64_t)nx * ((int64_t)y + (int64_t)ny * z);
+}
+
+// ---------------------------------------------------------------------------
+// Window statistics of a search image.
+template <int BWT>
+__global__ __launch_bounds__(kThreads) void search_stats_kernel(
+    const float* __restrict__ y, const uint8_t* __restrict__ sm, int nx, int ny,
+    int nz, int bw_rt, float2* __restrict__ out) {
+  const int bw = BWT > 0 ? BWT : bw_rt;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int yy = blockIdx.y;
+  const int z = blockIdx.z;
+  if (x > nx - bw || yy > ny - bw || z > nz - bw) return;
+  double sum = 0.0;
+  for (int dz = 0; dz < bw; ++dz)
+    for (int dy = 0; dy < bw; ++dy) {
+      const float* r = y + lin3(nx, ny, x, yy + dy, z + dz);
+#pragma unroll
+      for (int dx = 0; dx < (BWT > 0 ? BWT : 1); ++dx) sum += r[dx];
+      if (BWT == 0)
+        for (int dx = 1; dx < bw; ++dx) sum += r[dx];
+    }
+  const double mu = sum / (double)(bw * bw * bw);
+  double ss = 0.0;
+  for (int dz = 0; dz < bw; ++dz)
+    for (int dy = 0; dy < bw; ++dy) {
+      const int64_t o = lin3(nx, ny, x, yy + dy, z + dz);
+      for (int dx = 0; dx < bw; ++dx) {
+        if (sm && sm[o + dx] == 0) continue;
+        const double d = (double)y[o + dx] - mu;
+        ss += d * d;
+      }
+    }
+  out[lin3(nx, ny, x, yy, z)] = make_float2((float)mu, (float)ss);
+}
+
+// ---------------------------------------------------------------------------
+// Per-block preparation of a base image: one wavefront per block, a lane per
+// voxel (BW^3 <= 64 handled in one pass; larger blocks loop).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ int wave_sum_i(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__global__ __launch_bounds__(kThreads) void block_prep_kernel(
+    const float* __restrict__ x, const uint8_t* __restrict__ bm, BMGrid g,
+    float* __restrict__ xt, float* __restrict__ sxx, float* __restrict__ sumxt,
+    uint32_t* __restrict__ flags) {
+  const int lane = threadIdx.x & 63;
+  const int64_t b = (int64_t)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  if (b >= g.nblocks) return;  // wave-uniform
+  const int bw = g.bw, n = bw * bw * bw;
+  const int bxi = (int)(b % g.nb[0]);
+  const int byi = (int)((b / g.nb[0]) % g.nb[1]);
+  const int bzi = (int)(b / ((int64_t)g.nb[0] * g.nb[1]));
+  const int px = g.start[0] + g.ng * bxi;
+  const int py = g.start[1] + g.ng * byi;
+  const int pz = g.start[2] + g.ng * bzi;
+  double s = 0.0;
+  for (int k = lane; k < n; k += 64) {
+    const int dx = k % bw, dy = (k / bw) % bw, dz = k / (bw * bw);
+    s += x[lin3(g.dims[0], g.dims[1], px + dx, py + dy, pz + dz)];
+  }
+  const double mx = wave_sum(s) / (double)n;
+  double s2 = 0.0, st = 0.0;
+  int on = 0;
+  for (int k = lane; k < n; k += 64) {
+    const int dx = k % bw, dy = (k / bw) % bw, dz = k / (bw * bw);
+    const int64_t o = lin3(g.dims[0], g.dims[1], px + dx, py + dy, pz + dz);
+    const double d = (double)x[o] - mx;
+    const float f = (float)d;
+    xt[b * n + k] = f;
+    s2 += d * d;
+    st += (double)f;
+    on += (bm == nullptr || bm[o] != 0) ? 1 : 0;
+  }
+  s2 = wave_sum(s2);
+  st = wave_sum(st);
+  on = wave_sum_i(on);
+  if (lane == 0) {
+    sxx[b] = (float)s2;
+    sumxt[b] = (float)st;
+    flags[b] = (on == n ? 1u : 0u) | (on == 0 ? 2u : 0u);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Top-2 bookkeeping shared by the kernels.
+struct Top2 {
+  float best;
+  int scan;  // scan-order index of the best candidate
+  float second;
+};
+
+__device__ __forceinline__ void top2_init(Top2& t) {
+  t.best = -3.0f;
+  t.scan = INT_MAX;
+  t.second = -3.0f;
+}
+
+// Candidates reach a lane in increasing scan order: strict '>' keeps the
+// earliest (itkBlockMatcher.txx:823-833 on the float buffer).
+__device__ __forceinline__ void top2_push(Top2& t, float s, int scan) {
+  if (s > t.best) {
+    t.second = t.best;
+    t.best = s;
+    t.scan = scan;
+  } else {
+    t.second = fmaxf(t.second, s);
+  }
+}
+
+// Order-independent merge of per-lane records of one block: best = max
+// score, earliest scan index on ties; second = max of all seconds and of the
+// bests that did not win (equal bests make best == second, as in the
+// reference's sequential update).
+__device__ inline Top2 top2_merge(const float* tb, const int* ti, const float* ts,
+                                  int first, int n) {
+  Top2 r;
+  top2_init(r);
+  int win = -1;
+  for (int j = first; j < first + n; ++j) {
+    if (tb[j] > r.best || (tb[j] == r.best && ti[j] < r.scan)) {
+      r.best = tb[j];
+      r.scan = ti[j];
+      win = j;
+    }
+  }
+  float sec = -3.0f;
+  for (int j = first; j < first + n; ++j) {
+    sec = fmaxf(sec, ts[j]);
+    if (j != win) sec = fmaxf(sec, tb[j]);
+  }
+  r.second = sec;
+  return r;
+}
+
+__device__ __forceinline__ float ncc_score(float sxy, float sxx, float syy) {
+  // -2 when either sum of squares is not positive (itkBlockMatcher.txx:97-98).
+  if (!(sxx > 0.0f) || !(syy > 0.0f)) return -2.0f;
+  return sxy * sxy * __builtin_amdgcn_rcpf(sxx * syy);
+}
+
+// ---------------------------------------------------------------------------
+// Tiled kernels, BW = 4, BG = 1.
+template <int NW_, int KY_, int TY_, int TZ_>
+struct TileCfg {
+  static constexpr int BW = 4;
+  static constexpr int NW = NW_;
+  static constexpr int KY = KY_;
+  static constexpr int TY = TY_;
+  static constexpr int TZ = TZ_;
+  static constexpr int H = NW / 2;
+  static constexpr int NCY = (NW + KY - 1) / KY;
+  static constexpr int TPB = NW * NCY;  // fast-path tasks per block
+  static constexpr int NB = TY * TZ;    // blocks per tile
+  static constexpr int NT = TPB * NB;   // fast-path tasks per tile
+  static constexpr int WX = NW + BW - 1;
+  static constexpr int PX = (WX + 3) & ~3;  // LDS row pitch (floats)
+  static constexpr int NV = PX / 4;         // float4 per row
+  static constexpr int YPAD = NCY * KY - NW;
+  static constexpr int GT = NB * NW * NW;  // masked-path tasks per tile
+  static_assert(NT <= kThreads, "tile too large");
+};
+
+struct TileArgs {
+  const float* y;
+  const uint8_t* sm;
+  const uint8_t* bm;
+  BMGrid g;
+  const float* xt;
+  const float* sxx;
+  const float* sumxt;
+  const uint32_t* bflags;
+  const float2* stats;
+  int ntx, nty, ntz;
+  int32_t* queue;
+  int32_t* qcount;
+  float* out_best;
+  float* out_second;
+  int32_t* out_match;
+};
+
+template <class T>
+__device__ __forceinline__ void tile_origin(const TileArgs& a, int tile, int& bx,
+                                            int& by0, int& bz0) {
+  bx = tile % a.ntx;
+  by0 = ((tile / a.ntx) % a.nty) * T::TY;
+  bz0 = (tile / (a.ntx * a.nty)) * T::TZ;
+}
+
+template <class T>
+__device__ __forceinline__ void write_block_result(const TileArgs& a, int64_t b,
+                                                   int px, int py, int pz,
+                                                   const Top2& r) {
+  const int cx = r.scan % T::NW;
+  const int cy = (r.scan / T::NW) % T::NW;
+  const int cz = r.scan / (T::NW * T::NW);
+  a.out_best[b] = r.best;
+  a.out_second[b] = r.second;
+  a.out_match[3 * b + 0] = px - T::H + cx;
+  a.out_match[3 * b + 1] = py - T::H + cy;
+  a.out_match[3 * b + 2] = pz - T::H + cz;
+}
+
+// Orderable encodings for LDS atomics: max over the key is max over the
+// score, then min over the scan index.
+__device__ __forceinline__ uint32_t ord_f32(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float unord_f32(uint32_t u) {
+  return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+__device__ __forceinline__ unsigned long long best_key(float s, int scan) {
+  return ((unsigned long long)ord_f32(s) << 32) | (unsigned long long)(0xffffffffu - (uint32_t)scan);
+}
+
+// Per-block merge state in LDS.
+struct MergeSlot {
+  unsigned long long best;  // best_key of the winner
+  uint32_t second;          // ord_f32 of the second best
+  uint32_t pad;
+};
+
+__device__ __forceinline__ void merge_init(MergeSlot* m, int nb) {
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    m[i].best = 0ull;
+    m[i].second = ord_f32(-3.0f);
+  }
+}
+// Phase 1: the winner. Phase 2: the second = max of all seconds and of the
+// bests that did not win (equal bests give best == second, as the
+// reference's sequential update does).
+__device__ __forceinline__ void merge_phase1(MergeSlot* m, int bl, const Top2& t) {
+  if (t.scan != INT_MAX) atomicMax(&m[bl].best, best_key(t.best, t.scan));
+}
+__device__ __forceinline__ void merge_phase2(MergeSlot* m, int bl, const Top2& t) {
+  if (t.scan == INT_MAX) return;
+  uint32_t v = ord_f32(t.second);
+  if (best_key(t.best, t.scan) != m[bl].best) v = max(v, ord_f32(t.best));
+  atomicMax(&m[bl].second, v);
+}
+__device__ __forceinline__ Top2 merge_result(const MergeSlot& m) {
+  Top2 r;
+  r.best = unord_f32((uint32_t)(m.best >> 32));
+  r.scan = (int)(0xffffffffu - (uint32_t)(m.best & 0xffffffffull));
+  r.second = unord_f32(m.second);
+  return r;
+}
+
+// Fast path: all base and search masks of the tile are on.
+template <class T>
+__global__ __launch_bounds__(kThreads, 4) void ncc_tile_fast(TileArgs a) {
+  extern __shared__ float4 dyn4[];
+  float* win = reinterpret_cast<float*>(dyn4);
+  __shared__ float4 xs4[T::NB * 16];
+  __shared__ float bsxx[T::NB], bsum[T::NB];
+  __shared__ int bvalid[T::NB];
+  __shared__ MergeSlot mrg[T::NB];
+  __shared__ int s_ok;
+  __shared__ float s_ref;
+
+  const BMGrid& g = a.g;
+  const int nx = g.dims[0], ny = g.dims[1], nz = g.dims[2];
+  const int tid = threadIdx.x;
+  int bx, by0, bz0;
+  tile_origin<T>(a, blockIdx.x, bx, by0, bz0);
+  const int NG = g.ng;
+  const int WY = (T::TY - 1) * NG + T::WX + T::YPAD;
+  const int WZ = (T::TZ - 1) * NG + T::WX;
+  const int px = g.start[0] + NG * bx;
+  const int py0 = g.start[1] + NG * by0;
+  const int pz0 = g.start[2] + NG * bz0;
+  const int wx0 = px - T::H, wy0 = py0 - T::H, wz0 = pz0 - T::H;
+
+  if (tid == 0) {
+    s_ok = 1;
+    // Reference shift: window mean at the tile's middle block (clamped).
+    const int ry = min(py0 + NG * (T::TY / 2), ny - T::BW);
+    const int rz = min(pz0 + NG * (T::TZ / 2), nz - T::BW);
+    s_ref = a.stats[lin3(nx, ny, px, ry, rz)].x;
+  }
+  merge_init(mrg, T::NB);
+  __syncthreads();
+  if (tid < T::NB) {
+    const int gy = by0 + tid % T::TY, gz = bz0 + tid / T::TY;
+    const bool v = gy < g.nb[1] && gz < g.nb[2];
+    bvalid[tid] = v;
+    if (v) {
+      const int64_t b = bx + (int64_t)g.nb[0] * (gy + (int64_t)g.nb[1] * gz);
+      bsxx[tid] = a.sxx[b];
+      bsum[tid] = a.sumxt[b];
+      if ((a.bflags[b] & 1u) == 0u) s_ok = 0;
+    }
+  }
+  const float ref = s_ref;
+  // Stage the union search window (shifted by ref; zeros outside the image).
+  const int total = T::PX * WY * WZ;
+  bool ok = true;
+  for (int i = tid; i < total; i += kThreads) {
+    const int wx = i % T::PX;
+    const int wy = (i / T::PX) % WY;
+    const int wz = i / (T::PX * WY);
+    const int gx = wx0 + wx, gy = wy0 + wy, gz = wz0 + wz;
+    float v = 0.0f;
+    if (wx < T::WX && gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 && gz < nz) {
+      const int64_t o = lin3(nx, ny, gx, gy, gz);
+      v = a.y[o] - ref;
+      if (a.sm && a.sm[o] == 0) ok = false;
+    }
+    win[i] = v;
+  }
+  if (!ok) s_ok = 0;
+  for (int i = tid; i < T::NB * 16; i += kThreads) {
+    const int bl = i / 16;
+    const int gy = by0 + bl % T::TY, gz = bz0 + bl / T::TY;
+    if (gy < g.nb[1] && gz < g.nb[2]) {
+      const int64_t b = bx + (int64_t)g.nb[0] * (gy + (int64_t)g.nb[1] * gz);
+      xs4[i] = reinterpret_cast<const float4*>(a.xt)[b * 16 + (i % 16)];
+    }
+  }
+  __syncthreads();
+  if (!s_ok) {
+    if (tid == 0) {
+      const int q = atomicAdd(a.qcount, 1);
+      a.queue[q] = blockIdx.x;
+    }
+    return;  // workgroup-uniform
+  }
+
+  Top2 t2;
+  top2_init(t2);
+  const int bl = tid / T::TPB;
+  if (tid < T::NT && bvalid[bl]) {
+    const int r = tid % T::TPB;
+    const int cz = r / T::NCY;
+    const int cyg = r % T::NCY;
+    const int bly = bl % T::TY, blz = bl / T::TY;
+    float acc[T::KY][T::NW];
+#pragma unroll
+    for (int ky = 0; ky < T::KY; ++ky)
+#pragma unroll
+      for (int cx = 0; cx < T::NW; ++cx) acc[ky][cx] = 0.0f;
+    const float* rowbase = win + ((blz * NG + cz) * WY + (bly * NG + cyg * T::KY)) * T::PX;
+#pragma unroll 1
+    for (int iz = 0; iz < 4; ++iz) {
+      // One z-plane of the centred block (16 values) stays in VGPRs.
+      float xp[16];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const float4 q = xs4[bl * 16 + iz * 4 + k];
+        xp[4 * k + 0] = q.x;
+        xp[4 * k + 1] = q.y;
+        xp[4 * k + 2] = q.z;
+        xp[4 * k + 3] = q.w;
+      }
+      const float* plane = rowbase + iz * WY * T::PX;
+#pragma unroll
+      for (int rr = 0; rr < T::KY + 3; ++rr) {
+        float row[T::PX];
+        const float4* rp = reinterpret_cast<const float4*>(plane + rr * T::PX);
+#pragma unroll
+        for (int v = 0; v < T::NV; ++v) {
+          const float4 q = rp[v];
+          row[4 * v + 0] = q.x;
+          row[4 * v + 1] = q.y;
+          row[4 * v + 2] = q.z;
+          row[4 * v + 3] = q.w;
+        }
+#pragma unroll
+        for (int ky = 0; ky < T::KY; ++ky) {
+          const int iy = rr - ky;
+          if (iy < 0 || iy > 3) continue;
+#pragma unroll
+          for (int cx = 0; cx < T::NW; ++cx)
+#pragma unroll
+            for (int ix = 0; ix < 4; ++ix)
+              acc[ky][cx] = fmaf(xp[iy * 4 + ix], row[cx + ix], acc[ky][cx]);
+        }
+      }
+    }
+    // Epilogue: scores of the lane's candidates, in scan order.
+    const int py = py0 + NG * bly, pz = pz0 + NG * blz;
+    const int lox = max(px - T::H, 0), hix = min(px - T::H + T::NW - 1, nx - T::BW);
+    const int loy = max(py - T::H, 0), hiy = min(py - T::H + T::NW - 1, ny - T::BW);
+    const int loz = max(pz - T::H, 0), hiz = min(pz - T::H + T::NW - 1, nz - T::BW);
+    const int czg = pz - T::H + cz;
+    const float sxx = bsxx[bl], sumx = bsum[bl];
+    if (czg >= loz && czg <= hiz) {
+#pragma unroll
+      for (int ky = 0; ky < T::KY; ++ky) {
+        const int cy = cyg * T::KY + ky;
+        const int cyv = py - T::H + cy;
+        if (cy >= T::NW || cyv < loy || cyv > hiy) continue;
+        const float2* srow = a.stats + lin3(nx, ny, 0, cyv, czg);
+#pragma unroll
+        for (int cx = 0; cx < T::NW; ++cx) {
+          const int cxg = px - T::H + cx;
+          if (cxg < lox || cxg > hix) continue;
+          const float2 st = srow[cxg];
+          const float sxy = fmaf(-(st.x - ref), sumx, acc[ky][cx]);
+          const float s = ncc_score(sxy, sxx, st.y);
+          top2_push(t2, s, (cz * T::NW + cy) * T::NW + cx);
+        }
+      }
+    }
+  }
+  if (tid < T::NT) merge_phase1(mrg, bl, t2);
+  __syncthreads();
+  if (tid < T::NT) merge_phase2(mrg, bl, t2);
+  __syncthreads();
+  if (tid < T::NB && bvalid[tid]) {
+    const Top2 r = merge_result(mrg[tid]);
+    const int gy = by0 + tid % T::TY, gz = bz0 + tid / T::TY;
+    const int64_t b = bx + (int64_t)g.nb[0] * (gy + (int64_t)g.nb[1] * gz);
+    write_block_result<T>(a, b, px, py0 + NG * (tid % T::TY), pz0 + NG * (tid / T::TY), r);
+  }
+}
+
+// Masked path: exact sums over voxels where both masks are on, centred per
+// candidate on the window mean (about 6 VALU ops per voxel-eval).
+template <class T>
+__global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
+  constexpr int R = (T::GT + kThreads - 1) / kThreads;  // tasks per thread
+  extern __shared__ float4 dyn4[];
+  __shared__ float4 xh4[T::NB * 16];  // base-masked centred blocks
+  __shared__ unsigned long long bbits[T::NB];  // base mask bit per voxel
+  __shared__ int bvalid[T::NB];
+  __shared__ MergeSlot mrg[T::NB];
+
+  if (blockIdx.x >= *a.qcount) return;
+  const int tile = a.queue[blockIdx.x];
+  const BMGrid& g = a.g;
+  const int nx = g.dims[0], ny = g.dims[1], nz = g.dims[2];
+  const int tid = threadIdx.x;
+  int bx, by0, bz0;
+  tile_origin<T>(a, tile, bx, by0, bz0);
+  const int NG = g.ng;
+  const int WY = (T::TY - 1) * NG + T::WX;
+  const int WZ = (T::TZ - 1) * NG + T::WX;
+  const int px = g.start[0] + NG * bx;
+  const int py0 = g.start[1] + NG * by0;
+  const int pz0 = g.start[2] + NG * bz0;
+  const int wx0 = px - T::H, wy0 = py0 - T::H, wz0 = pz0 - T::H;
+  const int total = T::PX * WY * WZ;
+  float* wy_ = reinterpret_cast<float*>(dyn4);
+  uint8_t* wv_ = reinterpret_cast<uint8_t*>(wy_ + total);
+
+  merge_init(mrg, T::NB);
+  for (int i = tid; i < total; i += kThreads) {
+    const int wx = i % T::PX;
+    const int wy = (i / T::PX) % WY;
+    const int wz = i / (T::PX * WY);
+    const int gx = wx0 + wx, gy = wy0 + wy, gz = wz0 + wz;
+    float v = 0.0f;
+    uint8_t m = 0;
+    if (wx < T::WX && gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 && gz < nz) {
+      const int64_t o = lin3(nx, ny, gx, gy, gz);
+      v = a.y[o];
+      m = (a.sm == nullptr || a.sm[o] != 0) ? 1 : 0;
+    }
+    wy_[i] = v;
+    wv_[i] = m;
+  }
+  if (tid < T::NB) {
+    const int gy = by0 + tid % T::TY, gz = bz0 + tid / T::TY;
+    bvalid[tid] = gy < g.nb[1] && gz < g.nb[2];
+    bbits[tid] = 0ull;
+  }
+  __syncthreads();
+  for (int i = tid; i < T::NB * 64; i += kThreads) {
+    const int bl = i / 64, k = i % 64;
+    const int gy = by0 + bl % T::TY, gz = bz0 + bl / T::TY;
+    float xh = 0.0f;
+    if (gy < g.nb[1] && gz < g.nb[2]) {
+      const int64_t b = bx + (int64_t)g.nb[0] * (gy + (int64_t)g.nb[1] * gz);
+      bool on = true;
+      if (a.bm != nullptr && (a.bflags[b] & 1u) == 0u) {
+        const int qy = py0 + NG * (bl % T::TY), qz = pz0 + NG * (bl / T::TY);
+        on = a.bm[lin3(nx, ny, px + (k & 3), qy + ((k >> 2) & 3), qz + (k >> 4))] != 0;
+      }
+      if (on) {
+        xh = a.xt[b * 64 + k];
+        atomicOr(&bbits[bl], 1ull << k);
+      }
+    }
+    reinterpret_cast<float*>(xh4)[i] = xh;
+  }
+  __syncthreads();
+
+  Top2 rec[R];
+  int recb[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    top2_init(rec[j]);
+    recb[j] = 0;
+  }
+#pragma unroll 1
+  for (int j = 0; j < R; ++j) {
+    const int t = tid + j * kThreads;
+    if (t >= T::GT) break;
+    const int bl = t / (T::NW * T::NW);
+    const int r = t % (T::NW * T::NW);
+    const int cz = r / T::NW;
+    const int cy = r % T::NW;
+    recb[j] = bl;
+    if (!bvalid[bl]) continue;
+    const int bly = bl % T::TY, blz = bl / T::TY;
+    const int py = py0 + NG * bly, pz = pz0 + NG * blz;
+    const int lox = max(px - T::H, 0), hix = min(px - T::H + T::NW - 1, nx - T::BW);
+    const int loy = max(py - T::H, 0), hiy = min(py - T::H + T::NW - 1, ny - T::BW);
+    const int loz = max(pz - T::H, 0), hiz = min(pz - T::H + T::NW - 1, nz - T::BW);
+    const int czg = pz - T::H + cz, cyg = py - T::H + cy;
+    if (czg < loz || czg > hiz || cyg < loy || cyg > hiy) continue;
+    float myc[T::NW];
+    const float2* srow = a.stats + lin3(nx, ny, 0, cyg, czg);
+#pragma unroll
+    for (int cx = 0; cx < T::NW; ++cx) {
+      const int cxg = min(max(px - T::H + cx, lox), hix);
+      myc[cx] = srow[cxg].x;
+    }
+    float sxy[T::NW], syy[T::NW], sxx[T::NW];
+#pragma unroll
+    for (int cx = 0; cx < T::NW; ++cx) sxy[cx] = syy[cx] = sxx[cx] = 0.0f;
+    const unsigned long long bits = bbits[bl];
+    const int rowoff = ((blz * NG + cz) * WY + (bly * NG + cy)) * T::PX;
+#pragma unroll 1
+    for (int iz = 0; iz < 4; ++iz) {
+#pragma unroll
+      for (int iy = 0; iy < 4; ++iy) {
+        const int ro = rowoff + (iz * WY + iy) * T::PX;
+        float yr[T::PX], vr[T::PX];
+        const float4* yp = reinterpret_cast<const float4*>(wy_ + ro);
+        const uint32_t* vp = reinterpret_cast<const uint32_t*>(wv_ + ro);
+#pragma unroll
+        for (int v = 0; v < T::NV; ++v) {
+          const float4 q = yp[v];
+          const uint32_t m = vp[v];
+          yr[4 * v + 0] = q.x; yr[4 * v + 1] = q.y; yr[4 * v + 2] = q.z; yr[4 * v + 3] = q.w;
+          vr[4 * v + 0] = (float)(m & 0xffu);
+          vr[4 * v + 1] = (float)((m >> 8) & 0xffu);
+          vr[4 * v + 2] = (float)((m >> 16) & 0xffu);
+          vr[4 * v + 3] = (float)(m >> 24);
+        }
+        const float4 xq4 = xh4[bl * 16 + iz * 4 + iy];
+        const float xh[4] = {xq4.x, xq4.y, xq4.z, xq4.w};
+        const uint32_t nib = (uint32_t)(bits >> (iz * 16 + iy * 4)) & 0xfu;
+        float bb[4], xs[4];
+#pragma unroll
+        for (int ix = 0; ix < 4; ++ix) {
+          bb[ix] = (float)((nib >> ix) & 1u);
+          xs[ix] = xh[ix] * xh[ix];
+        }
+#pragma unroll
+        for (int cx = 0; cx < T::NW; ++cx)
+#pragma unroll
+          for (int ix = 0; ix < 4; ++ix) {
+            const float tm = (yr[cx + ix] - myc[cx]) * vr[cx + ix];
+            sxy[cx] = fmaf(xh[ix], tm, sxy[cx]);
+            syy[cx] = fmaf(tm * bb[ix], tm, syy[cx]);
+            sxx[cx] = fmaf(xs[ix], vr[cx + ix], sxx[cx]);
+          }
+      }
+    }
+#pragma unroll
+    for (int cx = 0; cx < T::NW; ++cx) {
+      const int cxg = px - T::H + cx;
+      if (cxg < lox || cxg > hix) continue;
+      top2_push(rec[j], ncc_score(sxy[cx], sxx[cx], syy[cx]), (cz * T::NW + cy) * T::NW + cx);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < R; ++j) merge_phase1(mrg, recb[j], rec[j]);
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < R; ++j) merge_phase2(mrg, recb[j], rec[j]);
+  __syncthreads();
+  if (tid < T::NB && bvalid[tid]) {
+    const Top2 r = merge_result(mrg[tid]);
+    const int gy = by0 + tid % T::TY, gz = bz0 + tid / T::TY;
+    const int64_t b = bx + (int64_t)g.nb[0] * (gy + (int64_t)g.nb[1] * gz);
+    write_block_result<T>(a, b, px, py0 + NG * (tid % T::TY), pz0 + NG * (tid / T::TY), r);
+  }
+}
+
+__global__ void reset_counter(int32_t* c) { *c = 0; }
+
+// ---------------------------------------------------------------------------
+// Generic kernel: any BW, BG, NW. One workgroup per block; candidates in
+// scan order strided over the threads; double two-pass sums as the reference.
+__global__ __launch_bounds__(kThreads) void ncc_generic_kernel(
+    const float* __restrict__ x, const uint8_t* __restrict__ bm,
+    const float* __restrict__ y, const uint8_t* __restrict__ sm, BMGrid g,
+    float* out_best, float* out_second, int32_t* out_match) {
+  __shared__ float tb[kThreads], ts[kThreads];
+  __shared__ int ti[kThreads];
+  const int64_t b = blockIdx.x;
+  const int nx = g.dims[0], ny = g.dims[1], nz = g.dims[2];
+  const int bxi = (int)(b % g.nb[0]);
+  const int byi = (int)((b / g.nb[0]) % g.nb[1]);
+  const int bzi = (int)(b / ((int64_t)g.nb[0] * g.nb[1]));
+  const int p[3] = {g.start[0] + g.ng * bxi, g.start[1] + g.ng * byi, g.start[2] + g.ng * bzi};
+  int first[3], cnt[3];
+  for (int k = 0; k < 3; ++k) cand_range(p[k], g.dims[k], g.bw, g.bg, g.nw, &first[k], &cnt[k]);
+  const int bw = g.bw;
+  const double n = (double)bw * bw * bw;
+  double mb = 0.0;
+  for (int dz = 0; dz < bw; ++dz)
+    for (int dy = 0; dy < bw; ++dy)
+      for (int dx = 0; dx < bw; ++dx) mb += x[lin3(nx, ny, p[0] + dx, p[1] + dy, p[2] + dz)];
+  mb /= n;
+  Top2 t2;
+  top2_init(t2);
+  const int total = cnt[0] * cnt[1] * cnt[2];
+  for (int k = threadIdx.x; k < total; k += kThreads) {
+    const int kx = k % cnt[0], ky = (k / cnt[0]) % cnt[1], kz = k / (cnt[0] * cnt[1]);
+    const int s[3] = {first[0] + kx * g.bg, first[1] + ky * g.bg, first[2] + kz * g.bg};
+    double ms = 0.0;
+    for (int dz = 0; dz < bw; ++dz)
+      for (int dy = 0; dy < bw; ++dy)
+        for (int dx = 0; dx < bw; ++dx) ms += y[lin3(nx, ny, s[0] + dx, s[1] + dy, s[2] + dz)];
+    ms /= n;
+    double sp = 0.0, sb = 0.0, ss = 0.0;
+    for (int dz = 0; dz < bw; ++dz)
+      for (int dy = 0; dy < bw; ++dy)
+        for (int dx = 0; dx < bw; ++dx) {
+          const int64_t ob = lin3(nx, ny, p[0] + dx, p[1] + dy, p[2] + dz);
+          const int64_t os = lin3(nx, ny, s[0] + dx, s[1] + dy, s[2] + dz);
+          if ((bm && bm[ob] == 0) || (sm && sm[os] == 0)) continue;
+          const double u = (double)x[ob] - mb, v = (double)y[os] - ms;
+          sp += u * v;
+          sb += u * u;
+          ss += v * v;
+        }
+    const float sc = (sb <= 0.0 || ss <= 0.0) ? -2.0f : (float)(sp * sp / sb / ss);
+    top2_push(t2, sc, k);
+  }
+  tb[threadIdx.x] = t2.best;
+  ti[threadIdx.x] = t2.scan;
+  ts[threadIdx.x] = t2.second;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const Top2 r = top2_merge(tb, ti, ts, 0, kThreads);
+    const int k = r.scan;
+    const int kx = k % cnt[0], ky = (k / cnt[0]) % cnt[1], kz = k / (cnt[0] * cnt[1]);
+    out_best[b] = r.best;
+    out_second[b] = r.second;
+    out_match[3 * b + 0] = first[0] + kx * g.bg;
+    out_match[3 * b + 1] = first[1] + ky * g.bg;
+    out_match[3 * b + 2] = first[2] + kz * g.bg;
+  }
+}
+
+// ---------------------------------------------------------------------------
+__global__ void postprocess_kernel(BMGrid g, Geom geom, const float* __restrict__ best,
+                                   const float* __restrict__ second,
+                                   const int32_t* __restrict__ match, uint8_t* kept,
+                                   double* base_mm, double* match_mm) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= g.nblocks) return;
+  const double cb = best[b], c2 = second[b];
+  kept[b] = (cb > 1e-7 && fabs(cb - c2) > 1e-4) ? 1 : 0;
+  const int bxi = (int)(b % g.nb[0]);
+  const int byi = (int)((b / g.nb[0]) % g.nb[1]);
+  const int bzi = (int)(b / ((int64_t)g.nb[0] * g.nb[1]));
+  const int p[3] = {g.start[0] + g.ng * bxi, g.start[1] + g.ng * byi, g.start[2] + g.ng * bzi};
+  const int m[3] = {match[3 * b], match[3 * b + 1], match[3 * b + 2]};
+  const double half = 0.5 * g.bw;
+  for (int r = 0; r < 3; ++r) {
+    double pb = geom.origin[r], pm = geom.origin[r];
+    for (int c = 0; c < 3; ++c) {
+      const double dsc = geom.dir[3 * r + c] * geom.spacing[c];
+      pb += dsc * (double)p[c];
+      pm += dsc * (double)m[c];
+    }
+    base_mm[3 * b + r] = pb + geom.spacing[r] * half;
+    match_mm[3 * b + r] = pm + geom.spacing[r] * half;
+  }
+}
+
+// ---------------------------------------------------------------------------
+constexpr int kChunk = 1024;  // elements per compaction workgroup
+
+__global__ __launch_bounds__(kThreads) void compact_count(const uint8_t* f, int64_t n,
+                                                         int32_t* counts) {
+  __shared__ int s;
+  if (threadIdx.x == 0) s = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * kChunk;
+  int c = 0;
+  for (int k = threadIdx.x; k < kChunk; k += kThreads) {
+    const int64_t i = base + k;
+    if (i < n && f[i]) ++c;
+  }
+  atomicAdd(&s, c);
+  __syncthreads();
+  if (threadIdx.x == 0) counts[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(1024) void compact_scan(const int32_t* counts, int nchunks,
+                                                     int32_t* offsets) {
+  __shared__ int32_t sh[1024];
+  __shared__ int32_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < nchunks; base += 1024) {
+    const int i = base + threadIdx.x;
+    const int32_t v = i < nchunks ? counts[i] : 0;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+      const int32_t t = threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
+      __syncthreads();
+      sh[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < nchunks) offsets[i] = carry + sh[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += sh[1023];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) offsets[nchunks] = carry;
+}
+
+__global__ __launch_bounds__(kThreads) void compact_scatter(const uint8_t* f, int64_t n,
+                                                           const int32_t* offsets,
+                                                           int32_t* out) {
+  // Each thread owns 4 consecutive elements of the chunk.
+  __shared__ int32_t sh[kThreads];
+  const int64_t base = (int64_t)blockIdx.x * kChunk + 4 * threadIdx.x;
+  int flags[4], c = 0;
+  for (int k = 0; k < 4; ++k) {
+    const int64_t i = base + k;
+    flags[k] = (i < n && f[i]) ? 1 : 0;
+    c += flags[k];
+  }
+  sh[threadIdx.x] = c;
+  __syncthreads();
+  for (int o = 1; o < kThreads; o <<= 1) {
+    const int32_t t = threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += t;
+    __syncthreads();
+  }
+  int pos = offsets[blockIdx.x] + sh[threadIdx.x] - c;
+  for (int k = 0; k < 4; ++k)
+    if (flags[k]) out[pos++] = (int32_t)(base + k);
+}
+
+// ---------------------------------------------------------------------------
+// Tile configurations per neighbourhood width (BW = 4, BG = 1).
+using Cfg5 = TileCfg<5, 5, 5, 10>;
+using Cfg7 = TileCfg<7, 4, 3, 6>;
+using Cfg9 = TileCfg<9, 3, 3, 3>;
+using Cfg11 = TileCfg<11, 4, 7, 1>;
+using Cfg13 = TileCfg<13, 5, 2, 3>;
+
+template <class T>
+int64_t tiles_of(const BMGrid& g) {
+  const int64_t nty = (g.nb[1] + T::TY - 1) / T::TY;
+  const int64_t ntz = (g.nb[2] + T::TZ - 1) / T::TZ;
+  return (int64_t)g.nb[0] * nty * ntz;
+}
+
+template <class T>
+void run_tiled(const BMGrid& g, TileArgs a, BMScratch& scr, hipStream_t s) {
+  a.ntx = g.nb[0];
+  a.nty = (g.nb[1] + T::TY - 1) / T::TY;
+  a.ntz = (g.nb[2] + T::TZ - 1) / T::TZ;
+  const int64_t nt = (int64_t)a.ntx * a.nty * a.ntz;
+  if (nt > scr.ntiles_cap) throw Error(kErrState, "tile queue too small");
+  a.queue = scr.queue;
+  a.qcount = scr.count;
+  hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, s, scr.count);
+  const int WYf = (T::TY - 1) * g.ng + T::WX + T::YPAD;
+  const int WZ = (T::TZ - 1) * g.ng + T::WX;
+  const size_t lds_fast = sizeof(float) * (size_t)T::PX * WYf * WZ;
+  hipLaunchKernelGGL(ncc_tile_fast<T>, dim3((unsigned)nt), dim3(kThreads), lds_fast, s, a);
+  MIRORR_CHECK_LAUNCH();
+  const int WYm = (T::TY - 1) * g.ng + T::WX;
+  const size_t lds_masked = (sizeof(float) + 1) * (size_t)T::PX * WYm * WZ;
+  hipLaunchKernelGGL(ncc_tile_masked<T>, dim3((unsigned)nt), dim3(kThreads), lds_masked, s, a);
+  MIRORR_CHECK_LAUNCH();
+}
+
+}  // namespace
+
+template <class T>
+bool lds_fits(int ng) {
+  // Dynamic windows plus the static arrays of each kernel, within 64 KB.
+  const size_t wyf = (T::TY - 1) * ng + T::WX + T::YPAD, wym = (T::TY - 1) * ng + T::WX;
+  const size_t wz = (T::TZ - 1) * ng + T::WX;
+  const size_t fast = 4 * T::PX * wyf * wz + T::NB * (16 * 16 + 12 + 16) + 16;
+  const size_t masked = 5 * T::PX * wym * wz + T::NB * (16 * 16 + 8 + 4 + 16);
+  return fast <= 65536 && masked <= 65536;
+}
+
+bool tiled_supported(const BMGrid& g) {
+  if (g.bw != 4 || g.bg != 1 || g.ng < 1) return false;
+  switch (g.nw) {
+    case 5: return lds_fits<Cfg5>(g.ng);
+    case 7: return lds_fits<Cfg7>(g.ng);
+    case 9: return lds_fits<Cfg9>(g.ng);
+    case 11: return lds_fits<Cfg11>(g.ng);
+    case 13: return lds_fits<Cfg13>(g.ng);
+    default: return false;
+  }
+}
+
+int64_t tile_count(const BMGrid& g) {
+  switch (g.nw) {
+    case 5: return tiles_of<Cfg5>(g);
+    case 7: return tiles_of<Cfg7>(g);
+    case 9: return tiles_of<Cfg9>(g);
+    case 11: return tiles_of<Cfg11>(g);
+    case 13: return tiles_of<Cfg13>(g);
+    default: return 0;
+  }
+}
+
+void launch_search_stats(const float* y, const uint8_t* smask, const BMGrid& g,
+                         SearchStats st, hipStream_t s) {
+  const int nx = g.dims[0], ny = g.dims[1], nz = g.dims[2];
+  if (nx < g.bw || ny < g.bw || nz < g.bw) return;
+  dim3 grid((nx + kThreads - 1) / kThreads, ny - g.bw + 1, nz - g.bw + 1);
+  if (g.bw == 4)
+    hipLaunchKernelGGL(search_stats_kernel<4>, grid, dim3(kThreads), 0, s, y, smask, nx, ny, nz, 4, st.ms);
+  else
+    hipLaunchKernelGGL(search_stats_kernel<0>, grid, dim3(kThreads), 0, s, y, smask, nx, ny, nz, g.bw, st.ms);
+  MIRORR_CHECK_LAUNCH();
+}
+
+void launch_block_prep(const float* x, const uint8_t* bmask, const BMGrid& g,
+                       BlockPrep bp, hipStream_t s) {
+  if (g.nblocks == 0) return;
+  const int waves = kThreads / 64;
+  dim3 grid((unsigned)((g.nblocks + waves - 1) / waves));
+  hipLaunchKernelGGL(block_prep_kernel, grid, dim3(kThreads), 0, s, x, bmask, g, bp.xt, bp.sxx,
+                     bp.sumxt, bp.flags);
+  MIRORR_CHECK_LAUNCH();
+}
+
+void launch_ncc_generic(const float* x, const uint8_t* bmask, const float* y,
+                        const uint8_t* smask, const BMGrid& g, BMResult res,
+                        hipStream_t s) {
+  if (g.nblocks == 0) return;
+  hipLaunchKernelGGL(ncc_generic_kernel, dim3((unsigned)g.nblocks), dim3(kThreads), 0, s, x,
+                     bmask, y, smask, g, res.best, res.second, res.match);
+  MIRORR_CHECK_LAUNCH();
+}
+
+void launch_ncc_search(const float* x, const uint8_t* bmask, const float* y,
+                       const uint8_t* smask, const BMGrid& g, const BlockPrep& bp,
+                       const SearchStats& st, BMScratch& scr, BMResult res,
+                       hipStream_t s) {
+  if (g.nblocks == 0) return;
+  if (!tiled_supported(g)) {
+    launch_ncc_generic(x, bmask, y, smask, g, res, s);
+    return;
+  }
+  TileArgs a{};
+  a.y = y;
+  a.sm = smask;
+  a.bm = bmask;
+  a.g = g;
+  a.xt = bp.xt;
+  a.sxx = bp.sxx;
+  a.sumxt = bp.sumxt;
+  a.bflags = bp.flags;
+  a.stats = st.ms;
+  a.out_best = res.best;
+  a.out_second = res.second;
+  a.out_match = res.match;
+  switch (g.nw) {
+    case 5: run_tiled<Cfg5>(g, a, scr, s); break;
+    case 7: run_tiled<Cfg7>(g, a, scr, s); break;
+    case 9: run_tiled<Cfg9>(g, a, scr, s); break;
+    case 11: run_tiled<Cfg11>(g, a, scr, s); break;
+    case 13: run_tiled<Cfg13>(g, a, scr, s); break;
+  }
+}
+
+void launch_postprocess(const BMGrid& g, const Geom& geom, const BMResult& res,
+                        uint8_t* kept, double* base_mm, double* match_mm,
+                        hipStream_t s) {
+  if (g.nblocks == 0) return;
+  hipLaunchKernelGGL(postprocess_kernel, dim3((unsigned)((g.nblocks + kThreads - 1) / kThreads)),
+                     dim3(kThreads), 0, s, g, geom, res.best, res.second, res.match, kept,
+                     base_mm, match_mm);
+  MIRORR_CHECK_LAUNCH();
+}
+
+int64_t compact_chunks(int64_t n) { return (n + kChunk - 1) / kChunk; }
+
+void launch_compact_index(const uint8_t* flags, int64_t n, Compactor& c,
+                          int32_t* out_index, hipStream_t s) {
+  const int64_t nc = compact_chunks(n);
+  if (nc == 0) return;
+  if (nc > c.cap_chunks) throw Error(kErrState, "compaction scratch too small");
+  hipLaunchKernelGGL(compact_count, dim3((unsigned)nc), dim3(kThreads), 0, s, flags, n,
+                     c.chunk_counts);
+  hipLaunchKernelGGL(compact_scan, dim3(1), dim3(1024), 0, s, c.chunk_counts, (int)nc,
+                     c.chunk_offsets);
+  hipLaunchKernelGGL(compact_scatter, dim3((unsigned)nc), dim3(kThreads), 0, s, flags, n,
+                     c.chunk_offsets, out_index);
+  MIRORR_CHECK_LAUNCH();
+}
+
+}  // namespace mirorr

@@ -1,0 +1,71 @@
+// Launchers of the block-matching kernels (implemented in bm_kernels.hip).
+#pragma once
+
+#include "common.hpp"
+
+namespace mirorr {
+
+// Per-block preparation of a base image (centred block, sums, mask bits).
+struct BlockPrep {
+  float* xt = nullptr;        // nblocks x BW^3 centred voxels x - mean (float)
+  float* sxx = nullptr;       // nblocks: sum (x - mean)^2 over all voxels
+  float* sumxt = nullptr;     // nblocks: sum of the float centred voxels
+  uint32_t* flags = nullptr;  // nblocks: bit0 base mask all on, bit1 all off
+};
+
+// Per-position window statistics of a search image (window = BW^3 voxels
+// starting at the position): x = mean over all voxels, y = sum over
+// search-mask-on voxels of (v - mean)^2. Computed in double, stored float.
+struct SearchStats {
+  float2* ms = nullptr;  // dims[0]*dims[1]*dims[2]
+};
+
+// Raw per-block results in block raster order.
+struct BMResult {
+  float* best = nullptr;
+  float* second = nullptr;
+  int32_t* match = nullptr;  // nblocks x 3 voxel index of the matched corner
+};
+
+// Working memory of one search (tile queue for the masked path).
+struct BMScratch {
+  int32_t* queue = nullptr;  // ntiles entries
+  int32_t* count = nullptr;  // 1 counter
+  int64_t ntiles_cap = 0;
+};
+
+void launch_search_stats(const float* y, const uint8_t* smask, const BMGrid& g,
+                         SearchStats st, hipStream_t s);
+void launch_block_prep(const float* x, const uint8_t* bmask, const BMGrid& g,
+                       BlockPrep bp, hipStream_t s);
+// Exhaustive NCC search. Chooses the tiled fast kernels for BW=4, BG=1 and a
+// supported NW, else the generic kernel.
+void launch_ncc_search(const float* x, const uint8_t* bmask, const float* y,
+                       const uint8_t* smask, const BMGrid& g, const BlockPrep& bp,
+                       const SearchStats& st, BMScratch& scratch, BMResult res,
+                       hipStream_t s);
+// Generic kernel only (any BW/BG/NW), double accumulation; used for
+// unsupported shapes and as an independent device cross-check.
+void launch_ncc_generic(const float* x, const uint8_t* bmask, const float* y,
+                        const uint8_t* smask, const BMGrid& g, BMResult res,
+                        hipStream_t s);
+int64_t tile_count(const BMGrid& g);
+bool tiled_supported(const BMGrid& g);
+
+// Score filter + mm conversion (itkBlockMatcher.txx:886-916,
+// itkAbstractBlockMatcher.h:271-287). kept[i] in {0,1}; pts are nblocks x 3.
+void launch_postprocess(const BMGrid& g, const Geom& geom, const BMResult& res,
+                        uint8_t* kept, double* base_mm, double* match_mm,
+                        hipStream_t s);
+
+// Ordered compaction of nblocks records by a 0/1 flag array.
+struct Compactor {
+  int32_t* chunk_counts = nullptr;  // nchunks
+  int32_t* chunk_offsets = nullptr; // nchunks + 1
+  int64_t cap_chunks = 0;
+};
+int64_t compact_chunks(int64_t n);
+void launch_compact_index(const uint8_t* flags, int64_t n, Compactor& c,
+                          int32_t* out_index, hipStream_t s);
+
+}  // namespace mirorr

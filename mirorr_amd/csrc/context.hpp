@@ -1,0 +1,53 @@
+// Context object behind the C-ABI handle mirorr_ctx.
+#pragma once
+
+#include <mutex>
+#include <string>
+
+#include "../../include/mirorr_gpu.h"
+#include "device_buffer.hpp"
+
+struct mirorr_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  mirorr::Budget budget;
+  std::string last_error;
+  std::mutex mu;
+};
+
+namespace mirorr {
+
+// Makes the context's device current for the calling host thread.
+inline void use_device(mirorr_ctx* c) { MIRORR_HIP(hipSetDevice(c->device)); }
+
+// Runs f(), converting library errors into a status recorded on the context.
+template <class F>
+mirorr_status guarded(mirorr_ctx* c, F&& f) {
+  try {
+    if (c) use_device(c);
+    f();
+    return MIRORR_OK;
+  } catch (const Error& e) {
+    if (c) c->last_error = e.what();
+    return static_cast<mirorr_status>(e.code);
+  } catch (const std::bad_alloc&) {
+    if (c) c->last_error = "host allocation failed";
+    return MIRORR_ERR_NOMEM;
+  } catch (const std::exception& e) {
+    if (c) c->last_error = e.what();
+    return MIRORR_ERR_PLATFORM;
+  }
+}
+
+inline Geom to_geom(const mirorr_geom& g) {
+  Geom o{};
+  for (int i = 0; i < 3; ++i) {
+    o.dims[i] = g.dims[i];
+    o.spacing[i] = g.spacing[i];
+    o.origin[i] = g.origin[i];
+  }
+  for (int i = 0; i < 9; ++i) o.dir[i] = g.direction[i];
+  return o;
+}
+
+}  // namespace mirorr
