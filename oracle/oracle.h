@@ -59,6 +59,23 @@ int orc_bm_get_offsets(const float* base, const uint8_t* base_mask,
 int64_t orc_bm_candidate_count(const int dims[3], int bw, int bg, int nw,
                                int ng);
 
+// ---- Resampling and pyramid (imaging.cpp) -------------------------------
+void orc_resample(const float* in, const uint8_t* in_mask, const orc_geom* gin,
+                  const double M[9], const double off[3], const orc_geom* gout,
+                  float* out, uint8_t* out_mask);
+int orc_gaussian_kernel(double variance, double max_error, int max_width, double* out,
+                        int cap);
+void orc_gaussian_smooth(const float* in, const int dims[3], const double var[3],
+                         double max_error, float* out);
+void orc_pyramid_level_geom(const orc_geom* in, const int factors[3], orc_geom* out);
+void orc_pyramid_step(const float* prev, const orc_geom* prev_geom, const int ratio[3],
+                      const orc_geom* out_geom, float* out);
+int orc_schedule_symmetric(const int msize[3], const double mspacing[3], const int fsize[3],
+                           const double fspacing[3], unsigned min_length, unsigned max_levels,
+                           int* msched, int* fsched, int cap);
+int orc_cropped_level(int level, int default_level, int total_levels);
+void orc_cropped_min_max(int level_min, int level_max, int total, int* min_out, int* max_out);
+
 #ifdef __cplusplus
 }
 #endif

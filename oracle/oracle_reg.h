@@ -1,0 +1,50 @@
+// Oracle registration API (test infrastructure only; see oracle.h).
+#pragma once
+#include <cstdint>
+
+#include "oracle.h"
+
+#ifdef __cplusplus
+#include <vector>
+
+#include "xform.hpp"
+namespace orc {
+extern int g_verbosity;
+int lts_classic(Xf& T, const std::vector<double>& X, const std::vector<double>& Y);
+int lts_symmetric(Xf& T, Xf& t12F, Xf& t12B, const std::vector<double>& X,
+                  const std::vector<double>& Y);
+}  // namespace orc
+extern "C" {
+#endif
+
+typedef struct orc_reg_params {
+  int symmetric;     // --reg-mode symmetric (1, default) or classic (0)
+  int affine;        // -t affine (1) or rigid Euler3D (0)
+  int bw, bg, nw, ng;
+  int iterations;    // -n: 0 keeps 5 per level, negative halves per level
+  int pyr_start, pyr_end, pyr_num, pyr_min_size;  // -a -c -d -e
+  double portion_kept;
+  int nthreads;      // block-matcher threads; 0 = single-threaded matcher
+  int verbosity;
+  int centre_init;   // CenteredTransformInitializer GeometryOn (default 1)
+} orc_reg_params;
+
+typedef struct orc_reg_stats {
+  int levels;
+  int iterations[16];        // outer iterations run per level
+  int64_t points[16];        // matched pairs in the last iteration per level
+  int inner_iterations;      // LTS inner iterations, all levels
+  int64_t voxel_evals;       // NCC voxel-evals, all levels and passes
+  int level_dims[16][3];     // half-space grid per level
+} orc_reg_stats;
+
+void orc_reg_default_params(orc_reg_params* p);
+int orc_register(const float* fixed, const uint8_t* fmask, const orc_geom* fg,
+                 const float* moving, const uint8_t* mmask, const orc_geom* mg,
+                 const orc_reg_params* p, const double* init_params,
+                 const double* init_center, double* out_params, double* out_center,
+                 orc_reg_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
