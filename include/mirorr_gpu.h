@@ -83,6 +83,16 @@ mirorr_status mirorr_ctx_synchronize(mirorr_ctx* ctx);
 /* The context's HIP stream (hipStream_t) for callers that interleave work. */
 void* mirorr_ctx_stream(mirorr_ctx* ctx);
 
+/* Per-kernel timing (no counterpart in the reference; the OpenCL host timed
+ * its kernel with CL_QUEUE_PROFILING_ENABLE, itkBlockMatcherGPU.txx:380-391).
+ * When on, instrumented kernels ("ncc_tile_fast", "ncc_tile_masked") are
+ * bracketed by HIP events on the context stream and report their work in
+ * voxel-evals; mirorr_ctx_profile_get sums launches, milliseconds and work. */
+mirorr_status mirorr_ctx_set_profiling(mirorr_ctx* ctx, int on);
+mirorr_status mirorr_ctx_profile_get(mirorr_ctx* ctx, const char* kernel, int64_t* launches,
+                                     double* total_ms, double* work);
+mirorr_status mirorr_ctx_profile_reset(mirorr_ctx* ctx);
+
 /* ---- block matcher (itk::AbstractBlockMatcher, itkAbstractBlockMatcher.h:41-313) */
 
 void mirorr_bm_default_params(mirorr_bm_params* p);
@@ -140,6 +150,83 @@ mirorr_status mirorr_bm_match_async(mirorr_bm* bm);
 /* Voxel-evals of one search over the current grid: sum over blocks of the
  * valid candidates times BW^3 (the unit of the throughput metric). */
 mirorr_status mirorr_bm_voxel_evals(mirorr_bm* bm, int64_t* evals);
+
+/* ---- device-resident registration ------------------------------------ */
+
+typedef struct mirorr_volume mirorr_volume;
+
+/* Registration parameters: the mirorr CLI options of the hot path
+ * (mirorr.cxx:99-281; MirorrPyramidImplement / MirorrRegistrationMethod
+ * setters, itkMirorrPyramidImplement.h:104-139, itkMirorrRegistrationMethod.h:135-276). */
+typedef struct mirorr_reg_params {
+  int symmetric;      /* --reg-mode symmetric (1, default) | classic (0)        */
+  int affine;         /* -t affine (1) | rigid Euler3D (0, default)             */
+  int block_width;    /* --blockwidth (4)                                        */
+  int block_gap;      /* --blockgap (1)                                          */
+  int nhood_width;    /* --nhoodwidth (7)                                        */
+  int nhood_gap;      /* --nhoodgap (3)                                          */
+  int iterations;     /* -n: 0 keeps 5 per level, negative halves per level      */
+  int pyr_start;      /* -a (1)                                                  */
+  int pyr_end;        /* -c: 0 runs every level                                  */
+  int pyr_num;        /* -d (1024)                                               */
+  int pyr_min_size;   /* -e (32)                                                 */
+  double portion_kept;/* --portion-kept (0.5); used by variance selection only   */
+  int variance_select;/* 0: reference behaviour                                  */
+  int verbosity;
+  int centre_init;    /* centred initial transform (1) unless --no-centering-init */
+} mirorr_reg_params;
+
+typedef struct mirorr_reg_stats {
+  int levels;
+  int iterations[16];      /* outer iterations run per level                 */
+  int64_t points[16];      /* matched pairs in the last iteration per level  */
+  int inner_iterations;    /* LTS inner iterations, all levels               */
+  int64_t voxel_evals;     /* NCC voxel-evals, all levels and passes         */
+  int level_dims[16][3];   /* half-way grid per level                        */
+  double seconds;          /* wall time of mirorr_register                   */
+} mirorr_reg_stats;
+
+void mirorr_reg_default_params(mirorr_reg_params* p);
+
+/* An image (and optional uint8 mask) uploaded once and kept in HBM; the
+ * reference's NIfTI images after ReadAndResampleImages
+ * (itkMirorrPyramidWrapper.txx:175-310). */
+mirorr_status mirorr_volume_create(mirorr_ctx* ctx, const float* data, const uint8_t* mask,
+                                   const mirorr_geom* geom, mirorr_volume** out);
+void mirorr_volume_destroy(mirorr_volume* v);
+
+/* MirorrPyramidImplement::RunMirorr (itkMirorrPyramidImplement.txx:71-158):
+ * pyramid build, level loop, resample / block-match / LTS iterations.
+ * init_params/init_center give the start transform (NULL: the centred
+ * initialisation of itkMirorrPyramidWrapper.txx:611-623). out_params holds
+ * 6 (rigid) or 12 (affine) ITK parameters; out_center their centre. */
+mirorr_status mirorr_register(mirorr_ctx* ctx, const mirorr_reg_params* p,
+                              const mirorr_volume* fixed, const mirorr_volume* moving,
+                              const double* init_params, const double* init_center,
+                              double* out_params, double* out_center, mirorr_reg_stats* stats);
+
+/* OptimizeTransform on given pairs (x_i block centres, y_i matches, n x 3 mm):
+ * classic (itkMirorrRegistrationMethod.txx:758-1003) composes the fit into
+ * params/center (in-out); symmetric (itkSymMirorrRegistrationMethod.txx:729-1442)
+ * starts from identity half-space transforms and returns them as 3x4
+ * [M | offset] in fwd34/bwd34 with params = t12F o t12B^-1. */
+mirorr_status mirorr_lts_fit(mirorr_ctx* ctx, int symmetric, int affine, const double* x,
+                             const double* y, size_t n, double* params, double* center,
+                             double* fwd34, double* bwd34, int* iterations);
+
+/* ResampleImageFilter with LinearInterpolateImageFunction as the registration
+ * uses it: out(p) = in(matrix * p + offset), 0 outside; out_mask (nullable)
+ * is 255 where the point falls inside the input grid. Host arrays. */
+mirorr_status mirorr_resample(mirorr_ctx* ctx, const float* data, const mirorr_geom* gin,
+                              const double matrix[9], const double offset[3],
+                              const mirorr_geom* gout, float* out, uint8_t* out_mask);
+
+/* Level `level` (0 = coarsest) of the recursive Gaussian pyramid of `data`
+ * for a schedule of rows x 3 shrink factors (coarsest first). out holds the
+ * level image; out_geom its geometry. */
+mirorr_status mirorr_pyramid_level(mirorr_ctx* ctx, const float* data, const mirorr_geom* geom,
+                                   const int* schedule, int rows, int level, float* out,
+                                   mirorr_geom* out_geom);
 
 #ifdef __cplusplus
 }

@@ -24,6 +24,7 @@ from . import capi
 # Handles are released before interpreter shutdown (the HIP runtime may be
 # torn down before Python's own finalisation reaches __del__).
 _live_matchers: "weakref.WeakSet" = weakref.WeakSet()
+_live_volumes: "weakref.WeakSet" = weakref.WeakSet()
 _live_contexts: "weakref.WeakSet" = weakref.WeakSet()
 
 
@@ -31,6 +32,8 @@ _live_contexts: "weakref.WeakSet" = weakref.WeakSet()
 def _release_all():
     for m in list(_live_matchers):
         m.close()
+    for v in list(_live_volumes):
+        v.close()
     for c in list(_live_contexts):
         c.close()
 
@@ -49,6 +52,21 @@ class Context:
 
     def synchronize(self):
         capi.check(capi.lib().mirorr_ctx_synchronize(self._h), self._h)
+
+    def set_profiling(self, on: bool):
+        capi.check(capi.lib().mirorr_ctx_set_profiling(self._h, 1 if on else 0), self._h)
+
+    def profile(self, kernel: str):
+        """(launches, total ms, work units) of an instrumented kernel."""
+        n = C.c_int64()
+        ms = C.c_double()
+        w = C.c_double()
+        capi.check(capi.lib().mirorr_ctx_profile_get(self._h, kernel.encode(), C.byref(n),
+                                                     C.byref(ms), C.byref(w)), self._h)
+        return int(n.value), float(ms.value), float(w.value)
+
+    def profile_reset(self):
+        capi.check(capi.lib().mirorr_ctx_profile_reset(self._h), self._h)
 
     def close(self):
         if self._h:

@@ -52,6 +52,20 @@ class BMParams(C.Structure):
                 ("variance_select", C.c_int), ("verbosity", C.c_int)]
 
 
+class RegParams(C.Structure):
+    _fields_ = [("symmetric", C.c_int), ("affine", C.c_int), ("block_width", C.c_int),
+                ("block_gap", C.c_int), ("nhood_width", C.c_int), ("nhood_gap", C.c_int),
+                ("iterations", C.c_int), ("pyr_start", C.c_int), ("pyr_end", C.c_int),
+                ("pyr_num", C.c_int), ("pyr_min_size", C.c_int), ("portion_kept", C.c_double),
+                ("variance_select", C.c_int), ("verbosity", C.c_int), ("centre_init", C.c_int)]
+
+
+class RegStats(C.Structure):
+    _fields_ = [("levels", C.c_int), ("iterations", C.c_int * 16), ("points", C.c_int64 * 16),
+                ("inner_iterations", C.c_int), ("voxel_evals", C.c_int64),
+                ("level_dims", (C.c_int * 3) * 16), ("seconds", C.c_double)]
+
+
 _P = C.c_void_p
 _lib = None
 
@@ -62,6 +76,10 @@ _SIGS = {
     "mirorr_last_error": (C.c_char_p, [_P]),
     "mirorr_ctx_synchronize": (C.c_int, [_P]),
     "mirorr_ctx_stream": (_P, [_P]),
+    "mirorr_ctx_set_profiling": (C.c_int, [_P, C.c_int]),
+    "mirorr_ctx_profile_get": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_int64),
+                                         C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "mirorr_ctx_profile_reset": (C.c_int, [_P]),
     "mirorr_bm_default_params": (None, [C.POINTER(BMParams)]),
     "mirorr_bm_create": (C.c_int, [_P, C.POINTER(BMParams), C.POINTER(_P)]),
     "mirorr_bm_destroy": (None, [_P]),
@@ -75,6 +93,16 @@ _SIGS = {
     "mirorr_bm_get_raw": (C.c_int, [_P, _P, _P, _P, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
     "mirorr_bm_match_async": (C.c_int, [_P]),
     "mirorr_bm_voxel_evals": (C.c_int, [_P, C.POINTER(C.c_int64)]),
+    "mirorr_reg_default_params": (None, [C.POINTER(RegParams)]),
+    "mirorr_volume_create": (C.c_int, [_P, _P, _P, C.POINTER(Geom), C.POINTER(_P)]),
+    "mirorr_volume_destroy": (None, [_P]),
+    "mirorr_register": (C.c_int, [_P, C.POINTER(RegParams), _P, _P, _P, _P, _P, _P,
+                                  C.POINTER(RegStats)]),
+    "mirorr_lts_fit": (C.c_int, [_P, C.c_int, C.c_int, _P, _P, C.c_size_t, _P, _P, _P, _P,
+                                 C.POINTER(C.c_int)]),
+    "mirorr_resample": (C.c_int, [_P, _P, C.POINTER(Geom), _P, _P, C.POINTER(Geom), _P, _P]),
+    "mirorr_pyramid_level": (C.c_int, [_P, _P, C.POINTER(Geom), _P, C.c_int, C.c_int, _P,
+                                       C.POINTER(Geom)]),
 }
 
 

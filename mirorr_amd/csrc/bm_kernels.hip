@@ -219,7 +219,26 @@ struct TileArgs {
   float* out_best;
   float* out_second;
   int32_t* out_match;
+  unsigned long long* work;  // optional: voxel-evals done by this launch
 };
+
+// Voxel-evals of the valid blocks of a tile (profiling only).
+template <class T>
+__device__ unsigned long long tile_evals(const TileArgs& a, int px, int py0, int pz0,
+                                         const int* bvalid) {
+  const BMGrid& g = a.g;
+  auto cnt = [&](int p, int L) {
+    return min(p - T::H + T::NW - 1, L - T::BW) - max(p - T::H, 0) + 1;
+  };
+  unsigned long long tot = 0;
+  const int cx = cnt(px, g.dims[0]);
+  for (int bl = 0; bl < T::NB; ++bl) {
+    if (!bvalid[bl]) continue;
+    const int py = py0 + g.ng * (bl % T::TY), pz = pz0 + g.ng * (bl / T::TY);
+    tot += (unsigned long long)cx * cnt(py, g.dims[1]) * cnt(pz, g.dims[2]);
+  }
+  return tot * 64ull;
+}
 
 template <class T>
 __device__ __forceinline__ void tile_origin(const TileArgs& a, int tile, int& bx,
@@ -368,6 +387,7 @@ __global__ __launch_bounds__(kThreads, 4) void ncc_tile_fast(TileArgs a) {
     }
     return;  // workgroup-uniform
   }
+  if (a.work && tid == 0) atomicAdd(a.work, tile_evals<T>(a, px, py0, pz0, bvalid));
 
   Top2 t2;
   top2_init(t2);
@@ -529,6 +549,7 @@ __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
   }
   __syncthreads();
 
+  if (a.work && tid == 0) atomicAdd(a.work, tile_evals<T>(a, px, py0, pz0, bvalid));
   Top2 rec[R];
   int recb[R];
 #pragma unroll
@@ -719,6 +740,42 @@ __global__ void postprocess_kernel(BMGrid g, Geom geom, const float* __restrict_
   }
 }
 
+// Pair output for the LTS: X gets the moving-side point and Y the fixed-side
+// point of every block; `swap` marks the second symmetric pass, whose base
+// image is the fixed one (itkSymMirorrRegistrationMethod.txx:595-602).
+__global__ void postprocess_pairs_kernel(BMGrid g, Geom geom, const float* __restrict__ best,
+                                         const float* __restrict__ second,
+                                         const int32_t* __restrict__ match, int swap,
+                                         uint8_t* kept, double* X, double* Y,
+                                         unsigned long long* count) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const double cb = b < g.nblocks ? (double)best[b] : 0.0;
+  const double c2 = b < g.nblocks ? (double)second[b] : 0.0;
+  const bool k = b < g.nblocks && cb > 1e-7 && fabs(cb - c2) > 1e-4;
+  const unsigned long long bal = __ballot(k);
+  if ((threadIdx.x & 63) == 0 && bal) atomicAdd(count, (unsigned long long)__popcll(bal));
+  if (b >= g.nblocks) return;
+  kept[b] = k ? 1 : 0;
+  const int bxi = (int)(b % g.nb[0]);
+  const int byi = (int)((b / g.nb[0]) % g.nb[1]);
+  const int bzi = (int)(b / ((int64_t)g.nb[0] * g.nb[1]));
+  const int p[3] = {g.start[0] + g.ng * bxi, g.start[1] + g.ng * byi, g.start[2] + g.ng * bzi};
+  const int m[3] = {match[3 * b], match[3 * b + 1], match[3 * b + 2]};
+  const double half = 0.5 * g.bw;
+  double* pb = swap ? Y : X;
+  double* pm = swap ? X : Y;
+  for (int r = 0; r < 3; ++r) {
+    double vb = geom.origin[r], vm = geom.origin[r];
+    for (int c = 0; c < 3; ++c) {
+      const double dsc = geom.dir[3 * r + c] * geom.spacing[c];
+      vb += dsc * (double)p[c];
+      vm += dsc * (double)m[c];
+    }
+    pb[3 * b + r] = vb + geom.spacing[r] * half;
+    pm[3 * b + r] = vm + geom.spacing[r] * half;
+  }
+}
+
 // ---------------------------------------------------------------------------
 constexpr int kChunk = 1024;  // elements per compaction workgroup
 
@@ -816,12 +873,16 @@ void run_tiled(const BMGrid& g, TileArgs a, BMScratch& scr, hipStream_t s) {
   const int WYf = (T::TY - 1) * g.ng + T::WX + T::YPAD;
   const int WZ = (T::TZ - 1) * g.ng + T::WX;
   const size_t lds_fast = sizeof(float) * (size_t)T::PX * WYf * WZ;
+  if (scr.prof) a.work = scr.prof->begin("ncc_tile_fast", s);
   hipLaunchKernelGGL(ncc_tile_fast<T>, dim3((unsigned)nt), dim3(kThreads), lds_fast, s, a);
   MIRORR_CHECK_LAUNCH();
+  if (scr.prof) scr.prof->end(s);
   const int WYm = (T::TY - 1) * g.ng + T::WX;
   const size_t lds_masked = (sizeof(float) + 1) * (size_t)T::PX * WYm * WZ;
+  if (scr.prof) a.work = scr.prof->begin("ncc_tile_masked", s);
   hipLaunchKernelGGL(ncc_tile_masked<T>, dim3((unsigned)nt), dim3(kThreads), lds_masked, s, a);
   MIRORR_CHECK_LAUNCH();
+  if (scr.prof) scr.prof->end(s);
 }
 
 }  // namespace
@@ -928,6 +989,16 @@ void launch_postprocess(const BMGrid& g, const Geom& geom, const BMResult& res,
   hipLaunchKernelGGL(postprocess_kernel, dim3((unsigned)((g.nblocks + kThreads - 1) / kThreads)),
                      dim3(kThreads), 0, s, g, geom, res.best, res.second, res.match, kept,
                      base_mm, match_mm);
+  MIRORR_CHECK_LAUNCH();
+}
+
+void launch_postprocess_pairs(const BMGrid& g, const Geom& geom, const BMResult& res, int swap,
+                              uint8_t* kept, double* X, double* Y, unsigned long long* count,
+                              hipStream_t s) {
+  if (g.nblocks == 0) return;
+  hipLaunchKernelGGL(postprocess_pairs_kernel,
+                     dim3((unsigned)((g.nblocks + kThreads - 1) / kThreads)), dim3(kThreads), 0, s,
+                     g, geom, res.best, res.second, res.match, swap, kept, X, Y, count);
   MIRORR_CHECK_LAUNCH();
 }
 

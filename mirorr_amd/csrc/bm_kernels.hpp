@@ -2,6 +2,7 @@
 #pragma once
 
 #include "common.hpp"
+#include "profiler.hpp"
 
 namespace mirorr {
 
@@ -32,6 +33,7 @@ struct BMScratch {
   int32_t* queue = nullptr;  // ntiles entries
   int32_t* count = nullptr;  // 1 counter
   int64_t ntiles_cap = 0;
+  Profiler* prof = nullptr;  // optional: event timing + voxel-evals per launch
 };
 
 void launch_search_stats(const float* y, const uint8_t* smask, const BMGrid& g,
@@ -57,6 +59,12 @@ bool tiled_supported(const BMGrid& g);
 void launch_postprocess(const BMGrid& g, const Geom& geom, const BMResult& res,
                         uint8_t* kept, double* base_mm, double* match_mm,
                         hipStream_t s);
+
+// Score filter + mm conversion written as LTS pairs: the base point goes to
+// X (Y when swap), the matched point to Y (X when swap); count += kept.
+void launch_postprocess_pairs(const BMGrid& g, const Geom& geom, const BMResult& res, int swap,
+                              uint8_t* kept, double* X, double* Y, unsigned long long* count,
+                              hipStream_t s);
 
 // Ordered compaction of nblocks records by a 0/1 flag array.
 struct Compactor {

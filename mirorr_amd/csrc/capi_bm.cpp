@@ -113,7 +113,7 @@ static void run_search(mirorr_bm* bm) {
     const int64_t nt = tile_count(g);
     bm->queue.ensure(sizeof(int32_t) * nt, bud);
     bm->qcount.ensure(sizeof(int32_t), bud);
-    BMScratch scr{bm->queue.as<int32_t>(), bm->qcount.as<int32_t>(), nt};
+    BMScratch scr{bm->queue.as<int32_t>(), bm->qcount.as<int32_t>(), nt, &bm->ctx->prof};
     BlockPrep bp{bm->prep_xt[bi].as<float>(), bm->prep_sxx[bi].as<float>(),
                  bm->prep_sum[bi].as<float>(), bm->prep_flags[bi].as<uint32_t>()};
     launch_ncc_search(B.data, B.mask, S.data, S.mask, g, bp,
@@ -170,6 +170,37 @@ mirorr_status mirorr_ctx_synchronize(mirorr_ctx* c) {
 }
 
 void* mirorr_ctx_stream(mirorr_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+mirorr_status mirorr_ctx_set_profiling(mirorr_ctx* c, int on) {
+  if (!c) return MIRORR_ERR_ARG;
+  return guarded(c, [&] {
+    c->prof.collect(c->stream);
+    c->prof.enable(on != 0, &c->budget);
+  });
+}
+
+mirorr_status mirorr_ctx_profile_get(mirorr_ctx* c, const char* kernel, int64_t* launches,
+                                     double* total_ms, double* work) {
+  if (!c || !kernel) return MIRORR_ERR_ARG;
+  return guarded(c, [&] {
+    c->prof.collect(c->stream);
+    const auto& a = c->prof.aggregates();
+    const auto it = a.find(kernel);
+    const ProfAgg z{};
+    const ProfAgg& g = it == a.end() ? z : it->second;
+    if (launches) *launches = g.launches;
+    if (total_ms) *total_ms = g.ms;
+    if (work) *work = g.units;
+  });
+}
+
+mirorr_status mirorr_ctx_profile_reset(mirorr_ctx* c) {
+  if (!c) return MIRORR_ERR_ARG;
+  return guarded(c, [&] {
+    c->prof.collect(c->stream);
+    c->prof.reset();
+  });
+}
 
 void mirorr_bm_default_params(mirorr_bm_params* p) {
   if (!p) return;

@@ -6,6 +6,7 @@
 
 #include "../../include/mirorr_gpu.h"
 #include "device_buffer.hpp"
+#include "profiler.hpp"
 
 struct mirorr_ctx {
   int device = 0;
@@ -13,6 +14,7 @@ struct mirorr_ctx {
   mirorr::Budget budget;
   std::string last_error;
   std::mutex mu;
+  mirorr::Profiler prof;  // optional per-kernel event timing
 };
 
 namespace mirorr {

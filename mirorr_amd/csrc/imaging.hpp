@@ -1,0 +1,66 @@
+// Resampling and Gaussian pyramid on the device (implemented in imaging.hip).
+#pragma once
+
+#include "common.hpp"
+#include "xform.hpp"
+
+namespace mirorr {
+
+// Output voxel index -> input continuous index, ci = A i + b (double).
+struct IndexMap {
+  double A[9];
+  double b[3];
+};
+
+// Builds the index map of y = M x + off between two grids:
+// A = S_in^-1 D_in^-1 M D_out S_out, b = S_in^-1 D_in^-1 (M o_out + off - o_in).
+__host__ __device__ inline IndexMap make_index_map(const xf::M3& M, const double off[3],
+                                                   const Geom& gin, const Geom& gout) {
+  xf::M3 Din, Dout;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      Din.a[r][c] = gin.dir[3 * r + c];
+      Dout.a[r][c] = gout.dir[3 * r + c] * gout.spacing[c];
+    }
+  const xf::M3 Dinv = xf::inverse(Din);
+  const xf::M3 P = xf::mul(Dinv, xf::mul(M, Dout));
+  double q[3];
+  xf::mulv(M, gout.origin, q);
+  for (int r = 0; r < 3; ++r) q[r] += off[r] - gin.origin[r];
+  double u[3];
+  xf::mulv(Dinv, q, u);
+  IndexMap m;
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) m.A[3 * r + c] = P.a[r][c] / gin.spacing[r];
+    m.b[r] = u[r] / gin.spacing[r];
+  }
+  return m;
+}
+
+enum MaskMode : int {
+  kMaskNone = 0,     // no mask output
+  kMaskInside = 1,   // the reference's default all-255 mask: 255 inside the source grid
+  kMaskLinear = 2,   // interpolate a uint8 mask and truncate
+};
+
+// Resample `in` (dims_in) onto an output grid of dims_out under `map`
+// (linear interpolation, neighbours clamped to the buffer, 0 outside
+// [-0.5, n-0.5)). A mask is written under `mmap` from the mask grid
+// (dims_m; a separate grid so full-resolution masks serve every level).
+// Maps are read from device memory (a transform the LTS updates on device).
+void launch_resample(const float* in, const int dims_in[3], const IndexMap* d_map,
+                     const uint8_t* mask, const int dims_m[3], const IndexMap* d_mmap,
+                     int mask_mode, const int dims_out[3], float* out, uint8_t* out_mask,
+                     hipStream_t s);
+
+// ITK GaussianOperator coefficients (centre first), normalised to unit sum.
+int gaussian_kernel(double variance, double max_error, int max_width, double* out, int cap);
+
+// One recursive pyramid step: smooth `prev` per axis with variance
+// (ratio/2)^2 voxels (zero-flux Neumann edges, double intermediates) and
+// resample onto `gout`. `tmp` holds 2 * nvox(prev) doubles.
+void launch_pyramid_step(const float* prev, const Geom& gprev, const int ratio[3],
+                         const Geom& gout, float* out, double* tmp, IndexMap* d_map,
+                         hipStream_t s);
+
+}  // namespace mirorr

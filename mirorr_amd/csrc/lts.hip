@@ -1,0 +1,669 @@
+// Least-trimmed-squares regression of block-match pairs on gfx950.
+//
+// Restates OptimizeTransform of the classic (itkMirorrRegistrationMethod.txx:758-1106)
+// and symmetric (itkSymMirorrRegistrationMethod.txx:729-1442) methods with the
+// per-point work on the GPU and the tiny FP64 solves in a one-thread tail:
+//  1. residual kernel: r_i = |F(x_i) - B(y_i)|^2 (classic |F(x_i) - y_i|^2),
+//     +inf for culled blocks; the first radix digit (bits 63..52 of the
+//     non-negative double) is histogrammed in LDS and merged globally;
+//  2. exact k-th smallest residual (std::nth_element in the reference) by a
+//     two-digit radix histogram, a compaction of the candidates that share
+//     the 24-bit prefix and a one-workgroup radix finish on the low 40 bits;
+//  3. trimmed moments in two passes (barycentres, then the 21 centred second
+//     moments x x^T, y y^T, x y^T), block-reduced in double; the rigid
+//     quaternion matrix sum Q^T Q is assembled from those moments;
+//  4. a one-thread solve (normal equations / quaternion eigenvector /
+//     polar factor / Denman-Beavers roots) updating the device state.
+// Every kernel is predicated on the state's `active` flag so the host can
+// queue several inner iterations before reading that flag back.
+#include <cfloat>
+#include <climits>
+
+#include "lts.hpp"
+
+namespace mirorr {
+namespace {
+
+constexpr int kT = 256;
+constexpr int kPart = 1024;  // points per moment workgroup (4 per thread)
+
+using namespace xf;
+
+__device__ __forceinline__ unsigned long long key_of(double r) {
+  return (unsigned long long)__double_as_longlong(r);
+}
+
+// ---- residuals + first digit ----------------------------------------------
+__global__ __launch_bounds__(kT) void residual_kernel(const double* __restrict__ X,
+                                                      const double* __restrict__ Y,
+                                                      const uint8_t* __restrict__ kept,
+                                                      int64_t n, LtsState* st,
+                                                      double* __restrict__ r,
+                                                      unsigned int* hist1, int do_hist) {
+  if (!st->active) return;
+  __shared__ unsigned int h[4096];
+  if (do_hist)
+    for (int i = threadIdx.x; i < 4096; i += kT) h[i] = 0;
+  __syncthreads();
+  const Xf F = st->F;
+  const Xf B = st->B;
+  const bool sym = st->symmetric != 0;
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT) {
+    double v = __longlong_as_double(0x7ff0000000000000ll);  // +inf: culled block
+    if (kept[i]) {
+      double p[3], q[3];
+      apply(F, X + 3 * i, p);
+      if (sym) {
+        apply(B, Y + 3 * i, q);
+      } else {
+        q[0] = Y[3 * i];
+        q[1] = Y[3 * i + 1];
+        q[2] = Y[3 * i + 2];
+      }
+      v = 0.0;
+      for (int k = 0; k < 3; ++k) {
+        const double d = p[k] - q[k];
+        v += d * d;
+      }
+      if (do_hist) atomicAdd(&h[(unsigned)(key_of(v) >> 52)], 1u);
+    }
+    r[i] = v;
+  }
+  if (!do_hist) return;
+  __syncthreads();
+  for (int i = threadIdx.x; i < 4096; i += kT)
+    if (h[i]) atomicAdd(&hist1[i], h[i]);
+}
+
+// Finds the bin holding rank k of a 4096-bin histogram (one workgroup).
+__device__ void find_bin(const unsigned int* hist, unsigned long long k, unsigned int* bin,
+                         unsigned long long* krem) {
+  __shared__ unsigned long long part[1024];
+  const int t = threadIdx.x;  // 1024 threads, 4 bins each
+  unsigned long long s = 0;
+  for (int j = 0; j < 4; ++j) s += hist[4 * t + j];
+  part[t] = s;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const unsigned long long v = t >= o ? part[t - o] : 0ull;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  const unsigned long long before = part[t] - s;  // exclusive prefix
+  if (k >= before && k < part[t]) {
+    unsigned long long c = before;
+    for (int j = 0; j < 4; ++j) {
+      const unsigned long long h = hist[4 * t + j];
+      if (k < c + h) {
+        *bin = 4 * t + j;
+        *krem = k - c;
+        break;
+      }
+      c += h;
+    }
+  }
+}
+
+__global__ __launch_bounds__(1024) void scan1_kernel(const unsigned int* hist1, LtsState* st) {
+  if (!st->active) return;
+  find_bin(hist1, st->k, &st->bin1, &st->k2);
+  if (threadIdx.x == 0) st->ncand = 0;
+}
+
+__global__ __launch_bounds__(kT) void hist2_kernel(const double* __restrict__ r, int64_t n,
+                                                   LtsState* st, unsigned int* hist2) {
+  if (!st->active) return;
+  __shared__ unsigned int h[4096];
+  for (int i = threadIdx.x; i < 4096; i += kT) h[i] = 0;
+  __syncthreads();
+  const unsigned int b1 = st->bin1;
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT) {
+    const unsigned long long k = key_of(r[i]);
+    if ((unsigned)(k >> 52) == b1) atomicAdd(&h[(unsigned)(k >> 40) & 0xfffu], 1u);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 4096; i += kT)
+    if (h[i]) atomicAdd(&hist2[i], h[i]);
+}
+
+__global__ __launch_bounds__(1024) void scan2_kernel(const unsigned int* hist2, LtsState* st) {
+  if (!st->active) return;
+  find_bin(hist2, st->k2, &st->bin2, &st->k3);
+}
+
+__global__ __launch_bounds__(kT) void compact_kernel(const double* __restrict__ r, int64_t n,
+                                                     LtsState* st,
+                                                     unsigned long long* __restrict__ cand) {
+  if (!st->active) return;
+  const unsigned long long pre = ((unsigned long long)st->bin1 << 12) | st->bin2;
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT) {
+    const unsigned long long k = key_of(r[i]);
+    if ((k >> 40) == pre) cand[atomicAdd(&st->ncand, 1u)] = k;
+  }
+}
+
+// Radix select of rank k3 among the candidates on bits 39..0, 8 bits a pass.
+__global__ __launch_bounds__(1024) void final_select_kernel(const unsigned long long* cand,
+                                                            LtsState* st) {
+  if (!st->active) return;
+  __shared__ unsigned int h[256];
+  __shared__ unsigned long long s_prefix, s_k;
+  const unsigned int nc = st->ncand;
+  if (threadIdx.x == 0) {
+    s_prefix = (((unsigned long long)st->bin1 << 12) | st->bin2) << 40;
+    s_k = st->k3;
+  }
+  for (int shift = 32; shift >= 0; shift -= 8) {
+    for (int i = threadIdx.x; i < 256; i += 1024) h[i] = 0;
+    __syncthreads();
+    const unsigned long long pre = s_prefix;
+    const unsigned long long hi_mask = ~((1ull << (shift + 8)) - 1ull);
+    for (unsigned int i = threadIdx.x; i < nc; i += 1024) {
+      const unsigned long long k = cand[i];
+      if ((k & hi_mask) == pre) atomicAdd(&h[(unsigned)(k >> shift) & 0xffu], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long c = 0, k = s_k;
+      for (int b = 0; b < 256; ++b) {
+        if (k < c + h[b]) {
+          s_prefix = pre | ((unsigned long long)b << shift);
+          s_k = k - c;
+          break;
+        }
+        c += h[b];
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) st->thr = __longlong_as_double((long long)s_prefix);
+}
+
+// ---- trimmed moments ---------------------------------------------------------
+template <int N>
+__device__ void block_reduce(double (&v)[N], double* out) {
+  __shared__ double sh[kT / 64][N];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    double x = v[j];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    if (lane == 0) sh[w][j] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < N) {
+    double s = 0.0;
+    for (int k = 0; k < kT / 64; ++k) s += sh[k][threadIdx.x];
+    out[threadIdx.x] = s;
+  }
+}
+
+__global__ __launch_bounds__(kT) void moments1_kernel(const double* __restrict__ X,
+                                                      const double* __restrict__ Y,
+                                                      const double* __restrict__ r, int64_t n,
+                                                      LtsState* st, double* part) {
+  if (!st->active) return;
+  const double thr = st->fraction > 0.99999 ? DBL_MAX : st->thr;
+  double v[7] = {0, 0, 0, 0, 0, 0, 0};
+  const int64_t base = (int64_t)blockIdx.x * kPart;
+  for (int j = threadIdx.x; j < kPart; j += kT) {
+    const int64_t i = base + j;
+    if (i < n && r[i] <= thr) {
+      v[0] += 1.0;
+      for (int k = 0; k < 3; ++k) {
+        v[1 + k] += X[3 * i + k];
+        v[4 + k] += Y[3 * i + k];
+      }
+    }
+  }
+  block_reduce<7>(v, part + 8 * blockIdx.x);
+}
+
+// Sums per-workgroup partials in a fixed order (deterministic).
+__device__ void sum_parts(const double* part, int stride, int nparts, int nval, double* out) {
+  __shared__ double sh[kT];
+  for (int j = 0; j < nval; ++j) {
+    double s = 0.0;
+    for (int p = threadIdx.x; p < nparts; p += kT) s += part[(int64_t)stride * p + j];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = kT / 2; o > 0; o >>= 1) {
+      if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[j] = sh[0];
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kT) void moments2_kernel(const double* __restrict__ X,
+                                                      const double* __restrict__ Y,
+                                                      const double* __restrict__ r, int64_t n,
+                                                      LtsState* st, const double* part1,
+                                                      int nparts, double* part2) {
+  if (!st->active) return;
+  __shared__ double s1[7];
+  sum_parts(part1, 8, nparts, 7, s1);
+  __syncthreads();
+  const double cnt = s1[0];
+  const double mx[3] = {s1[1] / cnt, s1[2] / cnt, s1[3] / cnt};
+  const double my[3] = {s1[4] / cnt, s1[5] / cnt, s1[6] / cnt};
+  const double thr = st->fraction > 0.99999 ? DBL_MAX : st->thr;
+  // xx (6), yy (6), xy (9)
+  double v[21];
+#pragma unroll
+  for (int j = 0; j < 21; ++j) v[j] = 0.0;
+  const int64_t base = (int64_t)blockIdx.x * kPart;
+  for (int j = threadIdx.x; j < kPart; j += kT) {
+    const int64_t i = base + j;
+    if (i < n && r[i] <= thr) {
+      const double x[3] = {X[3 * i] - mx[0], X[3 * i + 1] - mx[1], X[3 * i + 2] - mx[2]};
+      const double y[3] = {Y[3 * i] - my[0], Y[3 * i + 1] - my[1], Y[3 * i + 2] - my[2]};
+      int q = 0;
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = a; b < 3; ++b, ++q) {
+          v[q] += x[a] * x[b];
+          v[6 + q] += y[a] * y[b];
+        }
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) v[12 + 3 * a + b] += x[a] * y[b];
+    }
+  }
+  block_reduce<21>(v, part2 + 24 * blockIdx.x);
+}
+
+// ---- the one-thread solves -------------------------------------------------
+struct Moments {
+  double n, mx[3], my[3];
+  double xx[3][3], yy[3][3], xy[3][3];  // centred sums (not divided by n)
+};
+
+__device__ void unpack(const double* s1, const double* s2, Moments& m) {
+  m.n = s1[0];
+  for (int k = 0; k < 3; ++k) {
+    m.mx[k] = s1[1 + k] / m.n;
+    m.my[k] = s1[4 + k] / m.n;
+  }
+  int q = 0;
+  for (int a = 0; a < 3; ++a)
+    for (int b = a; b < 3; ++b, ++q) {
+      m.xx[a][b] = m.xx[b][a] = s2[q];
+      m.yy[a][b] = m.yy[b][a] = s2[6 + q];
+    }
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) m.xy[a][b] = s2[12 + 3 * a + b];
+}
+
+// CalculateRigidQuaternion from moments: sum Q^T Q with a = x - y, b = x + y.
+__device__ Quat rigid_quat(const Moments& m) {
+  double aa[3][3], bb[3][3], ab[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      aa[i][j] = m.xx[i][j] - m.xy[i][j] - m.xy[j][i] + m.yy[i][j];
+      bb[i][j] = m.xx[i][j] + m.xy[i][j] + m.xy[j][i] + m.yy[i][j];
+      ab[i][j] = m.xx[i][j] + m.xy[i][j] - m.xy[j][i] - m.yy[i][j];
+    }
+  double T[4][4];
+  T[0][0] = aa[0][0] + aa[1][1] + aa[2][2];
+  T[0][1] = -ab[1][2] + ab[2][1];
+  T[0][2] = ab[0][2] - ab[2][0];
+  T[0][3] = -ab[0][1] + ab[1][0];
+  T[1][1] = aa[0][0] + bb[2][2] + bb[1][1];
+  T[1][2] = aa[0][1] - bb[0][1];
+  T[1][3] = aa[0][2] - bb[0][2];
+  T[2][2] = aa[1][1] + bb[2][2] + bb[0][0];
+  T[2][3] = aa[1][2] - bb[1][2];
+  T[3][3] = aa[2][2] + bb[1][1] + bb[0][0];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < i; ++j) T[i][j] = T[j][i];
+  double ev[4], V[4][4];
+  eigen_sym<4>(T, ev, V);
+  return Quat{V[1][0], V[2][0], V[3][0], V[0][0]};
+}
+
+// CovarianceOfPoints(a, b) of centred sets: sum a b^T / n.
+__device__ M3 cov(const double s[3][3], double n, bool transpose_) {
+  M3 r;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) r.a[i][j] = (transpose_ ? s[j][i] : s[i][j]) / n;
+  return r;
+}
+
+__device__ double delta_of(const double* a, const double* b, int np) {
+  double s = 0.0;
+  for (int i = 0; i < np; ++i) s += (a[i] - b[i]) * (a[i] - b[i]);
+  return s;
+}
+
+__device__ M4 to4(const M3& m, const double t[3]) {
+  M4 r;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) r.a[i][j] = (i == j && i == 3) ? 1.0 : 0.0;
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < 3; ++j) r.a[i][j] = m.a[i][j];
+    r.a[i][3] = t[i];
+  }
+  return r;
+}
+__device__ M3 top3(const M4& m) {
+  M3 r;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) r.a[i][j] = m.a[i][j];
+  return r;
+}
+
+// Symmetric update (itkSymMirorrRegistrationMethod.txx:866-1379).
+__device__ void solve_symmetric(LtsState* st, const Moments& m) {
+  const int kind = st->kind, np = kind == kRigid ? 6 : 12;
+  double neu[12], neuF[12], neuB[12];
+  for (int i = 0; i < np; ++i) neu[i] = neuF[i] = neuB[i] = st->last[i];
+  double tr[3];
+  for (int k = 0; k < 3; ++k) tr[k] = m.my[k] - m.mx[k];
+  Xf fF = st->F, fB = st->B;
+  if (kind == kAffine) {
+    const M3 oxy = cov(m.xy, m.n, true), oxx = cov(m.xx, m.n, false);
+    M3 rF = oxy;
+    if (fabs(det(oxx)) >= 1e-4) rF = mul(oxy, inverse(oxx));
+    const M3 oyx = cov(m.xy, m.n, false), oyy = cov(m.yy, m.n, false);
+    M3 rB = oyx;
+    if (fabs(det(oyy)) >= 1e-4) rB = mul(oyx, inverse(oyy));
+    for (int i = 0; i < 9; ++i) {
+      neu[i] = neuF[i] = rF.a[i / 3][i % 3];
+      neuB[i] = rB.a[i / 3][i % 3];
+    }
+    for (int k = 0; k < 3; ++k) {
+      neu[9 + k] = neuF[9 + k] = tr[k];
+      neuB[9 + k] = -tr[k];
+    }
+    set_center(fF, m.mx);
+    set_params(fF, neuF);
+    set_center(fB, m.my);
+    set_params(fB, neuB);
+    M3 K = fF.M, Mi = inverse(fF.M);
+    const M3 L = inverse(fB.M), N = fB.M;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        K.a[i][j] = (K.a[i][j] + L.a[i][j]) / 2.0;
+        Mi.a[i][j] = (Mi.a[i][j] + N.a[i][j]) / 2.0;
+      }
+    set_matrix(fF, K);
+    set_matrix(fB, Mi);
+  } else {
+    const Quat q = rigid_quat(m);
+    double eu[3], euF[3], euB[3];
+    quat_euler_zyx(q, eu);
+    const double vm = sqrt(q.x * q.x + q.y * q.y + q.z * q.z);
+    double ax[3] = {0.0, 0.0, 1.0};
+    if (vm != 0.0) {
+      ax[0] = q.x / vm;
+      ax[1] = q.y / vm;
+      ax[2] = q.z / vm;
+    }
+    const double angle = 2.0 * atan2(vm, q.r);
+    quat_euler_zyx(quat_axis_angle(ax, angle), euF);
+    quat_euler_zyx(quat_axis_angle(ax, -angle), euB);
+    for (int k = 0; k < 3; ++k) {
+      neu[k] = eu[k];
+      neuF[k] = euF[k];
+      neuB[k] = euB[k];
+      neu[3 + k] = neuF[3 + k] = tr[k];
+      neuB[3 + k] = -tr[k];
+    }
+    set_center(fF, m.mx);
+    set_params(fF, neuF);
+    set_center(fB, m.my);
+    set_params(fB, neuB);
+    M3 K = fF.M;
+    const M3 L = inverse(fB.M);
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) K.a[i][j] = (K.a[i][j] + L.a[i][j]) / 2.0;
+    K = orthogonalize(K);
+    set_matrix(fF, K);
+    set_matrix(fB, transpose(K));
+  }
+  const double zero[3] = {0.0, 0.0, 0.0};
+  {
+    const double oF[3] = {fF.off[0], fF.off[1], fF.off[2]};
+    const double oB[3] = {fB.off[0], fB.off[1], fB.off[2]};
+    set_center(fF, zero);
+    set_center(fB, zero);
+    set_translation(fF, oF);
+    set_translation(fB, oB);
+  }
+  const Xf iF = inverse(fF), iB = inverse(fB);
+  M3 K, K2;
+  double tF[3], tB[3];
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < 3; ++j) {
+      K.a[i][j] = (fF.M.a[i][j] + iB.M.a[i][j]) / 2.0;
+      K2.a[i][j] = (iF.M.a[i][j] + fB.M.a[i][j]) / 2.0;
+    }
+    tF[i] = (fF.t[i] + iB.t[i]) / 2.0;
+    tB[i] = (iF.t[i] + fB.t[i]) / 2.0;
+  }
+  if (kind == kRigid) {
+    K = orthogonalize(K);
+    K2 = orthogonalize(K2);
+  }
+  const M4 m12 = sqrt_db(to4(K, tF), 1e-14, 50);
+  const M4 im12 = sqrt_db(to4(K2, tB), 1e-14, 50);
+  M3 A = top3(m12), Bm = top3(im12);
+  Xf sF = fF, sB = fB;
+  if (!set_matrix(sF, A) || !set_matrix(sB, Bm)) {
+    // the square roots were not orthogonal enough for a rigid transform
+    A = orthogonalize(A);
+    Bm = orthogonalize(Bm);
+    sF = fF;
+    sB = fB;
+    set_matrix(sF, A);
+    set_matrix(sB, Bm);
+    st->status |= 1;
+  }
+  const double t1[3] = {m12.a[0][3], m12.a[1][3], m12.a[2][3]};
+  const double t2[3] = {im12.a[0][3], im12.a[1][3], im12.a[2][3]};
+  set_translation(sF, t1);
+  set_translation(sB, t2);
+  st->F = sF;
+  st->B = sB;
+  st->last_delta = st->delta;
+  st->delta = delta_of(neu, st->last, np);
+  for (int i = 0; i < np; ++i) st->last[i] = neu[i];
+}
+
+// Classic update (itkMirorrRegistrationMethod.txx:841-999); last_parameters
+// keeps the identity as the reference never updates it.
+__device__ void solve_classic(LtsState* st, const Moments& m) {
+  const int kind = st->kind, np = kind == kRigid ? 6 : 12;
+  double neu[12];
+  for (int i = 0; i < np; ++i) neu[i] = st->last[i];
+  double a[3], b[3], ra[3];
+  for (int k = 0; k < 3; ++k) {
+    a[k] = m.my[k] - st->rc[k];
+    b[k] = m.mx[k] - st->rc[k];
+  }
+  if (kind == kAffine) {
+    const M3 oxy = cov(m.xy, m.n, true), oxx = cov(m.xx, m.n, false);
+    M3 rot = oxy;
+    if (fabs(det(oxx)) >= 1e-4) rot = mul(rot, inverse(oxx));
+    mulv(rot, b, ra);
+    for (int i = 0; i < 9; ++i) neu[i] = rot.a[i / 3][i % 3];
+    for (int k = 0; k < 3; ++k) neu[9 + k] = a[k] - ra[k];
+  } else {
+    const Quat q = rigid_quat(m);
+    const M3 rot = quat_rotation(q);
+    double ang[3];
+    mulv(rot, b, ra);
+    quat_euler_zyx(q, ang);
+    for (int k = 0; k < 3; ++k) {
+      neu[k] = ang[k];
+      neu[3 + k] = a[k] - ra[k];
+    }
+  }
+  st->last_delta = st->delta;
+  st->delta = delta_of(neu, st->last, np);
+  for (int k = 0; k < 3; ++k) st->rc[k] = m.mx[k];
+  set_center(st->F, st->rc);
+  set_params(st->F, neu);
+}
+
+__global__ __launch_bounds__(kT) void solve_kernel(LtsState* st, const double* part1,
+                                                   const double* part2, int nparts,
+                                                   unsigned int* hist1, unsigned int* hist2) {
+  if (!st->active) return;
+  __shared__ double s1[7], s2[21];
+  sum_parts(part1, 8, nparts, 7, s1);
+  sum_parts(part2, 24, nparts, 21, s2);
+  for (int i = threadIdx.x; i < 4096; i += kT) {
+    hist1[i] = 0;
+    hist2[i] = 0;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  st->iter += 1;  // the loop head of the reference
+  Moments m;
+  unpack(s1, s2, m);
+  if (st->symmetric)
+    solve_symmetric(st, m);
+  else
+    solve_classic(st, m);
+  st->fraction = 0.5;
+  st->active = (st->delta > 1e-3 && st->iter < 100 && fabs(st->delta - st->last_delta) > 1e-3) ? 1 : 0;
+}
+
+__global__ void init_kernel(LtsState* st, const OuterState* outer, int kind, int symmetric,
+                            unsigned int* hist1, unsigned int* hist2) {
+  // Runs with one workgroup; thread 0 writes the state.
+  for (int i = threadIdx.x; i < 4096; i += blockDim.x) {
+    hist1[i] = 0;
+    hist2[i] = 0;
+  }
+  if (threadIdx.x != 0) return;
+  st->kind = kind;
+  st->symmetric = symmetric;
+  const double* c = outer->T.c;
+  st->F = identity(kind);
+  set_center(st->F, c);
+  st->B = st->F;
+  for (int k = 0; k < 3; ++k) st->rc[k] = c[k];
+  get_params(st->F, st->last);
+  st->fraction = 1.0;
+  st->delta = 1.0;
+  st->last_delta = 0.0;
+  st->iter = -1;
+  st->status = 0;
+  const unsigned long long n = st->n_kept;
+  unsigned long long k = (unsigned long long)(0.5 * (double)n);
+  if (n > 0 && k > n - 1) k = n - 1;
+  st->k = k;
+  st->ncand = 0;
+  st->thr = DBL_MAX;
+  st->active = n > 0 ? 1 : 0;
+}
+
+__device__ void centered(const Xf& t12, const Xf& shift, Xf& out) {
+  double par[12];
+  get_params(t12, par);
+  out = identity(t12.kind);
+  set_params(out, par);
+  compose(out, shift, false);
+}
+
+__device__ void prepare_maps(OuterState* o, int symmetric) {
+  if (symmetric) {
+    Xf cF, cM;
+    centered(o->t12F, o->fixT, cF);
+    centered(o->t12B, o->movT, cM);
+    o->mapF = make_index_map(cF.M, cF.off, o->gF, o->gHF);
+    o->mmapF = make_index_map(cF.M, cF.off, o->gFfull, o->gHF);
+    o->mapM = make_index_map(cM.M, cM.off, o->gM, o->gHM);
+    o->mmapM = make_index_map(cM.M, cM.off, o->gMfull, o->gHM);
+  } else {
+    o->mapF = make_index_map(o->T.M, o->T.off, o->gF, o->gHF);
+    o->mmapF = make_index_map(o->T.M, o->T.off, o->gFfull, o->gHF);
+  }
+}
+
+__global__ void finish_kernel(LtsState* st, OuterState* o) {
+  const int kind = st->kind;
+  if (st->symmetric) {
+    compose(o->t12F, st->F, true);
+    compose(o->t12B, st->B, true);
+    const Xf ib = inverse(o->t12B);
+    o->T = identity(kind);
+    compose(o->T, ib, true);
+    compose(o->T, o->t12F, false);
+  } else {
+    compose(o->T, st->F, true);
+  }
+  double cur[12];
+  get_params(o->T, cur);
+  const int np = kind == kRigid ? 6 : 12;
+  o->sqdiff = delta_of(o->lastp, cur, np);
+  for (int i = 0; i < np; ++i) o->lastp[i] = cur[i];
+  o->stop = o->sqdiff < 1e-3 ? 1 : 0;
+  o->inner_total += st->iter + 1;
+  prepare_maps(o, st->symmetric);
+}
+
+__global__ void prepare_maps_kernel(OuterState* o, int symmetric) { prepare_maps(o, symmetric); }
+
+__global__ void reset_count_kernel(LtsState* st) { st->n_kept = 0ull; }
+
+}  // namespace
+
+int64_t lts_parts(int64_t npts) { return (npts + kPart - 1) / kPart; }
+
+void lts_enqueue_iteration(const double* X, const double* Y, const uint8_t* kept, int64_t npts,
+                           LtsState* st, OuterState* outer, LtsBuffers& b, int iteration,
+                           hipStream_t s) {
+  (void)outer;
+  const int nparts = (int)lts_parts(npts);
+  if (nparts > b.nparts_cap) throw Error(kErrState, "LTS partial buffer too small");
+  const int grid = (int)std::min<int64_t>((npts + kT - 1) / kT, 2048);
+  const bool select = iteration > 0;  // the first pass keeps every pair
+  hipLaunchKernelGGL(residual_kernel, dim3(grid), dim3(kT), 0, s, X, Y, kept, npts, st, b.r,
+                     b.hist1, select ? 1 : 0);
+  if (select) {
+    hipLaunchKernelGGL(scan1_kernel, dim3(1), dim3(1024), 0, s, b.hist1, st);
+    hipLaunchKernelGGL(hist2_kernel, dim3(grid), dim3(kT), 0, s, b.r, npts, st, b.hist2);
+    hipLaunchKernelGGL(scan2_kernel, dim3(1), dim3(1024), 0, s, b.hist2, st);
+    hipLaunchKernelGGL(compact_kernel, dim3(grid), dim3(kT), 0, s, b.r, npts, st, b.cand);
+    hipLaunchKernelGGL(final_select_kernel, dim3(1), dim3(1024), 0, s, b.cand, st);
+  }
+  hipLaunchKernelGGL(moments1_kernel, dim3(nparts), dim3(kT), 0, s, X, Y, b.r, npts, st, b.part1);
+  hipLaunchKernelGGL(moments2_kernel, dim3(nparts), dim3(kT), 0, s, X, Y, b.r, npts, st, b.part1,
+                     nparts, b.part2);
+  hipLaunchKernelGGL(solve_kernel, dim3(1), dim3(kT), 0, s, st, b.part1, b.part2, nparts, b.hist1,
+                     b.hist2);
+  MIRORR_CHECK_LAUNCH();
+}
+
+void lts_start(LtsState* st, const OuterState* outer, int kind, int symmetric, LtsBuffers& b,
+               hipStream_t s) {
+  hipLaunchKernelGGL(init_kernel, dim3(1), dim3(kT), 0, s, st, outer, kind, symmetric, b.hist1,
+                     b.hist2);
+  MIRORR_CHECK_LAUNCH();
+}
+
+void lts_finish(LtsState* st, OuterState* outer, hipStream_t s) {
+  hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(1), 0, s, st, outer);
+  MIRORR_CHECK_LAUNCH();
+}
+
+void outer_prepare_maps(OuterState* outer, int symmetric, hipStream_t s) {
+  hipLaunchKernelGGL(prepare_maps_kernel, dim3(1), dim3(1), 0, s, outer, symmetric);
+  MIRORR_CHECK_LAUNCH();
+}
+
+void lts_reset_count(LtsState* st, hipStream_t s) {
+  hipLaunchKernelGGL(reset_count_kernel, dim3(1), dim3(1), 0, s, st);
+  MIRORR_CHECK_LAUNCH();
+}
+
+}  // namespace mirorr

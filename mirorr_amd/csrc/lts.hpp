@@ -1,0 +1,73 @@
+// Least-trimmed-squares regression on the device (implemented in lts.hip).
+#pragma once
+
+#include "common.hpp"
+#include "imaging.hpp"
+#include "xform.hpp"
+
+namespace mirorr {
+
+// Device state of one LTS run (one OptimizeTransform call).
+struct LtsState {
+  xf::Xf F, B;       // current fits: final_transform(F) and final_transformB
+  double rc[3];      // rotation centre (classic)
+  double last[12];   // last_parameters
+  double fraction, delta, last_delta;
+  int iter, active;
+  int kind, symmetric;
+  int status;        // bit0: a rigid square root needed re-orthogonalising
+  unsigned long long n_kept;  // kept pairs (points with a finite residual)
+  unsigned long long k;       // order statistic of the trimmed threshold
+  unsigned long long k2, k3;  // remaining ranks after the 1st and 2nd digit
+  unsigned int bin1, bin2;
+  unsigned int ncand;
+  double thr;
+};
+
+// Device state of the outer iteration loop of one level.
+struct OuterState {
+  xf::Xf T;              // m_Transform (centred frame in symmetric mode)
+  xf::Xf t12F, t12B;     // half-space transforms (symmetric)
+  xf::Xf fixT, movT;     // translations by the image centres
+  double lastp[12];      // m_LastTransformParameters
+  double sqdiff;         // squared parameter change of the last iteration
+  int stop;              // early breakout (sqdiff < 1e-3)
+  int inner_total;       // LTS inner iterations so far
+  // resampling index maps for the next iteration
+  IndexMap mapF, mapM, mmapF, mmapM;
+  Geom gF, gM;           // level grids of the fixed and moving images
+  Geom gFfull, gMfull;   // full-resolution grids (masks)
+  Geom gHF, gHM;         // half-space grids (fixed / moving direction)
+};
+
+struct LtsBuffers {
+  double* r = nullptr;                // npts residuals
+  unsigned int* hist1 = nullptr;      // 4096
+  unsigned int* hist2 = nullptr;      // 4096
+  unsigned long long* cand = nullptr; // npts candidate keys
+  double* part1 = nullptr;            // nparts x 8
+  double* part2 = nullptr;            // nparts x 24
+  int64_t nparts_cap = 0;
+};
+
+int64_t lts_parts(int64_t npts);
+
+// One LTS run over pairs (X[i], Y[i]) with kept[i] != 0; st->n_kept must
+// hold the number of kept pairs. lts_start resets the state from the outer
+// transform; the host then queues iterations (kernels are predicated on
+// st->active, so a chunk can be queued before reading the flag back).
+void lts_start(LtsState* st, const OuterState* outer, int kind, int symmetric, LtsBuffers& b,
+               hipStream_t s);
+void lts_enqueue_iteration(const double* X, const double* Y, const uint8_t* kept, int64_t npts,
+                           LtsState* st, OuterState* outer, LtsBuffers& buf, int iteration,
+                           hipStream_t s);
+// Composes the fits into the outer transforms (symmetric: t12F/t12B and
+// T = t12F o t12B^-1; classic: T o F), updates the stop flag and the
+// resampling maps of the next iteration.
+void lts_finish(LtsState* st, OuterState* outer, hipStream_t s);
+// Computes the resampling maps of the current outer state (level start).
+void outer_prepare_maps(OuterState* outer, int symmetric, hipStream_t s);
+// Zeroes the kept-pair counter (before the block-matching passes).
+void lts_reset_count(LtsState* st, hipStream_t s);
+
+}  // namespace mirorr

@@ -1,0 +1,104 @@
+// Per-kernel timing with HIP events on the launching stream, plus a device
+// counter per launch for the algorithmic work the kernel reports.
+#pragma once
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "device_buffer.hpp"
+
+namespace mirorr {
+
+struct ProfAgg {
+  int64_t launches = 0;
+  double ms = 0.0;
+  double units = 0.0;  // algorithmic work units reported by the kernel
+};
+
+class Profiler {
+ public:
+  ~Profiler() { release(); }
+  bool on() const { return on_; }
+  void enable(bool v, Budget* bud) {
+    if (v && !counters_.bytes()) counters_.ensure(sizeof(unsigned long long) * kSlots, bud);
+    on_ = v;
+  }
+  // Records the start event; returns the launch's work counter (device) or
+  // nullptr when profiling is off.
+  unsigned long long* begin(const char* name, hipStream_t s) {
+    if (!on_) return nullptr;
+    if (used_ >= kSlots) collect(s);
+    Slot sl;
+    sl.name = name;
+    sl.start = take_event();
+    sl.stop = take_event();
+    sl.counter = used_++;
+    MIRORR_HIP(hipMemsetAsync(counters_.as<unsigned long long>() + sl.counter, 0,
+                              sizeof(unsigned long long), s));
+    MIRORR_HIP(hipEventRecord(sl.start, s));
+    pending_.push_back(sl);
+    return counters_.as<unsigned long long>() + sl.counter;
+  }
+  void end(hipStream_t s) {
+    if (!on_ || pending_.empty()) return;
+    MIRORR_HIP(hipEventRecord(pending_.back().stop, s));
+  }
+  // Waits for pending launches and folds them into the aggregates.
+  void collect(hipStream_t s) {
+    if (pending_.empty()) return;
+    MIRORR_HIP(hipStreamSynchronize(s));
+    std::vector<unsigned long long> c(used_);
+    MIRORR_HIP(hipMemcpy(c.data(), counters_.as<unsigned long long>(),
+                         sizeof(unsigned long long) * used_, hipMemcpyDeviceToHost));
+    for (Slot& sl : pending_) {
+      float ms = 0.f;
+      MIRORR_HIP(hipEventElapsedTime(&ms, sl.start, sl.stop));
+      ProfAgg& a = agg_[sl.name];
+      a.launches += 1;
+      a.ms += ms;
+      a.units += (double)c[sl.counter];
+      free_.push_back(sl.start);
+      free_.push_back(sl.stop);
+    }
+    pending_.clear();
+    used_ = 0;
+  }
+  const std::map<std::string, ProfAgg>& aggregates() const { return agg_; }
+  void reset() { agg_.clear(); }
+  void release() {
+    for (hipEvent_t e : free_) (void)hipEventDestroy(e);
+    for (Slot& s : pending_) {
+      (void)hipEventDestroy(s.start);
+      (void)hipEventDestroy(s.stop);
+    }
+    free_.clear();
+    pending_.clear();
+  }
+
+ private:
+  static constexpr int kSlots = 4096;
+  struct Slot {
+    std::string name;
+    hipEvent_t start, stop;
+    int counter;
+  };
+  hipEvent_t take_event() {
+    if (!free_.empty()) {
+      hipEvent_t e = free_.back();
+      free_.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    MIRORR_HIP(hipEventCreate(&e));
+    return e;
+  }
+  bool on_ = false;
+  int used_ = 0;
+  DeviceBuffer counters_;
+  std::vector<Slot> pending_;
+  std::vector<hipEvent_t> free_;
+  std::map<std::string, ProfAgg> agg_;
+};
+
+}  // namespace mirorr

@@ -1,0 +1,671 @@
+// Device-resident multi-resolution registration (host driver + C ABI).
+//
+// Mirrors MirorrPyramidImplement::RunMirorr (itkMirorrPyramidImplement.txx:71-158)
+// with the classic (itkMirorrRegistrationMethod.txx:422-659) and symmetric
+// (itkSymMirorrRegistrationMethod.txx:238-720) StartOptimization in
+// middle-space resampling mode. All per-voxel and per-block work runs on the
+// GPU and the transforms of the iteration live in device memory (OuterState,
+// LtsState); the host only reads two flags per outer iteration and one per
+// chunk of LTS inner iterations. Level set-up (grids, centring, half-space
+// square roots) is FP64 host code on 4x4 matrices.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "bm_kernels.hpp"
+#include "context.hpp"
+#include "imaging.hpp"
+#include "lts.hpp"
+#include "schedule.hpp"
+
+using namespace mirorr;
+
+struct mirorr_volume {
+  mirorr_ctx* ctx = nullptr;
+  DeviceBuffer data, mask;
+  Geom geom{};
+  bool has_mask = false;
+};
+
+namespace {
+
+size_t nvox(const Geom& g) { return (size_t)g.dims[0] * g.dims[1] * g.dims[2]; }
+
+void to_phys(const Geom& g, const double idx[3], double p[3]) {
+  for (int r = 0; r < 3; ++r) {
+    double s = g.origin[r];
+    for (int c = 0; c < 3; ++c) s += g.dir[3 * r + c] * g.spacing[c] * idx[c];
+    p[r] = s;
+  }
+}
+// ComputeImageCenter: physical point of continuous index (size - 1) / 2.
+void centre_of(const Geom& g, double c[3]) {
+  const double idx[3] = {(g.dims[0] - 1) / 2.0, (g.dims[1] - 1) / 2.0, (g.dims[2] - 1) / 2.0};
+  to_phys(g, idx, c);
+}
+
+// Output geometry of a pyramid level for total shrink factors f
+// (MultiResolutionPyramidImageFilter output information).
+Geom level_geom(const Geom& in, const int f[3]) {
+  Geom o = in;
+  double ds[3];
+  for (int a = 0; a < 3; ++a) {
+    o.spacing[a] = in.spacing[a] * f[a];
+    o.dims[a] = std::max(1, (int)std::floor((double)in.dims[a] / f[a]));
+    ds[a] = o.spacing[a] - in.spacing[a];
+  }
+  for (int r = 0; r < 3; ++r) {
+    double v = 0.0;
+    for (int c = 0; c < 3; ++c) v += in.dir[3 * r + c] * ds[c];
+    o.origin[r] = in.origin[r] + 0.5 * v;
+  }
+  return o;
+}
+
+// Middle (half-way) grid (itkSymMirorrRegistrationMethod.txx:255-283; the
+// classic method adds the moving image centre, itkMirorrRegistrationMethod.txx:495-514).
+Geom middle_grid(const Geom& F, const Geom& M, const double cF[3], const double cM[3],
+                 bool add_moving_centre, const double* dir) {
+  Geom h = F;
+  for (int a = 0; a < 3; ++a) {
+    h.origin[a] = ((F.origin[a] - cF[a]) + (M.origin[a] - cM[a])) / 2.0 +
+                  (add_moving_centre ? cM[a] : 0.0);
+    h.spacing[a] = (F.spacing[a] + M.spacing[a]) / 2.0;
+    h.dims[a] = (int)(((double)(F.dims[a] + M.dims[a]) + 0.5) / 2.0);
+  }
+  for (int i = 0; i < 9; ++i) h.dir[i] = dir[i];
+  return h;
+}
+
+struct Level {
+  DeviceBuffer data;
+  Geom geom;
+};
+
+// Recursive pyramid of one image (coarsest first).
+std::vector<Level> build_pyramid(const float* d_in, const Geom& g, const std::vector<int>& sched,
+                                 int rows, DeviceBuffer& tmp, DeviceBuffer& map, Budget* bud,
+                                 hipStream_t s) {
+  std::vector<Level> out(rows);
+  map.ensure(sizeof(IndexMap), bud);
+  for (int l = rows - 1; l >= 0; --l) {
+    const int f[3] = {sched[3 * l], sched[3 * l + 1], sched[3 * l + 2]};
+    out[l].geom = level_geom(g, f);
+    out[l].data.ensure(sizeof(float) * nvox(out[l].geom), bud);
+    int ratio[3];
+    const float* prev;
+    const Geom* pg;
+    if (l == rows - 1) {
+      for (int a = 0; a < 3; ++a) ratio[a] = f[a];
+      prev = d_in;
+      pg = &g;
+    } else {
+      for (int a = 0; a < 3; ++a) ratio[a] = f[a] / sched[3 * (l + 1) + a];
+      prev = out[l + 1].data.as<float>();
+      pg = &out[l + 1].geom;
+    }
+    tmp.ensure(sizeof(double) * (2 * nvox(*pg) + 3 * 64), bud);
+    launch_pyramid_step(prev, *pg, ratio, out[l].geom, out[l].data.as<float>(), tmp.as<double>(),
+                        map.as<IndexMap>(), s);
+  }
+  return out;
+}
+
+// Per-registration device workspace, grown to the largest level.
+struct Workspace {
+  DeviceBuffer Fh, Mh, Fmh, Mmh;              // resampled images and masks
+  DeviceBuffer statsF, statsM;                // window statistics
+  DeviceBuffer xtF, xtM, sxxF, sxxM, sumF, sumM, flF, flM;  // block preparation
+  DeviceBuffer best, second, match;           // raw BM results (one pass)
+  DeviceBuffer queue, qcount;
+  DeviceBuffer X, Y, kept;                    // LTS pairs
+  DeviceBuffer r, hist1, hist2, cand, part1, part2;
+  DeviceBuffer lts, outer, mapI;
+  DeviceBuffer pyr_tmp, pyr_map;
+};
+
+struct Flags {
+  int active;
+  int stop;
+};
+
+}  // namespace
+
+namespace mirorr {
+
+struct RegRun {
+  mirorr_ctx* ctx;
+  const mirorr_reg_params* p;
+  Workspace w;
+  mirorr_reg_stats* stats;
+  int kind;
+  bool symmetric;
+  hipStream_t s;
+  Budget* bud;
+
+  // One BM pass base -> search over the half grid, writing LTS pairs.
+  void match_pass(const float* base, const uint8_t* bmask, const float* search,
+                  const uint8_t* smask, const Geom& g_base, DeviceBuffer& xt, DeviceBuffer& sxx,
+                  DeviceBuffer& sum, DeviceBuffer& fl, DeviceBuffer& stats_search, int swap,
+                  LtsState* st) {
+    const BMGrid g = make_grid(g_base.dims, p->block_width, p->block_gap, p->nhood_width,
+                               p->nhood_gap);
+    if (g.nblocks <= 0) return;
+    BMResult res{w.best.as<float>(), w.second.as<float>(), w.match.as<int32_t>()};
+    if (tiled_supported(g)) {
+      BlockPrep bp{xt.as<float>(), sxx.as<float>(), sum.as<float>(), fl.as<uint32_t>()};
+      BMScratch scr{w.queue.as<int32_t>(), w.qcount.as<int32_t>(), tile_count(g), &ctx->prof};
+      launch_ncc_search(base, bmask, search, smask, g, bp, SearchStats{stats_search.as<float2>()},
+                        scr, res, s);
+    } else {
+      launch_ncc_generic(base, bmask, search, smask, g, res, s);
+    }
+    const int64_t off = swap ? g.nblocks : 0;
+    launch_postprocess_pairs(g, g_base, res, swap, w.kept.as<uint8_t>() + off,
+                             w.X.as<double>() + 3 * off, w.Y.as<double>() + 3 * off,
+                             &st->n_kept, s);
+    stats->voxel_evals += voxel_evals(g);
+  }
+
+  static int64_t voxel_evals(const BMGrid& g) {
+    int64_t per[3] = {0, 0, 0};
+    for (int a = 0; a < 3; ++a)
+      for (int i = 0; i < g.nb[a]; ++i) {
+        int f, c;
+        cand_range(g.start[a] + i * g.ng, g.dims[a], g.bw, g.bg, g.nw, &f, &c);
+        per[a] += c;
+      }
+    return per[0] * per[1] * per[2] * (int64_t)g.bw * g.bw * g.bw;
+  }
+
+  void ensure_level_buffers(const Geom& gh) {
+    const size_t n = nvox(gh);
+    const BMGrid g = make_grid(gh.dims, p->block_width, p->block_gap, p->nhood_width,
+                               p->nhood_gap);
+    const int64_t nb = std::max<int64_t>(g.nblocks, 1);
+    const int64_t n3 = (int64_t)g.bw * g.bw * g.bw;
+    for (DeviceBuffer* b : {&w.Fh, &w.Mh}) b->ensure(sizeof(float) * n, bud);
+    for (DeviceBuffer* b : {&w.Fmh, &w.Mmh}) b->ensure(n, bud);
+    for (DeviceBuffer* b : {&w.statsF, &w.statsM}) b->ensure(sizeof(float2) * n, bud);
+    for (DeviceBuffer* b : {&w.xtF, &w.xtM}) b->ensure(sizeof(float) * nb * n3, bud);
+    for (DeviceBuffer* b : {&w.sxxF, &w.sxxM, &w.sumF, &w.sumM}) b->ensure(sizeof(float) * nb, bud);
+    for (DeviceBuffer* b : {&w.flF, &w.flM}) b->ensure(sizeof(uint32_t) * nb, bud);
+    w.best.ensure(sizeof(float) * nb, bud);
+    w.second.ensure(sizeof(float) * nb, bud);
+    w.match.ensure(sizeof(int32_t) * 3 * nb, bud);
+    w.queue.ensure(sizeof(int32_t) * std::max<int64_t>(tile_count(g), 1), bud);
+    w.qcount.ensure(sizeof(int32_t), bud);
+    const int64_t np = 2 * nb;
+    w.X.ensure(sizeof(double) * 3 * np, bud);
+    w.Y.ensure(sizeof(double) * 3 * np, bud);
+    w.kept.ensure(np, bud);
+    w.r.ensure(sizeof(double) * np, bud);
+    w.cand.ensure(sizeof(unsigned long long) * np, bud);
+    const int64_t parts = lts_parts(np);
+    w.part1.ensure(sizeof(double) * 8 * parts, bud);
+    w.part2.ensure(sizeof(double) * 24 * parts, bud);
+    w.hist1.ensure(sizeof(unsigned int) * 4096, bud);
+    w.hist2.ensure(sizeof(unsigned int) * 4096, bud);
+  }
+
+  // Runs the LTS to convergence with predicated chunks; returns when done.
+  void run_lts(LtsState* st, OuterState* outer, int64_t npts, Flags* h_flags) {
+    LtsBuffers lb;
+    lb.r = w.r.as<double>();
+    lb.hist1 = w.hist1.as<unsigned int>();
+    lb.hist2 = w.hist2.as<unsigned int>();
+    lb.cand = w.cand.as<unsigned long long>();
+    lb.part1 = w.part1.as<double>();
+    lb.part2 = w.part2.as<double>();
+    lb.nparts_cap = lts_parts(npts);
+    lts_start(st, outer, kind, symmetric ? 1 : 0, lb, s);
+    const int chunk = 4;
+    for (int it = 0; it < 101; it += chunk) {
+      for (int j = it; j < std::min(it + chunk, 101); ++j)
+        lts_enqueue_iteration(w.X.as<double>(), w.Y.as<double>(), w.kept.as<uint8_t>(), npts, st,
+                              outer, lb, j, s);
+      MIRORR_HIP(hipMemcpyAsync(&h_flags->active, &st->active, sizeof(int),
+                                hipMemcpyDeviceToHost, s));
+      MIRORR_HIP(hipStreamSynchronize(s));
+      if (!h_flags->active) break;
+    }
+    lts_finish(st, outer, s);
+  }
+};
+
+}  // namespace mirorr
+
+namespace {
+
+void run_registration(mirorr_ctx* ctx, const mirorr_reg_params* p, const mirorr_volume* fixed,
+                      const mirorr_volume* moving, const double* init_params,
+                      const double* init_center, double* out_params, double* out_center,
+                      mirorr_reg_stats* stats) {
+  const auto t0 = std::chrono::steady_clock::now();
+  hipStream_t s = ctx->stream;
+  Budget* bud = &ctx->budget;
+  RegRun run{ctx, p, {}, stats, p->affine ? xf::kAffine : xf::kRigid, p->symmetric != 0, s, bud};
+  Workspace& w = run.w;
+  const Geom& FG = fixed->geom;
+  const Geom& MG = moving->geom;
+  // Schedule and cropping (itkPyramidScheduleTuner.h:148-161).
+  const Schedule sch = schedule_symmetric(MG.dims, MG.spacing, FG.dims, FG.spacing,
+                                          (unsigned)p->pyr_min_size, (unsigned)p->pyr_num);
+  if (sch.rows <= 0) throw Error(kErrArg, "empty pyramid schedule");
+  const int rows = sch.rows;
+  const int level_min = p->pyr_start == 0 ? 1 : p->pyr_start;
+  const int level_max = p->pyr_end == 0 ? rows : p->pyr_end;
+  int mn = cropped_level(level_min, 1, rows), mx = cropped_level(level_max, rows, rows);
+  if (mn > mx) mx = mn;
+  const int nlev = mx - mn + 1;
+  std::vector<int> ms(sch.moving.begin() + 3 * (mn - 1), sch.moving.begin() + 3 * mx);
+  std::vector<int> fs(sch.fixed.begin() + 3 * (mn - 1), sch.fixed.begin() + 3 * mx);
+  std::vector<Level> Fp = build_pyramid(fixed->data.as<float>(), FG, fs, nlev, w.pyr_tmp,
+                                        w.pyr_map, bud, s);
+  std::vector<Level> Mp = build_pyramid(moving->data.as<float>(), MG, ms, nlev, w.pyr_tmp,
+                                        w.pyr_map, bud, s);
+  w.pyr_tmp.reset();
+  const int t_max = std::min(cropped_level(level_max, nlev, rows), nlev);
+  // Initial transform: given, or centred (itkMirorrPyramidWrapper.txx:611-623:
+  // centre of the moving image, translation between the image centres).
+  xf::Xf T = xf::identity(run.kind);
+  if (init_params) {
+    if (init_center) xf::set_center(T, init_center);
+    xf::set_params(T, init_params);
+  } else if (p->centre_init) {
+    double cM[3], cF[3], t[3];
+    centre_of(MG, cM);
+    centre_of(FG, cF);
+    for (int k = 0; k < 3; ++k) t[k] = cF[k] - cM[k];
+    xf::set_center(T, cM);
+    xf::set_translation(T, t);
+  }
+  double cur[12];
+  xf::get_params(T, cur);
+  int max_it = p->iterations != 0 ? std::abs(p->iterations) : 5;
+  const bool halve = p->iterations < 0;
+  w.lts.ensure(sizeof(LtsState), bud);
+  w.outer.ensure(sizeof(OuterState), bud);
+  w.mapI.ensure(sizeof(IndexMap), bud);
+  LtsState* st = w.lts.as<LtsState>();
+  OuterState* outer = w.outer.as<OuterState>();
+  Flags* flags = nullptr;
+  MIRORR_HIP(hipHostMalloc((void**)&flags, sizeof(Flags), hipHostMallocDefault));
+  struct FreeHost {
+    void* p;
+    ~FreeHost() { (void)hipHostFree(p); }
+  } free_flags{flags};
+  std::memset(stats, 0, sizeof(*stats));
+  stats->levels = t_max;
+  const int np = xf::nparams(T);
+  for (int level = 0; level < t_max && level < 16; ++level) {
+    xf::set_params(T, cur);
+    const Geom& gF = Fp[level].geom;
+    const Geom& gM = Mp[level].geom;
+    double cF[3], cM[3];
+    centre_of(gF, cF);
+    centre_of(gM, cM);
+    OuterState o{};
+    o.gF = gF;
+    o.gM = gM;
+    o.gFfull = FG;
+    o.gMfull = MG;
+    const double c0[3] = {T.c[0], T.c[1], T.c[2]};
+    if (run.symmetric) {
+      o.gHF = middle_grid(gF, gM, cF, cM, false, gF.dir);
+      o.gHM = middle_grid(gF, gM, cF, cM, false, gM.dir);
+      // Centre m_Transform and express it between centred images.
+      xf::set_center_to_origin(T);
+      o.fixT = xf::translation(cF);
+      o.movT = xf::translation(cM);
+      xf::compose(T, xf::inverse(o.fixT), false);
+      xf::compose(T, o.movT, true);
+      // ComputeHalfSpaceTransforms (itkSymMirorrRegistrationMethod.txx:139-201).
+      xf::M4 m;
+      for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) m.a[i][j] = (i == 3 && j == 3) ? 1.0 : 0.0;
+      for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 3; ++j) m.a[i][j] = T.M.a[i][j];
+        m.a[i][3] = T.t[i];
+      }
+      const xf::M4 m12 = xf::sqrt_db(m, 1e-14, 50);
+      const xf::M4 im12 = xf::sqrt_db(xf::inverse(m), 1e-14, 50);
+      o.t12F = xf::identity(run.kind);
+      o.t12B = xf::identity(run.kind);
+      xf::M3 K, L;
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          K.a[i][j] = m12.a[i][j];
+          L.a[i][j] = im12.a[i][j];
+        }
+      if (!xf::set_matrix(o.t12F, K) || !xf::set_matrix(o.t12B, L))
+        throw Error(kErrAlgo, "Attempting to set a non-orthogonal rotation matrix");
+      const double a[3] = {m12.a[0][3], m12.a[1][3], m12.a[2][3]};
+      const double b[3] = {im12.a[0][3], im12.a[1][3], im12.a[2][3]};
+      xf::set_translation(o.t12F, a);
+      xf::set_translation(o.t12B, b);
+    } else {
+      o.gHF = middle_grid(gF, gM, cF, cM, true, gF.dir);
+      o.gHM = middle_grid(gF, gM, cF, cM, true, gM.dir);
+    }
+    o.T = T;
+    xf::get_params(T, o.lastp);
+    for (int a = 0; a < 3; ++a) stats->level_dims[level][a] = o.gHF.dims[a];
+    run.ensure_level_buffers(o.gHF);
+    MIRORR_HIP(hipMemcpyAsync(outer, &o, sizeof(o), hipMemcpyHostToDevice, s));
+    outer_prepare_maps(outer, run.symmetric ? 1 : 0, s);
+    const int64_t nb = make_grid(o.gHF.dims, p->block_width, p->block_gap, p->nhood_width,
+                                 p->nhood_gap).nblocks;
+    const int64_t npts = run.symmetric ? 2 * nb : nb;
+    const uint8_t* fmask = fixed->has_mask ? fixed->mask.as<uint8_t>() : nullptr;
+    const uint8_t* mmask = moving->has_mask ? moving->mask.as<uint8_t>() : nullptr;
+    const int fmode = fixed->has_mask ? kMaskLinear : kMaskInside;
+    const int mmode = moving->has_mask ? kMaskLinear : kMaskInside;
+    const BMGrid gh = make_grid(o.gHF.dims, p->block_width, p->block_gap, p->nhood_width,
+                                p->nhood_gap);
+    const bool tiled = tiled_supported(gh);
+    if (!run.symmetric) {
+      // The moving image and mask are resampled once onto the grid.
+      const xf::M3 I = xf::eye3();
+      const double z[3] = {0.0, 0.0, 0.0};
+      IndexMap mi[2] = {make_index_map(I, z, gM, o.gHM), make_index_map(I, z, MG, o.gHM)};
+      w.mapI.ensure(sizeof(mi), bud);
+      MIRORR_HIP(hipMemcpyAsync(w.mapI.as<IndexMap>(), mi, sizeof(mi), hipMemcpyHostToDevice, s));
+      launch_resample(Mp[level].data.as<float>(), gM.dims, w.mapI.as<IndexMap>(), mmask, MG.dims,
+                      w.mapI.as<IndexMap>() + 1, mmode, o.gHM.dims, w.Mh.as<float>(),
+                      w.Mmh.as<uint8_t>(), s);
+      if (tiled)
+        launch_block_prep(w.Mh.as<float>(), w.Mmh.as<uint8_t>(), gh,
+                          BlockPrep{w.xtM.as<float>(), w.sxxM.as<float>(), w.sumM.as<float>(),
+                                    w.flM.as<uint32_t>()},
+                          s);
+      // (the half-grid stays fixed through the level; hipMemcpy is ordered)
+      MIRORR_HIP(hipStreamSynchronize(s));
+    }
+    int it = 0;
+    for (; it < max_it; ++it) {
+      // 1. resample onto the half-way grid under the current transforms
+      launch_resample(Fp[level].data.as<float>(), gF.dims, &outer->mapF, fmask, FG.dims,
+                      &outer->mmapF, fmode, o.gHF.dims, w.Fh.as<float>(), w.Fmh.as<uint8_t>(), s);
+      if (run.symmetric)
+        launch_resample(Mp[level].data.as<float>(), gM.dims, &outer->mapM, mmask, MG.dims,
+                        &outer->mmapM, mmode, o.gHM.dims, w.Mh.as<float>(), w.Mmh.as<uint8_t>(),
+                        s);
+      // 2. block matching: moving half as base, then (symmetric) swapped
+      lts_reset_count(st, s);
+      if (tiled) {
+        launch_search_stats(w.Fh.as<float>(), w.Fmh.as<uint8_t>(), gh,
+                            SearchStats{w.statsF.as<float2>()}, s);
+        if (run.symmetric) {
+          launch_search_stats(w.Mh.as<float>(), w.Mmh.as<uint8_t>(), gh,
+                              SearchStats{w.statsM.as<float2>()}, s);
+          launch_block_prep(w.Mh.as<float>(), w.Mmh.as<uint8_t>(), gh,
+                            BlockPrep{w.xtM.as<float>(), w.sxxM.as<float>(), w.sumM.as<float>(),
+                                      w.flM.as<uint32_t>()},
+                            s);
+          launch_block_prep(w.Fh.as<float>(), w.Fmh.as<uint8_t>(), gh,
+                            BlockPrep{w.xtF.as<float>(), w.sxxF.as<float>(), w.sumF.as<float>(),
+                                      w.flF.as<uint32_t>()},
+                            s);
+        }
+      }
+      run.match_pass(w.Mh.as<float>(), w.Mmh.as<uint8_t>(), w.Fh.as<float>(), w.Fmh.as<uint8_t>(),
+                     o.gHM, w.xtM, w.sxxM, w.sumM, w.flM, w.statsF, 0, st);
+      if (run.symmetric)
+        run.match_pass(w.Fh.as<float>(), w.Fmh.as<uint8_t>(), w.Mh.as<float>(),
+                       w.Mmh.as<uint8_t>(), o.gHF, w.xtF, w.sxxF, w.sumF, w.flF, w.statsM, 1, st);
+      // 3. LTS and transform update on the device
+      run.run_lts(st, outer, npts, flags);
+      MIRORR_HIP(hipMemcpyAsync(&flags->stop, &outer->stop, sizeof(int), hipMemcpyDeviceToHost, s));
+      MIRORR_HIP(hipStreamSynchronize(s));
+      if (flags->stop) {
+        ++it;
+        break;
+      }
+    }
+    stats->iterations[level] = it;
+    MIRORR_HIP(hipMemcpyAsync(&o, outer, sizeof(o), hipMemcpyDeviceToHost, s));
+    unsigned long long nk = 0;
+    MIRORR_HIP(hipMemcpyAsync(&nk, &st->n_kept, sizeof(nk), hipMemcpyDeviceToHost, s));
+    MIRORR_HIP(hipStreamSynchronize(s));
+    stats->points[level] = (int64_t)nk;
+    stats->inner_iterations += o.inner_total;
+    if (run.symmetric) {
+      // Final composition and re-centring (itkSymMirorrRegistrationMethod.txx:661-692).
+      auto centered = [&](const xf::Xf& t12, const xf::Xf& shift) {
+        double par[12];
+        xf::get_params(t12, par);
+        xf::Xf c = xf::identity(run.kind);
+        xf::set_params(c, par);
+        xf::compose(c, shift, false);
+        return c;
+      };
+      const xf::Xf cF12 = centered(o.t12F, o.fixT), cM12 = centered(o.t12B, o.movT);
+      T = xf::identity(run.kind);
+      xf::compose(T, xf::inverse(cM12), true);
+      xf::compose(T, cF12, false);
+      xf::update_center(T, c0);
+    } else {
+      T = o.T;
+    }
+    xf::get_params(T, cur);
+    if (halve) max_it = std::max(1, max_it / 2);
+  }
+  for (int i = 0; i < np; ++i) out_params[i] = cur[i];
+  if (out_center)
+    for (int k = 0; k < 3; ++k) out_center[k] = T.c[k];
+  stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+
+extern "C" {
+
+void mirorr_reg_default_params(mirorr_reg_params* p) {
+  if (!p) return;
+  // mirorr.cxx:99-281 (pyr-end 0 = all levels, the usual choice; the CLI
+  // default of 1 runs only the coarsest level).
+  p->symmetric = 1;
+  p->affine = 0;
+  p->block_width = 4;
+  p->block_gap = 1;
+  p->nhood_width = 7;
+  p->nhood_gap = 3;
+  p->iterations = 0;
+  p->pyr_start = 1;
+  p->pyr_end = 0;
+  p->pyr_num = 1024;
+  p->pyr_min_size = 32;
+  p->portion_kept = 0.5;
+  p->variance_select = 0;
+  p->verbosity = 0;
+  p->centre_init = 1;
+}
+
+mirorr_status mirorr_volume_create(mirorr_ctx* c, const float* data, const uint8_t* mask,
+                                   const mirorr_geom* geom, mirorr_volume** out) {
+  if (!c || !out) return MIRORR_ERR_ARG;
+  *out = nullptr;
+  mirorr_volume* v = nullptr;
+  mirorr_status st = guarded(c, [&] {
+    if (!data || !geom) throw Error(kErrArg, "null volume data or geometry");
+    for (int i = 0; i < 3; ++i)
+      if (geom->dims[i] < 1 || !(geom->spacing[i] > 0.0)) throw Error(kErrArg, "bad geometry");
+    v = new mirorr_volume();
+    v->ctx = c;
+    v->geom = to_geom(*geom);
+    const size_t n = (size_t)geom->dims[0] * geom->dims[1] * geom->dims[2];
+    v->data.ensure(n * sizeof(float), &c->budget);
+    MIRORR_HIP(hipMemcpyAsync(v->data.as<float>(), data, n * sizeof(float), hipMemcpyHostToDevice,
+                              c->stream));
+    if (mask) {
+      v->mask.ensure(n, &c->budget);
+      MIRORR_HIP(hipMemcpyAsync(v->mask.as<uint8_t>(), mask, n, hipMemcpyHostToDevice, c->stream));
+      v->has_mask = true;
+    }
+    MIRORR_HIP(hipStreamSynchronize(c->stream));
+  });
+  if (st != MIRORR_OK) {
+    delete v;
+    return st;
+  }
+  *out = v;
+  return MIRORR_OK;
+}
+
+void mirorr_volume_destroy(mirorr_volume* v) {
+  if (!v) return;
+  (void)hipSetDevice(v->ctx->device);
+  (void)hipStreamSynchronize(v->ctx->stream);
+  delete v;
+}
+
+mirorr_status mirorr_register(mirorr_ctx* c, const mirorr_reg_params* p,
+                              const mirorr_volume* fixed, const mirorr_volume* moving,
+                              const double* init_params, const double* init_center,
+                              double* out_params, double* out_center, mirorr_reg_stats* stats) {
+  if (!c || !p || !fixed || !moving || !out_params) return MIRORR_ERR_ARG;
+  mirorr_reg_stats local{};
+  return guarded(c, [&] {
+    if (p->block_width < 1 || p->block_gap < 1 || p->nhood_width < 1 || p->nhood_gap < 1)
+      throw Error(kErrArg, "block-matcher parameters must be positive");
+    run_registration(c, p, fixed, moving, init_params, init_center, out_params, out_center,
+                     stats ? stats : &local);
+  });
+}
+
+mirorr_status mirorr_lts_fit(mirorr_ctx* c, int symmetric, int affine, const double* x,
+                             const double* y, size_t n, double* params, double* center,
+                             double* fwd34, double* bwd34, int* iterations) {
+  if (!c || !x || !y || !params) return MIRORR_ERR_ARG;
+  return guarded(c, [&] {
+    hipStream_t s = c->stream;
+    Budget* bud = &c->budget;
+    const int kind = affine ? xf::kAffine : xf::kRigid;
+    mirorr_reg_params rp;
+    mirorr_reg_default_params(&rp);
+    mirorr_reg_stats stats{};
+    RegRun run{c, &rp, {}, &stats, kind, symmetric != 0, s, bud};
+    Workspace& w = run.w;
+    const int64_t np = (int64_t)n;
+    w.X.ensure(sizeof(double) * 3 * std::max<int64_t>(np, 1), bud);
+    w.Y.ensure(sizeof(double) * 3 * std::max<int64_t>(np, 1), bud);
+    w.kept.ensure(std::max<int64_t>(np, 1), bud);
+    w.r.ensure(sizeof(double) * std::max<int64_t>(np, 1), bud);
+    w.cand.ensure(sizeof(unsigned long long) * std::max<int64_t>(np, 1), bud);
+    w.part1.ensure(sizeof(double) * 8 * std::max<int64_t>(lts_parts(np), 1), bud);
+    w.part2.ensure(sizeof(double) * 24 * std::max<int64_t>(lts_parts(np), 1), bud);
+    w.hist1.ensure(sizeof(unsigned int) * 4096, bud);
+    w.hist2.ensure(sizeof(unsigned int) * 4096, bud);
+    w.lts.ensure(sizeof(LtsState), bud);
+    w.outer.ensure(sizeof(OuterState), bud);
+    std::vector<uint8_t> ones(std::max<int64_t>(np, 1), 1);
+    if (np > 0) {
+      MIRORR_HIP(hipMemcpyAsync(w.X.as<double>(), x, sizeof(double) * 3 * np,
+                                hipMemcpyHostToDevice, s));
+      MIRORR_HIP(hipMemcpyAsync(w.Y.as<double>(), y, sizeof(double) * 3 * np,
+                                hipMemcpyHostToDevice, s));
+    }
+    MIRORR_HIP(hipMemcpyAsync(w.kept.as<uint8_t>(), ones.data(), ones.size(),
+                              hipMemcpyHostToDevice, s));
+    OuterState o{};
+    const Geom unit{{1, 1, 1}, {1, 1, 1}, {0, 0, 0}, {1, 0, 0, 0, 1, 0, 0, 0, 1}};
+    o.gF = o.gM = o.gFfull = o.gMfull = o.gHF = o.gHM = unit;
+    o.fixT = o.movT = xf::identity(xf::kRigid);
+    o.t12F = o.t12B = xf::identity(kind);
+    o.T = xf::identity(kind);
+    if (!symmetric) {
+      if (center) xf::set_center(o.T, center);
+      xf::set_params(o.T, params);
+    }
+    xf::get_params(o.T, o.lastp);
+    LtsState* st = w.lts.as<LtsState>();
+    OuterState* outer = w.outer.as<OuterState>();
+    MIRORR_HIP(hipMemcpyAsync(outer, &o, sizeof(o), hipMemcpyHostToDevice, s));
+    const unsigned long long nk = (unsigned long long)np;
+    MIRORR_HIP(hipMemcpyAsync(&st->n_kept, &nk, sizeof(nk), hipMemcpyHostToDevice, s));
+    Flags* flags = nullptr;
+    MIRORR_HIP(hipHostMalloc((void**)&flags, sizeof(Flags), hipHostMallocDefault));
+    struct FreeHost {
+      void* p;
+      ~FreeHost() { (void)hipHostFree(p); }
+    } free_flags{flags};
+    run.run_lts(st, outer, std::max<int64_t>(np, 1), flags);
+    MIRORR_HIP(hipMemcpyAsync(&o, outer, sizeof(o), hipMemcpyDeviceToHost, s));
+    MIRORR_HIP(hipStreamSynchronize(s));
+    xf::get_params(o.T, params);
+    if (center)
+      for (int k = 0; k < 3; ++k) center[k] = o.T.c[k];
+    for (int r = 0; r < 3; ++r) {
+      for (int k = 0; k < 3; ++k) {
+        if (fwd34) fwd34[4 * r + k] = o.t12F.M.a[r][k];
+        if (bwd34) bwd34[4 * r + k] = o.t12B.M.a[r][k];
+      }
+      if (fwd34) fwd34[4 * r + 3] = o.t12F.off[r];
+      if (bwd34) bwd34[4 * r + 3] = o.t12B.off[r];
+    }
+    if (iterations) *iterations = o.inner_total;
+  });
+}
+
+mirorr_status mirorr_resample(mirorr_ctx* c, const float* data, const mirorr_geom* gin,
+                              const double matrix[9], const double offset[3],
+                              const mirorr_geom* gout, float* out, uint8_t* out_mask) {
+  if (!c || !data || !gin || !matrix || !offset || !gout || !out) return MIRORR_ERR_ARG;
+  return guarded(c, [&] {
+    hipStream_t s = c->stream;
+    const Geom gi = to_geom(*gin), go = to_geom(*gout);
+    DeviceBuffer din, dout, dmask, dmap;
+    din.ensure(sizeof(float) * nvox(gi), &c->budget);
+    dout.ensure(sizeof(float) * nvox(go), &c->budget);
+    if (out_mask) dmask.ensure(nvox(go), &c->budget);
+    xf::M3 M;
+    for (int i = 0; i < 9; ++i) M.a[i / 3][i % 3] = matrix[i];
+    const IndexMap map = make_index_map(M, offset, gi, go);
+    dmap.ensure(sizeof(IndexMap), &c->budget);
+    MIRORR_HIP(hipMemcpyAsync(din.as<float>(), data, sizeof(float) * nvox(gi),
+                              hipMemcpyHostToDevice, s));
+    MIRORR_HIP(hipMemcpyAsync(dmap.as<IndexMap>(), &map, sizeof(map), hipMemcpyHostToDevice, s));
+    launch_resample(din.as<float>(), gi.dims, dmap.as<IndexMap>(), nullptr, gi.dims,
+                    dmap.as<IndexMap>(), out_mask ? kMaskInside : kMaskNone, go.dims,
+                    dout.as<float>(), out_mask ? dmask.as<uint8_t>() : nullptr, s);
+    MIRORR_HIP(hipMemcpyAsync(out, dout.as<float>(), sizeof(float) * nvox(go),
+                              hipMemcpyDeviceToHost, s));
+    if (out_mask)
+      MIRORR_HIP(hipMemcpyAsync(out_mask, dmask.as<uint8_t>(), nvox(go), hipMemcpyDeviceToHost, s));
+    MIRORR_HIP(hipStreamSynchronize(s));
+  });
+}
+
+mirorr_status mirorr_pyramid_level(mirorr_ctx* c, const float* data, const mirorr_geom* geom,
+                                   const int* schedule, int rows, int level, float* out,
+                                   mirorr_geom* out_geom) {
+  if (!c || !data || !geom || !schedule || rows < 1 || level < 0 || level >= rows)
+    return MIRORR_ERR_ARG;
+  return guarded(c, [&] {
+    hipStream_t s = c->stream;
+    const Geom g = to_geom(*geom);
+    DeviceBuffer din, tmp, map;
+    din.ensure(sizeof(float) * nvox(g), &c->budget);
+    MIRORR_HIP(hipMemcpyAsync(din.as<float>(), data, sizeof(float) * nvox(g),
+                              hipMemcpyHostToDevice, s));
+    const std::vector<int> sched(schedule, schedule + 3 * rows);
+    std::vector<Level> lv = build_pyramid(din.as<float>(), g, sched, rows, tmp, map, &c->budget, s);
+    const Geom& lg = lv[level].geom;
+    MIRORR_HIP(hipMemcpyAsync(out, lv[level].data.as<float>(), sizeof(float) * nvox(lg),
+                              hipMemcpyDeviceToHost, s));
+    MIRORR_HIP(hipStreamSynchronize(s));
+    if (out_geom) {
+      for (int i = 0; i < 3; ++i) {
+        out_geom->dims[i] = lg.dims[i];
+        out_geom->spacing[i] = lg.spacing[i];
+        out_geom->origin[i] = lg.origin[i];
+      }
+      for (int i = 0; i < 9; ++i) out_geom->direction[i] = lg.dir[i];
+    }
+  });
+}
+
+}  // extern "C"

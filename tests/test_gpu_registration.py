@@ -1,0 +1,135 @@
+"""Parity of the device resampler, pyramid, LTS and full registration with
+the CPU oracle (through the C ABI)."""
+import numpy as np
+import pytest
+
+from mirorr_amd import phantom, transforms
+from mirorr_amd.capi import Geom as CGeom
+from oracle.oracle import Geom
+
+pytestmark = pytest.mark.gpu
+
+
+def _geoms(dims, spacing=(1.0, 1.0, 1.0), origin=(0.0, 0.0, 0.0)):
+    return (CGeom.make(dims, spacing, origin), Geom(dims=dims, spacing=spacing, origin=origin))
+
+
+def test_resample_matches_oracle(oracle):
+    from mirorr_amd.registration import resample
+    rng = np.random.default_rng(0)
+    a = rng.uniform(0, 1000, (20, 24, 28)).astype(np.float32)
+    gi_c, gi_o = _geoms((28, 24, 20), (1.5, 1.2, 2.0), (-20.0, -14.0, -19.0))
+    go_c, go_o = _geoms((30, 26, 18), (1.4, 1.3, 2.1), (-21.0, -16.0, -18.0))
+    M = phantom.euler_zxy(0.1, -0.05, 0.15) @ np.diag([1.05, 0.97, 1.02])
+    off = np.array([1.3, -2.2, 0.7])
+    got, gm = resample(a, gi_c, M, off, go_c, want_mask=True)
+    ref, rm = oracle.resample(a, None, gi_o, M.ravel(), off, go_o, want_mask=True)
+    np.testing.assert_allclose(got, ref, rtol=1e-6, atol=1e-3)
+    assert np.mean(gm != rm) < 1e-3  # boundary points may round either way
+
+
+@pytest.mark.parametrize("sched", [[[2, 2, 2], [1, 1, 1]], [[4, 4, 2], [2, 2, 2], [1, 1, 1]],
+                                   [[2, 2, 1], [1, 1, 1]]])
+def test_pyramid_matches_oracle(oracle, sched):
+    from mirorr_amd.registration import pyramid_level
+    rng = np.random.default_rng(1)
+    a = rng.uniform(0, 100, (16, 32, 40)).astype(np.float32)
+    gc, go = _geoms((40, 32, 16), (1.0, 1.5, 2.0), (3.0, -4.0, 5.0))
+    rows = len(sched)
+    # oracle recursion
+    prev, pg = a, go
+    levels = {}
+    for lvl in range(rows - 1, -1, -1):
+        lg = oracle.pyramid_level_geom(go, sched[lvl])
+        ratio = sched[lvl] if lvl == rows - 1 else [sched[lvl][k] // sched[lvl + 1][k]
+                                                    for k in range(3)]
+        prev = oracle.pyramid_step(prev, pg, ratio, lg)
+        pg = lg
+        levels[lvl] = (prev, lg)
+    for lvl in range(rows):
+        got, gg = pyramid_level(a, gc, sched, lvl)
+        ref, rg = levels[lvl]
+        assert tuple(gg.dims) == tuple(rg.dims)
+        np.testing.assert_allclose(tuple(gg.origin), rg.origin, atol=1e-12)
+        np.testing.assert_allclose(got, ref, rtol=1e-6, atol=1e-4)
+
+
+def _pairs(seed, n=3000, affine=False, outliers=0.3):
+    rng = np.random.default_rng(seed)
+    X = rng.uniform(-80, 80, (n, 3))
+    if affine:
+        A = phantom.euler_zxy(0.08, -0.05, 0.1) @ np.diag([1.04, 0.97, 1.02])
+    else:
+        A = phantom.euler_zxy(0.08, -0.05, 0.1)
+    Y = X @ A.T + np.array([3.0, -2.0, 5.0]) + rng.normal(0, 0.5, X.shape)
+    bad = rng.choice(n, int(outliers * n), replace=False)
+    Y[bad] += rng.uniform(-30, 30, (len(bad), 3))
+    return X, Y
+
+
+@pytest.mark.parametrize("affine", [0, 1])
+def test_lts_classic_matches_oracle(oracle, affine):
+    from mirorr_amd.registration import lts_fit
+    X, Y = _pairs(2, affine=bool(affine))
+    n = 12 if affine else 6
+    p0 = np.zeros(n)
+    if affine:
+        p0[[0, 4, 8]] = 1.0
+    c0 = np.array([1.0, -2.0, 0.5])
+    rp, rc, rit = oracle.lts_classic(affine, X, Y, p0, c0)
+    gp, gc, _, _, git = lts_fit(False, bool(affine), X, Y, p0, c0)
+    assert git == rit
+    np.testing.assert_allclose(gp, rp, atol=1e-9)
+
+
+@pytest.mark.parametrize("affine", [0, 1])
+def test_lts_symmetric_matches_oracle(oracle, affine):
+    from mirorr_amd.registration import lts_fit
+    X, Y = _pairs(3, affine=bool(affine))
+    rf, rb, rp, rit = oracle.lts_symmetric(affine, X, Y)
+    gp, _, gf, gb, git = lts_fit(True, bool(affine), X, Y)
+    assert git == rit
+    np.testing.assert_allclose(gf, rf, atol=1e-9)
+    np.testing.assert_allclose(gb, rb, atol=1e-9)
+    np.testing.assert_allclose(gp, rp, atol=1e-9)
+
+
+def test_lts_symmetric_exact_on_clean_affine():
+    from mirorr_amd.registration import lts_fit
+    X, Y = _pairs(4, affine=True, outliers=0.0)
+    rng = np.random.default_rng(4)
+    A = phantom.euler_zxy(0.08, -0.05, 0.1) @ np.diag([1.04, 0.97, 1.02])
+    Y = X @ A.T + np.array([3.0, -2.0, 5.0])
+    _, _, f, b, _ = lts_fit(True, True, X, Y)
+    T = np.vstack([f, [0, 0, 0, 1]]) @ np.linalg.inv(np.vstack([b, [0, 0, 0, 1]]))
+    np.testing.assert_allclose(T[:3, :3], A, atol=1e-9)
+
+
+@pytest.mark.parametrize("symmetric,affine", [(1, 0), (1, 1), (0, 0), (0, 1)])
+def test_registration_matches_oracle(oracle, symmetric, affine):
+    from mirorr_amd.registration import MirorrPyramid
+    pair = phantom.make_pair(dims=(64, 64, 32), spacing=(3.0, 3.0, 3.0), n_ellipsoids=20,
+                             seed=11, model="affine" if affine else "rigid")
+    rp = oracle.reg_params(symmetric=symmetric, affine=affine, nw=7, ng=3, pyr_num=2,
+                           nthreads=8)
+    ref_p, ref_c, ref_st = oracle.register(pair.fixed, pair.moving, rp)
+    reg = MirorrPyramid()
+    if not symmetric:
+        reg.SetRegistrationTypeClassic()
+    reg.SetTransformType("affine" if affine else "rigid")
+    reg.SetNhoodWidth(7)
+    reg.SetNhoodGap(3)
+    reg.GetPyramidScheduleTuner().SetMaxLevelNumber(2)
+    reg.SetFixedImage(pair.fixed)
+    reg.SetMovingImage(pair.moving)
+    got_p, got_c = reg.RunMirorr()
+    st = reg.stats
+    print("oracle", ref_st["iterations"], ref_st["points"], ref_st["inner_iterations"])
+    print("gpu   ", st["iterations"], st["points"], st["inner_iterations"], st["seconds"])
+    assert st["voxel_evals"] > 0
+    M1, o1 = transforms.matrix_offset(got_p, got_c, affine)
+    M2, o2 = transforms.matrix_offset(ref_p, ref_c, affine)
+    dmm = transforms.max_corner_distance(M1, o1, M2, o2, pair.fixed)
+    ddeg = transforms.rotation_angle_deg(M1, M2)
+    print(f"corner distance {dmm:.3e} mm, rotation {ddeg:.3e} deg")
+    assert dmm < 0.01 and ddeg < 0.01
