@@ -21,7 +21,9 @@
 //  * Top-2 per block uses the float score buffer semantics of the reference
 //    with ties on the best broken by the earliest candidate in scan order.
 #include "bm_kernels.hpp"
+#include "select.hpp"
 
+#include <algorithm>
 #include <climits>
 #include <cmath>
 
@@ -66,6 +68,76 @@ __global__ __launch_bounds__(kThreads) void search_stats_kernel(
       }
     }
   out[lin3(nx, ny, x, yy, z)] = make_float2((float)mu, (float)ss);
+}
+
+// BW = 4: a thread owns one (x, y) column of window positions and slides
+// along z over a chunk, keeping the four 4x4 planes of the current window
+// in registers with their plane sums (double), so a position loads one
+// new plane (16 values) instead of the whole window twice.
+constexpr int kStatsZ = 16;  // z positions per thread
+
+template <bool MASK>
+__global__ __launch_bounds__(kThreads) void search_stats4_kernel(
+    const float* __restrict__ y, const uint8_t* __restrict__ sm, int nx, int ny, int nz,
+    float2* __restrict__ out) {
+  const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int yy = blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int z0 = blockIdx.z * kStatsZ;
+  if (x > nx - 4 || yy > ny - 4) return;
+  const int zend = min(z0 + kStatsZ, nz - 3);  // exclusive end of output z
+  float p[4][16];
+  uint32_t mk[4];  // mask bit per plane voxel
+  double ps[4];
+  auto load = [&](int slot, int z) {
+    double s = 0.0;
+    uint32_t bits = 0u;
+#pragma unroll
+    for (int dy = 0; dy < 4; ++dy) {
+      const int64_t o = lin3(nx, ny, x, yy + dy, z);
+#pragma unroll
+      for (int dx = 0; dx < 4; ++dx) {
+        const float v = y[o + dx];
+        p[slot][4 * dy + dx] = v;
+        s += (double)v;
+        if (MASK && sm[o + dx] != 0) bits |= 1u << (4 * dy + dx);
+      }
+    }
+    ps[slot] = s;
+    mk[slot] = bits;
+  };
+  auto emit = [&](int z) {
+    const double mu = (ps[0] + ps[1] + ps[2] + ps[3]) * (1.0 / 64.0);
+    double ss = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        float v = p[k][j];
+        asm volatile("" : "+v"(v));  // convert afresh: keeps 64 floats live, not 64 doubles
+        double d = (double)v - mu;
+        if (MASK && !((mk[k] >> j) & 1u)) d = 0.0;
+        ss = fma(d, d, ss);
+      }
+    out[lin3(nx, ny, x, yy, z)] = make_float2((float)mu, (float)ss);
+  };
+  if (z0 >= zend) return;
+  load(0, z0);
+  load(1, z0 + 1);
+  load(2, z0 + 2);
+  // Plane z + k + 3 goes to slot (k + 3) % 4; the slots are a ring.
+  for (int z = z0; z < zend; z += 4) {
+    load(3, z + 3);
+    emit(z);
+    if (z + 1 >= zend) break;
+    load(0, z + 4);
+    emit(z + 1);
+    if (z + 2 >= zend) break;
+    load(1, z + 5);
+    emit(z + 2);
+    if (z + 3 >= zend) break;
+    load(2, z + 6);
+    emit(z + 3);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -233,7 +305,7 @@ __device__ unsigned long long tile_evals(const TileArgs& a, int px, int py0, int
   unsigned long long tot = 0;
   const int cx = cnt(px, g.dims[0]);
   for (int bl = 0; bl < T::NB; ++bl) {
-    if (!bvalid[bl]) continue;
+    if (bvalid[bl] != 1) continue;
     const int py = py0 + g.ng * (bl % T::TY), pz = pz0 + g.ng * (bl / T::TY);
     tot += (unsigned long long)cx * cnt(py, g.dims[1]) * cnt(pz, g.dims[2]);
   }
@@ -260,6 +332,23 @@ __device__ __forceinline__ void write_block_result(const TileArgs& a, int64_t b,
   a.out_match[3 * b + 0] = px - T::H + cx;
   a.out_match[3 * b + 1] = py - T::H + cy;
   a.out_match[3 * b + 2] = pz - T::H + cz;
+}
+
+// Record of a block dropped by the variance selection: the initial score
+// buffer (-3) with the block's own position as match.
+__device__ __forceinline__ void write_skipped(const TileArgs& a, int64_t b, int px, int py,
+                                              int pz) {
+  a.out_best[b] = -3.0f;
+  a.out_second[b] = -3.0f;
+  a.out_match[3 * b + 0] = px;
+  a.out_match[3 * b + 1] = py;
+  a.out_match[3 * b + 2] = pz;
+}
+
+// Block states in a tile: 0 outside the grid, 1 searched, 2 skipped.
+__device__ __forceinline__ int block_state(const TileArgs& a, int gy, int gz, int64_t b) {
+  if (gy >= a.g.nb[1] || gz >= a.g.nb[2]) return 0;
+  return (a.bflags[b] & kBlockSkip) ? 2 : 1;
 }
 
 // Orderable encodings for LDS atomics: max over the key is max over the
@@ -317,7 +406,7 @@ __global__ __launch_bounds__(kThreads, 4) void ncc_tile_fast(TileArgs a) {
   __shared__ float bsxx[T::NB], bsum[T::NB];
   __shared__ int bvalid[T::NB];
   __shared__ MergeSlot mrg[T::NB];
-  __shared__ int s_ok;
+  __shared__ int s_ok, s_any;
   __shared__ float s_ref;
 
   const BMGrid& g = a.g;
@@ -335,6 +424,7 @@ __global__ __launch_bounds__(kThreads, 4) void ncc_tile_fast(TileArgs a) {
 
   if (tid == 0) {
     s_ok = 1;
+    s_any = 0;
     // Reference shift: window mean at the tile's middle block (clamped).
     const int ry = min(py0 + NG * (T::TY / 2), ny - T::BW);
     const int rz = min(pz0 + NG * (T::TZ / 2), nz - T::BW);
@@ -344,14 +434,24 @@ __global__ __launch_bounds__(kThreads, 4) void ncc_tile_fast(TileArgs a) {
   __syncthreads();
   if (tid < T::NB) {
     const int gy = by0 + tid % T::TY, gz = bz0 + tid / T::TY;
-    const bool v = gy < g.nb[1] && gz < g.nb[2];
-    bvalid[tid] = v;
-    if (v) {
-      const int64_t b = bx + (int64_t)g.nb[0] * (gy + (int64_t)g.nb[1] * gz);
+    const int64_t b = bx + (int64_t)g.nb[0] * (gy + (int64_t)g.nb[1] * gz);
+    const int st = block_state(a, gy, gz, b);
+    bvalid[tid] = st;
+    if (st == 1) {
+      s_any = 1;
       bsxx[tid] = a.sxx[b];
       bsum[tid] = a.sumxt[b];
       if ((a.bflags[b] & 1u) == 0u) s_ok = 0;
     }
+  }
+  __syncthreads();
+  if (!s_any) {  // every block of the tile is skipped (or outside the grid)
+    if (tid < T::NB && bvalid[tid] == 2) {
+      const int gy = by0 + tid % T::TY, gz = bz0 + tid / T::TY;
+      const int64_t b = bx + (int64_t)g.nb[0] * (gy + (int64_t)g.nb[1] * gz);
+      write_skipped(a, b, px, py0 + NG * (tid % T::TY), pz0 + NG * (tid / T::TY));
+    }
+    return;  // workgroup-uniform
   }
   const float ref = s_ref;
   // Stage the union search window (shifted by ref; zeros outside the image).
@@ -392,7 +492,7 @@ __global__ __launch_bounds__(kThreads, 4) void ncc_tile_fast(TileArgs a) {
   Top2 t2;
   top2_init(t2);
   const int bl = tid / T::TPB;
-  if (tid < T::NT && bvalid[bl]) {
+  if (tid < T::NT && bvalid[bl] == 1) {
     const int r = tid % T::TPB;
     const int cz = r / T::NCY;
     const int cyg = r % T::NCY;
@@ -471,15 +571,26 @@ __global__ __launch_bounds__(kThreads, 4) void ncc_tile_fast(TileArgs a) {
   if (tid < T::NT) merge_phase2(mrg, bl, t2);
   __syncthreads();
   if (tid < T::NB && bvalid[tid]) {
-    const Top2 r = merge_result(mrg[tid]);
     const int gy = by0 + tid % T::TY, gz = bz0 + tid / T::TY;
     const int64_t b = bx + (int64_t)g.nb[0] * (gy + (int64_t)g.nb[1] * gz);
-    write_block_result<T>(a, b, px, py0 + NG * (tid % T::TY), pz0 + NG * (tid / T::TY), r);
+    const int py = py0 + NG * (tid % T::TY), pz = pz0 + NG * (tid / T::TY);
+    if (bvalid[tid] == 2)
+      write_skipped(a, b, px, py, pz);
+    else
+      write_block_result<T>(a, b, px, py, pz, merge_result(mrg[tid]));
   }
 }
 
-// Masked path: exact sums over voxels where both masks are on, centred per
-// candidate on the window mean (about 6 VALU ops per voxel-eval).
+// Masked path: sums over voxels where both masks are on. The window is
+// staged as v * (y - ref) (v = search mask) plus the mask bytes.
+// With t = v (y - mu) = v (y - ref) - v (mu - ref) (one FMA per voxel):
+//  * Base block fully on (the usual case at image borders): Sxy = Sum x~ t,
+//    Syy = Sum t^2, Sxx = Sum v x~^2 (4 FMAs per voxel). Syy is summed from
+//    the same float t as Sxy rather than taken from the window statistic:
+//    with few voxels on, r^2 is a ratio of nearly equal terms and both must
+//    carry the same centring (one voxel on gives exactly 1).
+//  * Base block partly masked (base mask b): Sxy = Sum b x~ t,
+//    Syy = Sum b t^2, Sxx = Sum b x~^2 v (5 ops per voxel).
 template <class T>
 __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
   constexpr int R = (T::GT + kThreads - 1) / kThreads;  // tasks per thread
@@ -488,6 +599,7 @@ __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
   __shared__ unsigned long long bbits[T::NB];  // base mask bit per voxel
   __shared__ int bvalid[T::NB];
   __shared__ MergeSlot mrg[T::NB];
+  __shared__ float s_ref;
 
   if (blockIdx.x >= *a.qcount) return;
   const int tile = a.queue[blockIdx.x];
@@ -507,7 +619,19 @@ __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
   float* wy_ = reinterpret_cast<float*>(dyn4);
   uint8_t* wv_ = reinterpret_cast<uint8_t*>(wy_ + total);
 
+  if (tid == 0) {
+    const int ry = min(py0 + NG * (T::TY / 2), ny - T::BW);
+    const int rz = min(pz0 + NG * (T::TZ / 2), nz - T::BW);
+    s_ref = a.stats[lin3(nx, ny, px, ry, rz)].x;
+  }
   merge_init(mrg, T::NB);
+  if (tid < T::NB) {
+    const int gy = by0 + tid % T::TY, gz = bz0 + tid / T::TY;
+    bvalid[tid] = block_state(a, gy, gz, bx + (int64_t)g.nb[0] * (gy + (int64_t)g.nb[1] * gz));
+    bbits[tid] = 0ull;
+  }
+  __syncthreads();
+  const float ref = s_ref;
   for (int i = tid; i < total; i += kThreads) {
     const int wx = i % T::PX;
     const int wy = (i / T::PX) % WY;
@@ -517,18 +641,14 @@ __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
     uint8_t m = 0;
     if (wx < T::WX && gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 && gz < nz) {
       const int64_t o = lin3(nx, ny, gx, gy, gz);
-      v = a.y[o];
-      m = (a.sm == nullptr || a.sm[o] != 0) ? 1 : 0;
+      if (a.sm == nullptr || a.sm[o] != 0) {
+        v = a.y[o] - ref;
+        m = 1;
+      }
     }
     wy_[i] = v;
     wv_[i] = m;
   }
-  if (tid < T::NB) {
-    const int gy = by0 + tid % T::TY, gz = bz0 + tid / T::TY;
-    bvalid[tid] = gy < g.nb[1] && gz < g.nb[2];
-    bbits[tid] = 0ull;
-  }
-  __syncthreads();
   for (int i = tid; i < T::NB * 64; i += kThreads) {
     const int bl = i / 64, k = i % 64;
     const int gy = by0 + bl % T::TY, gz = bz0 + bl / T::TY;
@@ -566,7 +686,7 @@ __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
     const int cz = r / T::NW;
     const int cy = r % T::NW;
     recb[j] = bl;
-    if (!bvalid[bl]) continue;
+    if (bvalid[bl] != 1) continue;
     const int bly = bl % T::TY, blz = bl / T::TY;
     const int py = py0 + NG * bly, pz = pz0 + NG * blz;
     const int lox = max(px - T::H, 0), hix = min(px - T::H + T::NW - 1, nx - T::BW);
@@ -574,61 +694,105 @@ __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
     const int loz = max(pz - T::H, 0), hiz = min(pz - T::H + T::NW - 1, nz - T::BW);
     const int czg = pz - T::H + cz, cyg = py - T::H + cy;
     if (czg < loz || czg > hiz || cyg < loy || cyg > hiy) continue;
-    float myc[T::NW];
+    float dmu[T::NW];
     const float2* srow = a.stats + lin3(nx, ny, 0, cyg, czg);
 #pragma unroll
     for (int cx = 0; cx < T::NW; ++cx) {
       const int cxg = min(max(px - T::H + cx, lox), hix);
-      myc[cx] = srow[cxg].x;
+      dmu[cx] = srow[cxg].x - ref;
     }
-    float sxy[T::NW], syy[T::NW], sxx[T::NW];
-#pragma unroll
-    for (int cx = 0; cx < T::NW; ++cx) sxy[cx] = syy[cx] = sxx[cx] = 0.0f;
     const unsigned long long bits = bbits[bl];
     const int rowoff = ((blz * NG + cz) * WY + (bly * NG + cy)) * T::PX;
+    float sxy[T::NW], sxx[T::NW], s3[T::NW];
+#pragma unroll
+    for (int cx = 0; cx < T::NW; ++cx) sxy[cx] = sxx[cx] = s3[cx] = 0.0f;
+    if (bits == ~0ull) {
 #pragma unroll 1
-    for (int iz = 0; iz < 4; ++iz) {
+      for (int iz = 0; iz < 4; ++iz) {
 #pragma unroll
-      for (int iy = 0; iy < 4; ++iy) {
-        const int ro = rowoff + (iz * WY + iy) * T::PX;
-        float yr[T::PX], vr[T::PX];
-        const float4* yp = reinterpret_cast<const float4*>(wy_ + ro);
-        const uint32_t* vp = reinterpret_cast<const uint32_t*>(wv_ + ro);
+        for (int iy = 0; iy < 4; ++iy) {
+          const int ro = rowoff + (iz * WY + iy) * T::PX;
+          float yr[T::PX], vr[T::PX];
+          const float4* yp = reinterpret_cast<const float4*>(wy_ + ro);
+          const uint32_t* vp = reinterpret_cast<const uint32_t*>(wv_ + ro);
 #pragma unroll
-        for (int v = 0; v < T::NV; ++v) {
-          const float4 q = yp[v];
-          const uint32_t m = vp[v];
-          yr[4 * v + 0] = q.x; yr[4 * v + 1] = q.y; yr[4 * v + 2] = q.z; yr[4 * v + 3] = q.w;
-          vr[4 * v + 0] = (float)(m & 0xffu);
-          vr[4 * v + 1] = (float)((m >> 8) & 0xffu);
-          vr[4 * v + 2] = (float)((m >> 16) & 0xffu);
-          vr[4 * v + 3] = (float)(m >> 24);
+          for (int v = 0; v < T::NV; ++v) {
+            const float4 q = yp[v];
+            const uint32_t m = vp[v];
+            yr[4 * v + 0] = q.x; yr[4 * v + 1] = q.y; yr[4 * v + 2] = q.z; yr[4 * v + 3] = q.w;
+            vr[4 * v + 0] = (float)(m & 0xffu);
+            vr[4 * v + 1] = (float)((m >> 8) & 0xffu);
+            vr[4 * v + 2] = (float)((m >> 16) & 0xffu);
+            vr[4 * v + 3] = (float)(m >> 24);
+          }
+          const float4 xq4 = xh4[bl * 16 + iz * 4 + iy];
+          const float xh[4] = {xq4.x, xq4.y, xq4.z, xq4.w};
+          float xs[4];
+#pragma unroll
+          for (int ix = 0; ix < 4; ++ix) xs[ix] = xh[ix] * xh[ix];
+#pragma unroll
+          for (int cx = 0; cx < T::NW; ++cx)
+#pragma unroll
+            for (int ix = 0; ix < 4; ++ix) {
+              const float tm = fmaf(-vr[cx + ix], dmu[cx], yr[cx + ix]);
+              sxy[cx] = fmaf(xh[ix], tm, sxy[cx]);
+              s3[cx] = fmaf(tm, tm, s3[cx]);
+              sxx[cx] = fmaf(xs[ix], vr[cx + ix], sxx[cx]);
+            }
         }
-        const float4 xq4 = xh4[bl * 16 + iz * 4 + iy];
-        const float xh[4] = {xq4.x, xq4.y, xq4.z, xq4.w};
-        const uint32_t nib = (uint32_t)(bits >> (iz * 16 + iy * 4)) & 0xfu;
-        float bb[4], xs[4];
+      }
 #pragma unroll
-        for (int ix = 0; ix < 4; ++ix) {
-          bb[ix] = (float)((nib >> ix) & 1u);
-          xs[ix] = xh[ix] * xh[ix];
-        }
+      for (int cx = 0; cx < T::NW; ++cx) {
+        const int cxg = px - T::H + cx;
+        if (cxg < lox || cxg > hix) continue;
+        top2_push(rec[j], ncc_score(sxy[cx], sxx[cx], s3[cx]), (cz * T::NW + cy) * T::NW + cx);
+      }
+    } else {
+      // s3 = Sum b t^2 (Syy over both masks)
+#pragma unroll 1
+      for (int iz = 0; iz < 4; ++iz) {
 #pragma unroll
-        for (int cx = 0; cx < T::NW; ++cx)
+        for (int iy = 0; iy < 4; ++iy) {
+          const int ro = rowoff + (iz * WY + iy) * T::PX;
+          float yr[T::PX], vr[T::PX];
+          const float4* yp = reinterpret_cast<const float4*>(wy_ + ro);
+          const uint32_t* vp = reinterpret_cast<const uint32_t*>(wv_ + ro);
+#pragma unroll
+          for (int v = 0; v < T::NV; ++v) {
+            const float4 q = yp[v];
+            const uint32_t m = vp[v];
+            yr[4 * v + 0] = q.x; yr[4 * v + 1] = q.y; yr[4 * v + 2] = q.z; yr[4 * v + 3] = q.w;
+            vr[4 * v + 0] = (float)(m & 0xffu);
+            vr[4 * v + 1] = (float)((m >> 8) & 0xffu);
+            vr[4 * v + 2] = (float)((m >> 16) & 0xffu);
+            vr[4 * v + 3] = (float)(m >> 24);
+          }
+          const float4 xq4 = xh4[bl * 16 + iz * 4 + iy];
+          const float xh[4] = {xq4.x, xq4.y, xq4.z, xq4.w};
+          const uint32_t nib = (uint32_t)(bits >> (iz * 16 + iy * 4)) & 0xfu;
+          float bb[4], xs[4];
 #pragma unroll
           for (int ix = 0; ix < 4; ++ix) {
-            const float tm = (yr[cx + ix] - myc[cx]) * vr[cx + ix];
-            sxy[cx] = fmaf(xh[ix], tm, sxy[cx]);
-            syy[cx] = fmaf(tm * bb[ix], tm, syy[cx]);
-            sxx[cx] = fmaf(xs[ix], vr[cx + ix], sxx[cx]);
+            bb[ix] = (float)((nib >> ix) & 1u);
+            xs[ix] = xh[ix] * xh[ix];
           }
-      }
-    }
 #pragma unroll
-    for (int cx = 0; cx < T::NW; ++cx) {
-      const int cxg = px - T::H + cx;
-      if (cxg < lox || cxg > hix) continue;
-      top2_push(rec[j], ncc_score(sxy[cx], sxx[cx], syy[cx]), (cz * T::NW + cy) * T::NW + cx);
+          for (int cx = 0; cx < T::NW; ++cx)
+#pragma unroll
+            for (int ix = 0; ix < 4; ++ix) {
+              const float tm = fmaf(-vr[cx + ix], dmu[cx], yr[cx + ix]);
+              sxy[cx] = fmaf(xh[ix], tm, sxy[cx]);
+              s3[cx] = fmaf(tm * bb[ix], tm, s3[cx]);
+              sxx[cx] = fmaf(xs[ix], vr[cx + ix], sxx[cx]);
+            }
+        }
+      }
+#pragma unroll
+      for (int cx = 0; cx < T::NW; ++cx) {
+        const int cxg = px - T::H + cx;
+        if (cxg < lox || cxg > hix) continue;
+        top2_push(rec[j], ncc_score(sxy[cx], sxx[cx], s3[cx]), (cz * T::NW + cy) * T::NW + cx);
+      }
     }
   }
 #pragma unroll
@@ -638,10 +802,13 @@ __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
   for (int j = 0; j < R; ++j) merge_phase2(mrg, recb[j], rec[j]);
   __syncthreads();
   if (tid < T::NB && bvalid[tid]) {
-    const Top2 r = merge_result(mrg[tid]);
     const int gy = by0 + tid % T::TY, gz = bz0 + tid / T::TY;
     const int64_t b = bx + (int64_t)g.nb[0] * (gy + (int64_t)g.nb[1] * gz);
-    write_block_result<T>(a, b, px, py0 + NG * (tid % T::TY), pz0 + NG * (tid / T::TY), r);
+    const int py = py0 + NG * (tid % T::TY), pz = pz0 + NG * (tid / T::TY);
+    if (bvalid[tid] == 2)
+      write_skipped(a, b, px, py, pz);
+    else
+      write_block_result<T>(a, b, px, py, pz, merge_result(mrg[tid]));
   }
 }
 
@@ -653,7 +820,7 @@ __global__ void reset_counter(int32_t* c) { *c = 0; }
 __global__ __launch_bounds__(kThreads) void ncc_generic_kernel(
     const float* __restrict__ x, const uint8_t* __restrict__ bm,
     const float* __restrict__ y, const uint8_t* __restrict__ sm, BMGrid g,
-    float* out_best, float* out_second, int32_t* out_match) {
+    float* out_best, float* out_second, int32_t* out_match, const uint32_t* flags) {
   __shared__ float tb[kThreads], ts[kThreads];
   __shared__ int ti[kThreads];
   const int64_t b = blockIdx.x;
@@ -662,6 +829,14 @@ __global__ __launch_bounds__(kThreads) void ncc_generic_kernel(
   const int byi = (int)((b / g.nb[0]) % g.nb[1]);
   const int bzi = (int)(b / ((int64_t)g.nb[0] * g.nb[1]));
   const int p[3] = {g.start[0] + g.ng * bxi, g.start[1] + g.ng * byi, g.start[2] + g.ng * bzi};
+  if (flags && (flags[b] & kBlockSkip)) {  // dropped by the variance selection
+    if (threadIdx.x == 0) {
+      out_best[b] = -3.0f;
+      out_second[b] = -3.0f;
+      for (int k = 0; k < 3; ++k) out_match[3 * b + k] = p[k];
+    }
+    return;
+  }
   int first[3], cnt[3];
   for (int k = 0; k < 3; ++k) cand_range(p[k], g.dims[k], g.bw, g.bg, g.nw, &first[k], &cnt[k]);
   const int bw = g.bw;
@@ -846,6 +1021,86 @@ __global__ __launch_bounds__(kThreads) void compact_scatter(const uint8_t* f, in
 }
 
 // ---------------------------------------------------------------------------
+// Block-variance selection: exact K-th largest block key (the float bits of
+// the centred sum of squares, non-negative, so ordered as unsigned ints) by
+// three radix digits (bits 31..20, 19..8, 7..0), each histogram pass
+// finishing its bin search in the last workgroup to arrive.
+__global__ __launch_bounds__(kThreads) void vsel_init(VarSelState* st, unsigned int* hist,
+                                                      unsigned long long k) {
+  for (int i = threadIdx.x; i < kVarSelHistBins; i += kThreads) hist[i] = 0u;
+  if (threadIdx.x == 0) {
+    st->k = k;
+    st->k2 = st->k3 = 0ull;
+    st->bin1 = st->bin2 = st->bin3 = 0u;
+    for (int i = 0; i < 3; ++i) st->sync[i] = 0u;
+    st->kept = 0ull;
+  }
+}
+
+template <int PASS>
+__global__ __launch_bounds__(kThreads) void vsel_hist(const float* __restrict__ key, int64_t n,
+                                                      VarSelState* st, unsigned int* hist) {
+  constexpr int NB = PASS == 2 ? 256 : 4096;
+  __shared__ unsigned int h[NB];
+  for (int i = threadIdx.x; i < NB; i += kThreads) h[i] = 0u;
+  __syncthreads();
+  const unsigned int b1 = st->bin1, b2 = st->bin2;
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kThreads) {
+    const uint32_t u = __float_as_uint(key[i]);
+    if (PASS == 0) {
+      atomicAdd(&h[u >> 20], 1u);
+    } else if (PASS == 1) {
+      if ((u >> 20) == b1) atomicAdd(&h[(u >> 8) & 0xfffu], 1u);
+    } else {
+      if ((u >> 8) == ((b1 << 12) | b2)) atomicAdd(&h[u & 0xffu], 1u);
+    }
+  }
+  __syncthreads();
+  unsigned int* gh = hist + (PASS == 0 ? 0 : PASS == 1 ? 4096 : 8192);
+  for (int i = threadIdx.x; i < NB; i += kThreads)
+    if (h[i]) atomicAdd(&gh[i], h[i]);
+  if (!sel::arrive_last(&st->sync[PASS])) return;
+  if (PASS == 0) sel::find_bin<4096 / kThreads>(gh, st->k, &st->bin1, &st->k2);
+  if (PASS == 1) sel::find_bin<4096 / kThreads>(gh, st->k2, &st->bin2, &st->k3);
+  if (PASS == 2) sel::find_bin<256 / kThreads>(gh, st->k3, &st->bin3, &st->k3);
+}
+
+// Marks the blocks below the cut-off and accounts the kept ones.
+__global__ __launch_bounds__(kThreads) void vsel_mark(const float* __restrict__ key,
+                                                      uint32_t* __restrict__ flags, BMGrid g,
+                                                      VarSelState* st,
+                                                      unsigned long long* evals) {
+  const uint32_t cut = (st->bin1 << 20) | (st->bin2 << 8) | st->bin3;
+  unsigned long long kept = 0, ev = 0;
+  for (int64_t b = (int64_t)blockIdx.x * kThreads + threadIdx.x; b < g.nblocks;
+       b += (int64_t)gridDim.x * kThreads) {
+    const bool keep = __float_as_uint(key[b]) >= cut;
+    flags[b] = (flags[b] & ~kBlockSkip) | (keep ? 0u : kBlockSkip);
+    if (keep) {
+      const int bxi = (int)(b % g.nb[0]);
+      const int byi = (int)((b / g.nb[0]) % g.nb[1]);
+      const int bzi = (int)(b / ((int64_t)g.nb[0] * g.nb[1]));
+      int f, cx, cy, cz;
+      cand_range(g.start[0] + g.ng * bxi, g.dims[0], g.bw, g.bg, g.nw, &f, &cx);
+      cand_range(g.start[1] + g.ng * byi, g.dims[1], g.bw, g.bg, g.nw, &f, &cy);
+      cand_range(g.start[2] + g.ng * bzi, g.dims[2], g.bw, g.bg, g.nw, &f, &cz);
+      kept += 1;
+      ev += (unsigned long long)cx * cy * cz;
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    kept += __shfl_xor(kept, o, 64);
+    ev += __shfl_xor(ev, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (kept) atomicAdd(&st->kept, kept);
+    if (evals && ev) atomicAdd(evals, ev * (unsigned long long)(g.bw * g.bw * g.bw));
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Tile configurations per neighbourhood width (BW = 4, BG = 1).
 using Cfg5 = TileCfg<5, 5, 5, 10>;
 using Cfg7 = TileCfg<7, 4, 3, 6>;
@@ -924,12 +1179,39 @@ void launch_search_stats(const float* y, const uint8_t* smask, const BMGrid& g,
                          SearchStats st, hipStream_t s) {
   const int nx = g.dims[0], ny = g.dims[1], nz = g.dims[2];
   if (nx < g.bw || ny < g.bw || nz < g.bw) return;
-  dim3 grid((nx + kThreads - 1) / kThreads, ny - g.bw + 1, nz - g.bw + 1);
-  if (g.bw == 4)
-    hipLaunchKernelGGL(search_stats_kernel<4>, grid, dim3(kThreads), 0, s, y, smask, nx, ny, nz, 4, st.ms);
-  else
-    hipLaunchKernelGGL(search_stats_kernel<0>, grid, dim3(kThreads), 0, s, y, smask, nx, ny, nz, g.bw, st.ms);
+  if (g.bw == 4) {
+    dim3 grid((nx - 3 + 63) / 64, (ny - 3 + 3) / 4, (nz - 3 + kStatsZ - 1) / kStatsZ);
+    if (smask)
+      hipLaunchKernelGGL(search_stats4_kernel<true>, grid, dim3(kThreads), 0, s, y, smask, nx, ny,
+                         nz, st.ms);
+    else
+      hipLaunchKernelGGL(search_stats4_kernel<false>, grid, dim3(kThreads), 0, s, y, smask, nx,
+                         ny, nz, st.ms);
+  } else {
+    dim3 grid((nx + kThreads - 1) / kThreads, ny - g.bw + 1, nz - g.bw + 1);
+    hipLaunchKernelGGL(search_stats_kernel<0>, grid, dim3(kThreads), 0, s, y, smask, nx, ny, nz,
+                       g.bw, st.ms);
+  }
   MIRORR_CHECK_LAUNCH();
+}
+
+bool launch_variance_select(const BMGrid& g, const BlockPrep& bp, double portion,
+                            VarSelScratch& scr, hipStream_t s) {
+  const int64_t n = g.nblocks;
+  if (n <= 0) return false;
+  int64_t K = (int64_t)std::ceil(portion * (double)n);
+  K = std::min<int64_t>(std::max<int64_t>(K, 1), n);
+  if (K >= n) return false;
+  const unsigned grid = (unsigned)std::min<int64_t>((n + kThreads - 1) / kThreads, 1024);
+  hipLaunchKernelGGL(vsel_init, dim3(1), dim3(kThreads), 0, s, scr.st, scr.hist,
+                     (unsigned long long)(n - K));
+  hipLaunchKernelGGL(vsel_hist<0>, dim3(grid), dim3(kThreads), 0, s, bp.sxx, n, scr.st, scr.hist);
+  hipLaunchKernelGGL(vsel_hist<1>, dim3(grid), dim3(kThreads), 0, s, bp.sxx, n, scr.st, scr.hist);
+  hipLaunchKernelGGL(vsel_hist<2>, dim3(grid), dim3(kThreads), 0, s, bp.sxx, n, scr.st, scr.hist);
+  hipLaunchKernelGGL(vsel_mark, dim3(grid), dim3(kThreads), 0, s, bp.sxx, bp.flags, g, scr.st,
+                     scr.evals);
+  MIRORR_CHECK_LAUNCH();
+  return true;
 }
 
 void launch_block_prep(const float* x, const uint8_t* bmask, const BMGrid& g,
@@ -944,10 +1226,10 @@ void launch_block_prep(const float* x, const uint8_t* bmask, const BMGrid& g,
 
 void launch_ncc_generic(const float* x, const uint8_t* bmask, const float* y,
                         const uint8_t* smask, const BMGrid& g, BMResult res,
-                        hipStream_t s) {
+                        hipStream_t s, const uint32_t* flags) {
   if (g.nblocks == 0) return;
   hipLaunchKernelGGL(ncc_generic_kernel, dim3((unsigned)g.nblocks), dim3(kThreads), 0, s, x,
-                     bmask, y, smask, g, res.best, res.second, res.match);
+                     bmask, y, smask, g, res.best, res.second, res.match, flags);
   MIRORR_CHECK_LAUNCH();
 }
 
@@ -957,7 +1239,7 @@ void launch_ncc_search(const float* x, const uint8_t* bmask, const float* y,
                        hipStream_t s) {
   if (g.nblocks == 0) return;
   if (!tiled_supported(g)) {
-    launch_ncc_generic(x, bmask, y, smask, g, res, s);
+    launch_ncc_generic(x, bmask, y, smask, g, res, s, bp.flags);
     return;
   }
   TileArgs a{};

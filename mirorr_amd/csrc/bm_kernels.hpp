@@ -11,8 +11,25 @@ struct BlockPrep {
   float* xt = nullptr;        // nblocks x BW^3 centred voxels x - mean (float)
   float* sxx = nullptr;       // nblocks: sum (x - mean)^2 over all voxels
   float* sumxt = nullptr;     // nblocks: sum of the float centred voxels
-  uint32_t* flags = nullptr;  // nblocks: bit0 base mask all on, bit1 all off
+  uint32_t* flags = nullptr;  // nblocks: bit0 base mask all on, bit1 all off,
+                              // bit2 dropped by the variance selection
 };
+
+constexpr uint32_t kBlockSkip = 4u;  // flags bit: block not searched
+
+// Block-variance selection (device state and scratch).
+struct VarSelState {
+  unsigned long long k, k2, k3;  // ascending rank of the cut-off key, then within bins
+  unsigned int bin1, bin2, bin3;
+  unsigned int sync[3];
+  unsigned long long kept;       // blocks kept by the last selection
+};
+struct VarSelScratch {
+  VarSelState* st = nullptr;
+  unsigned int* hist = nullptr;        // 4096 + 4096 + 256 bins
+  unsigned long long* evals = nullptr; // optional: += voxel-evals of the kept blocks
+};
+constexpr int kVarSelHistBins = 4096 + 4096 + 256;
 
 // Per-position window statistics of a search image (window = BW^3 voxels
 // starting at the position): x = mean over all voxels, y = sum over
@@ -40,6 +57,15 @@ void launch_search_stats(const float* y, const uint8_t* smask, const BMGrid& g,
                          SearchStats st, hipStream_t s);
 void launch_block_prep(const float* x, const uint8_t* bmask, const BMGrid& g,
                        BlockPrep bp, hipStream_t s);
+// Keeps the top `portion` of the blocks by centred sum of squares (bp.sxx):
+// K = ceil(portion * nblocks) clamped to [1, nblocks], the cut-off is the
+// K-th largest value and every block at or above it is kept (ties kept);
+// the others get kBlockSkip in bp.flags. The reference has this culling
+// compiled out (itkBlockMatcher.txx:664-719, keeping the top 1 - portion
+// instead). Returns false (and touches nothing) when every block is kept.
+bool launch_variance_select(const BMGrid& g, const BlockPrep& bp, double portion,
+                            VarSelScratch& scr, hipStream_t s);
+
 // Exhaustive NCC search. Chooses the tiled fast kernels for BW=4, BG=1 and a
 // supported NW, else the generic kernel.
 void launch_ncc_search(const float* x, const uint8_t* bmask, const float* y,
@@ -50,7 +76,7 @@ void launch_ncc_search(const float* x, const uint8_t* bmask, const float* y,
 // unsupported shapes and as an independent device cross-check.
 void launch_ncc_generic(const float* x, const uint8_t* bmask, const float* y,
                         const uint8_t* smask, const BMGrid& g, BMResult res,
-                        hipStream_t s);
+                        hipStream_t s, const uint32_t* flags = nullptr);
 int64_t tile_count(const BMGrid& g);
 bool tiled_supported(const BMGrid& g);
 

@@ -29,6 +29,9 @@ struct mirorr_bm {
   DeviceBuffer prep_xt[2], prep_sxx[2], prep_sum[2], prep_flags[2];
   uint64_t prep_ver[2] = {0, 0};
   int prep_key[2][2] = {{0, 0}, {0, 0}};  // (bw, ng) the prep was made for
+  double prep_portion[2] = {1.0, 1.0};    // variance selection applied to the flags
+  bool prep_selected[2] = {false, false};
+  DeviceBuffer vsel_st, vsel_hist, vsel_evals;
   // Results of the last search.
   DeviceBuffer best, second, match, kept, base_mm, match_mm, index;
   DeviceBuffer queue, qcount, chunk_counts, chunk_offsets;
@@ -87,39 +90,56 @@ static void run_search(mirorr_bm* bm) {
   bm->second.ensure(sizeof(float) * nb, bud);
   bm->match.ensure(sizeof(int32_t) * 3 * nb, bud);
   BMResult res{bm->best.as<float>(), bm->second.as<float>(), bm->match.as<int32_t>()};
-  if (tiled_supported(g)) {
-    // Search-image statistics (slot SEARCH) and base-image preparation.
-    const int si = MIRORR_ROLE_SEARCH, bi = MIRORR_ROLE_BASE;
+  const int si = MIRORR_ROLE_SEARCH, bi = MIRORR_ROLE_BASE;
+  const bool tiled = tiled_supported(g);
+  const double portion = bm->p.variance_select ? bm->p.portion_kept : 1.0;
+  // Base-image preparation (tiled search, or sums for the variance selection).
+  if (tiled || portion < 1.0) {
+    if (bm->prep_ver[bi] != B.version || bm->prep_key[bi][0] != g.bw ||
+        bm->prep_key[bi][1] != g.ng || bm->prep_portion[bi] != portion) {
+      const int64_t n3 = (int64_t)g.bw * g.bw * g.bw;
+      bm->prep_xt[bi].ensure(sizeof(float) * nb * n3, bud);
+      bm->prep_sxx[bi].ensure(sizeof(float) * nb, bud);
+      bm->prep_sum[bi].ensure(sizeof(float) * nb, bud);
+      bm->prep_flags[bi].ensure(sizeof(uint32_t) * nb, bud);
+      const BlockPrep bp{bm->prep_xt[bi].as<float>(), bm->prep_sxx[bi].as<float>(),
+                         bm->prep_sum[bi].as<float>(), bm->prep_flags[bi].as<uint32_t>()};
+      launch_block_prep(B.data, B.mask, g, bp, s);
+      bm->prep_selected[bi] = false;
+      if (portion < 1.0) {
+        bm->vsel_st.ensure(sizeof(VarSelState), bud);
+        bm->vsel_hist.ensure(sizeof(unsigned int) * kVarSelHistBins, bud);
+        bm->vsel_evals.ensure(sizeof(unsigned long long), bud);
+        MIRORR_HIP(hipMemsetAsync(bm->vsel_evals.as<void>(), 0, sizeof(unsigned long long), s));
+        VarSelScratch vs{bm->vsel_st.as<VarSelState>(), bm->vsel_hist.as<unsigned int>(),
+                         bm->vsel_evals.as<unsigned long long>()};
+        bm->prep_selected[bi] = launch_variance_select(g, bp, portion, vs, s);
+      }
+      bm->prep_ver[bi] = B.version;
+      bm->prep_key[bi][0] = g.bw;
+      bm->prep_key[bi][1] = g.ng;
+      bm->prep_portion[bi] = portion;
+    }
+  }
+  const BlockPrep bp{bm->prep_xt[bi].as<float>(), bm->prep_sxx[bi].as<float>(),
+                     bm->prep_sum[bi].as<float>(), bm->prep_flags[bi].as<uint32_t>()};
+  if (tiled) {
+    // Search-image statistics (slot SEARCH).
     if (bm->stats_ver[si] != S.version || bm->stats_bw[si] != g.bw) {
       bm->stats[si].ensure(sizeof(float2) * nvox, bud);
       launch_search_stats(S.data, S.mask, g, SearchStats{bm->stats[si].as<float2>()}, s);
       bm->stats_ver[si] = S.version;
       bm->stats_bw[si] = g.bw;
     }
-    if (bm->prep_ver[bi] != B.version || bm->prep_key[bi][0] != g.bw || bm->prep_key[bi][1] != g.ng) {
-      const int64_t n3 = (int64_t)g.bw * g.bw * g.bw;
-      bm->prep_xt[bi].ensure(sizeof(float) * nb * n3, bud);
-      bm->prep_sxx[bi].ensure(sizeof(float) * nb, bud);
-      bm->prep_sum[bi].ensure(sizeof(float) * nb, bud);
-      bm->prep_flags[bi].ensure(sizeof(uint32_t) * nb, bud);
-      launch_block_prep(B.data, B.mask, g,
-                        BlockPrep{bm->prep_xt[bi].as<float>(), bm->prep_sxx[bi].as<float>(),
-                                  bm->prep_sum[bi].as<float>(), bm->prep_flags[bi].as<uint32_t>()},
-                        s);
-      bm->prep_ver[bi] = B.version;
-      bm->prep_key[bi][0] = g.bw;
-      bm->prep_key[bi][1] = g.ng;
-    }
     const int64_t nt = tile_count(g);
     bm->queue.ensure(sizeof(int32_t) * nt, bud);
     bm->qcount.ensure(sizeof(int32_t), bud);
     BMScratch scr{bm->queue.as<int32_t>(), bm->qcount.as<int32_t>(), nt, &bm->ctx->prof};
-    BlockPrep bp{bm->prep_xt[bi].as<float>(), bm->prep_sxx[bi].as<float>(),
-                 bm->prep_sum[bi].as<float>(), bm->prep_flags[bi].as<uint32_t>()};
     launch_ncc_search(B.data, B.mask, S.data, S.mask, g, bp,
                       SearchStats{bm->stats[si].as<float2>()}, scr, res, s);
   } else {
-    launch_ncc_generic(B.data, B.mask, S.data, S.mask, g, res, s);
+    launch_ncc_generic(B.data, B.mask, S.data, S.mask, g, res, s,
+                       bm->prep_selected[bi] ? bp.flags : nullptr);
   }
   bm->results_valid = true;
   bm->results_key[0] = key0;
@@ -318,6 +338,8 @@ mirorr_status mirorr_bm_swap(mirorr_bm* bm) {
     std::swap(bm->prep_flags[0], bm->prep_flags[1]);
     std::swap(bm->prep_ver[0], bm->prep_ver[1]);
     std::swap(bm->prep_key[0], bm->prep_key[1]);
+    std::swap(bm->prep_portion[0], bm->prep_portion[1]);
+    std::swap(bm->prep_selected[0], bm->prep_selected[1]);
     bm->results_valid = false;
   });
 }
@@ -349,6 +371,18 @@ mirorr_status mirorr_bm_voxel_evals(mirorr_bm* bm, int64_t* evals) {
   return guarded(bm->ctx, [&] {
     if (!bm->slot[0].set) throw Error(kErrState, "base image not set");
     const BMGrid g = current_grid(bm);
+    if (bm->p.variance_select && bm->p.portion_kept < 1.0) {
+      // Only the selected blocks are searched: counted on the device.
+      run_search(bm);
+      if (bm->prep_selected[MIRORR_ROLE_BASE]) {
+        unsigned long long e = 0;
+        MIRORR_HIP(hipMemcpyAsync(&e, bm->vsel_evals.as<void>(), sizeof(e),
+                                  hipMemcpyDeviceToHost, bm->ctx->stream));
+        MIRORR_HIP(hipStreamSynchronize(bm->ctx->stream));
+        *evals = (int64_t)e;
+        return;
+      }
+    }
     int64_t per[3] = {0, 0, 0};
     for (int a = 0; a < 3; ++a)
       for (int i = 0; i < g.nb[a]; ++i) {

@@ -1,12 +1,17 @@
 // Context object behind the C-ABI handle mirorr_ctx.
 #pragma once
 
+#include <memory>
 #include <mutex>
 #include <string>
 
 #include "../../include/mirorr_gpu.h"
 #include "device_buffer.hpp"
 #include "profiler.hpp"
+
+namespace mirorr {
+struct RegCache;  // registration workspace, defined with the driver
+}
 
 struct mirorr_ctx {
   int device = 0;
@@ -15,6 +20,10 @@ struct mirorr_ctx {
   std::string last_error;
   std::mutex mu;
   mirorr::Profiler prof;  // optional per-kernel event timing
+  // Device workspace, pyramid levels and pinned flags of registrations run on
+  // this context: grown to the largest request and reused, so a registration
+  // after the first performs no device allocation. Guarded by `mu`.
+  std::shared_ptr<mirorr::RegCache> reg;
 };
 
 namespace mirorr {

@@ -2,32 +2,40 @@
 //
 // Restates OptimizeTransform of the classic (itkMirorrRegistrationMethod.txx:758-1106)
 // and symmetric (itkSymMirorrRegistrationMethod.txx:729-1442) methods with the
-// per-point work on the GPU and the tiny FP64 solves in a one-thread tail:
+// per-point work on the GPU and the tiny FP64 solves in a one-thread tail.
+// One inner iteration is six launches; each of the first five finishes its cross-workgroup
+// reduction in the workgroup that arrives last (an arrival counter with
+// release/acquire fences), so no launch exists only to reduce:
 //  1. residual kernel: r_i = |F(x_i) - B(y_i)|^2 (classic |F(x_i) - y_i|^2),
 //     +inf for culled blocks; the first radix digit (bits 63..52 of the
-//     non-negative double) is histogrammed in LDS and merged globally;
-//  2. exact k-th smallest residual (std::nth_element in the reference) by a
-//     two-digit radix histogram, a compaction of the candidates that share
-//     the 24-bit prefix and a one-workgroup radix finish on the low 40 bits;
-//  3. trimmed moments in two passes (barycentres, then the 21 centred second
-//     moments x x^T, y y^T, x y^T), block-reduced in double; the rigid
-//     quaternion matrix sum Q^T Q is assembled from those moments;
-//  4. a one-thread solve (normal equations / quaternion eigenvector /
-//     polar factor / Denman-Beavers roots) updating the device state.
+//     non-negative double) is histogrammed in LDS and merged globally; the
+//     last workgroup finds the bin of rank k;
+//  2. second digit (bits 51..40) inside that bin, likewise;
+//  3. compaction of the candidates sharing the 24-bit prefix; the last
+//     workgroup finishes the exact k-th smallest residual (std::nth_element
+//     in the reference) by rank counting or 8-bit radix passes;
+//  4. trimmed barycentres (count and coordinate sums);
+//  5. the 21 centred second moments x x^T, y y^T, x y^T, block-reduced in
+//     double (the rigid quaternion matrix sum Q^T Q is assembled from them);
+//  6. a one-thread solve (normal equations / quaternion eigenvector / polar
+//     factor / Denman-Beavers roots) that updates the device state.
 // Every kernel is predicated on the state's `active` flag so the host can
 // queue several inner iterations before reading that flag back.
 #include <cfloat>
 #include <climits>
 
 #include "lts.hpp"
+#include "select.hpp"
 
 namespace mirorr {
 namespace {
 
-constexpr int kT = 256;
+constexpr int kT = sel::kSelT;
 constexpr int kPart = 1024;  // points per moment workgroup (4 per thread)
 
 using namespace xf;
+using sel::arrive_last;
+using sel::block_scan;
 
 __device__ __forceinline__ unsigned long long key_of(double r) {
   return (unsigned long long)__double_as_longlong(r);
@@ -73,44 +81,12 @@ __global__ __launch_bounds__(kT) void residual_kernel(const double* __restrict__
   __syncthreads();
   for (int i = threadIdx.x; i < 4096; i += kT)
     if (h[i]) atomicAdd(&hist1[i], h[i]);
-}
-
-// Finds the bin holding rank k of a 4096-bin histogram (one workgroup).
-__device__ void find_bin(const unsigned int* hist, unsigned long long k, unsigned int* bin,
-                         unsigned long long* krem) {
-  __shared__ unsigned long long part[1024];
-  const int t = threadIdx.x;  // 1024 threads, 4 bins each
-  unsigned long long s = 0;
-  for (int j = 0; j < 4; ++j) s += hist[4 * t + j];
-  part[t] = s;
-  __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {
-    const unsigned long long v = t >= o ? part[t - o] : 0ull;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
-  }
-  const unsigned long long before = part[t] - s;  // exclusive prefix
-  if (k >= before && k < part[t]) {
-    unsigned long long c = before;
-    for (int j = 0; j < 4; ++j) {
-      const unsigned long long h = hist[4 * t + j];
-      if (k < c + h) {
-        *bin = 4 * t + j;
-        *krem = k - c;
-        break;
-      }
-      c += h;
-    }
-  }
-}
-
-__global__ __launch_bounds__(1024) void scan1_kernel(const unsigned int* hist1, LtsState* st) {
-  if (!st->active) return;
-  find_bin(hist1, st->k, &st->bin1, &st->k2);
+  if (!arrive_last(&st->sync[0])) return;
+  sel::find_bin<4096 / kT>(hist1, st->k, &st->bin1, &st->k2);
   if (threadIdx.x == 0) st->ncand = 0;
 }
 
+// Second digit (bits 51..40) among residuals in the first digit's bin.
 __global__ __launch_bounds__(kT) void hist2_kernel(const double* __restrict__ r, int64_t n,
                                                    LtsState* st, unsigned int* hist2) {
   if (!st->active) return;
@@ -125,11 +101,59 @@ __global__ __launch_bounds__(kT) void hist2_kernel(const double* __restrict__ r,
   __syncthreads();
   for (int i = threadIdx.x; i < 4096; i += kT)
     if (h[i]) atomicAdd(&hist2[i], h[i]);
+  if (!arrive_last(&st->sync[1])) return;
+  sel::find_bin<4096 / kT>(hist2, st->k2, &st->bin2, &st->k3);
 }
 
-__global__ __launch_bounds__(1024) void scan2_kernel(const unsigned int* hist2, LtsState* st) {
-  if (!st->active) return;
-  find_bin(hist2, st->k2, &st->bin2, &st->k3);
+// Rank k3 among the candidate keys (all share the 24-bit prefix): by rank
+// counting when they are few, else by 8-bit radix passes on bits 39..0.
+__device__ void final_select(const unsigned long long* cand, LtsState* st) {
+  constexpr int kSmall = 512;
+  __shared__ unsigned long long sc[kSmall];
+  __shared__ unsigned int hb[kT];
+  __shared__ unsigned long long s_prefix, s_k;
+  const int t = threadIdx.x;
+  const unsigned int nc = __hip_atomic_load(&st->ncand, __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT);
+  const unsigned long long k = st->k3;
+  if (nc <= (unsigned)kSmall) {
+    for (unsigned int i = t; i < nc; i += kT) sc[i] = cand[i];
+    __syncthreads();
+    for (unsigned int i = t; i < nc; i += kT) {
+      const unsigned long long v = sc[i];
+      unsigned int rank = 0;
+      for (unsigned int j = 0; j < nc; ++j) {
+        const unsigned long long u = sc[j];
+        rank += (u < v || (u == v && j < i)) ? 1u : 0u;
+      }
+      if (rank == k) st->thr = __longlong_as_double((long long)v);
+    }
+    return;
+  }
+  if (t == 0) {
+    s_prefix = (((unsigned long long)st->bin1 << 12) | st->bin2) << 40;
+    s_k = k;
+  }
+  for (int shift = 32; shift >= 0; shift -= 8) {
+    hb[t] = 0;
+    __syncthreads();
+    const unsigned long long pre = s_prefix, kk = s_k;
+    const unsigned long long hi_mask = ~((1ull << (shift + 8)) - 1ull);
+    for (unsigned int i = t; i < nc; i += kT) {
+      const unsigned long long c = cand[i];
+      if ((c & hi_mask) == pre) atomicAdd(&hb[(unsigned)(c >> shift) & 0xffu], 1u);
+    }
+    __syncthreads();
+    const unsigned long long cnt = hb[t];
+    const unsigned long long inc = block_scan(cnt);
+    const unsigned long long before = inc - cnt;
+    if (kk >= before && kk < inc) {
+      s_prefix = pre | ((unsigned long long)t << shift);
+      s_k = kk - before;
+    }
+    __syncthreads();
+  }
+  if (t == 0) st->thr = __longlong_as_double((long long)s_prefix);
 }
 
 __global__ __launch_bounds__(kT) void compact_kernel(const double* __restrict__ r, int64_t n,
@@ -141,43 +165,8 @@ __global__ __launch_bounds__(kT) void compact_kernel(const double* __restrict__ 
     const unsigned long long k = key_of(r[i]);
     if ((k >> 40) == pre) cand[atomicAdd(&st->ncand, 1u)] = k;
   }
-}
-
-// Radix select of rank k3 among the candidates on bits 39..0, 8 bits a pass.
-__global__ __launch_bounds__(1024) void final_select_kernel(const unsigned long long* cand,
-                                                            LtsState* st) {
-  if (!st->active) return;
-  __shared__ unsigned int h[256];
-  __shared__ unsigned long long s_prefix, s_k;
-  const unsigned int nc = st->ncand;
-  if (threadIdx.x == 0) {
-    s_prefix = (((unsigned long long)st->bin1 << 12) | st->bin2) << 40;
-    s_k = st->k3;
-  }
-  for (int shift = 32; shift >= 0; shift -= 8) {
-    for (int i = threadIdx.x; i < 256; i += 1024) h[i] = 0;
-    __syncthreads();
-    const unsigned long long pre = s_prefix;
-    const unsigned long long hi_mask = ~((1ull << (shift + 8)) - 1ull);
-    for (unsigned int i = threadIdx.x; i < nc; i += 1024) {
-      const unsigned long long k = cand[i];
-      if ((k & hi_mask) == pre) atomicAdd(&h[(unsigned)(k >> shift) & 0xffu], 1u);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      unsigned long long c = 0, k = s_k;
-      for (int b = 0; b < 256; ++b) {
-        if (k < c + h[b]) {
-          s_prefix = pre | ((unsigned long long)b << shift);
-          s_k = k - c;
-          break;
-        }
-        c += h[b];
-      }
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) st->thr = __longlong_as_double((long long)s_prefix);
+  if (!arrive_last(&st->sync[2])) return;
+  final_select(cand, st);
 }
 
 // ---- trimmed moments ---------------------------------------------------------
@@ -200,6 +189,21 @@ __device__ void block_reduce(double (&v)[N], double* out) {
   }
 }
 
+// Sums per-workgroup partials in a fixed order (deterministic): wave w sums
+// values w, w + 4, ...; lanes stride over the partials.
+template <int N>
+__device__ void reduce_parts(const double* part, int stride, int nparts, double* out) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int j = w; j < N; j += kT / 64) {
+    double s = 0.0;
+    for (int p = lane; p < nparts; p += 64) s += part[(int64_t)stride * p + j];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) out[j] = s;
+  }
+  __syncthreads();
+}
+
 __global__ __launch_bounds__(kT) void moments1_kernel(const double* __restrict__ X,
                                                       const double* __restrict__ Y,
                                                       const double* __restrict__ r, int64_t n,
@@ -219,63 +223,10 @@ __global__ __launch_bounds__(kT) void moments1_kernel(const double* __restrict__
     }
   }
   block_reduce<7>(v, part + 8 * blockIdx.x);
-}
-
-// Sums per-workgroup partials in a fixed order (deterministic).
-__device__ void sum_parts(const double* part, int stride, int nparts, int nval, double* out) {
-  __shared__ double sh[kT];
-  for (int j = 0; j < nval; ++j) {
-    double s = 0.0;
-    for (int p = threadIdx.x; p < nparts; p += kT) s += part[(int64_t)stride * p + j];
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = kT / 2; o > 0; o >>= 1) {
-      if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) out[j] = sh[0];
-    __syncthreads();
-  }
-}
-
-__global__ __launch_bounds__(kT) void moments2_kernel(const double* __restrict__ X,
-                                                      const double* __restrict__ Y,
-                                                      const double* __restrict__ r, int64_t n,
-                                                      LtsState* st, const double* part1,
-                                                      int nparts, double* part2) {
-  if (!st->active) return;
+  if (!arrive_last(&st->sync[3])) return;
   __shared__ double s1[7];
-  sum_parts(part1, 8, nparts, 7, s1);
-  __syncthreads();
-  const double cnt = s1[0];
-  const double mx[3] = {s1[1] / cnt, s1[2] / cnt, s1[3] / cnt};
-  const double my[3] = {s1[4] / cnt, s1[5] / cnt, s1[6] / cnt};
-  const double thr = st->fraction > 0.99999 ? DBL_MAX : st->thr;
-  // xx (6), yy (6), xy (9)
-  double v[21];
-#pragma unroll
-  for (int j = 0; j < 21; ++j) v[j] = 0.0;
-  const int64_t base = (int64_t)blockIdx.x * kPart;
-  for (int j = threadIdx.x; j < kPart; j += kT) {
-    const int64_t i = base + j;
-    if (i < n && r[i] <= thr) {
-      const double x[3] = {X[3 * i] - mx[0], X[3 * i + 1] - mx[1], X[3 * i + 2] - mx[2]};
-      const double y[3] = {Y[3 * i] - my[0], Y[3 * i + 1] - my[1], Y[3 * i + 2] - my[2]};
-      int q = 0;
-#pragma unroll
-      for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int b = a; b < 3; ++b, ++q) {
-          v[q] += x[a] * x[b];
-          v[6 + q] += y[a] * y[b];
-        }
-#pragma unroll
-      for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int b = 0; b < 3; ++b) v[12 + 3 * a + b] += x[a] * y[b];
-    }
-  }
-  block_reduce<21>(v, part2 + 24 * blockIdx.x);
+  reduce_parts<7>(part, 8, (int)gridDim.x, s1);
+  if (threadIdx.x < 7) st->s1[threadIdx.x] = s1[threadIdx.x];
 }
 
 // ---- the one-thread solves -------------------------------------------------
@@ -512,22 +463,62 @@ __device__ void solve_classic(LtsState* st, const Moments& m) {
   set_params(st->F, neu);
 }
 
-__global__ __launch_bounds__(kT) void solve_kernel(LtsState* st, const double* part1,
-                                                   const double* part2, int nparts,
-                                                   unsigned int* hist1, unsigned int* hist2) {
+// Second pass of the moments (centred on the trimmed barycentres).
+__global__ __launch_bounds__(kT) void moments2_kernel(const double* __restrict__ X,
+                                                      const double* __restrict__ Y,
+                                                      const double* __restrict__ r, int64_t n,
+                                                      LtsState* st, double* part2,
+                                                      unsigned int* hist1, unsigned int* hist2) {
   if (!st->active) return;
-  __shared__ double s1[7], s2[21];
-  sum_parts(part1, 8, nparts, 7, s1);
-  sum_parts(part2, 24, nparts, 21, s2);
+  const double* s1 = st->s1;
+  const double cnt = s1[0];
+  const double mx[3] = {s1[1] / cnt, s1[2] / cnt, s1[3] / cnt};
+  const double my[3] = {s1[4] / cnt, s1[5] / cnt, s1[6] / cnt};
+  const double thr = st->fraction > 0.99999 ? DBL_MAX : st->thr;
+  // xx (6), yy (6), xy (9)
+  double v[21];
+#pragma unroll
+  for (int j = 0; j < 21; ++j) v[j] = 0.0;
+  const int64_t base = (int64_t)blockIdx.x * kPart;
+  for (int j = threadIdx.x; j < kPart; j += kT) {
+    const int64_t i = base + j;
+    if (i < n && r[i] <= thr) {
+      const double x[3] = {X[3 * i] - mx[0], X[3 * i + 1] - mx[1], X[3 * i + 2] - mx[2]};
+      const double y[3] = {Y[3 * i] - my[0], Y[3 * i + 1] - my[1], Y[3 * i + 2] - my[2]};
+      int q = 0;
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = a; b < 3; ++b, ++q) {
+          v[q] += x[a] * x[b];
+          v[6 + q] += y[a] * y[b];
+        }
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) v[12 + 3 * a + b] += x[a] * y[b];
+    }
+  }
+  block_reduce<21>(v, part2 + 24 * blockIdx.x);
+  if (!arrive_last(&st->sync[3])) return;
+  // The last workgroup: total moments and cleared histograms for the next
+  // iteration.
+  __shared__ double s2[21];
+  reduce_parts<21>(part2, 24, (int)gridDim.x, s2);
   for (int i = threadIdx.x; i < 4096; i += kT) {
     hist1[i] = 0;
     hist2[i] = 0;
   }
-  __syncthreads();
-  if (threadIdx.x != 0) return;
+  if (threadIdx.x < 21) st->s2[threadIdx.x] = s2[threadIdx.x];
+}
+
+// The one-thread solve (its own launch: its register footprint would
+// otherwise limit the occupancy of the streaming moments kernel).
+__global__ __launch_bounds__(64) void solve_kernel(LtsState* st) {
+  if (!st->active || threadIdx.x != 0) return;
   st->iter += 1;  // the loop head of the reference
   Moments m;
-  unpack(s1, s2, m);
+  unpack(st->s1, st->s2, m);
   if (st->symmetric)
     solve_symmetric(st, m);
   else
@@ -535,6 +526,7 @@ __global__ __launch_bounds__(kT) void solve_kernel(LtsState* st, const double* p
   st->fraction = 0.5;
   st->active = (st->delta > 1e-3 && st->iter < 100 && fabs(st->delta - st->last_delta) > 1e-3) ? 1 : 0;
 }
+
 
 __global__ void init_kernel(LtsState* st, const OuterState* outer, int kind, int symmetric,
                             unsigned int* hist1, unsigned int* hist2) {
@@ -562,6 +554,7 @@ __global__ void init_kernel(LtsState* st, const OuterState* outer, int kind, int
   if (n > 0 && k > n - 1) k = n - 1;
   st->k = k;
   st->ncand = 0;
+  for (int i = 0; i < 4; ++i) st->sync[i] = 0u;
   st->thr = DBL_MAX;
   st->active = n > 0 ? 1 : 0;
 }
@@ -589,7 +582,7 @@ __device__ void prepare_maps(OuterState* o, int symmetric) {
   }
 }
 
-__global__ void finish_kernel(LtsState* st, OuterState* o) {
+__global__ __launch_bounds__(64) void finish_kernel(LtsState* st, OuterState* o) {
   const int kind = st->kind;
   if (st->symmetric) {
     compose(o->t12F, st->F, true);
@@ -611,7 +604,7 @@ __global__ void finish_kernel(LtsState* st, OuterState* o) {
   prepare_maps(o, st->symmetric);
 }
 
-__global__ void prepare_maps_kernel(OuterState* o, int symmetric) { prepare_maps(o, symmetric); }
+__global__ __launch_bounds__(64) void prepare_maps_kernel(OuterState* o, int symmetric) { prepare_maps(o, symmetric); }
 
 __global__ void reset_count_kernel(LtsState* st) { st->n_kept = 0ull; }
 
@@ -630,17 +623,13 @@ void lts_enqueue_iteration(const double* X, const double* Y, const uint8_t* kept
   hipLaunchKernelGGL(residual_kernel, dim3(grid), dim3(kT), 0, s, X, Y, kept, npts, st, b.r,
                      b.hist1, select ? 1 : 0);
   if (select) {
-    hipLaunchKernelGGL(scan1_kernel, dim3(1), dim3(1024), 0, s, b.hist1, st);
     hipLaunchKernelGGL(hist2_kernel, dim3(grid), dim3(kT), 0, s, b.r, npts, st, b.hist2);
-    hipLaunchKernelGGL(scan2_kernel, dim3(1), dim3(1024), 0, s, b.hist2, st);
     hipLaunchKernelGGL(compact_kernel, dim3(grid), dim3(kT), 0, s, b.r, npts, st, b.cand);
-    hipLaunchKernelGGL(final_select_kernel, dim3(1), dim3(1024), 0, s, b.cand, st);
   }
   hipLaunchKernelGGL(moments1_kernel, dim3(nparts), dim3(kT), 0, s, X, Y, b.r, npts, st, b.part1);
-  hipLaunchKernelGGL(moments2_kernel, dim3(nparts), dim3(kT), 0, s, X, Y, b.r, npts, st, b.part1,
-                     nparts, b.part2);
-  hipLaunchKernelGGL(solve_kernel, dim3(1), dim3(kT), 0, s, st, b.part1, b.part2, nparts, b.hist1,
-                     b.hist2);
+  hipLaunchKernelGGL(moments2_kernel, dim3(nparts), dim3(kT), 0, s, X, Y, b.r, npts, st, b.part2,
+                     b.hist1, b.hist2);
+  hipLaunchKernelGGL(solve_kernel, dim3(1), dim3(64), 0, s, st);
   MIRORR_CHECK_LAUNCH();
 }
 

@@ -12,6 +12,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "bm_kernels.hpp"
@@ -84,11 +85,12 @@ struct Level {
   Geom geom;
 };
 
-// Recursive pyramid of one image (coarsest first).
-std::vector<Level> build_pyramid(const float* d_in, const Geom& g, const std::vector<int>& sched,
-                                 int rows, DeviceBuffer& tmp, DeviceBuffer& map, Budget* bud,
-                                 hipStream_t s) {
-  std::vector<Level> out(rows);
+// Recursive pyramid of one image (coarsest first) into `out`, whose level
+// buffers are reused when large enough.
+void build_pyramid(const float* d_in, const Geom& g, const std::vector<int>& sched, int rows,
+                   std::vector<Level>& out, DeviceBuffer& tmp, DeviceBuffer& map, Budget* bud,
+                   hipStream_t s) {
+  if ((int)out.size() < rows) out.resize(rows);
   map.ensure(sizeof(IndexMap), bud);
   for (int l = rows - 1; l >= 0; --l) {
     const int f[3] = {sched[3 * l], sched[3 * l + 1], sched[3 * l + 2]};
@@ -110,7 +112,6 @@ std::vector<Level> build_pyramid(const float* d_in, const Geom& g, const std::ve
     launch_pyramid_step(prev, *pg, ratio, out[l].geom, out[l].data.as<float>(), tmp.as<double>(),
                         map.as<IndexMap>(), s);
   }
-  return out;
 }
 
 // Per-registration device workspace, grown to the largest level.
@@ -124,6 +125,7 @@ struct Workspace {
   DeviceBuffer r, hist1, hist2, cand, part1, part2;
   DeviceBuffer lts, outer, mapI;
   DeviceBuffer pyr_tmp, pyr_map;
+  DeviceBuffer vsel_st, vsel_hist, vsel_evals;  // block-variance selection
 };
 
 struct Flags {
@@ -135,21 +137,51 @@ struct Flags {
 
 namespace mirorr {
 
+struct RegCache {
+  Workspace w;
+  std::vector<Level> F, M;  // pyramid levels of the fixed and moving image
+  Flags* flags = nullptr;   // pinned host flags read once per LTS chunk
+  RegCache() { MIRORR_HIP(hipHostMalloc((void**)&flags, sizeof(Flags), hipHostMallocDefault)); }
+  ~RegCache() {
+    if (flags) (void)hipHostFree(flags);
+  }
+};
+
+// The context's registration cache, created on first use (caller holds c->mu).
+RegCache& reg_cache(mirorr_ctx* c) {
+  if (!c->reg) c->reg = std::make_shared<RegCache>();
+  return *c->reg;
+}
+
 struct RegRun {
   mirorr_ctx* ctx;
   const mirorr_reg_params* p;
-  Workspace w;
+  Workspace& w;
   mirorr_reg_stats* stats;
   int kind;
   bool symmetric;
   hipStream_t s;
   Budget* bud;
 
-  // One BM pass base -> search over the half grid, writing LTS pairs.
+  // Block preparation of a base image and, with portion < 1, the variance
+  // selection over its blocks. Returns whether blocks were dropped.
+  bool prepare_base(const float* base, const uint8_t* bmask, const BMGrid& g, DeviceBuffer& xt,
+                    DeviceBuffer& sxx, DeviceBuffer& sum, DeviceBuffer& fl, double portion) {
+    const BlockPrep bp{xt.as<float>(), sxx.as<float>(), sum.as<float>(), fl.as<uint32_t>()};
+    launch_block_prep(base, bmask, g, bp, s);
+    if (!(portion < 1.0)) return false;
+    VarSelScratch vs{w.vsel_st.as<VarSelState>(), w.vsel_hist.as<unsigned int>(),
+                     w.vsel_evals.as<unsigned long long>()};
+    return launch_variance_select(g, bp, portion, vs, s);
+  }
+
+  // One BM pass base -> search over the half grid, writing LTS pairs. With
+  // `selected`, the prepared flags drop blocks and the pass's voxel-evals are
+  // counted on the device.
   void match_pass(const float* base, const uint8_t* bmask, const float* search,
                   const uint8_t* smask, const Geom& g_base, DeviceBuffer& xt, DeviceBuffer& sxx,
                   DeviceBuffer& sum, DeviceBuffer& fl, DeviceBuffer& stats_search, int swap,
-                  LtsState* st) {
+                  bool selected, LtsState* st) {
     const BMGrid g = make_grid(g_base.dims, p->block_width, p->block_gap, p->nhood_width,
                                p->nhood_gap);
     if (g.nblocks <= 0) return;
@@ -160,13 +192,14 @@ struct RegRun {
       launch_ncc_search(base, bmask, search, smask, g, bp, SearchStats{stats_search.as<float2>()},
                         scr, res, s);
     } else {
-      launch_ncc_generic(base, bmask, search, smask, g, res, s);
+      launch_ncc_generic(base, bmask, search, smask, g, res, s,
+                         selected ? fl.as<uint32_t>() : nullptr);
     }
     const int64_t off = swap ? g.nblocks : 0;
     launch_postprocess_pairs(g, g_base, res, swap, w.kept.as<uint8_t>() + off,
                              w.X.as<double>() + 3 * off, w.Y.as<double>() + 3 * off,
                              &st->n_kept, s);
-    stats->voxel_evals += voxel_evals(g);
+    if (!selected) stats->voxel_evals += voxel_evals(g);
   }
 
   static int64_t voxel_evals(const BMGrid& g) {
@@ -208,6 +241,8 @@ struct RegRun {
     w.part2.ensure(sizeof(double) * 24 * parts, bud);
     w.hist1.ensure(sizeof(unsigned int) * 4096, bud);
     w.hist2.ensure(sizeof(unsigned int) * 4096, bud);
+    w.vsel_st.ensure(sizeof(VarSelState), bud);
+    w.vsel_hist.ensure(sizeof(unsigned int) * kVarSelHistBins, bud);
   }
 
   // Runs the LTS to convergence with predicated chunks; returns when done.
@@ -246,7 +281,9 @@ void run_registration(mirorr_ctx* ctx, const mirorr_reg_params* p, const mirorr_
   const auto t0 = std::chrono::steady_clock::now();
   hipStream_t s = ctx->stream;
   Budget* bud = &ctx->budget;
-  RegRun run{ctx, p, {}, stats, p->affine ? xf::kAffine : xf::kRigid, p->symmetric != 0, s, bud};
+  RegCache& cache = reg_cache(ctx);
+  RegRun run{ctx, p, cache.w, stats, p->affine ? xf::kAffine : xf::kRigid, p->symmetric != 0, s,
+             bud};
   Workspace& w = run.w;
   const Geom& FG = fixed->geom;
   const Geom& MG = moving->geom;
@@ -262,11 +299,10 @@ void run_registration(mirorr_ctx* ctx, const mirorr_reg_params* p, const mirorr_
   const int nlev = mx - mn + 1;
   std::vector<int> ms(sch.moving.begin() + 3 * (mn - 1), sch.moving.begin() + 3 * mx);
   std::vector<int> fs(sch.fixed.begin() + 3 * (mn - 1), sch.fixed.begin() + 3 * mx);
-  std::vector<Level> Fp = build_pyramid(fixed->data.as<float>(), FG, fs, nlev, w.pyr_tmp,
-                                        w.pyr_map, bud, s);
-  std::vector<Level> Mp = build_pyramid(moving->data.as<float>(), MG, ms, nlev, w.pyr_tmp,
-                                        w.pyr_map, bud, s);
-  w.pyr_tmp.reset();
+  std::vector<Level>& Fp = cache.F;
+  std::vector<Level>& Mp = cache.M;
+  build_pyramid(fixed->data.as<float>(), FG, fs, nlev, Fp, w.pyr_tmp, w.pyr_map, bud, s);
+  build_pyramid(moving->data.as<float>(), MG, ms, nlev, Mp, w.pyr_tmp, w.pyr_map, bud, s);
   const int t_max = std::min(cropped_level(level_max, nlev, rows), nlev);
   // Initial transform: given, or centred (itkMirorrPyramidWrapper.txx:611-623:
   // centre of the moving image, translation between the image centres).
@@ -291,13 +327,10 @@ void run_registration(mirorr_ctx* ctx, const mirorr_reg_params* p, const mirorr_
   w.mapI.ensure(sizeof(IndexMap), bud);
   LtsState* st = w.lts.as<LtsState>();
   OuterState* outer = w.outer.as<OuterState>();
-  Flags* flags = nullptr;
-  MIRORR_HIP(hipHostMalloc((void**)&flags, sizeof(Flags), hipHostMallocDefault));
-  struct FreeHost {
-    void* p;
-    ~FreeHost() { (void)hipHostFree(p); }
-  } free_flags{flags};
+  Flags* flags = cache.flags;
   std::memset(stats, 0, sizeof(*stats));
+  w.vsel_evals.ensure(sizeof(unsigned long long), bud);
+  MIRORR_HIP(hipMemsetAsync(w.vsel_evals.as<void>(), 0, sizeof(unsigned long long), s));
   stats->levels = t_max;
   const int np = xf::nparams(T);
   for (int level = 0; level < t_max && level < 16; ++level) {
@@ -366,6 +399,13 @@ void run_registration(mirorr_ctx* ctx, const mirorr_reg_params* p, const mirorr_
     const BMGrid gh = make_grid(o.gHF.dims, p->block_width, p->block_gap, p->nhood_width,
                                 p->nhood_gap);
     const bool tiled = tiled_supported(gh);
+    // Portion of blocks searched: levels 0 and 1 use every block
+    // (itkMirorrPyramidImplement.txx:116-119); the selection is opt-in.
+    const double portion = (p->variance_select && level >= 2)
+                               ? (p->portion_kept == 0.0 ? 0.5 : p->portion_kept)
+                               : 1.0;
+    const bool prep = tiled || portion < 1.0;
+    bool selM = false, selF = false;
     if (!run.symmetric) {
       // The moving image and mask are resampled once onto the grid.
       const xf::M3 I = xf::eye3();
@@ -376,11 +416,9 @@ void run_registration(mirorr_ctx* ctx, const mirorr_reg_params* p, const mirorr_
       launch_resample(Mp[level].data.as<float>(), gM.dims, w.mapI.as<IndexMap>(), mmask, MG.dims,
                       w.mapI.as<IndexMap>() + 1, mmode, o.gHM.dims, w.Mh.as<float>(),
                       w.Mmh.as<uint8_t>(), s);
-      if (tiled)
-        launch_block_prep(w.Mh.as<float>(), w.Mmh.as<uint8_t>(), gh,
-                          BlockPrep{w.xtM.as<float>(), w.sxxM.as<float>(), w.sumM.as<float>(),
-                                    w.flM.as<uint32_t>()},
-                          s);
+      if (prep)
+        selM = run.prepare_base(w.Mh.as<float>(), w.Mmh.as<uint8_t>(), gh, w.xtM, w.sxxM, w.sumM,
+                                w.flM, portion);
       // (the half-grid stays fixed through the level; hipMemcpy is ordered)
       MIRORR_HIP(hipStreamSynchronize(s));
     }
@@ -398,24 +436,22 @@ void run_registration(mirorr_ctx* ctx, const mirorr_reg_params* p, const mirorr_
       if (tiled) {
         launch_search_stats(w.Fh.as<float>(), w.Fmh.as<uint8_t>(), gh,
                             SearchStats{w.statsF.as<float2>()}, s);
-        if (run.symmetric) {
+        if (run.symmetric)
           launch_search_stats(w.Mh.as<float>(), w.Mmh.as<uint8_t>(), gh,
                               SearchStats{w.statsM.as<float2>()}, s);
-          launch_block_prep(w.Mh.as<float>(), w.Mmh.as<uint8_t>(), gh,
-                            BlockPrep{w.xtM.as<float>(), w.sxxM.as<float>(), w.sumM.as<float>(),
-                                      w.flM.as<uint32_t>()},
-                            s);
-          launch_block_prep(w.Fh.as<float>(), w.Fmh.as<uint8_t>(), gh,
-                            BlockPrep{w.xtF.as<float>(), w.sxxF.as<float>(), w.sumF.as<float>(),
-                                      w.flF.as<uint32_t>()},
-                            s);
-        }
+      }
+      if (run.symmetric && prep) {
+        selM = run.prepare_base(w.Mh.as<float>(), w.Mmh.as<uint8_t>(), gh, w.xtM, w.sxxM, w.sumM,
+                                w.flM, portion);
+        selF = run.prepare_base(w.Fh.as<float>(), w.Fmh.as<uint8_t>(), gh, w.xtF, w.sxxF, w.sumF,
+                                w.flF, portion);
       }
       run.match_pass(w.Mh.as<float>(), w.Mmh.as<uint8_t>(), w.Fh.as<float>(), w.Fmh.as<uint8_t>(),
-                     o.gHM, w.xtM, w.sxxM, w.sumM, w.flM, w.statsF, 0, st);
+                     o.gHM, w.xtM, w.sxxM, w.sumM, w.flM, w.statsF, 0, selM, st);
       if (run.symmetric)
         run.match_pass(w.Fh.as<float>(), w.Fmh.as<uint8_t>(), w.Mh.as<float>(),
-                       w.Mmh.as<uint8_t>(), o.gHF, w.xtF, w.sxxF, w.sumF, w.flF, w.statsM, 1, st);
+                       w.Mmh.as<uint8_t>(), o.gHF, w.xtF, w.sxxF, w.sumF, w.flF, w.statsM, 1,
+                       selF, st);
       // 3. LTS and transform update on the device
       run.run_lts(st, outer, npts, flags);
       MIRORR_HIP(hipMemcpyAsync(&flags->stop, &outer->stop, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -453,6 +489,11 @@ void run_registration(mirorr_ctx* ctx, const mirorr_reg_params* p, const mirorr_
     xf::get_params(T, cur);
     if (halve) max_it = std::max(1, max_it / 2);
   }
+  unsigned long long sel_evals = 0;
+  MIRORR_HIP(hipMemcpyAsync(&sel_evals, w.vsel_evals.as<void>(), sizeof(sel_evals),
+                            hipMemcpyDeviceToHost, s));
+  MIRORR_HIP(hipStreamSynchronize(s));
+  stats->voxel_evals += (int64_t)sel_evals;
   for (int i = 0; i < np; ++i) out_params[i] = cur[i];
   if (out_center)
     for (int k = 0; k < 3; ++k) out_center[k] = T.c[k];
@@ -528,9 +569,13 @@ mirorr_status mirorr_register(mirorr_ctx* c, const mirorr_reg_params* p,
                               double* out_params, double* out_center, mirorr_reg_stats* stats) {
   if (!c || !p || !fixed || !moving || !out_params) return MIRORR_ERR_ARG;
   mirorr_reg_stats local{};
+  std::lock_guard<std::mutex> lk(c->mu);
   return guarded(c, [&] {
     if (p->block_width < 1 || p->block_gap < 1 || p->nhood_width < 1 || p->nhood_gap < 1)
       throw Error(kErrArg, "block-matcher parameters must be positive");
+    // SetPortionMatchesKept (itkAbstractBlockMatcher.txx:83-99): 0 means 0.5.
+    if (p->portion_kept < 0.0 || p->portion_kept > 1.0)
+      throw Error(kErrArg, "portion_kept should be between 0 and 1.0");
     run_registration(c, p, fixed, moving, init_params, init_center, out_params, out_center,
                      stats ? stats : &local);
   });
@@ -540,6 +585,7 @@ mirorr_status mirorr_lts_fit(mirorr_ctx* c, int symmetric, int affine, const dou
                              const double* y, size_t n, double* params, double* center,
                              double* fwd34, double* bwd34, int* iterations) {
   if (!c || !x || !y || !params) return MIRORR_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->mu);
   return guarded(c, [&] {
     hipStream_t s = c->stream;
     Budget* bud = &c->budget;
@@ -547,7 +593,8 @@ mirorr_status mirorr_lts_fit(mirorr_ctx* c, int symmetric, int affine, const dou
     mirorr_reg_params rp;
     mirorr_reg_default_params(&rp);
     mirorr_reg_stats stats{};
-    RegRun run{c, &rp, {}, &stats, kind, symmetric != 0, s, bud};
+    RegCache& cache = reg_cache(c);
+    RegRun run{c, &rp, cache.w, &stats, kind, symmetric != 0, s, bud};
     Workspace& w = run.w;
     const int64_t np = (int64_t)n;
     w.X.ensure(sizeof(double) * 3 * std::max<int64_t>(np, 1), bud);
@@ -586,13 +633,7 @@ mirorr_status mirorr_lts_fit(mirorr_ctx* c, int symmetric, int affine, const dou
     MIRORR_HIP(hipMemcpyAsync(outer, &o, sizeof(o), hipMemcpyHostToDevice, s));
     const unsigned long long nk = (unsigned long long)np;
     MIRORR_HIP(hipMemcpyAsync(&st->n_kept, &nk, sizeof(nk), hipMemcpyHostToDevice, s));
-    Flags* flags = nullptr;
-    MIRORR_HIP(hipHostMalloc((void**)&flags, sizeof(Flags), hipHostMallocDefault));
-    struct FreeHost {
-      void* p;
-      ~FreeHost() { (void)hipHostFree(p); }
-    } free_flags{flags};
-    run.run_lts(st, outer, std::max<int64_t>(np, 1), flags);
+    run.run_lts(st, outer, std::max<int64_t>(np, 1), cache.flags);
     MIRORR_HIP(hipMemcpyAsync(&o, outer, sizeof(o), hipMemcpyDeviceToHost, s));
     MIRORR_HIP(hipStreamSynchronize(s));
     xf::get_params(o.T, params);
@@ -652,7 +693,8 @@ mirorr_status mirorr_pyramid_level(mirorr_ctx* c, const float* data, const miror
     MIRORR_HIP(hipMemcpyAsync(din.as<float>(), data, sizeof(float) * nvox(g),
                               hipMemcpyHostToDevice, s));
     const std::vector<int> sched(schedule, schedule + 3 * rows);
-    std::vector<Level> lv = build_pyramid(din.as<float>(), g, sched, rows, tmp, map, &c->budget, s);
+    std::vector<Level> lv;
+    build_pyramid(din.as<float>(), g, sched, rows, lv, tmp, map, &c->budget, s);
     const Geom& lg = lv[level].geom;
     MIRORR_HIP(hipMemcpyAsync(out, lv[level].data.as<float>(), sizeof(float) * nvox(lg),
                               hipMemcpyDeviceToHost, s));

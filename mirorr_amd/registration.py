@@ -95,6 +95,7 @@ class MirorrPyramid:
     def SetNhoodWidth(self, v): self._p.nhood_width = int(v)
     def SetNhoodGap(self, v): self._p.nhood_gap = int(v)
     def SetPortionMatchesKept(self, v): self._p.portion_kept = float(v)
+    def SetVarianceSelection(self, on: bool): self._p.variance_select = 1 if on else 0
     def SetVerbosity(self, v): self._p.verbosity = int(v)
     def SetCentreInitialisation(self, on: bool): self._p.centre_init = 1 if on else 0
     def GetPyramidScheduleTuner(self): return self._tuner

@@ -10,6 +10,7 @@
 //   BlockMatcherThreaded split and reduction    itkBlockMatcherThreaded.txx:22-205
 #include "oracle.h"
 
+#include <algorithm>
 #include <cmath>
 #include <thread>
 #include <vector>
@@ -119,6 +120,7 @@ struct BMArgs {
   int32_t* match_idx;
   float* best;
   float* second;
+  const uint8_t* keep;  // per block (raster order): 0 = not selected, skipped
 };
 
 // blockPositionToPoint (itkAbstractBlockMatcher.h:271-287): the physical
@@ -146,6 +148,18 @@ void match_region(const BMArgs& a, const Region& r, std::vector<double>& out_b,
       for (int kx = 0; kx < cnt[0]; ++kx) {
         const int p[3] = {r.index[0] + kx * a.ng, r.index[1] + ky * a.ng,
                           r.index[2] + kz * a.ng};
+        const int gb[3] = {(p[0] - a.bstart[0]) / a.ng, (p[1] - a.bstart[1]) / a.ng,
+                           (p[2] - a.bstart[2]) / a.ng};
+        const int64_t id = gb[0] + (int64_t)a.nb[0] * (gb[1] + (int64_t)a.nb[1] * gb[2]);
+        if (a.keep && !a.keep[id]) {
+          // A block dropped by the variance selection is not searched: its
+          // record keeps the initial score -3 and the block's own position.
+          if (a.block_idx) for (int k = 0; k < 3; ++k) a.block_idx[3 * id + k] = p[k];
+          if (a.match_idx) for (int k = 0; k < 3; ++k) a.match_idx[3 * id + k] = p[k];
+          if (a.best) a.best[id] = -3.0f;
+          if (a.second) a.second[id] = -3.0f;
+          continue;
+        }
         int first[3], ncand[3];
         for (int k = 0; k < 3; ++k)
           cand_range(p[k], dims[k], a.bw, a.bg, a.nw, &first[k], &ncand[k]);
@@ -170,9 +184,6 @@ void match_region(const BMArgs& a, const Region& r, std::vector<double>& out_b,
               }
             }
         // Raw per-block record in global raster order.
-        const int gb[3] = {(p[0] - a.bstart[0]) / a.ng, (p[1] - a.bstart[1]) / a.ng,
-                           (p[2] - a.bstart[2]) / a.ng};
-        const int64_t id = gb[0] + (int64_t)a.nb[0] * (gb[1] + (int64_t)a.nb[1] * gb[2]);
         if (a.block_idx) for (int k = 0; k < 3; ++k) a.block_idx[3 * id + k] = p[k];
         if (a.match_idx) for (int k = 0; k < 3; ++k) a.match_idx[3 * id + k] = bidx[k];
         if (a.best) a.best[id] = best;
@@ -211,6 +222,25 @@ unsigned split_region(const Region& full, int bw, int ng, unsigned tid,
 
 }  // namespace
 
+extern "C" int64_t orc_bm_candidate_count_sel(const int dims[3], int bw, int bg, int nw, int ng,
+                                              const uint8_t* keep) {
+  int nb[3], st[3];
+  orc_block_grid(dims, bw, ng, nb, st);
+  int64_t tot = 0;
+  for (int z = 0; z < nb[2]; ++z)
+    for (int y = 0; y < nb[1]; ++y)
+      for (int x = 0; x < nb[0]; ++x) {
+        const int64_t b = x + (int64_t)nb[0] * (y + (int64_t)nb[1] * z);
+        if (keep && !keep[b]) continue;
+        int f, cx, cy, cz;
+        cand_range(st[0] + x * ng, dims[0], bw, bg, nw, &f, &cx);
+        cand_range(st[1] + y * ng, dims[1], bw, bg, nw, &f, &cy);
+        cand_range(st[2] + z * ng, dims[2], bw, bg, nw, &f, &cz);
+        tot += (int64_t)cx * cy * cz;
+      }
+  return tot;
+}
+
 extern "C" int64_t orc_bm_candidate_count(const int dims[3], int bw, int bg,
                                           int nw, int ng) {
   int nb[3], st[3];
@@ -232,8 +262,71 @@ extern "C" int orc_bm_get_offsets(const float* base, const uint8_t* bmask,
                                   int32_t* match_idx, float* best,
                                   float* second, double* base_mm,
                                   double* match_mm, int64_t* n_kept) {
+  return orc_bm_get_offsets_sel(base, bmask, search, smask, g, bw, bg, nw, ng, nthreads, nullptr,
+                                block_idx, match_idx, best, second, base_mm, match_mm, n_kept);
+}
+
+// Block-variance selection. The reference's culling is compiled out
+// (itkBlockMatcher.txx:664-719: variance sum(x^2)/n - mean^2, cut-off by
+// nth_element at floor(portion * N), blocks at or below it skipped, which
+// keeps the top 1 - portion and drops everything at portion 1.0). The
+// selection restated here keeps the top `portion` instead:
+//   key_b  = float(sum over the block of (x - mean)^2), two-pass in double
+//            over all BW^3 voxels (masks ignored, as in the dead code);
+//   K      = ceil(portion * N) clamped to [1, N];
+//   cutoff = the K-th largest key; keep every block with key >= cutoff
+//            (blocks tied with the cut-off are all kept).
+extern "C" int64_t orc_variance_select(const float* base, const int dims[3], int bw, int ng,
+                                       double portion, uint8_t* keep, float* keys) {
+  int nb[3], st[3];
+  orc_block_grid(dims, bw, ng, nb, st);
+  if (nb[0] <= 0 || nb[1] <= 0 || nb[2] <= 0) return 0;
+  const int64_t N = (int64_t)nb[0] * nb[1] * nb[2];
+  std::vector<float> key(N);
+  const int n3 = bw * bw * bw;
+  for (int64_t b = 0; b < N; ++b) {
+    const int p[3] = {st[0] + ng * (int)(b % nb[0]), st[1] + ng * (int)((b / nb[0]) % nb[1]),
+                      st[2] + ng * (int)(b / ((int64_t)nb[0] * nb[1]))};
+    double s = 0.0;
+    for (int z = 0; z < bw; ++z)
+      for (int y = 0; y < bw; ++y)
+        for (int x = 0; x < bw; ++x)
+          s += base[(p[0] + x) + (int64_t)dims[0] * ((p[1] + y) + (int64_t)dims[1] * (p[2] + z))];
+    const double mu = s / n3;
+    double ss = 0.0;
+    for (int z = 0; z < bw; ++z)
+      for (int y = 0; y < bw; ++y)
+        for (int x = 0; x < bw; ++x) {
+          const double d =
+              base[(p[0] + x) + (int64_t)dims[0] * ((p[1] + y) + (int64_t)dims[1] * (p[2] + z))] -
+              mu;
+          ss += d * d;
+        }
+    key[b] = (float)ss;
+    if (keys) keys[b] = key[b];
+  }
+  int64_t K = (int64_t)std::ceil(portion * (double)N);
+  K = std::min<int64_t>(std::max<int64_t>(K, 1), N);
+  std::vector<float> sorted(key);
+  std::nth_element(sorted.begin(), sorted.begin() + (N - K), sorted.end());
+  const float cutoff = sorted[N - K];
+  int64_t kept = 0;
+  for (int64_t b = 0; b < N; ++b) {
+    keep[b] = key[b] >= cutoff ? 1 : 0;
+    kept += keep[b];
+  }
+  return kept;
+}
+
+extern "C" int orc_bm_get_offsets_sel(const float* base, const uint8_t* bmask,
+                                      const float* search, const uint8_t* smask,
+                                      const orc_geom* g, int bw, int bg, int nw, int ng,
+                                      int nthreads, const uint8_t* block_keep,
+                                      int32_t* block_idx, int32_t* match_idx, float* best,
+                                      float* second, double* base_mm, double* match_mm,
+                                      int64_t* n_kept) {
   BMArgs a{base, bmask, search, smask, g, bw, bg, nw, ng, {0, 0, 0}, {0, 0, 0},
-           block_idx, match_idx, best, second};
+           block_idx, match_idx, best, second, block_keep};
   orc_block_grid(g->dims, bw, ng, a.nb, a.bstart);
   // ComputeRegionForMatching: region of block starts (itkBlockMatcher.txx:566-574).
   Region full;

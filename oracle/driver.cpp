@@ -87,22 +87,31 @@ struct Matches {
   int64_t evals = 0;
 };
 
+// One block-matching pass. With portion < 1 (variance selection on, at a
+// level whose portion is not forced to 1.0) only the selected base blocks
+// are searched.
 void match(const std::vector<float>& base, const std::vector<uint8_t>& bmask,
            const std::vector<float>& search, const std::vector<uint8_t>& smask,
-           const orc_geom& bg, const orc_reg_params& p, std::vector<double>& out_b,
-           std::vector<double>& out_m, int64_t* evals) {
+           const orc_geom& bg, const orc_reg_params& p, double portion,
+           std::vector<double>& out_b, std::vector<double>& out_m, int64_t* evals) {
   int nb[3], st[3];
   orc_block_grid(bg.dims, p.bw, p.ng, nb, st);
   const int64_t n = (int64_t)nb[0] * nb[1] * nb[2];
   std::vector<double> b(3 * std::max<int64_t>(n, 1)), m(3 * std::max<int64_t>(n, 1));
+  std::vector<uint8_t> keep;
+  if (portion < 1.0 && n > 0) {
+    keep.resize(n);
+    orc_variance_select(base.data(), bg.dims, p.bw, p.ng, portion, keep.data(), nullptr);
+  }
+  const uint8_t* kp = keep.empty() ? nullptr : keep.data();
   int64_t k = 0;
-  if (orc_bm_get_offsets(base.data(), bmask.data(), search.data(), smask.data(), &bg, p.bw, p.bg,
-                         p.nw, p.ng, p.nthreads, nullptr, nullptr, nullptr, nullptr, b.data(),
-                         m.data(), &k) != 0) {
+  if (orc_bm_get_offsets_sel(base.data(), bmask.data(), search.data(), smask.data(), &bg, p.bw,
+                             p.bg, p.nw, p.ng, p.nthreads, kp, nullptr, nullptr, nullptr,
+                             nullptr, b.data(), m.data(), &k) != 0) {
     std::fprintf(stderr, "oracle: no block matches occurred\n");
     k = 0;
   }
-  *evals += orc_bm_candidate_count(bg.dims, p.bw, p.bg, p.nw, p.ng) * p.bw * p.bw * p.bw;
+  *evals += orc_bm_candidate_count_sel(bg.dims, p.bw, p.bg, p.nw, p.ng, kp) * p.bw * p.bw * p.bw;
   out_b.assign(b.begin(), b.begin() + 3 * k);
   out_m.assign(m.begin(), m.begin() + 3 * k);
 }
@@ -172,9 +181,18 @@ int params_sq_delta(const double* a, const double* b, int n, double* out) {
   return 0;
 }
 
+// Portion of blocks kept by the variance selection at a level: levels 0 and
+// 1 (the two coarsest) use every block (itkMirorrPyramidImplement.txx:116-119);
+// 1.0 when the selection is off.
+double level_portion(const orc_reg_params& p, int level) {
+  if (!p.variance_select || level <= 1) return 1.0;
+  return p.portion_kept == 0.0 ? 0.5 : p.portion_kept;
+}
+
 // Symmetric StartOptimization for one level; T is updated in place.
 void start_opt_symmetric(Xf& T, const LevelIn& L, int max_iter, const orc_reg_params& p,
                          orc_reg_stats* st, int level) {
+  const double portion = level_portion(p, level);
   double cF[3], cM[3];
   image_centre(L.F->g, cF);
   image_centre(L.M->g, cM);
@@ -210,8 +228,8 @@ void start_opt_symmetric(Xf& T, const LevelIn& L, int max_iter, const orc_reg_pa
     resample(L.Ffull, L.Fmask, *L.Fg, cF12, gF, nullptr, &Fmh, L.Fmask == nullptr);
     resample(L.Mfull, L.Mmask, *L.Mg, cM12, gM, nullptr, &Mmh, L.Mmask == nullptr);
     std::vector<double> xb, ym, fb, fm;
-    match(Mh, Mmh, Fh, Fmh, gM, p, xb, ym, &st->voxel_evals);  // base = moving
-    match(Fh, Fmh, Mh, Mmh, gF, p, fb, fm, &st->voxel_evals);  // swapped
+    match(Mh, Mmh, Fh, Fmh, gM, p, portion, xb, ym, &st->voxel_evals);  // base = moving
+    match(Fh, Fmh, Mh, Mmh, gF, p, portion, fb, fm, &st->voxel_evals);  // swapped
     xb.insert(xb.end(), fm.begin(), fm.end());
     ym.insert(ym.end(), fb.begin(), fb.end());
     st->points[level] = (int64_t)xb.size() / 3;
@@ -248,6 +266,7 @@ void start_opt_symmetric(Xf& T, const LevelIn& L, int max_iter, const orc_reg_pa
 // Classic StartOptimization with middle resampling.
 void start_opt_classic(Xf& T, const LevelIn& L, int max_iter, const orc_reg_params& p,
                        orc_reg_stats* st, int level) {
+  const double portion = level_portion(p, level);
   double cF[3], cM[3];
   image_centre(L.F->g, cF);
   image_centre(L.M->g, cM);
@@ -269,7 +288,7 @@ void start_opt_classic(Xf& T, const LevelIn& L, int max_iter, const orc_reg_para
     resample(L.F->v.data(), nullptr, L.F->g, T, gF, &Fh, nullptr, true);
     resample(L.Ffull, L.Fmask, *L.Fg, T, gF, nullptr, &Fmh, L.Fmask == nullptr);
     std::vector<double> xb, ym;
-    match(Mh, Mmh, Fh, Fmh, gM, p, xb, ym, &st->voxel_evals);
+    match(Mh, Mmh, Fh, Fmh, gM, p, portion, xb, ym, &st->voxel_evals);
     st->points[level] = (int64_t)xb.size() / 3;
     st->inner_iterations += lts_classic(T, xb, ym);
     T.get_params(cur);
@@ -305,6 +324,7 @@ void orc_reg_default_params(orc_reg_params* p) {
   p->pyr_num = 1024;
   p->pyr_min_size = 32;
   p->portion_kept = 0.5;
+  p->variance_select = 0;
   p->nthreads = 0;
   p->verbosity = 0;
   p->centre_init = 1;

@@ -55,6 +55,26 @@ int orc_bm_get_offsets(const float* base, const uint8_t* base_mask,
                        float* best, float* second, double* base_mm,
                        double* match_mm, int64_t* n_kept);
 
+// GetOffsets restricted to the blocks with block_keep[b] != 0 (NULL: all).
+// Skipped blocks get best = second = -3 and their own position as match.
+int orc_bm_get_offsets_sel(const float* base, const uint8_t* base_mask,
+                           const float* search, const uint8_t* search_mask,
+                           const orc_geom* geom, int bw, int bg, int nw, int ng,
+                           int nthreads, const uint8_t* block_keep, int32_t* block_idx,
+                           int32_t* match_idx, float* best, float* second, double* base_mm,
+                           double* match_mm, int64_t* n_kept);
+
+// Block-variance selection (a live version of the culling compiled out at
+// itkBlockMatcher.txx:664-719, keeping the top `portion` of the blocks by
+// centred sum of squares; ties with the cut-off are kept). Writes keep[b]
+// and (nullable) keys[b] in block raster order; returns the number kept.
+int64_t orc_variance_select(const float* base, const int dims[3], int bw, int ng,
+                            double portion, uint8_t* keep, float* keys);
+
+// Candidates of the blocks with keep[b] != 0 (NULL: all blocks).
+int64_t orc_bm_candidate_count_sel(const int dims[3], int bw, int bg, int nw, int ng,
+                                   const uint8_t* keep);
+
 // Number of candidates evaluated for the whole grid (voxel-evals / bw^3).
 int64_t orc_bm_candidate_count(const int dims[3], int bw, int bg, int nw,
                                int ng);

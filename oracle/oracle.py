@@ -74,6 +74,14 @@ def _declare(L):
                                      _P, _P, _P, _P]
     L.orc_bm_candidate_count.restype = C.c_int64
     L.orc_bm_candidate_count.argtypes = [_P, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.orc_bm_candidate_count_sel.restype = C.c_int64
+    L.orc_bm_candidate_count_sel.argtypes = [_P, C.c_int, C.c_int, C.c_int, C.c_int, _P]
+    L.orc_bm_get_offsets_sel.restype = C.c_int
+    L.orc_bm_get_offsets_sel.argtypes = [_P, _P, _P, _P, C.POINTER(OracleGeom), C.c_int,
+                                         C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P,
+                                         _P, _P, _P, _P]
+    L.orc_variance_select.restype = C.c_int64
+    L.orc_variance_select.argtypes = [_P, _P, C.c_int, C.c_int, C.c_double, _P, _P]
     for name, rt, at in _EXTRA_DECLS:
         if hasattr(L, name):
             f = getattr(L, name)
@@ -86,7 +94,8 @@ class RegParams(C.Structure):
                 ("nw", C.c_int), ("ng", C.c_int), ("iterations", C.c_int),
                 ("pyr_start", C.c_int), ("pyr_end", C.c_int), ("pyr_num", C.c_int),
                 ("pyr_min_size", C.c_int), ("portion_kept", C.c_double), ("nthreads", C.c_int),
-                ("verbosity", C.c_int), ("centre_init", C.c_int)]
+                ("verbosity", C.c_int), ("centre_init", C.c_int),
+                ("variance_select", C.c_int)]
 
 
 class RegStats(C.Structure):
@@ -156,10 +165,30 @@ def candidate_count(dims, bw, bg, nw, ng) -> int:
     return int(lib().orc_bm_candidate_count(_ptr(_i3(dims)), bw, bg, nw, ng))
 
 
+def variance_select(base, bw=4, ng=3, portion=0.5):
+    """Oracle block-variance selection of a base volume (z, y, x array).
+    Returns (keep uint8 per block, keys float32 per block, number kept)."""
+    base = _vol(base, np.float32)
+    dims = (base.shape[2], base.shape[1], base.shape[0])
+    nb, _ = block_grid(dims, bw, ng)
+    n = nb[0] * nb[1] * nb[2]
+    keep = np.zeros(max(n, 1), np.uint8)
+    keys = np.zeros(max(n, 1), np.float32)
+    k = lib().orc_variance_select(_ptr(base), _ptr(_i3(dims)), bw, ng, float(portion),
+                                  _ptr(keep), _ptr(keys))
+    return keep[:n], keys[:n], int(k)
+
+
+def candidate_count_sel(dims, bw, bg, nw, ng, keep=None):
+    kp = None if keep is None else _vol(keep, np.uint8)
+    return int(lib().orc_bm_candidate_count_sel(_ptr(_i3(dims)), bw, bg, nw, ng, _ptr(kp)))
+
+
 def bm_get_offsets(base, base_mask, search, search_mask, geom: Geom, bw=4, bg=1, nw=7,
-                   ng=3, nthreads=0):
+                   ng=3, nthreads=0, keep=None):
     """Oracle GetOffsets. Returns a dict with raw per-block arrays (block raster
-    order) and the kept point lists in the reference's thread order."""
+    order) and the kept point lists in the reference's thread order. `keep`
+    (uint8 per block, optional) restricts the search to the selected blocks."""
     base = _vol(base, np.float32)
     search = _vol(search, np.float32)
     if search.shape != base.shape:
@@ -176,11 +205,13 @@ def bm_get_offsets(base, base_mask, search, search_mask, geom: Geom, bw=4, bg=1,
     mmm = np.zeros((n, 3), np.float64)
     nk = C.c_int64(0)
     g = geom.to_c()
-    rc = lib().orc_bm_get_offsets(_ptr(base), _ptr(_vol(base_mask, np.uint8)), _ptr(search),
-                                  _ptr(_vol(search_mask, np.uint8)), C.byref(g), bw, bg,
-                                  nw, ng, nthreads, _ptr(out["block_idx"]),
-                                  _ptr(out["match_idx"]), _ptr(out["best"]),
-                                  _ptr(out["second"]), _ptr(bmm), _ptr(mmm), C.byref(nk))
+    kp = None if keep is None else _vol(keep, np.uint8)
+    rc = lib().orc_bm_get_offsets_sel(_ptr(base), _ptr(_vol(base_mask, np.uint8)),
+                                      _ptr(search), _ptr(_vol(search_mask, np.uint8)),
+                                      C.byref(g), bw, bg, nw, ng, nthreads, _ptr(kp),
+                                      _ptr(out["block_idx"]), _ptr(out["match_idx"]),
+                                      _ptr(out["best"]), _ptr(out["second"]), _ptr(bmm),
+                                      _ptr(mmm), C.byref(nk))
     if rc != 0:
         raise RuntimeError("oracle: no block matches occurred (empty block grid)")
     out["base_mm"] = bmm[: nk.value].copy()

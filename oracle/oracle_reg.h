@@ -27,6 +27,7 @@ typedef struct orc_reg_params {
   int nthreads;      // block-matcher threads; 0 = single-threaded matcher
   int verbosity;
   int centre_init;   // CenteredTransformInitializer GeometryOn (default 1)
+  int variance_select;  // block-variance selection at levels >= 2 (default 0)
 } orc_reg_params;
 
 typedef struct orc_reg_stats {

@@ -171,3 +171,80 @@ def test_integer_shift_recovered_with_mm_points(oracle):
     for pb, pm in zip(out["base_mm"], out["match_mm"]):
         ib = (pb - np.array(org)) / np.array(sp) - 2.0
         assert np.allclose(ib, np.round(ib))
+
+
+# ---- block-variance selection ----------------------------------------------
+
+def _checker_blocks(amps, bw=4):
+    """Volume whose blocks (ng = bw, grid from the corner) are +/-a checkerboards,
+    so each block's centred sum of squares is exactly bw^3 * a^2."""
+    nb = amps.shape  # (z, y, x) blocks
+    vol = np.zeros((nb[0] * bw, nb[1] * bw, nb[2] * bw), np.float32)
+    z, y, x = np.indices((bw, bw, bw))
+    sign = np.where((x + y + z) % 2 == 0, 1.0, -1.0).astype(np.float32)
+    for bz in range(nb[0]):
+        for by in range(nb[1]):
+            for bx in range(nb[2]):
+                vol[bz * bw:(bz + 1) * bw, by * bw:(by + 1) * bw, bx * bw:(bx + 1) * bw] = \
+                    100.0 + amps[bz, by, bx] * sign
+    return vol
+
+
+@pytest.mark.parametrize("portion", [0.1, 0.25, 0.5, 0.9])
+def test_variance_select_keeps_top_portion(oracle, portion):
+    rng = np.random.default_rng(11)
+    amps = rng.permutation(np.arange(1, 4 * 5 * 6 + 1)).reshape(4, 5, 6).astype(np.float32)
+    vol = _checker_blocks(amps)
+    keep, keys, n = oracle.variance_select(vol, bw=4, ng=4, portion=portion)
+    flat = amps.reshape(-1)  # z, y, x raster = block raster order (x fastest)
+    np.testing.assert_allclose(keys, 64.0 * flat ** 2, rtol=1e-6)
+    K = int(np.ceil(portion * flat.size))
+    expect = np.zeros(flat.size, np.uint8)
+    expect[np.argsort(-flat)[:K]] = 1
+    np.testing.assert_array_equal(keep, expect)
+    assert n == K
+
+
+def test_variance_select_ties_and_limits(oracle):
+    amps = np.ones((2, 2, 3), np.float32)
+    amps[0, 0, 0] = 5.0
+    vol = _checker_blocks(amps)
+    keep, _, n = oracle.variance_select(vol, bw=4, ng=4, portion=0.5)
+    assert n == amps.size and keep.all()  # cut-off ties with 11 blocks: all kept
+    keep, _, n = oracle.variance_select(vol, bw=4, ng=4, portion=1.0 / 24)
+    assert n == 1 and keep[0] == 1  # at least one block, the largest
+    keep, _, n = oracle.variance_select(vol, bw=4, ng=4, portion=1.0)
+    assert keep.all()
+
+
+def test_variance_key_is_population_variance(oracle):
+    rng = np.random.default_rng(5)
+    vol = rng.normal(300.0, 50.0, (12, 16, 20)).astype(np.float32)
+    keep, keys, _ = oracle.variance_select(vol, bw=4, ng=3, portion=0.5)
+    nb, st = oracle.block_grid((20, 16, 12), 4, 3)
+    i = 0
+    for z in range(nb[2]):
+        for y in range(nb[1]):
+            for x in range(nb[0]):
+                blk = vol[st[2] + 3 * z:st[2] + 3 * z + 4, st[1] + 3 * y:st[1] + 3 * y + 4,
+                          st[0] + 3 * x:st[0] + 3 * x + 4].astype(np.float64)
+                assert keys[i] == pytest.approx(64.0 * blk.var(), rel=1e-6)
+                i += 1
+
+
+def test_selected_search_skips_dropped_blocks(oracle):
+    rng = np.random.default_rng(9)
+    base = rng.normal(0.0, 1.0, (16, 20, 24)).astype(np.float32)
+    search = np.roll(base, 1, axis=2).copy()
+    g = oracle.Geom(dims=(24, 20, 16))
+    keep, _, n = oracle.variance_select(base, bw=4, ng=3, portion=0.5)
+    full = oracle.bm_get_offsets(base, None, search, None, g, bw=4, bg=1, nw=5, ng=3)
+    sel = oracle.bm_get_offsets(base, None, search, None, g, bw=4, bg=1, nw=5, ng=3, keep=keep)
+    on = keep.astype(bool)
+    np.testing.assert_array_equal(sel["match_idx"][on], full["match_idx"][on])
+    np.testing.assert_array_equal(sel["best"][on], full["best"][on])
+    assert np.all(sel["best"][~on] == -3.0)
+    np.testing.assert_array_equal(sel["match_idx"][~on], sel["block_idx"][~on])
+    assert sel["base_mm"].shape[0] <= n
+    cc = oracle.candidate_count_sel(g.dims, 4, 1, 5, 3, keep)
+    assert cc < oracle.candidate_count(g.dims, 4, 1, 5, 3)

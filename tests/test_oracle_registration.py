@@ -28,3 +28,27 @@ def test_oracle_recovers_phantom_transform(oracle, symmetric, affine):
     init = transforms.max_corner_distance(np.eye(3), np.zeros(3), pair.matrix,
                                           pair.translation, pair.fixed)
     assert err < 6.0 and err < init / 5.0, (err, init)
+
+
+def test_oracle_variance_selection_applies_from_level_two(oracle):
+    # Levels 0 and 1 use every block (itkMirorrPyramidImplement.txx:116-119):
+    # with two levels the selection changes nothing; with three it thins the
+    # finest level and the transform is still recovered.
+    pair = phantom.make_pair(dims=(96, 96, 48), spacing=(2.0, 2.0, 2.0), n_ellipsoids=20,
+                             seed=12, model="rigid")
+    two = dict(symmetric=1, affine=0, nw=7, ng=3, pyr_num=2, nthreads=8)
+    _, _, a = oracle.register(pair.fixed, pair.moving, oracle.reg_params(**two))
+    _, _, b = oracle.register(pair.fixed, pair.moving,
+                              oracle.reg_params(variance_select=1, **two))
+    assert a["voxel_evals"] == b["voxel_evals"] and a["points"] == b["points"]
+    three = dict(two, pyr_num=3, pyr_min_size=16)
+    _, _, full = oracle.register(pair.fixed, pair.moving, oracle.reg_params(**three))
+    par, c, sel = oracle.register(pair.fixed, pair.moving,
+                                  oracle.reg_params(variance_select=1, **three))
+    assert sel["levels"] == 3
+    assert sel["voxel_evals"] < full["voxel_evals"]
+    assert sel["points"][2] < full["points"][2]
+    err = _corner_error(par, c, 0, pair)
+    init = transforms.max_corner_distance(np.eye(3), np.zeros(3), pair.matrix,
+                                          pair.translation, pair.fixed)
+    assert err < 4.0 and err < init / 5.0, (err, init)
