@@ -55,6 +55,21 @@ def configure(reg, cfg):
     reg.GetPyramidScheduleTuner().SetLevelMax(0)
 
 
+TRAFFIC_SUMMARY = os.path.join(HERE, "profiles", "ncc_tile_fast_pair256_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per ncc_tile_fast launch from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), or None."""
+    import re
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", f.read())
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
 def cpu_baseline(pair, cfg, threads):
     """Oracle threaded block matcher, one full-resolution pass (moving blocks
     searched in the fixed image), timed on the host cores."""
@@ -153,7 +168,7 @@ def main():
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": None,
+            "traffic": measured_traffic(),
             "launches": n_fast,
             "avg_launch_ms": avg_ms,
             "flop_per_voxel_eval": FLOP_PER_VOXEL_EVAL,

@@ -16,8 +16,9 @@
 //    KY y offsets, all NW x offsets) and accumulates Sum x~ * y' with one FMA
 //    per voxel-eval, the centred block held in 64 VGPRs and rows of the window
 //    read as 16-byte LDS loads shared by 4*KY*NW FMAs. Sxy, Syy then follow
-//    from the window statistics. Tiles whose masks are not all on go to a
-//    queue served by the masked kernel (exact masked sums, 6 ops/voxel-eval).
+//    from the window statistics. Blocks whose base mask or own search window
+//    is not all on are queued (tile + block bitmask) for the masked kernel
+//    (masked sums, 4-5 ops/voxel-eval); the rest of the tile stays here.
 //  * Top-2 per block uses the float score buffer semantics of the reference
 //    with ties on the best broken by the earliest candidate in scan order.
 #include "bm_kernels.hpp"
@@ -31,6 +32,7 @@ namespace mirorr {
 namespace {
 
 constexpr int kThreads = 256;
+constexpr int kMaxWindow = 64;  // window planes (and rows) a tile may span
 
 __device__ __forceinline__ int64_t lin3(int nx, int ny, int x, int y, int z) {
   return (int64_t)x + (int64_t)nx * ((int64_t)y + (int64_t)ny * z);
@@ -273,6 +275,7 @@ struct TileCfg {
   static constexpr int YPAD = NCY * KY - NW;
   static constexpr int GT = NB * NW * NW;  // masked-path tasks per tile
   static_assert(NT <= kThreads, "tile too large");
+  static_assert(NB <= 64, "block bitmasks are 64-bit");
 };
 
 struct TileArgs {
@@ -287,6 +290,7 @@ struct TileArgs {
   const float2* stats;
   int ntx, nty, ntz;
   int32_t* queue;
+  unsigned long long* qbits;  // per queued tile: the blocks left to the masked kernel
   int32_t* qcount;
   float* out_best;
   float* out_second;
@@ -345,7 +349,92 @@ __device__ __forceinline__ void write_skipped(const TileArgs& a, int64_t b, int 
   a.out_match[3 * b + 2] = pz;
 }
 
-// Block states in a tile: 0 outside the grid, 1 searched, 2 skipped.
+// Stages a tile's search window into LDS: rows of PX floats (wx fastest,
+// then wy, then wz), y - ref inside the image and 0 outside. One item is a
+// 4-wide column group of one row, so the row index needs one division by
+// the runtime height WY, done with a float reciprocal (exact here: rows are
+// small integers and the quotient's fractional part is at least 0.5 / WY
+// away from an integer).
+//  * MASKED: voxels whose search mask is off stage as 0, and the mask bits
+//    go to `wm` (one byte per voxel, same layout).
+//  * otherwise: a voxel whose search mask is off sets bit wy of zrow[wz].
+template <class T, bool MASKED>
+__device__ __forceinline__ void stage_window(const TileArgs& a, float* win, uint8_t* wm,
+                                             unsigned long long* zrow, int WY, int WZ, int wx0,
+                                             int wy0, int wz0, float ref) {
+  const int nx = a.g.dims[0], ny = a.g.dims[1], nz = a.g.dims[2];
+  const int items = T::NV * WY * WZ;
+  const float inv_wy = 1.0f / (float)WY;
+  for (int it = threadIdx.x; it < items; it += kThreads) {
+    const int v = it % T::NV;
+    const int row = it / T::NV;
+    const int wz = (int)(((float)row + 0.5f) * inv_wy);
+    const int wy = row - wz * WY;
+    const int gy = wy0 + wy, gz = wz0 + wz;
+    float q[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    uint32_t mbits = 0u;
+    if (gy >= 0 && gy < ny && gz >= 0 && gz < nz) {
+      const int64_t rb = lin3(nx, ny, 0, gy, gz);
+      const int gx0 = wx0 + 4 * v;
+      float y[4];
+      uint32_t inside, on;  // bit k: voxel k inside image and window / search mask on
+      if (gx0 >= 0 && gx0 + 3 < nx && 4 * v + 3 < T::WX) {
+        // interior: the four loads issue together
+        inside = 0xfu;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) y[k] = a.y[rb + gx0 + k];
+        on = inside;
+        if (a.sm) {
+          uint8_t m[4];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) m[k] = a.sm[rb + gx0 + k];
+#pragma unroll
+          for (int k = 0; k < 4; ++k)
+            if (m[k] == 0) on &= ~(1u << k);
+        }
+      } else {
+        inside = 0u;
+        on = 0u;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int gx = gx0 + k;
+          y[k] = 0.0f;
+          if (4 * v + k < T::WX && gx >= 0 && gx < nx) {
+            inside |= 1u << k;
+            y[k] = a.y[rb + gx];
+            if (a.sm == nullptr || a.sm[rb + gx] != 0) on |= 1u << k;
+          }
+        }
+      }
+      const uint32_t use = MASKED ? on : inside;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) q[k] = ((use >> k) & 1u) ? y[k] - ref : 0.0f;
+      if (MASKED) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) mbits |= ((on >> k) & 1u) << (8 * k);
+      } else if (inside & ~on) {
+        atomicOr(&zrow[wz], 1ull << wy);
+      }
+    }
+    reinterpret_cast<float4*>(win)[row * T::NV + v] = make_float4(q[0], q[1], q[2], q[3]);
+    if (MASKED) reinterpret_cast<uint32_t*>(wm)[row * T::NV + v] = mbits;
+  }
+}
+
+// Blocks of a tile whose own search window (WX rows from the block's offset
+// in y and z) holds a search-mask-off voxel, from the per-plane row bits.
+template <class T>
+__device__ __forceinline__ bool window_has_off(const unsigned long long* zrow, int bl, int ng) {
+  const int bly = bl % T::TY, blz = bl / T::TY;
+  const unsigned long long ym = ((1ull << T::WX) - 1ull) << (bly * ng);
+  unsigned long long any = 0ull;
+#pragma unroll
+  for (int k = 0; k < T::WX; ++k) any |= zrow[blz * ng + k];
+  return (any & ym) != 0ull;
+}
+
+// Block states in a tile: 0 outside the grid, 1 searched, 2 skipped,
+// 3 left to the masked kernel.
 __device__ __forceinline__ int block_state(const TileArgs& a, int gy, int gz, int64_t b) {
   if (gy >= a.g.nb[1] || gz >= a.g.nb[2]) return 0;
   return (a.bflags[b] & kBlockSkip) ? 2 : 1;
@@ -406,7 +495,9 @@ __global__ __launch_bounds__(kThreads, 4) void ncc_tile_fast(TileArgs a) {
   __shared__ float bsxx[T::NB], bsum[T::NB];
   __shared__ int bvalid[T::NB];
   __shared__ MergeSlot mrg[T::NB];
-  __shared__ int s_ok, s_any;
+  __shared__ int s_any;
+  __shared__ unsigned long long s_bad, s_valid;
+  __shared__ unsigned long long zrow[kMaxWindow];  // off-mask rows per window plane
   __shared__ float s_ref;
 
   const BMGrid& g = a.g;
@@ -423,14 +514,16 @@ __global__ __launch_bounds__(kThreads, 4) void ncc_tile_fast(TileArgs a) {
   const int wx0 = px - T::H, wy0 = py0 - T::H, wz0 = pz0 - T::H;
 
   if (tid == 0) {
-    s_ok = 1;
     s_any = 0;
+    s_bad = 0ull;
+    s_valid = 0ull;
     // Reference shift: window mean at the tile's middle block (clamped).
     const int ry = min(py0 + NG * (T::TY / 2), ny - T::BW);
     const int rz = min(pz0 + NG * (T::TZ / 2), nz - T::BW);
     s_ref = a.stats[lin3(nx, ny, px, ry, rz)].x;
   }
   merge_init(mrg, T::NB);
+  for (int i = tid; i < WZ; i += kThreads) zrow[i] = 0ull;
   __syncthreads();
   if (tid < T::NB) {
     const int gy = by0 + tid % T::TY, gz = bz0 + tid / T::TY;
@@ -439,38 +532,21 @@ __global__ __launch_bounds__(kThreads, 4) void ncc_tile_fast(TileArgs a) {
     bvalid[tid] = st;
     if (st == 1) {
       s_any = 1;
+      atomicOr(&s_valid, 1ull << tid);
       bsxx[tid] = a.sxx[b];
       bsum[tid] = a.sumxt[b];
-      if ((a.bflags[b] & 1u) == 0u) s_ok = 0;
+      if ((a.bflags[b] & 1u) == 0u) atomicOr(&s_bad, 1ull << tid);  // base partly masked
+    } else if (st == 2) {
+      write_skipped(a, b, px, py0 + NG * (tid % T::TY), pz0 + NG * (tid / T::TY));
     }
   }
   __syncthreads();
-  if (!s_any) {  // every block of the tile is skipped (or outside the grid)
-    if (tid < T::NB && bvalid[tid] == 2) {
-      const int gy = by0 + tid % T::TY, gz = bz0 + tid / T::TY;
-      const int64_t b = bx + (int64_t)g.nb[0] * (gy + (int64_t)g.nb[1] * gz);
-      write_skipped(a, b, px, py0 + NG * (tid % T::TY), pz0 + NG * (tid / T::TY));
-    }
-    return;  // workgroup-uniform
-  }
+  if (!s_any) return;  // every block is skipped or outside the grid (workgroup-uniform)
   const float ref = s_ref;
   // Stage the union search window (shifted by ref; zeros outside the image).
-  const int total = T::PX * WY * WZ;
-  bool ok = true;
-  for (int i = tid; i < total; i += kThreads) {
-    const int wx = i % T::PX;
-    const int wy = (i / T::PX) % WY;
-    const int wz = i / (T::PX * WY);
-    const int gx = wx0 + wx, gy = wy0 + wy, gz = wz0 + wz;
-    float v = 0.0f;
-    if (wx < T::WX && gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 && gz < nz) {
-      const int64_t o = lin3(nx, ny, gx, gy, gz);
-      v = a.y[o] - ref;
-      if (a.sm && a.sm[o] == 0) ok = false;
-    }
-    win[i] = v;
-  }
-  if (!ok) s_ok = 0;
+  // A search-mask-off voxel sends the blocks whose own window holds it to the
+  // masked kernel; the other blocks stay here.
+  stage_window<T, false>(a, win, nullptr, zrow, WY, WZ, wx0, wy0, wz0, ref);
   for (int i = tid; i < T::NB * 16; i += kThreads) {
     const int bl = i / 16;
     const int gy = by0 + bl % T::TY, gz = bz0 + bl / T::TY;
@@ -480,12 +556,19 @@ __global__ __launch_bounds__(kThreads, 4) void ncc_tile_fast(TileArgs a) {
     }
   }
   __syncthreads();
-  if (!s_ok) {
+  if (tid < T::NB && ((s_valid >> tid) & 1ull) && window_has_off<T>(zrow, tid, NG))
+    atomicOr(&s_bad, 1ull << tid);
+  __syncthreads();
+  const unsigned long long todo = s_bad & s_valid;
+  if (todo) {
     if (tid == 0) {
       const int q = atomicAdd(a.qcount, 1);
       a.queue[q] = blockIdx.x;
+      a.qbits[q] = todo;
     }
-    return;  // workgroup-uniform
+    if (todo == s_valid) return;  // workgroup-uniform: nothing left here
+    if (tid < T::NB && ((todo >> tid) & 1ull)) bvalid[tid] = 3;
+    __syncthreads();
   }
   if (a.work && tid == 0) atomicAdd(a.work, tile_evals<T>(a, px, py0, pz0, bvalid));
 
@@ -545,24 +628,29 @@ __global__ __launch_bounds__(kThreads, 4) void ncc_tile_fast(TileArgs a) {
     const int lox = max(px - T::H, 0), hix = min(px - T::H + T::NW - 1, nx - T::BW);
     const int loy = max(py - T::H, 0), hiy = min(py - T::H + T::NW - 1, ny - T::BW);
     const int loz = max(pz - T::H, 0), hiz = min(pz - T::H + T::NW - 1, nz - T::BW);
+    // Branch-free: candidates outside the image score -inf, which changes
+    // neither the best nor the second (the scores start at -3); their
+    // statistics are read at a clamped position.
     const int czg = pz - T::H + cz;
     const float sxx = bsxx[bl], sumx = bsum[bl];
-    if (czg >= loz && czg <= hiz) {
+    const bool xok = sxx > 0.0f;  // -2 for every candidate otherwise
+    const float isxx = xok ? __builtin_amdgcn_rcpf(sxx) : 0.0f;
+    const bool zok = czg >= loz && czg <= hiz;
+    const int czc = min(max(czg, 0), nz - T::BW);
 #pragma unroll
-      for (int ky = 0; ky < T::KY; ++ky) {
-        const int cy = cyg * T::KY + ky;
-        const int cyv = py - T::H + cy;
-        if (cy >= T::NW || cyv < loy || cyv > hiy) continue;
-        const float2* srow = a.stats + lin3(nx, ny, 0, cyv, czg);
+    for (int ky = 0; ky < T::KY; ++ky) {
+      const int cy = cyg * T::KY + ky;
+      const int cyv = py - T::H + cy;
+      const bool yok = zok && cy < T::NW && cyv >= loy && cyv <= hiy;
+      const float2* srow = a.stats + lin3(nx, ny, 0, min(max(cyv, 0), ny - T::BW), czc);
 #pragma unroll
-        for (int cx = 0; cx < T::NW; ++cx) {
-          const int cxg = px - T::H + cx;
-          if (cxg < lox || cxg > hix) continue;
-          const float2 st = srow[cxg];
-          const float sxy = fmaf(-(st.x - ref), sumx, acc[ky][cx]);
-          const float s = ncc_score(sxy, sxx, st.y);
-          top2_push(t2, s, (cz * T::NW + cy) * T::NW + cx);
-        }
+      for (int cx = 0; cx < T::NW; ++cx) {
+        const int cxg = px - T::H + cx;
+        const float2 st = srow[min(max(cxg, 0), nx - T::BW)];
+        const float sxy = fmaf(-(st.x - ref), sumx, acc[ky][cx]);
+        float s = (xok && st.y > 0.0f) ? sxy * sxy * isxx * __builtin_amdgcn_rcpf(st.y) : -2.0f;
+        s = (yok && cxg >= lox && cxg <= hix) ? s : -__builtin_huge_valf();
+        top2_push(t2, s, (cz * T::NW + cy) * T::NW + cx);
       }
     }
   }
@@ -570,14 +658,11 @@ __global__ __launch_bounds__(kThreads, 4) void ncc_tile_fast(TileArgs a) {
   __syncthreads();
   if (tid < T::NT) merge_phase2(mrg, bl, t2);
   __syncthreads();
-  if (tid < T::NB && bvalid[tid]) {
+  if (tid < T::NB && bvalid[tid] == 1) {
     const int gy = by0 + tid % T::TY, gz = bz0 + tid / T::TY;
     const int64_t b = bx + (int64_t)g.nb[0] * (gy + (int64_t)g.nb[1] * gz);
-    const int py = py0 + NG * (tid % T::TY), pz = pz0 + NG * (tid / T::TY);
-    if (bvalid[tid] == 2)
-      write_skipped(a, b, px, py, pz);
-    else
-      write_block_result<T>(a, b, px, py, pz, merge_result(mrg[tid]));
+    write_block_result<T>(a, b, px, py0 + NG * (tid % T::TY), pz0 + NG * (tid / T::TY),
+                          merge_result(mrg[tid]));
   }
 }
 
@@ -603,6 +688,7 @@ __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
 
   if (blockIdx.x >= *a.qcount) return;
   const int tile = a.queue[blockIdx.x];
+  const unsigned long long todo = a.qbits[blockIdx.x];  // blocks to search here
   const BMGrid& g = a.g;
   const int nx = g.dims[0], ny = g.dims[1], nz = g.dims[2];
   const int tid = threadIdx.x;
@@ -626,34 +712,17 @@ __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
   }
   merge_init(mrg, T::NB);
   if (tid < T::NB) {
-    const int gy = by0 + tid % T::TY, gz = bz0 + tid / T::TY;
-    bvalid[tid] = block_state(a, gy, gz, bx + (int64_t)g.nb[0] * (gy + (int64_t)g.nb[1] * gz));
+    bvalid[tid] = ((todo >> tid) & 1ull) ? 1 : 0;
     bbits[tid] = 0ull;
   }
   __syncthreads();
   const float ref = s_ref;
-  for (int i = tid; i < total; i += kThreads) {
-    const int wx = i % T::PX;
-    const int wy = (i / T::PX) % WY;
-    const int wz = i / (T::PX * WY);
-    const int gx = wx0 + wx, gy = wy0 + wy, gz = wz0 + wz;
-    float v = 0.0f;
-    uint8_t m = 0;
-    if (wx < T::WX && gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 && gz < nz) {
-      const int64_t o = lin3(nx, ny, gx, gy, gz);
-      if (a.sm == nullptr || a.sm[o] != 0) {
-        v = a.y[o] - ref;
-        m = 1;
-      }
-    }
-    wy_[i] = v;
-    wv_[i] = m;
-  }
+  stage_window<T, true>(a, wy_, wv_, nullptr, WY, WZ, wx0, wy0, wz0, ref);
   for (int i = tid; i < T::NB * 64; i += kThreads) {
     const int bl = i / 64, k = i % 64;
     const int gy = by0 + bl % T::TY, gz = bz0 + bl / T::TY;
     float xh = 0.0f;
-    if (gy < g.nb[1] && gz < g.nb[2]) {
+    if ((todo >> bl) & 1ull) {
       const int64_t b = bx + (int64_t)g.nb[0] * (gy + (int64_t)g.nb[1] * gz);
       bool on = true;
       if (a.bm != nullptr && (a.bflags[b] & 1u) == 0u) {
@@ -801,14 +870,11 @@ __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
 #pragma unroll
   for (int j = 0; j < R; ++j) merge_phase2(mrg, recb[j], rec[j]);
   __syncthreads();
-  if (tid < T::NB && bvalid[tid]) {
+  if (tid < T::NB && bvalid[tid] == 1) {
     const int gy = by0 + tid % T::TY, gz = bz0 + tid / T::TY;
     const int64_t b = bx + (int64_t)g.nb[0] * (gy + (int64_t)g.nb[1] * gz);
-    const int py = py0 + NG * (tid % T::TY), pz = pz0 + NG * (tid / T::TY);
-    if (bvalid[tid] == 2)
-      write_skipped(a, b, px, py, pz);
-    else
-      write_block_result<T>(a, b, px, py, pz, merge_result(mrg[tid]));
+    write_block_result<T>(a, b, px, py0 + NG * (tid % T::TY), pz0 + NG * (tid / T::TY),
+                          merge_result(mrg[tid]));
   }
 }
 
@@ -824,7 +890,7 @@ __global__ __launch_bounds__(kThreads) void ncc_generic_kernel(
   __shared__ float tb[kThreads], ts[kThreads];
   __shared__ int ti[kThreads];
   const int64_t b = blockIdx.x;
-  const int nx = g.dims[0], ny = g.dims[1], nz = g.dims[2];
+  const int nx = g.dims[0], ny = g.dims[1];
   const int bxi = (int)(b % g.nb[0]);
   const int byi = (int)((b / g.nb[0]) % g.nb[1]);
   const int bzi = (int)(b / ((int64_t)g.nb[0] * g.nb[1]));
@@ -1023,8 +1089,8 @@ __global__ __launch_bounds__(kThreads) void compact_scatter(const uint8_t* f, in
 // ---------------------------------------------------------------------------
 // Block-variance selection: exact K-th largest block key (the float bits of
 // the centred sum of squares, non-negative, so ordered as unsigned ints) by
-// three radix digits (bits 31..20, 19..8, 7..0), each histogram pass
-// finishing its bin search in the last workgroup to arrive.
+// three radix digits (bits 31..20, 19..8, 7..0): a histogram pass over the
+// blocks, then a one-workgroup bin search, per digit.
 __global__ __launch_bounds__(kThreads) void vsel_init(VarSelState* st, unsigned int* hist,
                                                       unsigned long long k) {
   for (int i = threadIdx.x; i < kVarSelHistBins; i += kThreads) hist[i] = 0u;
@@ -1032,7 +1098,6 @@ __global__ __launch_bounds__(kThreads) void vsel_init(VarSelState* st, unsigned 
     st->k = k;
     st->k2 = st->k3 = 0ull;
     st->bin1 = st->bin2 = st->bin3 = 0u;
-    for (int i = 0; i < 3; ++i) st->sync[i] = 0u;
     st->kept = 0ull;
   }
 }
@@ -1060,10 +1125,14 @@ __global__ __launch_bounds__(kThreads) void vsel_hist(const float* __restrict__ 
   unsigned int* gh = hist + (PASS == 0 ? 0 : PASS == 1 ? 4096 : 8192);
   for (int i = threadIdx.x; i < NB; i += kThreads)
     if (h[i]) atomicAdd(&gh[i], h[i]);
-  if (!sel::arrive_last(&st->sync[PASS])) return;
-  if (PASS == 0) sel::find_bin<4096 / kThreads>(gh, st->k, &st->bin1, &st->k2);
-  if (PASS == 1) sel::find_bin<4096 / kThreads>(gh, st->k2, &st->bin2, &st->k3);
-  if (PASS == 2) sel::find_bin<256 / kThreads>(gh, st->k3, &st->bin3, &st->k3);
+}
+
+// Bin search of one digit (one workgroup).
+template <int PASS>
+__global__ __launch_bounds__(kThreads) void vsel_scan(VarSelState* st, const unsigned int* hist) {
+  if (PASS == 0) sel::find_bin<4096 / kThreads>(hist, st->k, &st->bin1, &st->k2);
+  if (PASS == 1) sel::find_bin<4096 / kThreads>(hist + 4096, st->k2, &st->bin2, &st->k3);
+  if (PASS == 2) sel::find_bin<256 / kThreads>(hist + 8192, st->k3, &st->bin3, &st->k3);
 }
 
 // Marks the blocks below the cut-off and accounts the kept ones.
@@ -1123,6 +1192,7 @@ void run_tiled(const BMGrid& g, TileArgs a, BMScratch& scr, hipStream_t s) {
   const int64_t nt = (int64_t)a.ntx * a.nty * a.ntz;
   if (nt > scr.ntiles_cap) throw Error(kErrState, "tile queue too small");
   a.queue = scr.queue;
+  a.qbits = scr.qbits;
   a.qcount = scr.count;
   hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, s, scr.count);
   const int WYf = (T::TY - 1) * g.ng + T::WX + T::YPAD;
@@ -1144,9 +1214,11 @@ void run_tiled(const BMGrid& g, TileArgs a, BMScratch& scr, hipStream_t s) {
 
 template <class T>
 bool lds_fits(int ng) {
-  // Dynamic windows plus the static arrays of each kernel, within 64 KB.
+  // Dynamic windows plus the static arrays of each kernel, within 64 KB; the
+  // window spans at most kMaxWindow rows and planes (64-bit row masks).
   const size_t wyf = (T::TY - 1) * ng + T::WX + T::YPAD, wym = (T::TY - 1) * ng + T::WX;
   const size_t wz = (T::TZ - 1) * ng + T::WX;
+  if (wyf > (size_t)kMaxWindow || wz > (size_t)kMaxWindow) return false;
   const size_t fast = 4 * T::PX * wyf * wz + T::NB * (16 * 16 + 12 + 16) + 16;
   const size_t masked = 5 * T::PX * wym * wz + T::NB * (16 * 16 + 8 + 4 + 16);
   return fast <= 65536 && masked <= 65536;
@@ -1206,8 +1278,11 @@ bool launch_variance_select(const BMGrid& g, const BlockPrep& bp, double portion
   hipLaunchKernelGGL(vsel_init, dim3(1), dim3(kThreads), 0, s, scr.st, scr.hist,
                      (unsigned long long)(n - K));
   hipLaunchKernelGGL(vsel_hist<0>, dim3(grid), dim3(kThreads), 0, s, bp.sxx, n, scr.st, scr.hist);
+  hipLaunchKernelGGL(vsel_scan<0>, dim3(1), dim3(kThreads), 0, s, scr.st, scr.hist);
   hipLaunchKernelGGL(vsel_hist<1>, dim3(grid), dim3(kThreads), 0, s, bp.sxx, n, scr.st, scr.hist);
+  hipLaunchKernelGGL(vsel_scan<1>, dim3(1), dim3(kThreads), 0, s, scr.st, scr.hist);
   hipLaunchKernelGGL(vsel_hist<2>, dim3(grid), dim3(kThreads), 0, s, bp.sxx, n, scr.st, scr.hist);
+  hipLaunchKernelGGL(vsel_scan<2>, dim3(1), dim3(kThreads), 0, s, scr.st, scr.hist);
   hipLaunchKernelGGL(vsel_mark, dim3(grid), dim3(kThreads), 0, s, bp.sxx, bp.flags, g, scr.st,
                      scr.evals);
   MIRORR_CHECK_LAUNCH();

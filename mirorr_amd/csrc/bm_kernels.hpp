@@ -21,7 +21,6 @@ constexpr uint32_t kBlockSkip = 4u;  // flags bit: block not searched
 struct VarSelState {
   unsigned long long k, k2, k3;  // ascending rank of the cut-off key, then within bins
   unsigned int bin1, bin2, bin3;
-  unsigned int sync[3];
   unsigned long long kept;       // blocks kept by the last selection
 };
 struct VarSelScratch {
@@ -49,6 +48,7 @@ struct BMResult {
 struct BMScratch {
   int32_t* queue = nullptr;  // ntiles entries
   int32_t* count = nullptr;  // 1 counter
+  unsigned long long* qbits = nullptr;  // ntiles entries: blocks per queued tile
   int64_t ntiles_cap = 0;
   Profiler* prof = nullptr;  // optional: event timing + voxel-evals per launch
 };

@@ -34,7 +34,7 @@ struct mirorr_bm {
   DeviceBuffer vsel_st, vsel_hist, vsel_evals;
   // Results of the last search.
   DeviceBuffer best, second, match, kept, base_mm, match_mm, index;
-  DeviceBuffer queue, qcount, chunk_counts, chunk_offsets;
+  DeviceBuffer queue, qbits, qcount, chunk_counts, chunk_offsets;
   bool results_valid = false;
   uint64_t results_key[2] = {0, 0};
   BMGrid grid{};
@@ -133,8 +133,10 @@ static void run_search(mirorr_bm* bm) {
     }
     const int64_t nt = tile_count(g);
     bm->queue.ensure(sizeof(int32_t) * nt, bud);
+    bm->qbits.ensure(sizeof(unsigned long long) * nt, bud);
     bm->qcount.ensure(sizeof(int32_t), bud);
-    BMScratch scr{bm->queue.as<int32_t>(), bm->qcount.as<int32_t>(), nt, &bm->ctx->prof};
+    BMScratch scr{bm->queue.as<int32_t>(), bm->qcount.as<int32_t>(),
+                  bm->qbits.as<unsigned long long>(), nt, &bm->ctx->prof};
     launch_ncc_search(B.data, B.mask, S.data, S.mask, g, bp,
                       SearchStats{bm->stats[si].as<float2>()}, scr, res, s);
   } else {

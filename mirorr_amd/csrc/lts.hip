@@ -3,22 +3,22 @@
 // Restates OptimizeTransform of the classic (itkMirorrRegistrationMethod.txx:758-1106)
 // and symmetric (itkSymMirorrRegistrationMethod.txx:729-1442) methods with the
 // per-point work on the GPU and the tiny FP64 solves in a one-thread tail.
-// One inner iteration is six launches; each of the first five finishes its cross-workgroup
-// reduction in the workgroup that arrives last (an arrival counter with
-// release/acquire fences), so no launch exists only to reduce:
+// Per inner iteration (the reductions finish in one-workgroup launches:
+// kernel boundaries publish the partial results, where a per-workgroup
+// agent-scope fence would write back L2 for every workgroup):
 //  1. residual kernel: r_i = |F(x_i) - B(y_i)|^2 (classic |F(x_i) - y_i|^2),
 //     +inf for culled blocks; the first radix digit (bits 63..52 of the
-//     non-negative double) is histogrammed in LDS and merged globally; the
-//     last workgroup finds the bin of rank k;
-//  2. second digit (bits 51..40) inside that bin, likewise;
-//  3. compaction of the candidates sharing the 24-bit prefix; the last
-//     workgroup finishes the exact k-th smallest residual (std::nth_element
-//     in the reference) by rank counting or 8-bit radix passes;
-//  4. trimmed barycentres (count and coordinate sums);
-//  5. the 21 centred second moments x x^T, y y^T, x y^T, block-reduced in
-//     double (the rigid quaternion matrix sum Q^T Q is assembled from them);
-//  6. a one-thread solve (normal equations / quaternion eigenvector / polar
-//     factor / Denman-Beavers roots) that updates the device state.
+//     non-negative double) is histogrammed in LDS and merged globally;
+//  2. exact k-th smallest residual (std::nth_element in the reference): the
+//     bin of rank k, a second 12-bit digit inside it, a compaction of the
+//     candidates sharing the 24-bit prefix and a one-workgroup finish by rank
+//     counting or 8-bit radix passes;
+//  3. trimmed moments in two passes (barycentres, then the 21 centred second
+//     moments x x^T, y y^T, x y^T), block-reduced in double and totalled in
+//     a fixed order; the rigid quaternion matrix sum Q^T Q is assembled from
+//     those moments;
+//  4. a one-thread solve (normal equations / quaternion eigenvector / polar
+//     factor / Denman-Beavers roots) updating the device state.
 // Every kernel is predicated on the state's `active` flag so the host can
 // queue several inner iterations before reading that flag back.
 #include <cfloat>
@@ -34,7 +34,6 @@ constexpr int kT = sel::kSelT;
 constexpr int kPart = 1024;  // points per moment workgroup (4 per thread)
 
 using namespace xf;
-using sel::arrive_last;
 using sel::block_scan;
 
 __device__ __forceinline__ unsigned long long key_of(double r) {
@@ -81,7 +80,11 @@ __global__ __launch_bounds__(kT) void residual_kernel(const double* __restrict__
   __syncthreads();
   for (int i = threadIdx.x; i < 4096; i += kT)
     if (h[i]) atomicAdd(&hist1[i], h[i]);
-  if (!arrive_last(&st->sync[0])) return;
+}
+
+// Bin of rank k in the first-digit histogram (one workgroup).
+__global__ __launch_bounds__(kT) void scan1_kernel(const unsigned int* hist1, LtsState* st) {
+  if (!st->active) return;
   sel::find_bin<4096 / kT>(hist1, st->k, &st->bin1, &st->k2);
   if (threadIdx.x == 0) st->ncand = 0;
 }
@@ -101,7 +104,10 @@ __global__ __launch_bounds__(kT) void hist2_kernel(const double* __restrict__ r,
   __syncthreads();
   for (int i = threadIdx.x; i < 4096; i += kT)
     if (h[i]) atomicAdd(&hist2[i], h[i]);
-  if (!arrive_last(&st->sync[1])) return;
+}
+
+__global__ __launch_bounds__(kT) void scan2_kernel(const unsigned int* hist2, LtsState* st) {
+  if (!st->active) return;
   sel::find_bin<4096 / kT>(hist2, st->k2, &st->bin2, &st->k3);
 }
 
@@ -165,7 +171,11 @@ __global__ __launch_bounds__(kT) void compact_kernel(const double* __restrict__ 
     const unsigned long long k = key_of(r[i]);
     if ((k >> 40) == pre) cand[atomicAdd(&st->ncand, 1u)] = k;
   }
-  if (!arrive_last(&st->sync[2])) return;
+}
+
+__global__ __launch_bounds__(kT) void final_select_kernel(const unsigned long long* cand,
+                                                          LtsState* st) {
+  if (!st->active) return;
   final_select(cand, st);
 }
 
@@ -194,7 +204,7 @@ __device__ void block_reduce(double (&v)[N], double* out) {
 template <int N>
 __device__ void reduce_parts(const double* part, int stride, int nparts, double* out) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (int j = w; j < N; j += kT / 64) {
+  for (int j = w; j < N; j += blockDim.x / 64) {
     double s = 0.0;
     for (int p = lane; p < nparts; p += 64) s += part[(int64_t)stride * p + j];
 #pragma unroll
@@ -223,9 +233,13 @@ __global__ __launch_bounds__(kT) void moments1_kernel(const double* __restrict__
     }
   }
   block_reduce<7>(v, part + 8 * blockIdx.x);
-  if (!arrive_last(&st->sync[3])) return;
+}
+
+// Totals of the first moments pass (one workgroup, fixed order).
+__global__ __launch_bounds__(kT) void means_kernel(const double* part, int nparts, LtsState* st) {
+  if (!st->active) return;
   __shared__ double s1[7];
-  reduce_parts<7>(part, 8, (int)gridDim.x, s1);
+  reduce_parts<7>(part, 8, nparts, s1);
   if (threadIdx.x < 7) st->s1[threadIdx.x] = s1[threadIdx.x];
 }
 
@@ -467,8 +481,7 @@ __device__ void solve_classic(LtsState* st, const Moments& m) {
 __global__ __launch_bounds__(kT) void moments2_kernel(const double* __restrict__ X,
                                                       const double* __restrict__ Y,
                                                       const double* __restrict__ r, int64_t n,
-                                                      LtsState* st, double* part2,
-                                                      unsigned int* hist1, unsigned int* hist2) {
+                                                      LtsState* st, double* part2) {
   if (!st->active) return;
   const double* s1 = st->s1;
   const double cnt = s1[0];
@@ -500,25 +513,24 @@ __global__ __launch_bounds__(kT) void moments2_kernel(const double* __restrict__
     }
   }
   block_reduce<21>(v, part2 + 24 * blockIdx.x);
-  if (!arrive_last(&st->sync[3])) return;
-  // The last workgroup: total moments and cleared histograms for the next
-  // iteration.
+}
+
+// Totals of the second moments pass, cleared histograms for the next
+// iteration, then the one-thread solve (one wave; its register footprint
+// stays out of the streaming kernels).
+__global__ __launch_bounds__(64) void solve_kernel(LtsState* st, const double* part2, int nparts,
+                                                   unsigned int* hist1, unsigned int* hist2) {
+  if (!st->active) return;
   __shared__ double s2[21];
-  reduce_parts<21>(part2, 24, (int)gridDim.x, s2);
-  for (int i = threadIdx.x; i < 4096; i += kT) {
+  reduce_parts<21>(part2, 24, nparts, s2);
+  for (int i = threadIdx.x; i < 4096; i += blockDim.x) {
     hist1[i] = 0;
     hist2[i] = 0;
   }
-  if (threadIdx.x < 21) st->s2[threadIdx.x] = s2[threadIdx.x];
-}
-
-// The one-thread solve (its own launch: its register footprint would
-// otherwise limit the occupancy of the streaming moments kernel).
-__global__ __launch_bounds__(64) void solve_kernel(LtsState* st) {
-  if (!st->active || threadIdx.x != 0) return;
+  if (threadIdx.x != 0) return;
   st->iter += 1;  // the loop head of the reference
   Moments m;
-  unpack(st->s1, st->s2, m);
+  unpack(st->s1, s2, m);
   if (st->symmetric)
     solve_symmetric(st, m);
   else
@@ -554,7 +566,6 @@ __global__ void init_kernel(LtsState* st, const OuterState* outer, int kind, int
   if (n > 0 && k > n - 1) k = n - 1;
   st->k = k;
   st->ncand = 0;
-  for (int i = 0; i < 4; ++i) st->sync[i] = 0u;
   st->thr = DBL_MAX;
   st->active = n > 0 ? 1 : 0;
 }
@@ -623,13 +634,17 @@ void lts_enqueue_iteration(const double* X, const double* Y, const uint8_t* kept
   hipLaunchKernelGGL(residual_kernel, dim3(grid), dim3(kT), 0, s, X, Y, kept, npts, st, b.r,
                      b.hist1, select ? 1 : 0);
   if (select) {
+    hipLaunchKernelGGL(scan1_kernel, dim3(1), dim3(kT), 0, s, b.hist1, st);
     hipLaunchKernelGGL(hist2_kernel, dim3(grid), dim3(kT), 0, s, b.r, npts, st, b.hist2);
+    hipLaunchKernelGGL(scan2_kernel, dim3(1), dim3(kT), 0, s, b.hist2, st);
     hipLaunchKernelGGL(compact_kernel, dim3(grid), dim3(kT), 0, s, b.r, npts, st, b.cand);
+    hipLaunchKernelGGL(final_select_kernel, dim3(1), dim3(kT), 0, s, b.cand, st);
   }
   hipLaunchKernelGGL(moments1_kernel, dim3(nparts), dim3(kT), 0, s, X, Y, b.r, npts, st, b.part1);
-  hipLaunchKernelGGL(moments2_kernel, dim3(nparts), dim3(kT), 0, s, X, Y, b.r, npts, st, b.part2,
-                     b.hist1, b.hist2);
-  hipLaunchKernelGGL(solve_kernel, dim3(1), dim3(64), 0, s, st);
+  hipLaunchKernelGGL(means_kernel, dim3(1), dim3(kT), 0, s, b.part1, nparts, st);
+  hipLaunchKernelGGL(moments2_kernel, dim3(nparts), dim3(kT), 0, s, X, Y, b.r, npts, st, b.part2);
+  hipLaunchKernelGGL(solve_kernel, dim3(1), dim3(64), 0, s, st, b.part2, nparts, b.hist1,
+                     b.hist2);
   MIRORR_CHECK_LAUNCH();
 }
 

@@ -21,10 +21,8 @@ struct LtsState {
   unsigned long long k2, k3;  // remaining ranks after the 1st and 2nd digit
   unsigned int bin1, bin2;
   unsigned int ncand;
-  unsigned int sync[4];       // arrival counters of the last-workgroup reductions
   double thr;
   double s1[8];               // trimmed count and coordinate sums (moments pass 1)
-  double s2[21];              // centred second moments (moments pass 2)
 };
 
 // Device state of the outer iteration loop of one level.

@@ -120,7 +120,7 @@ struct Workspace {
   DeviceBuffer statsF, statsM;                // window statistics
   DeviceBuffer xtF, xtM, sxxF, sxxM, sumF, sumM, flF, flM;  // block preparation
   DeviceBuffer best, second, match;           // raw BM results (one pass)
-  DeviceBuffer queue, qcount;
+  DeviceBuffer queue, qbits, qcount;
   DeviceBuffer X, Y, kept;                    // LTS pairs
   DeviceBuffer r, hist1, hist2, cand, part1, part2;
   DeviceBuffer lts, outer, mapI;
@@ -188,7 +188,8 @@ struct RegRun {
     BMResult res{w.best.as<float>(), w.second.as<float>(), w.match.as<int32_t>()};
     if (tiled_supported(g)) {
       BlockPrep bp{xt.as<float>(), sxx.as<float>(), sum.as<float>(), fl.as<uint32_t>()};
-      BMScratch scr{w.queue.as<int32_t>(), w.qcount.as<int32_t>(), tile_count(g), &ctx->prof};
+      BMScratch scr{w.queue.as<int32_t>(), w.qcount.as<int32_t>(),
+                    w.qbits.as<unsigned long long>(), tile_count(g), &ctx->prof};
       launch_ncc_search(base, bmask, search, smask, g, bp, SearchStats{stats_search.as<float2>()},
                         scr, res, s);
     } else {
@@ -229,6 +230,7 @@ struct RegRun {
     w.second.ensure(sizeof(float) * nb, bud);
     w.match.ensure(sizeof(int32_t) * 3 * nb, bud);
     w.queue.ensure(sizeof(int32_t) * std::max<int64_t>(tile_count(g), 1), bud);
+    w.qbits.ensure(sizeof(unsigned long long) * std::max<int64_t>(tile_count(g), 1), bud);
     w.qcount.ensure(sizeof(int32_t), bud);
     const int64_t np = 2 * nb;
     w.X.ensure(sizeof(double) * 3 * np, bud);

@@ -1,4 +1,4 @@
-// Device helpers for cross-workgroup reductions and radix selection, shared
+// Device helpers for radix selection (workgroup scan and bin search), shared
 // by the LTS and the block-variance selection. Workgroups have kSelT threads.
 #pragma once
 
@@ -8,22 +8,6 @@ namespace mirorr {
 namespace sel {
 
 constexpr int kSelT = 256;
-
-// Every workgroup publishes its results (release), then counts itself in;
-// the one that arrives last sees all results (acquire) and finishes the
-// reduction in the same launch. It also re-arms the counter. Every thread
-// of every workgroup must call it.
-__device__ inline bool arrive_last(unsigned int* counter) {
-  __shared__ int s_last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) s_last = atomicAdd(counter, 1u) == gridDim.x - 1 ? 1 : 0;
-  __syncthreads();
-  if (!s_last) return false;
-  __threadfence();
-  if (threadIdx.x == 0) *counter = 0u;
-  return true;
-}
 
 // Inclusive scan over the kSelT threads of a workgroup.
 __device__ inline unsigned long long block_scan(unsigned long long v) {

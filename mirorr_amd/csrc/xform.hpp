@@ -82,36 +82,43 @@ XF_HD M3 inverse(const M3& m) {
   r.a[2][2] = (a[0][0] * a[1][1] - a[0][1] * a[1][0]) / d;
   return r;
 }
-// Gauss-Jordan with partial pivoting.
+// Closed-form inverse by 2x2 minors (Laplace expansion along the first two
+// rows): no pivoting and only fixed indices, so a device thread keeps it in
+// registers. The matrices inverted here (homogeneous transforms and their
+// Denman-Beavers iterates) are well conditioned.
 XF_HD M4 inverse(const M4& m) {
-  double w[4][8];
-  for (int i = 0; i < 4; ++i)
-    for (int j = 0; j < 4; ++j) {
-      w[i][j] = m.a[i][j];
-      w[i][j + 4] = i == j ? 1.0 : 0.0;
-    }
-  for (int c = 0; c < 4; ++c) {
-    int p = c;
-    for (int r = c + 1; r < 4; ++r)
-      if (fabs(w[r][c]) > fabs(w[p][c])) p = r;
-    if (p != c)
-      for (int j = 0; j < 8; ++j) {
-        const double t = w[p][j];
-        w[p][j] = w[c][j];
-        w[c][j] = t;
-      }
-    const double d = w[c][c];
-    for (int j = 0; j < 8; ++j) w[c][j] /= d;
-    for (int r = 0; r < 4; ++r) {
-      if (r == c) continue;
-      const double f = w[r][c];
-      if (f == 0.0) continue;
-      for (int j = 0; j < 8; ++j) w[r][j] -= f * w[c][j];
-    }
-  }
+  const auto& a = m.a;
+  const double s0 = a[0][0] * a[1][1] - a[1][0] * a[0][1];
+  const double s1 = a[0][0] * a[1][2] - a[1][0] * a[0][2];
+  const double s2 = a[0][0] * a[1][3] - a[1][0] * a[0][3];
+  const double s3 = a[0][1] * a[1][2] - a[1][1] * a[0][2];
+  const double s4 = a[0][1] * a[1][3] - a[1][1] * a[0][3];
+  const double s5 = a[0][2] * a[1][3] - a[1][2] * a[0][3];
+  const double c5 = a[2][2] * a[3][3] - a[3][2] * a[2][3];
+  const double c4 = a[2][1] * a[3][3] - a[3][1] * a[2][3];
+  const double c3 = a[2][1] * a[3][2] - a[3][1] * a[2][2];
+  const double c2 = a[2][0] * a[3][3] - a[3][0] * a[2][3];
+  const double c1 = a[2][0] * a[3][2] - a[3][0] * a[2][2];
+  const double c0 = a[2][0] * a[3][1] - a[3][0] * a[2][1];
+  const double det = s0 * c5 - s1 * c4 + s2 * c3 + s3 * c2 - s4 * c1 + s5 * c0;
+  const double id = 1.0 / det;
   M4 r;
-  for (int i = 0; i < 4; ++i)
-    for (int j = 0; j < 4; ++j) r.a[i][j] = w[i][j + 4];
+  r.a[0][0] = (a[1][1] * c5 - a[1][2] * c4 + a[1][3] * c3) * id;
+  r.a[0][1] = (-a[0][1] * c5 + a[0][2] * c4 - a[0][3] * c3) * id;
+  r.a[0][2] = (a[3][1] * s5 - a[3][2] * s4 + a[3][3] * s3) * id;
+  r.a[0][3] = (-a[2][1] * s5 + a[2][2] * s4 - a[2][3] * s3) * id;
+  r.a[1][0] = (-a[1][0] * c5 + a[1][2] * c2 - a[1][3] * c1) * id;
+  r.a[1][1] = (a[0][0] * c5 - a[0][2] * c2 + a[0][3] * c1) * id;
+  r.a[1][2] = (-a[3][0] * s5 + a[3][2] * s2 - a[3][3] * s1) * id;
+  r.a[1][3] = (a[2][0] * s5 - a[2][2] * s2 + a[2][3] * s1) * id;
+  r.a[2][0] = (a[1][0] * c4 - a[1][1] * c2 + a[1][3] * c0) * id;
+  r.a[2][1] = (-a[0][0] * c4 + a[0][1] * c2 - a[0][3] * c0) * id;
+  r.a[2][2] = (a[3][0] * s4 - a[3][1] * s2 + a[3][3] * s0) * id;
+  r.a[2][3] = (-a[2][0] * s4 + a[2][1] * s2 - a[2][3] * s0) * id;
+  r.a[3][0] = (-a[1][0] * c3 + a[1][1] * c1 - a[1][2] * c0) * id;
+  r.a[3][1] = (a[0][0] * c3 - a[0][1] * c1 + a[0][2] * c0) * id;
+  r.a[3][2] = (-a[3][0] * s3 + a[3][1] * s1 - a[3][2] * s0) * id;
+  r.a[3][3] = (a[2][0] * s3 - a[2][1] * s1 + a[2][2] * s0) * id;
   return r;
 }
 
@@ -148,25 +155,32 @@ XF_HD void eigen_sym(const double (&A)[N][N], double (&ev)[N], double (&V)[N][N]
     }
   for (int sweep = 0; sweep < 64; ++sweep) {
     double off = 0.0;
+#pragma unroll
     for (int p = 0; p < N; ++p)
+#pragma unroll
       for (int q = p + 1; q < N; ++q) off += a[p][q] * a[p][q];
     if (off < 1e-300) break;
+#pragma unroll
     for (int p = 0; p < N; ++p)
+#pragma unroll
       for (int q = p + 1; q < N; ++q) {
         if (a[p][q] == 0.0) continue;
         const double th = (a[q][q] - a[p][p]) / (2.0 * a[p][q]);
         const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
         const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+#pragma unroll
         for (int k = 0; k < N; ++k) {
           const double x = a[k][p], y = a[k][q];
           a[k][p] = c * x - s * y;
           a[k][q] = s * x + c * y;
         }
+#pragma unroll
         for (int k = 0; k < N; ++k) {
           const double x = a[p][k], y = a[q][k];
           a[p][k] = c * x - s * y;
           a[q][k] = s * x + c * y;
         }
+#pragma unroll
         for (int k = 0; k < N; ++k) {
           const double x = V[k][p], y = V[k][q];
           V[k][p] = c * x - s * y;
@@ -174,23 +188,26 @@ XF_HD void eigen_sym(const double (&A)[N][N], double (&ev)[N], double (&V)[N][N]
         }
       }
   }
+#pragma unroll
   for (int i = 0; i < N; ++i) ev[i] = a[i][i];
-  // selection sort, ascending
-  for (int i = 0; i < N; ++i) {
-    int m = i;
-    for (int j = i + 1; j < N; ++j)
-      if (ev[j] < ev[m]) m = j;
-    if (m != i) {
-      const double t = ev[i];
-      ev[i] = ev[m];
-      ev[m] = t;
-      for (int k = 0; k < N; ++k) {
-        const double u = V[k][i];
-        V[k][i] = V[k][m];
-        V[k][m] = u;
+  // Ascending order by compare-exchange on fixed indices (the order of
+  // distinct eigenvalues and their vectors is what any sort gives).
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = i + 1; j < N; ++j) {
+      if (ev[j] < ev[i]) {
+        const double t = ev[i];
+        ev[i] = ev[j];
+        ev[j] = t;
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+          const double u = V[k][i];
+          V[k][i] = V[k][j];
+          V[k][j] = u;
+        }
       }
     }
-  }
 }
 
 // Orthogonal polar factor U V^T of a non-singular m, as m (m^T m)^(-1/2).
