@@ -224,6 +224,22 @@ __device__ __forceinline__ void top2_push(Top2& t, float s, int scan) {
   }
 }
 
+// The record of a candidate sequence followed by a later one in scan order
+// (the same result as pushing b's candidates after a's).
+__device__ __forceinline__ Top2 top2_append(const Top2& a, const Top2& b) {
+  Top2 r;
+  if (b.best > a.best) {
+    r.best = b.best;
+    r.scan = b.scan;
+    r.second = fmaxf(b.second, a.best);
+  } else {
+    r.best = a.best;
+    r.scan = a.scan;
+    r.second = fmaxf(a.second, b.best);
+  }
+  return r;
+}
+
 // Order-independent merge of per-lane records of one block: best = max
 // score, earliest scan index on ties; second = max of all seconds and of the
 // bests that did not win (equal bests make best == second, as in the
@@ -314,6 +330,17 @@ __device__ unsigned long long tile_evals(const TileArgs& a, int px, int py0, int
     tot += (unsigned long long)cx * cnt(py, g.dims[1]) * cnt(pz, g.dims[2]);
   }
   return tot * 64ull;
+}
+
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup i runs on XCD
+// i % 8, each XCD with its own L2). Renumbering them so that each XCD walks a
+// contiguous run of tiles keeps neighbouring tiles, which re-read the same
+// window rows and statistics, in one L2. A bijection for any tile count.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_tile(int bid, int nt) {
+  const int q = nt / kXcds, r = nt % kXcds;
+  const int x = bid % kXcds;
+  return x * q + min(x, r) + bid / kXcds;
 }
 
 template <class T>
@@ -504,7 +531,8 @@ __global__ __launch_bounds__(kThreads, 4) void ncc_tile_fast(TileArgs a) {
   const int nx = g.dims[0], ny = g.dims[1], nz = g.dims[2];
   const int tid = threadIdx.x;
   int bx, by0, bz0;
-  tile_origin<T>(a, blockIdx.x, bx, by0, bz0);
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  tile_origin<T>(a, tile, bx, by0, bz0);
   const int NG = g.ng;
   const int WY = (T::TY - 1) * NG + T::WX + T::YPAD;
   const int WZ = (T::TZ - 1) * NG + T::WX;
@@ -563,7 +591,7 @@ __global__ __launch_bounds__(kThreads, 4) void ncc_tile_fast(TileArgs a) {
   if (todo) {
     if (tid == 0) {
       const int q = atomicAdd(a.qcount, 1);
-      a.queue[q] = blockIdx.x;
+      a.queue[q] = tile;
       a.qbits[q] = todo;
     }
     if (todo == s_valid) return;  // workgroup-uniform: nothing left here
@@ -739,23 +767,25 @@ __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
   __syncthreads();
 
   if (a.work && tid == 0) atomicAdd(a.work, tile_evals<T>(a, px, py0, pz0, bvalid));
-  Top2 rec[R];
-  int recb[R];
-#pragma unroll
-  for (int j = 0; j < R; ++j) {
-    top2_init(rec[j]);
-    recb[j] = 0;
-  }
+  // A thread owns R consecutive tasks (scan order within a block), so they
+  // span at most two blocks: one running record each, in registers.
+  static_assert(R <= T::NW * T::NW, "a thread's tasks span at most two blocks");
+  const int t0 = tid * R;
+  const int bl0 = t0 / (T::NW * T::NW);
+  Top2 rec0, rec1;
+  top2_init(rec0);
+  top2_init(rec1);
 #pragma unroll 1
   for (int j = 0; j < R; ++j) {
-    const int t = tid + j * kThreads;
+    const int t = t0 + j;
     if (t >= T::GT) break;
     const int bl = t / (T::NW * T::NW);
     const int r = t % (T::NW * T::NW);
     const int cz = r / T::NW;
     const int cy = r % T::NW;
-    recb[j] = bl;
     if (bvalid[bl] != 1) continue;
+    Top2 tr;
+    top2_init(tr);
     const int bly = bl % T::TY, blz = bl / T::TY;
     const int py = py0 + NG * bly, pz = pz0 + NG * blz;
     const int lox = max(px - T::H, 0), hix = min(px - T::H + T::NW - 1, nx - T::BW);
@@ -814,7 +844,7 @@ __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
       for (int cx = 0; cx < T::NW; ++cx) {
         const int cxg = px - T::H + cx;
         if (cxg < lox || cxg > hix) continue;
-        top2_push(rec[j], ncc_score(sxy[cx], sxx[cx], s3[cx]), (cz * T::NW + cy) * T::NW + cx);
+        top2_push(tr, ncc_score(sxy[cx], sxx[cx], s3[cx]), (cz * T::NW + cy) * T::NW + cx);
       }
     } else {
       // s3 = Sum b t^2 (Syy over both masks)
@@ -860,15 +890,20 @@ __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
       for (int cx = 0; cx < T::NW; ++cx) {
         const int cxg = px - T::H + cx;
         if (cxg < lox || cxg > hix) continue;
-        top2_push(rec[j], ncc_score(sxy[cx], sxx[cx], s3[cx]), (cz * T::NW + cy) * T::NW + cx);
+        top2_push(tr, ncc_score(sxy[cx], sxx[cx], s3[cx]), (cz * T::NW + cy) * T::NW + cx);
       }
     }
+    if (bl == bl0)
+      rec0 = top2_append(rec0, tr);
+    else
+      rec1 = top2_append(rec1, tr);
   }
-#pragma unroll
-  for (int j = 0; j < R; ++j) merge_phase1(mrg, recb[j], rec[j]);
+  // (threads past the last task hold empty records: no merge)
+  if (bl0 < T::NB) merge_phase1(mrg, bl0, rec0);
+  if (bl0 + 1 < T::NB) merge_phase1(mrg, bl0 + 1, rec1);
   __syncthreads();
-#pragma unroll
-  for (int j = 0; j < R; ++j) merge_phase2(mrg, recb[j], rec[j]);
+  if (bl0 < T::NB) merge_phase2(mrg, bl0, rec0);
+  if (bl0 + 1 < T::NB) merge_phase2(mrg, bl0 + 1, rec1);
   __syncthreads();
   if (tid < T::NB && bvalid[tid] == 1) {
     const int gy = by0 + tid % T::TY, gz = bz0 + tid / T::TY;
