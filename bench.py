@@ -108,8 +108,10 @@ def main():
     cfg = workload()
     pair = make_inputs(cfg)
     ctx = default_context()
+    t_up = time.perf_counter()
     fixed = DeviceVolume(pair.fixed, ctx)
     moving = DeviceVolume(pair.moving, ctx)
+    upload_ms = (time.perf_counter() - t_up) * 1e3  # host -> HBM, outside the timed region
     reg = MirorrPyramid(ctx)
     configure(reg, cfg)
     print(f"bench: warmup {args.warmup}", file=sys.stderr, flush=True)
@@ -160,6 +162,7 @@ def main():
         "latency_ms_p99": float(np.percentile(lat_ms, 99)),
         "voxel_evals_per_registration": evals / args.steps,
         "transform_error_mm": err_mm,
+        "upload_ms": upload_ms,
         "roofline": {
             "bound": "mfma",
             "compute": "fp32 vector ALU (the search uses no MFMA)",

@@ -16,7 +16,7 @@
 //    KY y offsets, all NW x offsets) and accumulates Sum x~ * y' with one FMA
 //    per voxel-eval, the centred block held in 64 VGPRs and rows of the window
 //    read as 16-byte LDS loads shared by 4*KY*NW FMAs. Sxy, Syy then follow
-//    from the window statistics. Blocks whose base mask or own search window
+//    from the window statistics (1/Syy stored). Blocks whose base mask or own search window
 //    is not all on are queued (tile + block bitmask) for the masked kernel
 //    (masked sums, 4-5 ops/voxel-eval); the rest of the tile stays here.
 //  * Top-2 per block uses the float score buffer semantics of the reference
@@ -69,7 +69,7 @@ __global__ __launch_bounds__(kThreads) void search_stats_kernel(
         ss += d * d;
       }
     }
-  out[lin3(nx, ny, x, yy, z)] = make_float2((float)mu, (float)ss);
+  out[lin3(nx, ny, x, yy, z)] = make_float2((float)mu, ss > 0.0 ? (float)(1.0 / ss) : -1.0f);
 }
 
 // BW = 4: a thread owns one (x, y) column of window positions and slides
@@ -110,17 +110,29 @@ __global__ __launch_bounds__(kThreads) void search_stats4_kernel(
   auto emit = [&](int z) {
     const double mu = (ps[0] + ps[1] + ps[2] + ps[3]) * (1.0 / 64.0);
     double ss = 0.0;
+    if (!MASK || (mk[0] & mk[1] & mk[2] & mk[3]) == 0xffffu) {  // every voxel on
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
+      for (int k = 0; k < 4; ++k)
 #pragma unroll
-      for (int j = 0; j < 16; ++j) {
-        float v = p[k][j];
-        asm volatile("" : "+v"(v));  // convert afresh: keeps 64 floats live, not 64 doubles
-        double d = (double)v - mu;
-        if (MASK && !((mk[k] >> j) & 1u)) d = 0.0;
-        ss = fma(d, d, ss);
-      }
-    out[lin3(nx, ny, x, yy, z)] = make_float2((float)mu, (float)ss);
+        for (int j = 0; j < 16; ++j) {
+          float v = p[k][j];
+          asm volatile("" : "+v"(v));  // convert afresh: keeps 64 floats live, not 64 doubles
+          const double d = (double)v - mu;
+          ss = fma(d, d, ss);
+        }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+          float v = p[k][j];
+          asm volatile("" : "+v"(v));
+          double d = (double)v - mu;
+          if (!((mk[k] >> j) & 1u)) d = 0.0;
+          ss = fma(d, d, ss);
+        }
+    }
+    out[lin3(nx, ny, x, yy, z)] = make_float2((float)mu, ss > 0.0 ? (float)(1.0 / ss) : -1.0f);
   };
   if (z0 >= zend) return;
   load(0, z0);
@@ -156,34 +168,35 @@ __device__ __forceinline__ int wave_sum_i(int v) {
   return v;
 }
 
+template <int BWT>
 __global__ __launch_bounds__(kThreads) void block_prep_kernel(
     const float* __restrict__ x, const uint8_t* __restrict__ bm, BMGrid g,
     float* __restrict__ xt, float* __restrict__ sxx, float* __restrict__ sumxt,
     uint32_t* __restrict__ flags) {
   const int lane = threadIdx.x & 63;
-  const int64_t b = (int64_t)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  const int b = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);  // nblocks < 2^31
   if (b >= g.nblocks) return;  // wave-uniform
-  const int bw = g.bw, n = bw * bw * bw;
-  const int bxi = (int)(b % g.nb[0]);
-  const int byi = (int)((b / g.nb[0]) % g.nb[1]);
-  const int bzi = (int)(b / ((int64_t)g.nb[0] * g.nb[1]));
+  const int bw = BWT > 0 ? BWT : g.bw, n = bw * bw * bw;
+  const int bxi = b % g.nb[0];
+  const int byi = (b / g.nb[0]) % g.nb[1];
+  const int bzi = b / (g.nb[0] * g.nb[1]);
   const int px = g.start[0] + g.ng * bxi;
   const int py = g.start[1] + g.ng * byi;
   const int pz = g.start[2] + g.ng * bzi;
-  double s = 0.0;
-  for (int k = lane; k < n; k += 64) {
+  auto voxel = [&](int k) {
     const int dx = k % bw, dy = (k / bw) % bw, dz = k / (bw * bw);
-    s += x[lin3(g.dims[0], g.dims[1], px + dx, py + dy, pz + dz)];
-  }
+    return lin3(g.dims[0], g.dims[1], px + dx, py + dy, pz + dz);
+  };
+  double s = 0.0;
+  for (int k = lane; k < n; k += 64) s += x[voxel(k)];
   const double mx = wave_sum(s) / (double)n;
   double s2 = 0.0, st = 0.0;
   int on = 0;
   for (int k = lane; k < n; k += 64) {
-    const int dx = k % bw, dy = (k / bw) % bw, dz = k / (bw * bw);
-    const int64_t o = lin3(g.dims[0], g.dims[1], px + dx, py + dy, pz + dz);
+    const int64_t o = voxel(k);
     const double d = (double)x[o] - mx;
     const float f = (float)d;
-    xt[b * n + k] = f;
+    xt[(int64_t)b * n + k] = f;
     s2 += d * d;
     st += (double)f;
     on += (bm == nullptr || bm[o] != 0) ? 1 : 0;
@@ -676,7 +689,7 @@ __global__ __launch_bounds__(kThreads, 4) void ncc_tile_fast(TileArgs a) {
         const int cxg = px - T::H + cx;
         const float2 st = srow[min(max(cxg, 0), nx - T::BW)];
         const float sxy = fmaf(-(st.x - ref), sumx, acc[ky][cx]);
-        float s = (xok && st.y > 0.0f) ? sxy * sxy * isxx * __builtin_amdgcn_rcpf(st.y) : -2.0f;
+        float s = (xok && st.y > 0.0f) ? sxy * sxy * isxx * st.y : -2.0f;
         s = (yok && cxg >= lox && cxg <= hix) ? s : -__builtin_huge_valf();
         top2_push(t2, s, (cz * T::NW + cy) * T::NW + cx);
       }
@@ -710,6 +723,8 @@ __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
   extern __shared__ float4 dyn4[];
   __shared__ float4 xh4[T::NB * 16];  // base-masked centred blocks
   __shared__ unsigned long long bbits[T::NB];  // base mask bit per voxel
+  __shared__ int todo_list[T::NB];
+  __shared__ int s_ntodo;
   __shared__ int bvalid[T::NB];
   __shared__ MergeSlot mrg[T::NB];
   __shared__ float s_ref;
@@ -743,6 +758,12 @@ __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
     bvalid[tid] = ((todo >> tid) & 1ull) ? 1 : 0;
     bbits[tid] = 0ull;
   }
+  if (tid == 0) {  // the queued blocks, densely numbered
+    int n = 0;
+    for (int bl = 0; bl < T::NB; ++bl)
+      if ((todo >> bl) & 1ull) todo_list[n++] = bl;
+    s_ntodo = n;
+  }
   __syncthreads();
   const float ref = s_ref;
   stage_window<T, true>(a, wy_, wv_, nullptr, WY, WZ, wx0, wy0, wz0, ref);
@@ -767,23 +788,27 @@ __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
   __syncthreads();
 
   if (a.work && tid == 0) atomicAdd(a.work, tile_evals<T>(a, px, py0, pz0, bvalid));
-  // A thread owns R consecutive tasks (scan order within a block), so they
-  // span at most two blocks: one running record each, in registers.
+  // Tasks (block, cz, cy) run over the queued blocks only. A thread owns rr
+  // consecutive tasks (scan order within a block), so they span at most two
+  // blocks: one running record each, in registers.
   static_assert(R <= T::NW * T::NW, "a thread's tasks span at most two blocks");
-  const int t0 = tid * R;
-  const int bl0 = t0 / (T::NW * T::NW);
+  const int ntodo = s_ntodo;
+  const int ntasks = ntodo * T::NW * T::NW;
+  const int rr = (ntasks + kThreads - 1) / kThreads;
+  const int t0 = tid * rr;
+  const int k0 = t0 / (T::NW * T::NW);
   Top2 rec0, rec1;
   top2_init(rec0);
   top2_init(rec1);
 #pragma unroll 1
-  for (int j = 0; j < R; ++j) {
+  for (int j = 0; j < rr; ++j) {
     const int t = t0 + j;
-    if (t >= T::GT) break;
-    const int bl = t / (T::NW * T::NW);
+    if (t >= ntasks) break;
+    const int k = t / (T::NW * T::NW);
+    const int bl = todo_list[k];
     const int r = t % (T::NW * T::NW);
     const int cz = r / T::NW;
     const int cy = r % T::NW;
-    if (bvalid[bl] != 1) continue;
     Top2 tr;
     top2_init(tr);
     const int bly = bl % T::TY, blz = bl / T::TY;
@@ -893,17 +918,17 @@ __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
         top2_push(tr, ncc_score(sxy[cx], sxx[cx], s3[cx]), (cz * T::NW + cy) * T::NW + cx);
       }
     }
-    if (bl == bl0)
+    if (k == k0)
       rec0 = top2_append(rec0, tr);
     else
       rec1 = top2_append(rec1, tr);
   }
   // (threads past the last task hold empty records: no merge)
-  if (bl0 < T::NB) merge_phase1(mrg, bl0, rec0);
-  if (bl0 + 1 < T::NB) merge_phase1(mrg, bl0 + 1, rec1);
+  if (k0 < ntodo) merge_phase1(mrg, todo_list[k0], rec0);
+  if (k0 + 1 < ntodo) merge_phase1(mrg, todo_list[k0 + 1], rec1);
   __syncthreads();
-  if (bl0 < T::NB) merge_phase2(mrg, bl0, rec0);
-  if (bl0 + 1 < T::NB) merge_phase2(mrg, bl0 + 1, rec1);
+  if (k0 < ntodo) merge_phase2(mrg, todo_list[k0], rec0);
+  if (k0 + 1 < ntodo) merge_phase2(mrg, todo_list[k0 + 1], rec1);
   __syncthreads();
   if (tid < T::NB && bvalid[tid] == 1) {
     const int gy = by0 + tid % T::TY, gz = bz0 + tid / T::TY;
@@ -1327,10 +1352,15 @@ bool launch_variance_select(const BMGrid& g, const BlockPrep& bp, double portion
 void launch_block_prep(const float* x, const uint8_t* bmask, const BMGrid& g,
                        BlockPrep bp, hipStream_t s) {
   if (g.nblocks == 0) return;
+  if (g.nblocks > (int64_t)INT_MAX / 2) throw Error(kErrArg, "block grid too large");
   const int waves = kThreads / 64;
   dim3 grid((unsigned)((g.nblocks + waves - 1) / waves));
-  hipLaunchKernelGGL(block_prep_kernel, grid, dim3(kThreads), 0, s, x, bmask, g, bp.xt, bp.sxx,
-                     bp.sumxt, bp.flags);
+  if (g.bw == 4)
+    hipLaunchKernelGGL(block_prep_kernel<4>, grid, dim3(kThreads), 0, s, x, bmask, g, bp.xt,
+                       bp.sxx, bp.sumxt, bp.flags);
+  else
+    hipLaunchKernelGGL(block_prep_kernel<0>, grid, dim3(kThreads), 0, s, x, bmask, g, bp.xt,
+                       bp.sxx, bp.sumxt, bp.flags);
   MIRORR_CHECK_LAUNCH();
 }
 

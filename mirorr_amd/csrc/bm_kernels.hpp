@@ -31,8 +31,9 @@ struct VarSelScratch {
 constexpr int kVarSelHistBins = 4096 + 4096 + 256;
 
 // Per-position window statistics of a search image (window = BW^3 voxels
-// starting at the position): x = mean over all voxels, y = sum over
-// search-mask-on voxels of (v - mean)^2. Computed in double, stored float.
+// starting at the position): x = mean over all voxels, y = 1 / (sum over
+// search-mask-on voxels of (v - mean)^2), or -1 when that sum is zero (the
+// score is then -2). Computed in double, stored float.
 struct SearchStats {
   float2* ms = nullptr;  // dims[0]*dims[1]*dims[2]
 };

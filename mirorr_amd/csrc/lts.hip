@@ -13,10 +13,10 @@
 //     bin of rank k, a second 12-bit digit inside it, a compaction of the
 //     candidates sharing the 24-bit prefix and a one-workgroup finish by rank
 //     counting or 8-bit radix passes;
-//  3. trimmed moments in two passes (barycentres, then the 21 centred second
-//     moments x x^T, y y^T, x y^T), block-reduced in double and totalled in
-//     a fixed order; the rigid quaternion matrix sum Q^T Q is assembled from
-//     those moments;
+//  3. trimmed moments in one pass (count, sums and the raw second moments
+//     x x^T, y y^T, x y^T), block-reduced in double, totalled in a fixed
+//     order and centred in the solve; the rigid quaternion matrix sum Q^T Q
+//     is assembled from those moments;
 //  4. a one-thread solve (normal equations / quaternion eigenvector / polar
 //     factor / Denman-Beavers roots) updating the device state.
 // Every kernel is predicated on the state's `active` flag so the host can
@@ -214,33 +214,50 @@ __device__ void reduce_parts(const double* part, int stride, int nparts, double*
   __syncthreads();
 }
 
-__global__ __launch_bounds__(kT) void moments1_kernel(const double* __restrict__ X,
-                                                      const double* __restrict__ Y,
-                                                      const double* __restrict__ r, int64_t n,
-                                                      LtsState* st, double* part) {
+// One pass over the trimmed pairs: count, coordinate sums and the raw
+// second moments x x^T (6), y y^T (6), x y^T (9), block-reduced in double.
+// They are centred in the solve (sum (x-mx)(x-mx)^T = sum x x^T - n mx mx^T):
+// coordinates are mm within a few hundred of the origin, so the subtraction
+// costs a few digits of the 16 FP64 carries.
+constexpr int kMoments = 28;   // values per workgroup partial
+constexpr int kPartStride = 32;
+
+__global__ __launch_bounds__(kT) void moments_kernel(const double* __restrict__ X,
+                                                     const double* __restrict__ Y,
+                                                     const double* __restrict__ r, int64_t n,
+                                                     LtsState* st, double* part) {
   if (!st->active) return;
   const double thr = st->fraction > 0.99999 ? DBL_MAX : st->thr;
-  double v[7] = {0, 0, 0, 0, 0, 0, 0};
+  double v[kMoments];
+#pragma unroll
+  for (int j = 0; j < kMoments; ++j) v[j] = 0.0;
   const int64_t base = (int64_t)blockIdx.x * kPart;
   for (int j = threadIdx.x; j < kPart; j += kT) {
     const int64_t i = base + j;
     if (i < n && r[i] <= thr) {
+      const double x[3] = {X[3 * i], X[3 * i + 1], X[3 * i + 2]};
+      const double y[3] = {Y[3 * i], Y[3 * i + 1], Y[3 * i + 2]};
       v[0] += 1.0;
+#pragma unroll
       for (int k = 0; k < 3; ++k) {
-        v[1 + k] += X[3 * i + k];
-        v[4 + k] += Y[3 * i + k];
+        v[1 + k] += x[k];
+        v[4 + k] += y[k];
       }
+      int q = 0;
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = a; b < 3; ++b, ++q) {
+          v[7 + q] += x[a] * x[b];
+          v[13 + q] += y[a] * y[b];
+        }
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) v[19 + 3 * a + b] += x[a] * y[b];
     }
   }
-  block_reduce<7>(v, part + 8 * blockIdx.x);
-}
-
-// Totals of the first moments pass (one workgroup, fixed order).
-__global__ __launch_bounds__(kT) void means_kernel(const double* part, int nparts, LtsState* st) {
-  if (!st->active) return;
-  __shared__ double s1[7];
-  reduce_parts<7>(part, 8, nparts, s1);
-  if (threadIdx.x < 7) st->s1[threadIdx.x] = s1[threadIdx.x];
+  block_reduce<kMoments>(v, part + kPartStride * blockIdx.x);
 }
 
 // ---- the one-thread solves -------------------------------------------------
@@ -477,60 +494,33 @@ __device__ void solve_classic(LtsState* st, const Moments& m) {
   set_params(st->F, neu);
 }
 
-// Second pass of the moments (centred on the trimmed barycentres).
-__global__ __launch_bounds__(kT) void moments2_kernel(const double* __restrict__ X,
-                                                      const double* __restrict__ Y,
-                                                      const double* __restrict__ r, int64_t n,
-                                                      LtsState* st, double* part2) {
-  if (!st->active) return;
-  const double* s1 = st->s1;
-  const double cnt = s1[0];
-  const double mx[3] = {s1[1] / cnt, s1[2] / cnt, s1[3] / cnt};
-  const double my[3] = {s1[4] / cnt, s1[5] / cnt, s1[6] / cnt};
-  const double thr = st->fraction > 0.99999 ? DBL_MAX : st->thr;
-  // xx (6), yy (6), xy (9)
-  double v[21];
-#pragma unroll
-  for (int j = 0; j < 21; ++j) v[j] = 0.0;
-  const int64_t base = (int64_t)blockIdx.x * kPart;
-  for (int j = threadIdx.x; j < kPart; j += kT) {
-    const int64_t i = base + j;
-    if (i < n && r[i] <= thr) {
-      const double x[3] = {X[3 * i] - mx[0], X[3 * i + 1] - mx[1], X[3 * i + 2] - mx[2]};
-      const double y[3] = {Y[3 * i] - my[0], Y[3 * i + 1] - my[1], Y[3 * i + 2] - my[2]};
-      int q = 0;
-#pragma unroll
-      for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int b = a; b < 3; ++b, ++q) {
-          v[q] += x[a] * x[b];
-          v[6 + q] += y[a] * y[b];
-        }
-#pragma unroll
-      for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int b = 0; b < 3; ++b) v[12 + 3 * a + b] += x[a] * y[b];
-    }
-  }
-  block_reduce<21>(v, part2 + 24 * blockIdx.x);
-}
-
-// Totals of the second moments pass, cleared histograms for the next
-// iteration, then the one-thread solve (one wave; its register footprint
-// stays out of the streaming kernels).
-__global__ __launch_bounds__(64) void solve_kernel(LtsState* st, const double* part2, int nparts,
+// Totals of the moments pass, cleared histograms for the next iteration,
+// then the one-thread solve (one wave; its register footprint stays out of
+// the streaming kernels).
+__global__ __launch_bounds__(64) void solve_kernel(LtsState* st, const double* part, int nparts,
                                                    unsigned int* hist1, unsigned int* hist2) {
   if (!st->active) return;
-  __shared__ double s2[21];
-  reduce_parts<21>(part2, 24, nparts, s2);
+  __shared__ double t[kMoments];
+  reduce_parts<kMoments>(part, kPartStride, nparts, t);
   for (int i = threadIdx.x; i < 4096; i += blockDim.x) {
     hist1[i] = 0;
     hist2[i] = 0;
   }
   if (threadIdx.x != 0) return;
   st->iter += 1;  // the loop head of the reference
+  // Centred second moments in the layout unpack() reads: xx (6), yy (6), xy (9).
+  double s2[21];
+  const double cnt = t[0];
+  int q = 0;
+  for (int a = 0; a < 3; ++a)
+    for (int b = a; b < 3; ++b, ++q) {
+      s2[q] = t[7 + q] - t[1 + a] * t[1 + b] / cnt;
+      s2[6 + q] = t[13 + q] - t[4 + a] * t[4 + b] / cnt;
+    }
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) s2[12 + 3 * a + b] = t[19 + 3 * a + b] - t[1 + a] * t[4 + b] / cnt;
   Moments m;
-  unpack(st->s1, s2, m);
+  unpack(t, s2, m);
   if (st->symmetric)
     solve_symmetric(st, m);
   else
@@ -640,10 +630,8 @@ void lts_enqueue_iteration(const double* X, const double* Y, const uint8_t* kept
     hipLaunchKernelGGL(compact_kernel, dim3(grid), dim3(kT), 0, s, b.r, npts, st, b.cand);
     hipLaunchKernelGGL(final_select_kernel, dim3(1), dim3(kT), 0, s, b.cand, st);
   }
-  hipLaunchKernelGGL(moments1_kernel, dim3(nparts), dim3(kT), 0, s, X, Y, b.r, npts, st, b.part1);
-  hipLaunchKernelGGL(means_kernel, dim3(1), dim3(kT), 0, s, b.part1, nparts, st);
-  hipLaunchKernelGGL(moments2_kernel, dim3(nparts), dim3(kT), 0, s, X, Y, b.r, npts, st, b.part2);
-  hipLaunchKernelGGL(solve_kernel, dim3(1), dim3(64), 0, s, st, b.part2, nparts, b.hist1,
+  hipLaunchKernelGGL(moments_kernel, dim3(nparts), dim3(kT), 0, s, X, Y, b.r, npts, st, b.part);
+  hipLaunchKernelGGL(solve_kernel, dim3(1), dim3(64), 0, s, st, b.part, nparts, b.hist1,
                      b.hist2);
   MIRORR_CHECK_LAUNCH();
 }

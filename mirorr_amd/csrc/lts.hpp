@@ -22,7 +22,6 @@ struct LtsState {
   unsigned int bin1, bin2;
   unsigned int ncand;
   double thr;
-  double s1[8];               // trimmed count and coordinate sums (moments pass 1)
 };
 
 // Device state of the outer iteration loop of one level.
@@ -46,8 +45,7 @@ struct LtsBuffers {
   unsigned int* hist1 = nullptr;      // 4096, zero between iterations
   unsigned int* hist2 = nullptr;      // 4096, zero between iterations
   unsigned long long* cand = nullptr; // npts candidate keys
-  double* part1 = nullptr;            // nparts x 8
-  double* part2 = nullptr;            // nparts x 24
+  double* part = nullptr;             // nparts x 32 moment partials
   int64_t nparts_cap = 0;
 };
 

@@ -122,7 +122,7 @@ struct Workspace {
   DeviceBuffer best, second, match;           // raw BM results (one pass)
   DeviceBuffer queue, qbits, qcount;
   DeviceBuffer X, Y, kept;                    // LTS pairs
-  DeviceBuffer r, hist1, hist2, cand, part1, part2;
+  DeviceBuffer r, hist1, hist2, cand, part;
   DeviceBuffer lts, outer, mapI;
   DeviceBuffer pyr_tmp, pyr_map;
   DeviceBuffer vsel_st, vsel_hist, vsel_evals;  // block-variance selection
@@ -239,8 +239,7 @@ struct RegRun {
     w.r.ensure(sizeof(double) * np, bud);
     w.cand.ensure(sizeof(unsigned long long) * np, bud);
     const int64_t parts = lts_parts(np);
-    w.part1.ensure(sizeof(double) * 8 * parts, bud);
-    w.part2.ensure(sizeof(double) * 24 * parts, bud);
+    w.part.ensure(sizeof(double) * 32 * parts, bud);
     w.hist1.ensure(sizeof(unsigned int) * 4096, bud);
     w.hist2.ensure(sizeof(unsigned int) * 4096, bud);
     w.vsel_st.ensure(sizeof(VarSelState), bud);
@@ -254,8 +253,7 @@ struct RegRun {
     lb.hist1 = w.hist1.as<unsigned int>();
     lb.hist2 = w.hist2.as<unsigned int>();
     lb.cand = w.cand.as<unsigned long long>();
-    lb.part1 = w.part1.as<double>();
-    lb.part2 = w.part2.as<double>();
+    lb.part = w.part.as<double>();
     lb.nparts_cap = lts_parts(npts);
     lts_start(st, outer, kind, symmetric ? 1 : 0, lb, s);
     const int chunk = 4;
@@ -604,8 +602,7 @@ mirorr_status mirorr_lts_fit(mirorr_ctx* c, int symmetric, int affine, const dou
     w.kept.ensure(std::max<int64_t>(np, 1), bud);
     w.r.ensure(sizeof(double) * std::max<int64_t>(np, 1), bud);
     w.cand.ensure(sizeof(unsigned long long) * std::max<int64_t>(np, 1), bud);
-    w.part1.ensure(sizeof(double) * 8 * std::max<int64_t>(lts_parts(np), 1), bud);
-    w.part2.ensure(sizeof(double) * 24 * std::max<int64_t>(lts_parts(np), 1), bud);
+    w.part.ensure(sizeof(double) * 32 * std::max<int64_t>(lts_parts(np), 1), bud);
     w.hist1.ensure(sizeof(unsigned int) * 4096, bud);
     w.hist2.ensure(sizeof(unsigned int) * 4096, bud);
     w.lts.ensure(sizeof(LtsState), bud);
