@@ -30,15 +30,29 @@ PEAK_HBM_GBS = 8000.0
 FLOP_PER_VOXEL_EVAL = 6       # two-pass NCC: mean add, centring, 2 FMAs
 
 
-def workload():
-    return dict(workload="pair256_affine_symmetric", dims=[256, 256, 128], spacing_mm=1.0,
-                ellipsoids=60, model="affine", mode="symmetric", block_width=4,
-                nhood_width=9, nhood_gap=3, block_gap=1, pyramid_levels=3,
-                iterations_per_level=5, seed=2)
+WORKLOADS = {
+    # The measured workload (default).
+    "pair256_affine_symmetric": dict(
+        workload="pair256_affine_symmetric", dims=[256, 256, 128], spacing_mm=1.0,
+        ellipsoids=60, model="affine", mode="symmetric", block_width=4, nhood_width=9,
+        nhood_gap=3, block_gap=1, pyramid_levels=3, iterations_per_level=5, seed=2),
+    # CT-like int16 / MR-like uint16 rigid pair (float32 on the device).
+    "pair512_ctmr_rigid_symmetric": dict(
+        workload="pair512_ctmr_rigid_symmetric", dims=[512, 512, 256], spacing_mm=1.0,
+        ellipsoids=24, model="rigid", mode="symmetric", block_width=4, nhood_width=13,
+        nhood_gap=3, block_gap=1, pyramid_levels=4, iterations_per_level=5, seed=3),
+}
+
+
+def workload(name="pair256_affine_symmetric"):
+    return dict(WORKLOADS[name])
 
 
 def make_inputs(cfg):
     from mirorr_amd import phantom
+    if cfg["workload"].startswith("pair512_ctmr"):
+        return phantom.make_ct_mr_pair(dims=tuple(cfg["dims"]), spacing=(cfg["spacing_mm"],) * 3,
+                                       n_ellipsoids=cfg["ellipsoids"], seed=cfg["seed"])
     return phantom.make_pair(dims=tuple(cfg["dims"]), spacing=(cfg["spacing_mm"],) * 3,
                              n_ellipsoids=cfg["ellipsoids"], seed=cfg["seed"], model=cfg["model"])
 
@@ -46,6 +60,7 @@ def make_inputs(cfg):
 def configure(reg, cfg):
     reg.SetRegistrationTypeSymmetrical()
     reg.SetTransformType(cfg["model"])
+    reg.GetPyramidScheduleTuner().SetMinLength(32)
     reg.SetBlockWidth(cfg["block_width"])
     reg.SetBlockGap(cfg["block_gap"])
     reg.SetNhoodWidth(cfg["nhood_width"])
@@ -70,22 +85,30 @@ def measured_traffic():
         return None
 
 
-def cpu_baseline(pair, cfg, threads):
-    """Oracle threaded block matcher, one full-resolution pass (moving blocks
-    searched in the fixed image), timed on the host cores."""
+def cpu_baseline(pair, cfg, threads, max_evals=20e9):
+    """Oracle threaded block matcher on one full-resolution block-matching
+    pass (moving blocks searched in the fixed image), timed on the host
+    cores. A pass larger than max_evals is timed on a centred sub-volume
+    (half of each axis) of the same images."""
     from oracle import oracle as orc
     if not os.path.exists(orc._LIB_PATH):
         orc.build()
-    g = orc.geom_of(pair.moving)
-    evals = orc.candidate_count(g.dims, cfg["block_width"], cfg["block_gap"], cfg["nhood_width"],
-                                cfg["nhood_gap"]) * cfg["block_width"] ** 3
+    mov, fix = pair.moving.data, pair.fixed.data
+    bw, bg, nw, ng = cfg["block_width"], cfg["block_gap"], cfg["nhood_width"], cfg["nhood_gap"]
+    what = "full-resolution"
+    evals = orc.candidate_count(mov.shape[::-1], bw, bg, nw, ng) * bw ** 3
+    if evals > max_evals:
+        sl = tuple(slice(n // 4, n // 4 + n // 2) for n in mov.shape)
+        mov, fix = np.ascontiguousarray(mov[sl]), np.ascontiguousarray(fix[sl])
+        evals = orc.candidate_count(mov.shape[::-1], bw, bg, nw, ng) * bw ** 3
+        what = "centred half-size sub-volume of the full-resolution"
+    g = orc.Geom(dims=mov.shape[::-1], spacing=pair.moving.spacing, origin=pair.moving.origin)
     t = time.perf_counter()
-    orc.bm_get_offsets(pair.moving.data, None, pair.fixed.data, None, g, bw=cfg["block_width"],
-                       bg=cfg["block_gap"], nw=cfg["nhood_width"], ng=cfg["nhood_gap"],
-                       nthreads=threads)
+    orc.bm_get_offsets(mov, None, fix, None, g, bw=bw, bg=bg, nw=nw, ng=ng, nthreads=threads)
     dt = time.perf_counter() - t
+    dims = "x".join(str(d) for d in mov.shape[::-1])
     return {"value": evals / dt, "unit": "voxel-evals/s", "cores": threads, "kind": "port",
-            "sample": f"one full-resolution 256x256x128 block-matching pass "
+            "sample": f"one {what} block-matching pass, {dims} voxels "
                       f"({evals / 1e9:.2f} G voxel-evals, {dt:.1f} s)"}
 
 
@@ -95,6 +118,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default="pair256_affine_symmetric", choices=sorted(WORKLOADS))
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported "
@@ -105,7 +129,7 @@ def main():
     from mirorr_amd.blockmatcher import default_context
     from mirorr_amd.registration import DeviceVolume, MirorrPyramid
 
-    cfg = workload()
+    cfg = workload(args.workload)
     pair = make_inputs(cfg)
     ctx = default_context()
     t_up = time.perf_counter()
@@ -137,7 +161,7 @@ def main():
     ctx.set_profiling(False)
 
     from mirorr_amd import transforms
-    M, off = transforms.matrix_offset(params, center, True)
+    M, off = transforms.matrix_offset(params, center, cfg["model"] == "affine")
     err_mm = transforms.max_corner_distance(M, off, pair.matrix, pair.translation, pair.fixed)
 
     avg_ms = ms_fast / max(n_fast, 1)

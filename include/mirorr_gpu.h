@@ -174,6 +174,13 @@ typedef struct mirorr_reg_params {
   int variance_select;/* 0: reference behaviour                                  */
   int verbosity;
   int centre_init;    /* centred initial transform (1) unless --no-centering-init */
+  /* Early-stop thresholds, 1e-3 in the reference: the outer loop stops when
+   * the squared parameter change falls below stop_tolerance
+   * (itkSymMirorrRegistrationMethod.txx:456-692); the LTS inner loop runs
+   * while the change and its change exceed lts_tolerance
+   * (itkMirorrRegistrationMethod.txx:785-786). 0 disables early stopping. */
+  double stop_tolerance;
+  double lts_tolerance;
 } mirorr_reg_params;
 
 typedef struct mirorr_reg_stats {

@@ -57,7 +57,8 @@ class RegParams(C.Structure):
                 ("block_gap", C.c_int), ("nhood_width", C.c_int), ("nhood_gap", C.c_int),
                 ("iterations", C.c_int), ("pyr_start", C.c_int), ("pyr_end", C.c_int),
                 ("pyr_num", C.c_int), ("pyr_min_size", C.c_int), ("portion_kept", C.c_double),
-                ("variance_select", C.c_int), ("verbosity", C.c_int), ("centre_init", C.c_int)]
+                ("variance_select", C.c_int), ("verbosity", C.c_int), ("centre_init", C.c_int),
+                ("stop_tolerance", C.c_double), ("lts_tolerance", C.c_double)]
 
 
 class RegStats(C.Structure):

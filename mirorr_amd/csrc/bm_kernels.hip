@@ -110,28 +110,16 @@ __global__ __launch_bounds__(kThreads) void search_stats4_kernel(
   auto emit = [&](int z) {
     const double mu = (ps[0] + ps[1] + ps[2] + ps[3]) * (1.0 / 64.0);
     double ss = 0.0;
-    if (!MASK || (mk[0] & mk[1] & mk[2] & mk[3]) == 0xffffu) {  // every voxel on
 #pragma unroll
-      for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < 4; ++k)
 #pragma unroll
-        for (int j = 0; j < 16; ++j) {
-          float v = p[k][j];
-          asm volatile("" : "+v"(v));  // convert afresh: keeps 64 floats live, not 64 doubles
-          const double d = (double)v - mu;
-          ss = fma(d, d, ss);
-        }
-    } else {
-#pragma unroll
-      for (int k = 0; k < 4; ++k)
-#pragma unroll
-        for (int j = 0; j < 16; ++j) {
-          float v = p[k][j];
-          asm volatile("" : "+v"(v));
-          double d = (double)v - mu;
-          if (!((mk[k] >> j) & 1u)) d = 0.0;
-          ss = fma(d, d, ss);
-        }
-    }
+      for (int j = 0; j < 16; ++j) {
+        float v = p[k][j];
+        asm volatile("" : "+v"(v));  // convert afresh: keeps 64 floats live, not 64 doubles
+        double d = (double)v - mu;
+        if (MASK && !((mk[k] >> j) & 1u)) d = 0.0;
+        ss = fma(d, d, ss);
+      }
     out[lin3(nx, ny, x, yy, z)] = make_float2((float)mu, ss > 0.0 ? (float)(1.0 / ss) : -1.0f);
   };
   if (z0 >= zend) return;

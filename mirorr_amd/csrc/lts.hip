@@ -341,7 +341,10 @@ __device__ M3 top3(const M4& m) {
 }
 
 // Symmetric update (itkSymMirorrRegistrationMethod.txx:866-1379).
-__device__ void solve_symmetric(LtsState* st, const Moments& m) {
+// Run by lanes 0 and 1 of one wave on identical inputs: they compute the
+// same values, except that the two Denman-Beavers roots run one per lane;
+// lane 0 writes the state.
+__device__ void solve_symmetric(LtsState* st, const Moments& m, int lane) {
   const int kind = st->kind, np = kind == kRigid ? 6 : 12;
   double neu[12], neuF[12], neuB[12];
   for (int i = 0; i < np; ++i) neu[i] = neuF[i] = neuB[i] = st->last[i];
@@ -433,8 +436,13 @@ __device__ void solve_symmetric(LtsState* st, const Moments& m) {
     K = orthogonalize(K);
     K2 = orthogonalize(K2);
   }
-  const M4 m12 = sqrt_db(to4(K, tF), 1e-14, 50);
-  const M4 im12 = sqrt_db(to4(K2, tB), 1e-14, 50);
+  __shared__ M4 roots[2];
+  roots[lane] = sqrt_db(lane == 0 ? to4(K, tF) : to4(K2, tB), 1e-14, 50);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  const M4 m12 = roots[0];
+  const M4 im12 = roots[1];
   M3 A = top3(m12), Bm = top3(im12);
   Xf sF = fF, sB = fB;
   if (!set_matrix(sF, A) || !set_matrix(sB, Bm)) {
@@ -445,12 +453,13 @@ __device__ void solve_symmetric(LtsState* st, const Moments& m) {
     sB = fB;
     set_matrix(sF, A);
     set_matrix(sB, Bm);
-    st->status |= 1;
+    if (lane == 0) st->status |= 1;
   }
   const double t1[3] = {m12.a[0][3], m12.a[1][3], m12.a[2][3]};
   const double t2[3] = {im12.a[0][3], im12.a[1][3], im12.a[2][3]};
   set_translation(sF, t1);
   set_translation(sB, t2);
+  if (lane != 0) return;
   st->F = sF;
   st->B = sB;
   st->last_delta = st->delta;
@@ -494,10 +503,10 @@ __device__ void solve_classic(LtsState* st, const Moments& m) {
   set_params(st->F, neu);
 }
 
-// Totals of the moments pass, cleared histograms for the next iteration,
-// then the one-thread solve (one wave; its register footprint stays out of
-// the streaming kernels).
-__global__ __launch_bounds__(64) void solve_kernel(LtsState* st, const double* part, int nparts,
+// Totals of the moments pass (four waves), cleared histograms for the next
+// iteration, then the solve on lanes 0 and 1 (its register footprint stays
+// out of the streaming kernels).
+__global__ __launch_bounds__(kT) void solve_kernel(LtsState* st, const double* part, int nparts,
                                                    unsigned int* hist1, unsigned int* hist2) {
   if (!st->active) return;
   __shared__ double t[kMoments];
@@ -506,8 +515,8 @@ __global__ __launch_bounds__(64) void solve_kernel(LtsState* st, const double* p
     hist1[i] = 0;
     hist2[i] = 0;
   }
-  if (threadIdx.x != 0) return;
-  st->iter += 1;  // the loop head of the reference
+  const int lane = threadIdx.x;
+  if (lane >= 2 || (lane == 1 && !st->symmetric)) return;
   // Centred second moments in the layout unpack() reads: xx (6), yy (6), xy (9).
   double s2[21];
   const double cnt = t[0];
@@ -522,16 +531,18 @@ __global__ __launch_bounds__(64) void solve_kernel(LtsState* st, const double* p
   Moments m;
   unpack(t, s2, m);
   if (st->symmetric)
-    solve_symmetric(st, m);
+    solve_symmetric(st, m, lane);
   else
     solve_classic(st, m);
+  if (lane != 0) return;
+  st->iter += 1;  // the loop head of the reference
   st->fraction = 0.5;
-  st->active = (st->delta > 1e-3 && st->iter < 100 && fabs(st->delta - st->last_delta) > 1e-3) ? 1 : 0;
+  st->active = (st->delta > st->tol && st->iter < 100 && fabs(st->delta - st->last_delta) > st->tol) ? 1 : 0;
 }
 
 
 __global__ void init_kernel(LtsState* st, const OuterState* outer, int kind, int symmetric,
-                            unsigned int* hist1, unsigned int* hist2) {
+                            double tol, unsigned int* hist1, unsigned int* hist2) {
   // Runs with one workgroup; thread 0 writes the state.
   for (int i = threadIdx.x; i < 4096; i += blockDim.x) {
     hist1[i] = 0;
@@ -540,6 +551,7 @@ __global__ void init_kernel(LtsState* st, const OuterState* outer, int kind, int
   if (threadIdx.x != 0) return;
   st->kind = kind;
   st->symmetric = symmetric;
+  st->tol = tol;
   const double* c = outer->T.c;
   st->F = identity(kind);
   set_center(st->F, c);
@@ -600,7 +612,7 @@ __global__ __launch_bounds__(64) void finish_kernel(LtsState* st, OuterState* o)
   const int np = kind == kRigid ? 6 : 12;
   o->sqdiff = delta_of(o->lastp, cur, np);
   for (int i = 0; i < np; ++i) o->lastp[i] = cur[i];
-  o->stop = o->sqdiff < 1e-3 ? 1 : 0;
+  o->stop = o->sqdiff < o->stop_tol ? 1 : 0;
   o->inner_total += st->iter + 1;
   prepare_maps(o, st->symmetric);
 }
@@ -631,15 +643,15 @@ void lts_enqueue_iteration(const double* X, const double* Y, const uint8_t* kept
     hipLaunchKernelGGL(final_select_kernel, dim3(1), dim3(kT), 0, s, b.cand, st);
   }
   hipLaunchKernelGGL(moments_kernel, dim3(nparts), dim3(kT), 0, s, X, Y, b.r, npts, st, b.part);
-  hipLaunchKernelGGL(solve_kernel, dim3(1), dim3(64), 0, s, st, b.part, nparts, b.hist1,
+  hipLaunchKernelGGL(solve_kernel, dim3(1), dim3(kT), 0, s, st, b.part, nparts, b.hist1,
                      b.hist2);
   MIRORR_CHECK_LAUNCH();
 }
 
-void lts_start(LtsState* st, const OuterState* outer, int kind, int symmetric, LtsBuffers& b,
-               hipStream_t s) {
-  hipLaunchKernelGGL(init_kernel, dim3(1), dim3(kT), 0, s, st, outer, kind, symmetric, b.hist1,
-                     b.hist2);
+void lts_start(LtsState* st, const OuterState* outer, int kind, int symmetric, double tol,
+               LtsBuffers& b, hipStream_t s) {
+  hipLaunchKernelGGL(init_kernel, dim3(1), dim3(kT), 0, s, st, outer, kind, symmetric, tol,
+                     b.hist1, b.hist2);
   MIRORR_CHECK_LAUNCH();
 }
 

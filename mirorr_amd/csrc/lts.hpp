@@ -13,6 +13,7 @@ struct LtsState {
   double rc[3];      // rotation centre (classic)
   double last[12];   // last_parameters
   double fraction, delta, last_delta;
+  double tol;        // inner-loop threshold (1e-3 in the reference)
   int iter, active;
   int kind, symmetric;
   int status;        // bit0: a rigid square root needed re-orthogonalising
@@ -31,7 +32,8 @@ struct OuterState {
   xf::Xf fixT, movT;     // translations by the image centres
   double lastp[12];      // m_LastTransformParameters
   double sqdiff;         // squared parameter change of the last iteration
-  int stop;              // early breakout (sqdiff < 1e-3)
+  int stop;              // early breakout (sqdiff < stop_tol)
+  double stop_tol;       // outer-loop threshold (1e-3 in the reference)
   int inner_total;       // LTS inner iterations so far
   // resampling index maps for the next iteration
   IndexMap mapF, mapM, mmapF, mmapM;
@@ -55,8 +57,8 @@ int64_t lts_parts(int64_t npts);
 // hold the number of kept pairs. lts_start resets the state from the outer
 // transform; the host then queues iterations (kernels are predicated on
 // st->active, so a chunk can be queued before reading the flag back).
-void lts_start(LtsState* st, const OuterState* outer, int kind, int symmetric, LtsBuffers& b,
-               hipStream_t s);
+void lts_start(LtsState* st, const OuterState* outer, int kind, int symmetric, double tol,
+               LtsBuffers& b, hipStream_t s);
 void lts_enqueue_iteration(const double* X, const double* Y, const uint8_t* kept, int64_t npts,
                            LtsState* st, OuterState* outer, LtsBuffers& buf, int iteration,
                            hipStream_t s);

@@ -255,7 +255,7 @@ struct RegRun {
     lb.cand = w.cand.as<unsigned long long>();
     lb.part = w.part.as<double>();
     lb.nparts_cap = lts_parts(npts);
-    lts_start(st, outer, kind, symmetric ? 1 : 0, lb, s);
+    lts_start(st, outer, kind, symmetric ? 1 : 0, p->lts_tolerance, lb, s);
     const int chunk = 4;
     for (int it = 0; it < 101; it += chunk) {
       for (int j = it; j < std::min(it + chunk, 101); ++j)
@@ -384,6 +384,7 @@ void run_registration(mirorr_ctx* ctx, const mirorr_reg_params* p, const mirorr_
       o.gHM = middle_grid(gF, gM, cF, cM, true, gM.dir);
     }
     o.T = T;
+    o.stop_tol = p->stop_tolerance;
     xf::get_params(T, o.lastp);
     for (int a = 0; a < 3; ++a) stats->level_dims[level][a] = o.gHF.dims[a];
     run.ensure_level_buffers(o.gHF);
@@ -523,6 +524,8 @@ void mirorr_reg_default_params(mirorr_reg_params* p) {
   p->variance_select = 0;
   p->verbosity = 0;
   p->centre_init = 1;
+  p->stop_tolerance = 1e-3;
+  p->lts_tolerance = 1e-3;
 }
 
 mirorr_status mirorr_volume_create(mirorr_ctx* c, const float* data, const uint8_t* mask,
@@ -576,6 +579,8 @@ mirorr_status mirorr_register(mirorr_ctx* c, const mirorr_reg_params* p,
     // SetPortionMatchesKept (itkAbstractBlockMatcher.txx:83-99): 0 means 0.5.
     if (p->portion_kept < 0.0 || p->portion_kept > 1.0)
       throw Error(kErrArg, "portion_kept should be between 0 and 1.0");
+    if (!(p->stop_tolerance >= 0.0) || !(p->lts_tolerance >= 0.0))
+      throw Error(kErrArg, "stop tolerances must be non-negative");
     run_registration(c, p, fixed, moving, init_params, init_center, out_params, out_center,
                      stats ? stats : &local);
   });

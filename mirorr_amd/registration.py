@@ -98,6 +98,10 @@ class MirorrPyramid:
     def SetVarianceSelection(self, on: bool): self._p.variance_select = 1 if on else 0
     def SetVerbosity(self, v): self._p.verbosity = int(v)
     def SetCentreInitialisation(self, on: bool): self._p.centre_init = 1 if on else 0
+    def SetStopTolerances(self, outer: float, lts: float):
+        """Early-stop thresholds (1e-3 each by default); 0 disables them."""
+        self._p.stop_tolerance = float(outer)
+        self._p.lts_tolerance = float(lts)
     def GetPyramidScheduleTuner(self): return self._tuner
 
     def SetFixedImage(self, vol): self._fixed = vol

@@ -233,7 +233,7 @@ void start_opt_symmetric(Xf& T, const LevelIn& L, int max_iter, const orc_reg_pa
     xb.insert(xb.end(), fm.begin(), fm.end());
     ym.insert(ym.end(), fb.begin(), fb.end());
     st->points[level] = (int64_t)xb.size() / 3;
-    st->inner_iterations += lts_symmetric(T, t12F, t12B, xb, ym);
+    st->inner_iterations += lts_symmetric(T, t12F, t12B, xb, ym, p.lts_tolerance);
     T.get_params(cur);
     double d;
     params_sq_delta(lastp, cur, np, &d);
@@ -241,7 +241,7 @@ void start_opt_symmetric(Xf& T, const LevelIn& L, int max_iter, const orc_reg_pa
       std::printf("oracle level %d iteration %d: %lld blocks, delta %.6g\n", level, it + 1,
                   (long long)st->points[level], d);
     std::memcpy(lastp, cur, sizeof(double) * np);
-    if (d < 1e-3) {
+    if (d < p.stop_tolerance) {
       ++it;
       break;
     }
@@ -290,7 +290,7 @@ void start_opt_classic(Xf& T, const LevelIn& L, int max_iter, const orc_reg_para
     std::vector<double> xb, ym;
     match(Mh, Mmh, Fh, Fmh, gM, p, portion, xb, ym, &st->voxel_evals);
     st->points[level] = (int64_t)xb.size() / 3;
-    st->inner_iterations += lts_classic(T, xb, ym);
+    st->inner_iterations += lts_classic(T, xb, ym, p.lts_tolerance);
     T.get_params(cur);
     double d;
     params_sq_delta(lastp, cur, np, &d);
@@ -298,7 +298,7 @@ void start_opt_classic(Xf& T, const LevelIn& L, int max_iter, const orc_reg_para
       std::printf("oracle level %d iteration %d: %lld blocks, delta %.6g\n", level, it + 1,
                   (long long)st->points[level], d);
     std::memcpy(lastp, cur, sizeof(double) * np);
-    if (d < 1e-3) {
+    if (d < p.stop_tolerance) {
       ++it;
       break;
     }
@@ -325,6 +325,8 @@ void orc_reg_default_params(orc_reg_params* p) {
   p->pyr_min_size = 32;
   p->portion_kept = 0.5;
   p->variance_select = 0;
+  p->stop_tolerance = 1e-3;
+  p->lts_tolerance = 1e-3;
   p->nthreads = 0;
   p->verbosity = 0;
   p->centre_init = 1;

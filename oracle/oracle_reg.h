@@ -10,9 +10,12 @@
 #include "xform.hpp"
 namespace orc {
 extern int g_verbosity;
-int lts_classic(Xf& T, const std::vector<double>& X, const std::vector<double>& Y);
+// tol: the inner loop's thresholds on the parameter change (1e-3 in the
+// reference, itkMirorrRegistrationMethod.txx:785-786).
+int lts_classic(Xf& T, const std::vector<double>& X, const std::vector<double>& Y,
+                double tol = 1e-3);
 int lts_symmetric(Xf& T, Xf& t12F, Xf& t12B, const std::vector<double>& X,
-                  const std::vector<double>& Y);
+                  const std::vector<double>& Y, double tol = 1e-3);
 }  // namespace orc
 extern "C" {
 #endif
@@ -28,6 +31,12 @@ typedef struct orc_reg_params {
   int verbosity;
   int centre_init;   // CenteredTransformInitializer GeometryOn (default 1)
   int variance_select;  // block-variance selection at levels >= 2 (default 0)
+  // Early-stop thresholds, 1e-3 in the reference: outer loop on the squared
+  // parameter change (itkSymMirorrRegistrationMethod.txx:456-692), inner LTS
+  // loop on the change and its change (itkMirorrRegistrationMethod.txx:785-786).
+  // 0 disables early stopping (fixed iteration counts), for comparisons.
+  double stop_tolerance;
+  double lts_tolerance;
 } orc_reg_params;
 
 typedef struct orc_reg_stats {

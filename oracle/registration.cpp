@@ -117,7 +117,7 @@ int g_verbosity = 0;
 
 // Classic LTS (itkMirorrRegistrationMethod.txx:758-1003). T is composed with
 // the fitted transform (pre-multiplied). Returns the inner iteration count.
-int lts_classic(Xf& T, const Pts& X, const Pts& Y) {
+int lts_classic(Xf& T, const Pts& X, const Pts& Y, double tol) {
   Xf fin;
   fin.kind = T.kind;
   double rc[3] = {T.c[0], T.c[1], T.c[2]};
@@ -129,7 +129,7 @@ int lts_classic(Xf& T, const Pts& X, const Pts& Y) {
   double fraction = 1.0, last_delta = 0.0, delta = 1.0;
   int iter = -1;
   const size_t n = X.size() / 3;
-  while (delta > 1e-3 && iter < 100 && std::fabs(delta - last_delta) > 1e-3) {
+  while (delta > tol && iter < 100 && std::fabs(delta - last_delta) > tol) {
     ++iter;
     if (n == 0) break;
     std::vector<double> r(n, 0.0);
@@ -203,7 +203,7 @@ int lts_classic(Xf& T, const Pts& X, const Pts& Y) {
 
 // Symmetric LTS (itkSymMirorrRegistrationMethod.txx:729-1442). Updates the
 // half-space transforms and T = t12F o t12B^-1. Returns inner iterations.
-int lts_symmetric(Xf& T, Xf& t12F, Xf& t12B, const Pts& X, const Pts& Y) {
+int lts_symmetric(Xf& T, Xf& t12F, Xf& t12B, const Pts& X, const Pts& Y, double tol) {
   Xf fF, fB;
   fF.kind = fB.kind = T.kind;
   double rc[3] = {T.c[0], T.c[1], T.c[2]};
@@ -224,7 +224,7 @@ int lts_symmetric(Xf& T, Xf& t12F, Xf& t12B, const Pts& X, const Pts& Y) {
   int iter = -1;
   const size_t n = X.size() / 3;
   const double zero[3] = {0, 0, 0};
-  while (delta > 1e-3 && iter < 100 && std::fabs(delta - last_delta) > 1e-3) {
+  while (delta > tol && iter < 100 && std::fabs(delta - last_delta) > tol) {
     ++iter;
     if (n == 0) break;
     std::vector<double> r(n, 0.0);
