@@ -69,13 +69,19 @@ __global__ __launch_bounds__(kThreads) void search_stats_kernel(
         ss += d * d;
       }
     }
-  out[lin3(nx, ny, x, yy, z)] = make_float2((float)mu, ss > 0.0 ? (float)(1.0 / ss) : -1.0f);
+  out[lin3(nx, ny, x, yy, z)] = make_float2((float)mu, ss > 0.0 ? (float)(1.0 / ss) : -__builtin_huge_valf());
 }
 
 // BW = 4: a thread owns one (x, y) column of window positions and slides
-// along z over a chunk, keeping the four 4x4 planes of the current window
-// in registers with their plane sums (double), so a position loads one
-// new plane (16 values) instead of the whole window twice.
+// along z over a chunk. Each of the four 4x4 planes of the current window is
+// summarised once, when it enters, by FP64 sums: all 16 values (for the mean)
+// and, over the voxels whose search mask is on, the count and the first and
+// second moments of y - c0 (c0 = the column's first value, which keeps the
+// terms small). Then
+//   Syy = sum_on (y - mu)^2 = S2 - 2 m S1 + m^2 n,   m = mu - c0,
+// where S1, S2, n are the on-voxel sums over the window. A window whose 64
+// values are all equal has Syy = 0 exactly, as the reference's two-pass sum
+// gives, so the per-plane minimum and maximum detect that case.
 constexpr int kStatsZ = 16;  // z positions per thread
 
 template <bool MASK>
@@ -87,42 +93,52 @@ __global__ __launch_bounds__(kThreads) void search_stats4_kernel(
   const int z0 = blockIdx.z * kStatsZ;
   if (x > nx - 4 || yy > ny - 4) return;
   const int zend = min(z0 + kStatsZ, nz - 3);  // exclusive end of output z
-  float p[4][16];
-  uint32_t mk[4];  // mask bit per plane voxel
-  double ps[4];
+  if (z0 >= zend) return;
+  const double c0 = (double)y[lin3(nx, ny, x, yy, z0)];
+  double ps[4], p1[4], p2[4];
+  int pn[4];
+  float pmin[4], pmax[4];
   auto load = [&](int slot, int z) {
-    double s = 0.0;
-    uint32_t bits = 0u;
+    double s = 0.0, s1 = 0.0, s2 = 0.0;
+    int cnt = 0;
+    float lo = __builtin_huge_valf(), hi = -__builtin_huge_valf();
 #pragma unroll
     for (int dy = 0; dy < 4; ++dy) {
       const int64_t o = lin3(nx, ny, x, yy + dy, z);
 #pragma unroll
       for (int dx = 0; dx < 4; ++dx) {
         const float v = y[o + dx];
-        p[slot][4 * dy + dx] = v;
-        s += (double)v;
-        if (MASK && sm[o + dx] != 0) bits |= 1u << (4 * dy + dx);
+        lo = fminf(lo, v);
+        hi = fmaxf(hi, v);
+        const double dv = (double)v;
+        s += dv;
+        if (!MASK || sm[o + dx] != 0) {
+          const double d = dv - c0;
+          s1 += d;
+          s2 = fma(d, d, s2);
+          ++cnt;
+        }
       }
     }
     ps[slot] = s;
-    mk[slot] = bits;
+    p1[slot] = s1;
+    p2[slot] = s2;
+    pn[slot] = cnt;
+    pmin[slot] = lo;
+    pmax[slot] = hi;
   };
   auto emit = [&](int z) {
     const double mu = (ps[0] + ps[1] + ps[2] + ps[3]) * (1.0 / 64.0);
-    double ss = 0.0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-#pragma unroll
-      for (int j = 0; j < 16; ++j) {
-        float v = p[k][j];
-        asm volatile("" : "+v"(v));  // convert afresh: keeps 64 floats live, not 64 doubles
-        double d = (double)v - mu;
-        if (MASK && !((mk[k] >> j) & 1u)) d = 0.0;
-        ss = fma(d, d, ss);
-      }
-    out[lin3(nx, ny, x, yy, z)] = make_float2((float)mu, ss > 0.0 ? (float)(1.0 / ss) : -1.0f);
+    const double m = mu - c0;
+    const double s1 = p1[0] + p1[1] + p1[2] + p1[3];
+    const double s2 = p2[0] + p2[1] + p2[2] + p2[3];
+    const double n = (double)(pn[0] + pn[1] + pn[2] + pn[3]);
+    const float lo = fminf(fminf(pmin[0], pmin[1]), fminf(pmin[2], pmin[3]));
+    const float hi = fmaxf(fmaxf(pmax[0], pmax[1]), fmaxf(pmax[2], pmax[3]));
+    double ss = fma(m * n - 2.0 * s1, m, s2);
+    if (lo == hi || n == 0.0) ss = 0.0;
+    out[lin3(nx, ny, x, yy, z)] = make_float2((float)mu, ss > 0.0 ? (float)(1.0 / ss) : -__builtin_huge_valf());
   };
-  if (z0 >= zend) return;
   load(0, z0);
   load(1, z0 + 1);
   load(2, z0 + 2);
@@ -199,6 +215,79 @@ __global__ __launch_bounds__(kThreads) void block_prep_kernel(
   }
 }
 
+// BW = 4: four lanes per block, lane q owning the block's plane dz = q (four
+// rows of four voxels). Sums are reduced within the quad; the centred voxels
+// of a wave's 16 blocks form one contiguous 4 KB run of xt.
+__device__ __forceinline__ double quad_sum(double v) {
+  v += __shfl_xor(v, 1, 4);
+  v += __shfl_xor(v, 2, 4);
+  return v;
+}
+__device__ __forceinline__ int quad_sum_i(int v) {
+  v += __shfl_xor(v, 1, 4);
+  v += __shfl_xor(v, 2, 4);
+  return v;
+}
+
+__global__ __launch_bounds__(kThreads) void block_prep4_kernel(
+    const float* __restrict__ x, const uint8_t* __restrict__ bm, BMGrid g,
+    float* __restrict__ xt, float* __restrict__ sxx, float* __restrict__ sumxt,
+    uint32_t* __restrict__ flags) {
+  const int t = blockIdx.x * kThreads + threadIdx.x;  // nblocks * 4 < 2^31
+  const int b = t >> 2, dz = t & 3;
+  if (b >= g.nblocks) return;  // the four lanes of a block leave together
+  const int bxi = b % g.nb[0];
+  const int byi = (b / g.nb[0]) % g.nb[1];
+  const int bzi = b / (g.nb[0] * g.nb[1]);
+  const int px = g.start[0] + g.ng * bxi;
+  const int py = g.start[1] + g.ng * byi;
+  const int pz = g.start[2] + g.ng * bzi + dz;
+  float v[16];
+  double s = 0.0;
+#pragma unroll
+  for (int dy = 0; dy < 4; ++dy) {
+    const float* row = x + lin3(g.dims[0], g.dims[1], px, py + dy, pz);
+#pragma unroll
+    for (int dx = 0; dx < 4; ++dx) {
+      v[dy * 4 + dx] = row[dx];
+      s += (double)row[dx];
+    }
+  }
+  const double mx = quad_sum(s) * (1.0 / 64.0);
+  double s2 = 0.0, st = 0.0;
+  float4* out = reinterpret_cast<float4*>(xt + (int64_t)b * 64 + dz * 16);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    float f[4];
+#pragma unroll
+    for (int dx = 0; dx < 4; ++dx) {
+      const double d = (double)v[r * 4 + dx] - mx;
+      f[dx] = (float)d;
+      s2 = fma(d, d, s2);
+      st += (double)f[dx];
+    }
+    out[r] = make_float4(f[0], f[1], f[2], f[3]);
+  }
+  int on = 16;
+  if (bm != nullptr) {
+    on = 0;
+#pragma unroll
+    for (int dy = 0; dy < 4; ++dy) {
+      const uint8_t* row = bm + lin3(g.dims[0], g.dims[1], px, py + dy, pz);
+#pragma unroll
+      for (int dx = 0; dx < 4; ++dx) on += row[dx] != 0 ? 1 : 0;
+    }
+  }
+  s2 = quad_sum(s2);
+  st = quad_sum(st);
+  on = quad_sum_i(on);
+  if (dz == 0) {
+    sxx[b] = (float)s2;
+    sumxt[b] = (float)st;
+    flags[b] = (on == 64 ? 1u : 0u) | (on == 0 ? 2u : 0u);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Top-2 bookkeeping shared by the kernels.
 struct Top2 {
@@ -223,6 +312,14 @@ __device__ __forceinline__ void top2_push(Top2& t, float s, int scan) {
   } else {
     t.second = fmaxf(t.second, s);
   }
+}
+
+// The same update as top2_push in three operations: the new second is the
+// median of (best, second, s) since second <= best.
+__device__ __forceinline__ void top2_step(Top2& t, float s, int scan) {
+  t.scan = s > t.best ? scan : t.scan;
+  t.second = __builtin_amdgcn_fmed3f(t.best, t.second, s);
+  t.best = fmaxf(t.best, s);
 }
 
 // The record of a candidate sequence followed by a later one in scan order
@@ -383,8 +480,9 @@ __device__ __forceinline__ void write_skipped(const TileArgs& a, int64_t b, int 
 // the runtime height WY, done with a float reciprocal (exact here: rows are
 // small integers and the quotient's fractional part is at least 0.5 / WY
 // away from an integer).
-//  * MASKED: voxels whose search mask is off stage as 0, and the mask bits
-//    go to `wm` (one byte per voxel, same layout).
+//  * MASKED: voxels whose search mask is off stage as 0, the mask bits go
+//    to `wm` (one byte per voxel, same layout), and a voxel whose search
+//    mask is on sets bit wy of zrow[wz].
 //  * otherwise: a voxel whose search mask is off sets bit wy of zrow[wz].
 template <class T, bool MASKED>
 __device__ __forceinline__ void stage_window(const TileArgs& a, float* win, uint8_t* wm,
@@ -440,6 +538,7 @@ __device__ __forceinline__ void stage_window(const TileArgs& a, float* win, uint
       if (MASKED) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) mbits |= ((on >> k) & 1u) << (8 * k);
+        if (on) atomicOr(&zrow[wz], 1ull << wy);
       } else if (inside & ~on) {
         atomicOr(&zrow[wz], 1ull << wy);
       }
@@ -461,8 +560,29 @@ __device__ __forceinline__ bool window_has_off(const unsigned long long* zrow, i
   return (any & ym) != 0ull;
 }
 
+// Record of a block whose base mask is off at every voxel: each candidate
+// inside the image scores -2 (the base sum of squares over both masks is 0,
+// itkBlockMatcher.txx:96-97), so the first candidate in scan order wins and
+// the second is -2 when there are at least two candidates, else -3.
+template <class T>
+__device__ __forceinline__ void write_all_off(const TileArgs& a, int64_t b, int px, int py,
+                                              int pz) {
+  const int p[3] = {px, py, pz};
+  int lo[3], n = 1;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = max(p[d] - T::H, 0);
+    n *= min(p[d] - T::H + T::NW - 1, a.g.dims[d] - T::BW) - lo[d] + 1;
+  }
+  a.out_best[b] = -2.0f;
+  a.out_second[b] = n >= 2 ? -2.0f : -3.0f;
+  a.out_match[3 * b + 0] = lo[0];
+  a.out_match[3 * b + 1] = lo[1];
+  a.out_match[3 * b + 2] = lo[2];
+}
+
 // Block states in a tile: 0 outside the grid, 1 searched, 2 skipped,
-// 3 left to the masked kernel.
+// 3 left to the masked kernel, 4 base mask off everywhere (no search).
 __device__ __forceinline__ int block_state(const TileArgs& a, int gy, int gz, int64_t b) {
   if (gy >= a.g.nb[1] || gz >= a.g.nb[2]) return 0;
   return (a.bflags[b] & kBlockSkip) ? 2 : 1;
@@ -557,7 +677,11 @@ __global__ __launch_bounds__(kThreads, 4) void ncc_tile_fast(TileArgs a) {
   if (tid < T::NB) {
     const int gy = by0 + tid % T::TY, gz = bz0 + tid / T::TY;
     const int64_t b = bx + (int64_t)g.nb[0] * (gy + (int64_t)g.nb[1] * gz);
-    const int st = block_state(a, gy, gz, b);
+    int st = block_state(a, gy, gz, b);
+    if (st == 1 && (a.bflags[b] & 2u)) {  // base mask off everywhere: no search
+      write_all_off<T>(a, b, px, py0 + NG * (tid % T::TY), pz0 + NG * (tid / T::TY));
+      st = 4;
+    }
     bvalid[tid] = st;
     if (st == 1) {
       s_any = 1;
@@ -601,6 +725,12 @@ __global__ __launch_bounds__(kThreads, 4) void ncc_tile_fast(TileArgs a) {
   }
   if (a.work && tid == 0) atomicAdd(a.work, tile_evals<T>(a, px, py0, pz0, bvalid));
 
+  // Workgroup-uniform: every candidate window of the tile's blocks (those
+  // inside the grid or not) lies inside the image.
+  const bool interior = px - T::H >= 0 && px - T::H + T::NW - 1 <= nx - T::BW &&
+                        py0 - T::H >= 0 &&
+                        py0 + NG * (T::TY - 1) - T::H + T::NW - 1 <= ny - T::BW &&
+                        pz0 - T::H >= 0 && pz0 + NG * (T::TZ - 1) - T::H + T::NW - 1 <= nz - T::BW;
   Top2 t2;
   top2_init(t2);
   const int bl = tid / T::TPB;
@@ -652,34 +782,57 @@ __global__ __launch_bounds__(kThreads, 4) void ncc_tile_fast(TileArgs a) {
         }
       }
     }
-    // Epilogue: scores of the lane's candidates, in scan order.
+    // Epilogue: scores of the lane's candidates, in scan order. With
+    // t = Sxy^2 / Sxx, the score is max(t * iyy, -2): iyy = 1 / Syy, or -inf
+    // for a window with Syy = 0, whose product is -inf or NaN, both taken to
+    // -2 by fmaxf. A block with Sxx = 0 scores -2 everywhere: 1 / Sxx is
+    // then NaN.
     const int py = py0 + NG * bly, pz = pz0 + NG * blz;
-    const int lox = max(px - T::H, 0), hix = min(px - T::H + T::NW - 1, nx - T::BW);
-    const int loy = max(py - T::H, 0), hiy = min(py - T::H + T::NW - 1, ny - T::BW);
-    const int loz = max(pz - T::H, 0), hiz = min(pz - T::H + T::NW - 1, nz - T::BW);
-    // Branch-free: candidates outside the image score -inf, which changes
-    // neither the best nor the second (the scores start at -3); their
-    // statistics are read at a clamped position.
-    const int czg = pz - T::H + cz;
     const float sxx = bsxx[bl], sumx = bsum[bl];
-    const bool xok = sxx > 0.0f;  // -2 for every candidate otherwise
-    const float isxx = xok ? __builtin_amdgcn_rcpf(sxx) : 0.0f;
-    const bool zok = czg >= loz && czg <= hiz;
-    const int czc = min(max(czg, 0), nz - T::BW);
+    const float isxx = sxx > 0.0f ? __builtin_amdgcn_rcpf(sxx) : __builtin_nanf("");
+    if (interior) {
+      // Every candidate of every block of the tile lies inside the image:
+      // no range checks, and the statistics of a row are read at constant
+      // offsets from one address.
 #pragma unroll
-    for (int ky = 0; ky < T::KY; ++ky) {
-      const int cy = cyg * T::KY + ky;
-      const int cyv = py - T::H + cy;
-      const bool yok = zok && cy < T::NW && cyv >= loy && cyv <= hiy;
-      const float2* srow = a.stats + lin3(nx, ny, 0, min(max(cyv, 0), ny - T::BW), czc);
+      for (int ky = 0; ky < T::KY; ++ky) {
+        const int cy = cyg * T::KY + ky;
+        const float2* srow =
+            a.stats + lin3(nx, ny, px - T::H, min(py - T::H + cy, ny - T::BW), pz - T::H + cz);
 #pragma unroll
-      for (int cx = 0; cx < T::NW; ++cx) {
-        const int cxg = px - T::H + cx;
-        const float2 st = srow[min(max(cxg, 0), nx - T::BW)];
-        const float sxy = fmaf(-(st.x - ref), sumx, acc[ky][cx]);
-        float s = (xok && st.y > 0.0f) ? sxy * sxy * isxx * st.y : -2.0f;
-        s = (yok && cxg >= lox && cxg <= hix) ? s : -__builtin_huge_valf();
-        top2_push(t2, s, (cz * T::NW + cy) * T::NW + cx);
+        for (int cx = 0; cx < T::NW; ++cx) {
+          const float2 st = srow[cx];
+          const float sxy = fmaf(-(st.x - ref), sumx, acc[ky][cx]);
+          float sc = fmaxf(sxy * sxy * isxx * st.y, -2.0f);
+          if (T::YPAD > 0 && cy >= T::NW) sc = -__builtin_huge_valf();
+          top2_step(t2, sc, (cz * T::NW + cy) * T::NW + cx);
+        }
+      }
+    } else {
+      const int lox = max(px - T::H, 0), hix = min(px - T::H + T::NW - 1, nx - T::BW);
+      const int loy = max(py - T::H, 0), hiy = min(py - T::H + T::NW - 1, ny - T::BW);
+      const int loz = max(pz - T::H, 0), hiz = min(pz - T::H + T::NW - 1, nz - T::BW);
+      // Candidates outside the image score -inf, which changes neither the
+      // best nor the second (the scores start at -3); their statistics are
+      // read at a clamped position.
+      const int czg = pz - T::H + cz;
+      const bool zok = czg >= loz && czg <= hiz;
+      const int czc = min(max(czg, 0), nz - T::BW);
+#pragma unroll
+      for (int ky = 0; ky < T::KY; ++ky) {
+        const int cy = cyg * T::KY + ky;
+        const int cyv = py - T::H + cy;
+        const bool yok = zok && cy < T::NW && cyv >= loy && cyv <= hiy;
+        const float2* srow = a.stats + lin3(nx, ny, 0, min(max(cyv, 0), ny - T::BW), czc);
+#pragma unroll
+        for (int cx = 0; cx < T::NW; ++cx) {
+          const int cxg = px - T::H + cx;
+          const float2 st = srow[min(max(cxg, 0), nx - T::BW)];
+          const float sxy = fmaf(-(st.x - ref), sumx, acc[ky][cx]);
+          float sc = fmaxf(sxy * sxy * isxx * st.y, -2.0f);
+          sc = (yok && cxg >= lox && cxg <= hix) ? sc : -__builtin_huge_valf();
+          top2_step(t2, sc, (cz * T::NW + cy) * T::NW + cx);
+        }
       }
     }
   }
@@ -713,6 +866,7 @@ __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
   __shared__ unsigned long long bbits[T::NB];  // base mask bit per voxel
   __shared__ int todo_list[T::NB];
   __shared__ int s_ntodo;
+  __shared__ unsigned long long zon[kMaxWindow];  // rows holding a search-mask-on voxel
   __shared__ int bvalid[T::NB];
   __shared__ MergeSlot mrg[T::NB];
   __shared__ float s_ref;
@@ -746,6 +900,7 @@ __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
     bvalid[tid] = ((todo >> tid) & 1ull) ? 1 : 0;
     bbits[tid] = 0ull;
   }
+  for (int i = tid; i < WZ; i += kThreads) zon[i] = 0ull;
   if (tid == 0) {  // the queued blocks, densely numbered
     int n = 0;
     for (int bl = 0; bl < T::NB; ++bl)
@@ -754,7 +909,7 @@ __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
   }
   __syncthreads();
   const float ref = s_ref;
-  stage_window<T, true>(a, wy_, wv_, nullptr, WY, WZ, wx0, wy0, wz0, ref);
+  stage_window<T, true>(a, wy_, wv_, zon, WY, WZ, wx0, wy0, wz0, ref);
   for (int i = tid; i < T::NB * 64; i += kThreads) {
     const int bl = i / 64, k = i % 64;
     const int gy = by0 + bl % T::TY, gz = bz0 + bl / T::TY;
@@ -806,6 +961,24 @@ __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
     const int loz = max(pz - T::H, 0), hiz = min(pz - T::H + T::NW - 1, nz - T::BW);
     const int czg = pz - T::H + cz, cyg = py - T::H + cy;
     if (czg < loz || czg > hiz || cyg < loy || cyg > hiy) continue;
+    // A task whose 4 planes x 4 rows of window hold no search-mask-on
+    // voxel: every candidate has Syy = 0 and scores -2.
+    unsigned long long rows_on = 0ull;
+#pragma unroll
+    for (int iz = 0; iz < 4; ++iz) rows_on |= zon[blz * NG + cz + iz];
+    if (((rows_on >> (bly * NG + cy)) & 0xfull) == 0ull) {
+#pragma unroll
+      for (int cx = 0; cx < T::NW; ++cx) {
+        const int cxg = px - T::H + cx;
+        if (cxg < lox || cxg > hix) continue;
+        top2_step(tr, -2.0f, (cz * T::NW + cy) * T::NW + cx);
+      }
+      if (k == k0)
+        rec0 = top2_append(rec0, tr);
+      else
+        rec1 = top2_append(rec1, tr);
+      continue;
+    }
     float dmu[T::NW];
     const float2* srow = a.stats + lin3(nx, ny, 0, cyg, czg);
 #pragma unroll
@@ -1032,36 +1205,49 @@ __global__ void postprocess_kernel(BMGrid g, Geom geom, const float* __restrict_
 // Pair output for the LTS: X gets the moving-side point and Y the fixed-side
 // point of every block; `swap` marks the second symmetric pass, whose base
 // image is the fixed one (itkSymMirorrRegistrationMethod.txx:595-602).
-__global__ void postprocess_pairs_kernel(BMGrid g, Geom geom, const float* __restrict__ best,
-                                         const float* __restrict__ second,
-                                         const int32_t* __restrict__ match, int swap,
-                                         uint8_t* kept, double* X, double* Y,
-                                         unsigned long long* count) {
-  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const double cb = b < g.nblocks ? (double)best[b] : 0.0;
-  const double c2 = b < g.nblocks ? (double)second[b] : 0.0;
-  const bool k = b < g.nblocks && cb > 1e-7 && fabs(cb - c2) > 1e-4;
-  const unsigned long long bal = __ballot(k);
-  if ((threadIdx.x & 63) == 0 && bal) atomicAdd(count, (unsigned long long)__popcll(bal));
-  if (b >= g.nblocks) return;
-  kept[b] = k ? 1 : 0;
-  const int bxi = (int)(b % g.nb[0]);
-  const int byi = (int)((b / g.nb[0]) % g.nb[1]);
-  const int bzi = (int)(b / ((int64_t)g.nb[0] * g.nb[1]));
-  const int p[3] = {g.start[0] + g.ng * bxi, g.start[1] + g.ng * byi, g.start[2] + g.ng * bzi};
-  const int m[3] = {match[3 * b], match[3 * b + 1], match[3 * b + 2]};
+__global__ __launch_bounds__(kThreads) void postprocess_pairs_kernel(
+    BMGrid g, Geom geom, const float* __restrict__ best, const float* __restrict__ second,
+    const int32_t* __restrict__ match, int swap, uint8_t* kept, double* X, double* Y,
+    unsigned long long* count) {
+  // Grid-stride over the blocks with one count update per workgroup: the
+  // counter is a single device-wide address, so updates are kept few.
+  __shared__ unsigned int wsum[kThreads / 64];
   const double half = 0.5 * g.bw;
   double* pb = swap ? Y : X;
   double* pm = swap ? X : Y;
-  for (int r = 0; r < 3; ++r) {
-    double vb = geom.origin[r], vm = geom.origin[r];
-    for (int c = 0; c < 3; ++c) {
-      const double dsc = geom.dir[3 * r + c] * geom.spacing[c];
-      vb += dsc * (double)p[c];
-      vm += dsc * (double)m[c];
+  unsigned int nk = 0;
+  for (int64_t b = (int64_t)blockIdx.x * kThreads + threadIdx.x; b < g.nblocks;
+       b += (int64_t)gridDim.x * kThreads) {
+    const double cb = (double)best[b];
+    const double c2 = (double)second[b];
+    const bool k = cb > 1e-7 && fabs(cb - c2) > 1e-4;
+    nk += k ? 1u : 0u;
+    kept[b] = k ? 1 : 0;
+    const int bxi = (int)(b % g.nb[0]);
+    const int byi = (int)((b / g.nb[0]) % g.nb[1]);
+    const int bzi = (int)(b / ((int64_t)g.nb[0] * g.nb[1]));
+    const int p[3] = {g.start[0] + g.ng * bxi, g.start[1] + g.ng * byi,
+                      g.start[2] + g.ng * bzi};
+    const int m[3] = {match[3 * b], match[3 * b + 1], match[3 * b + 2]};
+    for (int r = 0; r < 3; ++r) {
+      double vb = geom.origin[r], vm = geom.origin[r];
+      for (int c = 0; c < 3; ++c) {
+        const double dsc = geom.dir[3 * r + c] * geom.spacing[c];
+        vb += dsc * (double)p[c];
+        vm += dsc * (double)m[c];
+      }
+      pb[3 * b + r] = vb + geom.spacing[r] * half;
+      pm[3 * b + r] = vm + geom.spacing[r] * half;
     }
-    pb[3 * b + r] = vb + geom.spacing[r] * half;
-    pm[3 * b + r] = vm + geom.spacing[r] * half;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) nk += __shfl_xor(nk, o, 64);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = nk;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned int tot = 0;
+    for (int w = 0; w < kThreads / 64; ++w) tot += wsum[w];
+    if (tot) atomicAdd(count, (unsigned long long)tot);
   }
 }
 
@@ -1341,14 +1527,16 @@ void launch_block_prep(const float* x, const uint8_t* bmask, const BMGrid& g,
                        BlockPrep bp, hipStream_t s) {
   if (g.nblocks == 0) return;
   if (g.nblocks > (int64_t)INT_MAX / 2) throw Error(kErrArg, "block grid too large");
-  const int waves = kThreads / 64;
-  dim3 grid((unsigned)((g.nblocks + waves - 1) / waves));
-  if (g.bw == 4)
-    hipLaunchKernelGGL(block_prep_kernel<4>, grid, dim3(kThreads), 0, s, x, bmask, g, bp.xt,
+  if (g.bw == 4) {
+    dim3 grid((unsigned)((g.nblocks * 4 + kThreads - 1) / kThreads));
+    hipLaunchKernelGGL(block_prep4_kernel, grid, dim3(kThreads), 0, s, x, bmask, g, bp.xt,
                        bp.sxx, bp.sumxt, bp.flags);
-  else
+  } else {
+    const int waves = kThreads / 64;
+    dim3 grid((unsigned)((g.nblocks + waves - 1) / waves));
     hipLaunchKernelGGL(block_prep_kernel<0>, grid, dim3(kThreads), 0, s, x, bmask, g, bp.xt,
                        bp.sxx, bp.sumxt, bp.flags);
+  }
   MIRORR_CHECK_LAUNCH();
 }
 
@@ -1406,9 +1594,9 @@ void launch_postprocess_pairs(const BMGrid& g, const Geom& geom, const BMResult&
                               uint8_t* kept, double* X, double* Y, unsigned long long* count,
                               hipStream_t s) {
   if (g.nblocks == 0) return;
-  hipLaunchKernelGGL(postprocess_pairs_kernel,
-                     dim3((unsigned)((g.nblocks + kThreads - 1) / kThreads)), dim3(kThreads), 0, s,
-                     g, geom, res.best, res.second, res.match, swap, kept, X, Y, count);
+  const int64_t grid = std::min<int64_t>((g.nblocks + kThreads - 1) / kThreads, 512);
+  hipLaunchKernelGGL(postprocess_pairs_kernel, dim3((unsigned)grid), dim3(kThreads), 0, s, g, geom,
+                     res.best, res.second, res.match, swap, kept, X, Y, count);
   MIRORR_CHECK_LAUNCH();
 }
 

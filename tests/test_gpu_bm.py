@@ -110,6 +110,55 @@ def test_masked_matches_oracle(oracle, nw, ng):
     _compare(bm.get_raw(), _oracle_raw(oracle, vb, mb, vs, ms, 4, 1, nw, ng))
 
 
+@pytest.mark.parametrize("nw,ng", [(5, 2), (9, 3), (13, 4)])
+def test_all_off_base_blocks_match_oracle(oracle, nw, ng):
+    # Blocks whose base mask is off everywhere score -2 at every candidate
+    # (itkBlockMatcher.txx:96-97); the device writes their record without a
+    # search. A slab of off planes plus a partial search mask covers them
+    # next to partly masked and fully on blocks.
+    shape = (30, 32, 36)
+    base = _smooth_random(shape, 70 + nw)
+    search = (np.roll(base, (1, 0, 1), (0, 1, 2)) * 0.8 + 20 +
+              np.random.default_rng(nw).normal(0, 4, shape)).astype(np.float32)
+    mb = np.full(shape, 255, np.uint8)
+    mb[:11] = 0
+    mb[:, :, 30:] = 0
+    ms = _fov_mask(shape, (0.2, -1.0, 0.3), 20.0)
+    vb, vs = _vol(base), _vol(search)
+    bm = _matcher(nw=nw, ng=ng)
+    bm.SetBaseImage(vb)
+    bm.SetSearchImage(vs)
+    bm.SetBaseMask(mb)
+    bm.SetSearchMask(ms)
+    ref = _oracle_raw(oracle, vb, mb, vs, ms, 4, 1, nw, ng)
+    got = bm.get_raw()
+    off = np.array([not mb[z:z + 4, y:y + 4, x:x + 4].any() for x, y, z in ref["block_idx"]])
+    assert off.sum() > 20 and (~off).sum() > 20
+    np.testing.assert_array_equal(got["match_idx"][off], ref["match_idx"][off])
+    np.testing.assert_array_equal(got["best"][off], ref["best"][off])
+    np.testing.assert_array_equal(got["second"][off], ref["second"][off])
+    assert np.all(ref["best"][off] == -2.0)
+    _compare(got, ref)
+
+
+def test_all_off_base_single_candidate(oracle):
+    # One block, one candidate: best -2, second stays at the initial -3.
+    shape = (4, 4, 4)
+    base = _smooth_random(shape, 5)
+    vb, vs = _vol(base), _vol(base * 2.0)
+    mb = np.zeros(shape, np.uint8)
+    bm = _matcher(nw=5, ng=2)
+    bm.SetBaseImage(vb)
+    bm.SetSearchImage(vs)
+    bm.SetBaseMask(mb)
+    got = bm.get_raw()
+    ref = _oracle_raw(oracle, vb, mb, vs, None, 4, 1, 5, 2)
+    assert ref["best"].tolist() == [-2.0] and ref["second"].tolist() == [-3.0]
+    np.testing.assert_array_equal(got["match_idx"], ref["match_idx"])
+    np.testing.assert_array_equal(got["best"], ref["best"])
+    np.testing.assert_array_equal(got["second"], ref["second"])
+
+
 @pytest.mark.parametrize("bw,bg,nw,ng", [(3, 2, 4, 3), (4, 1, 3, 2), (5, 1, 5, 4), (4, 2, 5, 3)])
 def test_generic_kernel_matches_oracle(oracle, bw, bg, nw, ng):
     shape = (18, 20, 22)
