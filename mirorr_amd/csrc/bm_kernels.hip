@@ -406,6 +406,7 @@ struct TileArgs {
   int32_t* queue;
   unsigned long long* qbits;  // per queued tile: the blocks left to the masked kernel
   int32_t* qcount;
+  int msplit;                 // masked kernel: workgroups per queued tile
   float* out_best;
   float* out_second;
   int32_t* out_match;
@@ -871,9 +872,19 @@ __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
   __shared__ MergeSlot mrg[T::NB];
   __shared__ float s_ref;
 
-  if (blockIdx.x >= *a.qcount) return;
-  const int tile = a.queue[blockIdx.x];
-  const unsigned long long todo = a.qbits[blockIdx.x];  // blocks to search here
+  // Workgroup (q, part) searches every msplit-th queued block of tile q
+  // (more, shorter workgroups where the level has few tiles).
+  const int q = blockIdx.x / a.msplit, part = blockIdx.x % a.msplit;
+  if (q >= *a.qcount) return;
+  const int tile = a.queue[q];
+  const unsigned long long queued = a.qbits[q];
+  unsigned long long todo = 0ull;  // blocks to search here
+  for (int bl = 0, c = 0; bl < T::NB; ++bl) {
+    if (((queued >> bl) & 1ull) == 0ull) continue;
+    if (c % a.msplit == part) todo |= 1ull << bl;
+    ++c;
+  }
+  if (todo == 0ull) return;
   const BMGrid& g = a.g;
   const int nx = g.dims[0], ny = g.dims[1], nz = g.dims[2];
   const int tid = threadIdx.x;
@@ -1438,8 +1449,11 @@ void run_tiled(const BMGrid& g, TileArgs a, BMScratch& scr, hipStream_t s) {
   if (scr.prof) scr.prof->end(s);
   const int WYm = (T::TY - 1) * g.ng + T::WX;
   const size_t lds_masked = (sizeof(float) + 1) * (size_t)T::PX * WYm * WZ;
+  // Few tiles (a coarse level): split each queued tile over four workgroups.
+  a.msplit = nt < 2048 ? 4 : 1;
   if (scr.prof) a.work = scr.prof->begin("ncc_tile_masked", s);
-  hipLaunchKernelGGL(ncc_tile_masked<T>, dim3((unsigned)nt), dim3(kThreads), lds_masked, s, a);
+  hipLaunchKernelGGL(ncc_tile_masked<T>, dim3((unsigned)(nt * a.msplit)), dim3(kThreads),
+                     lds_masked, s, a);
   MIRORR_CHECK_LAUNCH();
   if (scr.prof) scr.prof->end(s);
 }

@@ -3,9 +3,10 @@
 // Restates OptimizeTransform of the classic (itkMirorrRegistrationMethod.txx:758-1106)
 // and symmetric (itkSymMirorrRegistrationMethod.txx:729-1442) methods with the
 // per-point work on the GPU and the tiny FP64 solves in a one-thread tail.
-// Per inner iteration (the reductions finish in one-workgroup launches:
-// kernel boundaries publish the partial results, where a per-workgroup
-// agent-scope fence would write back L2 for every workgroup):
+// Per inner iteration (the reductions finish in one-workgroup launches or
+// are repeated by every workgroup of the next kernel: kernel boundaries
+// publish the partial results, where a per-workgroup agent-scope fence
+// would write back L2 for every workgroup):
 //  1. residual kernel: r_i = |F(x_i) - B(y_i)|^2 (classic |F(x_i) - y_i|^2),
 //     +inf for culled blocks; the first radix digit (bits 63..52 of the
 //     non-negative double) is histogrammed in LDS and merged globally;
@@ -82,21 +83,25 @@ __global__ __launch_bounds__(kT) void residual_kernel(const double* __restrict__
     if (h[i]) atomicAdd(&hist1[i], h[i]);
 }
 
-// Bin of rank k in the first-digit histogram (one workgroup).
-__global__ __launch_bounds__(kT) void scan1_kernel(const unsigned int* hist1, LtsState* st) {
-  if (!st->active) return;
-  sel::find_bin<4096 / kT>(hist1, st->k, &st->bin1, &st->k2);
-  if (threadIdx.x == 0) st->ncand = 0;
-}
-
-// Second digit (bits 51..40) among residuals in the first digit's bin.
+// Second digit (bits 51..40) among residuals in the first digit's bin. Each
+// workgroup finds that bin (rank k in the first-digit histogram) itself;
+// workgroup 0 records it.
 __global__ __launch_bounds__(kT) void hist2_kernel(const double* __restrict__ r, int64_t n,
-                                                   LtsState* st, unsigned int* hist2) {
+                                                   LtsState* st, const unsigned int* hist1,
+                                                   unsigned int* hist2) {
   if (!st->active) return;
   __shared__ unsigned int h[4096];
+  __shared__ unsigned int s_bin;
+  __shared__ unsigned long long s_krem;
   for (int i = threadIdx.x; i < 4096; i += kT) h[i] = 0;
+  sel::find_bin<4096 / kT>(hist1, st->k, &s_bin, &s_krem);
   __syncthreads();
-  const unsigned int b1 = st->bin1;
+  const unsigned int b1 = s_bin;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->bin1 = b1;
+    st->k2 = s_krem;
+    st->ncand = 0;
+  }
   for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT) {
     const unsigned long long k = key_of(r[i]);
     if ((unsigned)(k >> 52) == b1) atomicAdd(&h[(unsigned)(k >> 40) & 0xfffu], 1u);
@@ -104,11 +109,6 @@ __global__ __launch_bounds__(kT) void hist2_kernel(const double* __restrict__ r,
   __syncthreads();
   for (int i = threadIdx.x; i < 4096; i += kT)
     if (h[i]) atomicAdd(&hist2[i], h[i]);
-}
-
-__global__ __launch_bounds__(kT) void scan2_kernel(const unsigned int* hist2, LtsState* st) {
-  if (!st->active) return;
-  sel::find_bin<4096 / kT>(hist2, st->k2, &st->bin2, &st->k3);
 }
 
 // Rank k3 among the candidate keys (all share the 24-bit prefix): by rank
@@ -162,11 +162,22 @@ __device__ void final_select(const unsigned long long* cand, LtsState* st) {
   if (t == 0) st->thr = __longlong_as_double((long long)s_prefix);
 }
 
+// Candidates sharing the 24-bit prefix of the k-th residual. Each workgroup
+// finds the second digit's bin (rank k2 in the second histogram) itself;
+// workgroup 0 records it.
 __global__ __launch_bounds__(kT) void compact_kernel(const double* __restrict__ r, int64_t n,
-                                                     LtsState* st,
+                                                     LtsState* st, const unsigned int* hist2,
                                                      unsigned long long* __restrict__ cand) {
   if (!st->active) return;
-  const unsigned long long pre = ((unsigned long long)st->bin1 << 12) | st->bin2;
+  __shared__ unsigned int s_bin;
+  __shared__ unsigned long long s_krem;
+  sel::find_bin<4096 / kT>(hist2, st->k2, &s_bin, &s_krem);
+  __syncthreads();
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->bin2 = s_bin;
+    st->k3 = s_krem;
+  }
+  const unsigned long long pre = ((unsigned long long)st->bin1 << 12) | s_bin;
   for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT) {
     const unsigned long long k = key_of(r[i]);
     if ((k >> 40) == pre) cand[atomicAdd(&st->ncand, 1u)] = k;
@@ -199,17 +210,35 @@ __device__ void block_reduce(double (&v)[N], double* out) {
   }
 }
 
-// Sums per-workgroup partials in a fixed order (deterministic): wave w sums
-// values w, w + 4, ...; lanes stride over the partials.
+// Sums per-workgroup partials in a fixed order (deterministic). Thread t
+// handles value t % 32 of the partials t / 32, t / 32 + 8, ... (independent
+// loads), then the eight groups are added in order.
 template <int N>
 __device__ void reduce_parts(const double* part, int stride, int nparts, double* out) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (int j = w; j < N; j += blockDim.x / 64) {
-    double s = 0.0;
-    for (int p = lane; p < nparts; p += 64) s += part[(int64_t)stride * p + j];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-    if (lane == 0) out[j] = s;
+  static_assert(N <= 32 && kT == 256, "32 values x 8 groups");
+  __shared__ double grp[8][32];
+  const int j = threadIdx.x & 31, gi = threadIdx.x >> 5;
+  double s = 0.0;
+  if (j < N) {
+    int p = gi;
+    for (; p + 24 < nparts; p += 32) {  // four independent loads per step
+      const double a0 = part[(int64_t)stride * p + j];
+      const double a1 = part[(int64_t)stride * (p + 8) + j];
+      const double a2 = part[(int64_t)stride * (p + 16) + j];
+      const double a3 = part[(int64_t)stride * (p + 24) + j];
+      s += a0;
+      s += a1;
+      s += a2;
+      s += a3;
+    }
+    for (; p < nparts; p += 8) s += part[(int64_t)stride * p + j];
+  }
+  grp[gi][j] = s;
+  __syncthreads();
+  if (threadIdx.x < N) {
+    double t = 0.0;
+    for (int k = 0; k < 8; ++k) t += grp[k][threadIdx.x];
+    out[threadIdx.x] = t;
   }
   __syncthreads();
 }
@@ -226,17 +255,32 @@ __global__ __launch_bounds__(kT) void moments_kernel(const double* __restrict__ 
                                                      const double* __restrict__ Y,
                                                      const double* __restrict__ r, int64_t n,
                                                      LtsState* st, double* part) {
-  if (!st->active) return;
+  if (!st->active || n < 1) return;
   const double thr = st->fraction > 0.99999 ? DBL_MAX : st->thr;
   double v[kMoments];
 #pragma unroll
   for (int j = 0; j < kMoments; ++j) v[j] = 0.0;
   const int64_t base = (int64_t)blockIdx.x * kPart;
-  for (int j = threadIdx.x; j < kPart; j += kT) {
-    const int64_t i = base + j;
-    if (i < n && r[i] <= thr) {
-      const double x[3] = {X[3 * i], X[3 * i + 1], X[3 * i + 2]};
-      const double y[3] = {Y[3 * i], Y[3 * i + 1], Y[3 * i + 2]};
+  // The thread's points are loaded together (one round trip), then summed
+  // in order.
+  constexpr int kPer = kPart / kT;
+  double xs[kPer][3], ys[kPer][3];
+  bool in[kPer];
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const int64_t i = min(base + threadIdx.x + (int64_t)u * kT, n - 1);
+    in[u] = base + threadIdx.x + (int64_t)u * kT < n && r[i] <= thr;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      xs[u][k] = X[3 * i + k];
+      ys[u][k] = Y[3 * i + k];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    if (in[u]) {
+      const double* x = xs[u];
+      const double* y = ys[u];
       v[0] += 1.0;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
@@ -636,10 +680,10 @@ void lts_enqueue_iteration(const double* X, const double* Y, const uint8_t* kept
   hipLaunchKernelGGL(residual_kernel, dim3(grid), dim3(kT), 0, s, X, Y, kept, npts, st, b.r,
                      b.hist1, select ? 1 : 0);
   if (select) {
-    hipLaunchKernelGGL(scan1_kernel, dim3(1), dim3(kT), 0, s, b.hist1, st);
-    hipLaunchKernelGGL(hist2_kernel, dim3(grid), dim3(kT), 0, s, b.r, npts, st, b.hist2);
-    hipLaunchKernelGGL(scan2_kernel, dim3(1), dim3(kT), 0, s, b.hist2, st);
-    hipLaunchKernelGGL(compact_kernel, dim3(grid), dim3(kT), 0, s, b.r, npts, st, b.cand);
+    // At most 256 workgroups: each re-reads a 16 KB histogram to find its bin.
+    const int g2 = std::min(grid, 256);
+    hipLaunchKernelGGL(hist2_kernel, dim3(g2), dim3(kT), 0, s, b.r, npts, st, b.hist1, b.hist2);
+    hipLaunchKernelGGL(compact_kernel, dim3(g2), dim3(kT), 0, s, b.r, npts, st, b.hist2, b.cand);
     hipLaunchKernelGGL(final_select_kernel, dim3(1), dim3(kT), 0, s, b.cand, st);
   }
   hipLaunchKernelGGL(moments_kernel, dim3(nparts), dim3(kT), 0, s, X, Y, b.r, npts, st, b.part);
