@@ -392,6 +392,12 @@ struct TileCfg {
   static_assert(NB <= 64, "block bitmasks are 64-bit");
 };
 
+// Static LDS of the masked kernel (bytes, rounded up).
+template <class T>
+constexpr size_t masked_static_lds() {
+  return T::NB * (16 * 16 + 8 + 4 + 16 + 4) + 8 * kMaxWindow + 64;
+}
+
 struct TileArgs {
   const float* y;
   const uint8_t* sm;
@@ -481,11 +487,11 @@ __device__ __forceinline__ void write_skipped(const TileArgs& a, int64_t b, int 
 // the runtime height WY, done with a float reciprocal (exact here: rows are
 // small integers and the quotient's fractional part is at least 0.5 / WY
 // away from an integer).
-//  * MASKED: voxels whose search mask is off stage as 0, the mask bits go
-//    to `wm` (one byte per voxel, same layout), and a voxel whose search
-//    mask is on sets bit wy of zrow[wz].
+//  * MASKED: voxels whose search mask is off stage as 0, the mask goes to
+//    `wm` (one byte per voxel, or with FMASK one float 0/1, same layout),
+//    and a voxel whose search mask is on sets bit wy of zrow[wz].
 //  * otherwise: a voxel whose search mask is off sets bit wy of zrow[wz].
-template <class T, bool MASKED>
+template <class T, bool MASKED, bool FMASK = false>
 __device__ __forceinline__ void stage_window(const TileArgs& a, float* win, uint8_t* wm,
                                              unsigned long long* zrow, int WY, int WZ, int wx0,
                                              int wy0, int wz0, float ref) {
@@ -545,7 +551,12 @@ __device__ __forceinline__ void stage_window(const TileArgs& a, float* win, uint
       }
     }
     reinterpret_cast<float4*>(win)[row * T::NV + v] = make_float4(q[0], q[1], q[2], q[3]);
-    if (MASKED) reinterpret_cast<uint32_t*>(wm)[row * T::NV + v] = mbits;
+    if (MASKED && FMASK)
+      reinterpret_cast<float4*>(wm)[row * T::NV + v] =
+          make_float4((float)(mbits & 1u), (float)((mbits >> 8) & 1u), (float)((mbits >> 16) & 1u),
+                      (float)((mbits >> 24) & 1u));
+    else if (MASKED)
+      reinterpret_cast<uint32_t*>(wm)[row * T::NV + v] = mbits;
   }
 }
 
@@ -849,6 +860,24 @@ __global__ __launch_bounds__(kThreads, 4) void ncc_tile_fast(TileArgs a) {
   }
 }
 
+// Search mask values 4v .. 4v+3 of the staged row at float offset ro.
+template <class T, bool FMASK>
+__device__ __forceinline__ void load_mask(const uint8_t* wv, int ro, int v, float* vr) {
+  if (FMASK) {
+    const float4 m = reinterpret_cast<const float4*>(reinterpret_cast<const float*>(wv) + ro)[v];
+    vr[4 * v + 0] = m.x;
+    vr[4 * v + 1] = m.y;
+    vr[4 * v + 2] = m.z;
+    vr[4 * v + 3] = m.w;
+  } else {
+    const uint32_t m = reinterpret_cast<const uint32_t*>(wv + ro)[v];
+    vr[4 * v + 0] = (float)(m & 0xffu);
+    vr[4 * v + 1] = (float)((m >> 8) & 0xffu);
+    vr[4 * v + 2] = (float)((m >> 16) & 0xffu);
+    vr[4 * v + 3] = (float)(m >> 24);
+  }
+}
+
 // Masked path: sums over voxels where both masks are on. The window is
 // staged as v * (y - ref) (v = search mask) plus the mask bytes.
 // With t = v (y - mu) = v (y - ref) - v (mu - ref) (one FMA per voxel):
@@ -859,7 +888,7 @@ __global__ __launch_bounds__(kThreads, 4) void ncc_tile_fast(TileArgs a) {
 //    carry the same centring (one voxel on gives exactly 1).
 //  * Base block partly masked (base mask b): Sxy = Sum b x~ t,
 //    Syy = Sum b t^2, Sxx = Sum b x~^2 v (5 ops per voxel).
-template <class T>
+template <class T, bool FMASK>
 __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
   constexpr int R = (T::GT + kThreads - 1) / kThreads;  // tasks per thread
   extern __shared__ float4 dyn4[];
@@ -867,6 +896,7 @@ __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
   __shared__ unsigned long long bbits[T::NB];  // base mask bit per voxel
   __shared__ int todo_list[T::NB];
   __shared__ int s_ntodo;
+  __shared__ unsigned long long s_fullbase;  // queued blocks whose base mask is fully on
   __shared__ unsigned long long zon[kMaxWindow];  // rows holding a search-mask-on voxel
   __shared__ int bvalid[T::NB];
   __shared__ MergeSlot mrg[T::NB];
@@ -899,7 +929,7 @@ __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
   const int wx0 = px - T::H, wy0 = py0 - T::H, wz0 = pz0 - T::H;
   const int total = T::PX * WY * WZ;
   float* wy_ = reinterpret_cast<float*>(dyn4);
-  uint8_t* wv_ = reinterpret_cast<uint8_t*>(wy_ + total);
+  uint8_t* wv_ = reinterpret_cast<uint8_t*>(wy_ + total);  // mask bytes, or floats with FMASK
 
   if (tid == 0) {
     const int ry = min(py0 + NG * (T::TY / 2), ny - T::BW);
@@ -912,15 +942,28 @@ __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
     bbits[tid] = 0ull;
   }
   for (int i = tid; i < WZ; i += kThreads) zon[i] = 0ull;
-  if (tid == 0) {  // the queued blocks, densely numbered
+  if (tid == 0) s_fullbase = 0ull;
+  __syncthreads();
+  if (tid < T::NB && ((todo >> tid) & 1ull)) {
+    const int gy = by0 + tid % T::TY, gz = bz0 + tid / T::TY;
+    const int64_t b = bx + (int64_t)g.nb[0] * (gy + (int64_t)g.nb[1] * gz);
+    if (a.bm == nullptr || (a.bflags[b] & 1u) != 0u) atomicOr(&s_fullbase, 1ull << tid);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    // The queued blocks, densely numbered: fully-on bases first, then the
+    // partly masked ones, so that few waves run both task loops.
+    const unsigned long long fb = s_fullbase;
     int n = 0;
     for (int bl = 0; bl < T::NB; ++bl)
-      if ((todo >> bl) & 1ull) todo_list[n++] = bl;
+      if (((todo & fb) >> bl) & 1ull) todo_list[n++] = bl;
+    for (int bl = 0; bl < T::NB; ++bl)
+      if (((todo & ~fb) >> bl) & 1ull) todo_list[n++] = bl;
     s_ntodo = n;
   }
   __syncthreads();
   const float ref = s_ref;
-  stage_window<T, true>(a, wy_, wv_, zon, WY, WZ, wx0, wy0, wz0, ref);
+  stage_window<T, true, FMASK>(a, wy_, wv_, zon, WY, WZ, wx0, wy0, wz0, ref);
   for (int i = tid; i < T::NB * 64; i += kThreads) {
     const int bl = i / 64, k = i % 64;
     const int gy = by0 + bl % T::TY, gz = bz0 + bl / T::TY;
@@ -1010,16 +1053,11 @@ __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
           const int ro = rowoff + (iz * WY + iy) * T::PX;
           float yr[T::PX], vr[T::PX];
           const float4* yp = reinterpret_cast<const float4*>(wy_ + ro);
-          const uint32_t* vp = reinterpret_cast<const uint32_t*>(wv_ + ro);
 #pragma unroll
           for (int v = 0; v < T::NV; ++v) {
             const float4 q = yp[v];
-            const uint32_t m = vp[v];
             yr[4 * v + 0] = q.x; yr[4 * v + 1] = q.y; yr[4 * v + 2] = q.z; yr[4 * v + 3] = q.w;
-            vr[4 * v + 0] = (float)(m & 0xffu);
-            vr[4 * v + 1] = (float)((m >> 8) & 0xffu);
-            vr[4 * v + 2] = (float)((m >> 16) & 0xffu);
-            vr[4 * v + 3] = (float)(m >> 24);
+            load_mask<T, FMASK>(wv_, ro, v, vr);
           }
           const float4 xq4 = xh4[bl * 16 + iz * 4 + iy];
           const float xh[4] = {xq4.x, xq4.y, xq4.z, xq4.w};
@@ -1052,16 +1090,11 @@ __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
           const int ro = rowoff + (iz * WY + iy) * T::PX;
           float yr[T::PX], vr[T::PX];
           const float4* yp = reinterpret_cast<const float4*>(wy_ + ro);
-          const uint32_t* vp = reinterpret_cast<const uint32_t*>(wv_ + ro);
 #pragma unroll
           for (int v = 0; v < T::NV; ++v) {
             const float4 q = yp[v];
-            const uint32_t m = vp[v];
             yr[4 * v + 0] = q.x; yr[4 * v + 1] = q.y; yr[4 * v + 2] = q.z; yr[4 * v + 3] = q.w;
-            vr[4 * v + 0] = (float)(m & 0xffu);
-            vr[4 * v + 1] = (float)((m >> 8) & 0xffu);
-            vr[4 * v + 2] = (float)((m >> 16) & 0xffu);
-            vr[4 * v + 3] = (float)(m >> 24);
+            load_mask<T, FMASK>(wv_, ro, v, vr);
           }
           const float4 xq4 = xh4[bl * 16 + iz * 4 + iy];
           const float xh[4] = {xq4.x, xq4.y, xq4.z, xq4.w};
@@ -1448,12 +1481,20 @@ void run_tiled(const BMGrid& g, TileArgs a, BMScratch& scr, hipStream_t s) {
   MIRORR_CHECK_LAUNCH();
   if (scr.prof) scr.prof->end(s);
   const int WYm = (T::TY - 1) * g.ng + T::WX;
-  const size_t lds_masked = (sizeof(float) + 1) * (size_t)T::PX * WYm * WZ;
+  // The search mask is staged as floats (no conversions in the inner loop)
+  // when that window fits, else as bytes.
+  const size_t lds_fmask = 2 * sizeof(float) * (size_t)T::PX * WYm * WZ;
+  const bool fmask = lds_fmask + masked_static_lds<T>() <= 65536;
+  const size_t lds_masked = fmask ? lds_fmask : (sizeof(float) + 1) * (size_t)T::PX * WYm * WZ;
   // Few tiles (a coarse level): split each queued tile over four workgroups.
   a.msplit = nt < 2048 ? 4 : 1;
   if (scr.prof) a.work = scr.prof->begin("ncc_tile_masked", s);
-  hipLaunchKernelGGL(ncc_tile_masked<T>, dim3((unsigned)(nt * a.msplit)), dim3(kThreads),
-                     lds_masked, s, a);
+  if (fmask)
+    hipLaunchKernelGGL((ncc_tile_masked<T, true>), dim3((unsigned)(nt * a.msplit)), dim3(kThreads),
+                       lds_masked, s, a);
+  else
+    hipLaunchKernelGGL((ncc_tile_masked<T, false>), dim3((unsigned)(nt * a.msplit)),
+                       dim3(kThreads), lds_masked, s, a);
   MIRORR_CHECK_LAUNCH();
   if (scr.prof) scr.prof->end(s);
 }
@@ -1468,7 +1509,7 @@ bool lds_fits(int ng) {
   const size_t wz = (T::TZ - 1) * ng + T::WX;
   if (wyf > (size_t)kMaxWindow || wz > (size_t)kMaxWindow) return false;
   const size_t fast = 4 * T::PX * wyf * wz + T::NB * (16 * 16 + 12 + 16) + 16;
-  const size_t masked = 5 * T::PX * wym * wz + T::NB * (16 * 16 + 8 + 4 + 16);
+  const size_t masked = 5 * T::PX * wym * wz + masked_static_lds<T>();
   return fast <= 65536 && masked <= 65536;
 }
 

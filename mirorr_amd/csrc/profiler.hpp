@@ -21,7 +21,10 @@ class Profiler {
   ~Profiler() { release(); }
   bool on() const { return on_; }
   void enable(bool v, Budget* bud) {
-    if (v && !counters_.bytes()) counters_.ensure(sizeof(unsigned long long) * kSlots, bud);
+    if (v && !counters_.bytes()) {
+      counters_.ensure(sizeof(unsigned long long) * kSlots, bud);
+      MIRORR_HIP(hipMemset(counters_.as<void>(), 0, sizeof(unsigned long long) * kSlots));
+    }
     on_ = v;
   }
   // Records the start event; returns the launch's work counter (device) or
@@ -33,9 +36,7 @@ class Profiler {
     sl.name = name;
     sl.start = take_event();
     sl.stop = take_event();
-    sl.counter = used_++;
-    MIRORR_HIP(hipMemsetAsync(counters_.as<unsigned long long>() + sl.counter, 0,
-                              sizeof(unsigned long long), s));
+    sl.counter = used_++;  // zero: slots are cleared together after each collection
     MIRORR_HIP(hipEventRecord(sl.start, s));
     pending_.push_back(sl);
     return counters_.as<unsigned long long>() + sl.counter;
@@ -62,6 +63,7 @@ class Profiler {
       free_.push_back(sl.stop);
     }
     pending_.clear();
+    MIRORR_HIP(hipMemsetAsync(counters_.as<void>(), 0, sizeof(unsigned long long) * used_, s));
     used_ = 0;
   }
   const std::map<std::string, ProfAgg>& aggregates() const { return agg_; }
