@@ -142,7 +142,10 @@ def main():
     for _ in range(args.warmup):
         reg.RunMirorr(fixed_dev=fixed, moving_dev=moving)
     ctx.synchronize()
-    ctx.set_profiling(True)
+    # The timed steps carry HIP events around the fast search kernel only
+    # (its launches are priced on the roofline); the masked kernel's share
+    # is measured on one more, untimed registration below.
+    ctx.set_profiling(ctx.PROF_FAST)
     ctx.profile_reset()
     lat = []
     evals = 0
@@ -157,7 +160,11 @@ def main():
     ctx.synchronize()
     total = time.perf_counter() - t0
     n_fast, ms_fast, work_fast = ctx.profile("ncc_tile_fast")
-    n_mask, ms_mask, work_mask = ctx.profile("ncc_tile_masked")
+    ctx.set_profiling(ctx.PROF_FAST | ctx.PROF_MASKED)
+    ctx.profile_reset()
+    reg.RunMirorr(fixed_dev=fixed, moving_dev=moving)
+    _, _, work_fast1 = ctx.profile("ncc_tile_fast")
+    _, ms_mask, work_mask = ctx.profile("ncc_tile_masked")
     ctx.set_profiling(False)
 
     from mirorr_amd import transforms
@@ -199,8 +206,8 @@ def main():
             "launches": n_fast,
             "avg_launch_ms": avg_ms,
             "flop_per_voxel_eval": FLOP_PER_VOXEL_EVAL,
-            "masked_kernel_ms_total": ms_mask,
-            "masked_kernel_share_of_evals": work_mask / max(work_fast + work_mask, 1.0),
+            "masked_kernel_ms_per_registration": ms_mask,
+            "masked_kernel_share_of_evals": work_mask / max(work_fast1 + work_mask, 1.0),
         },
         "cpu_baseline": None,
     }

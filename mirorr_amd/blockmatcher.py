@@ -53,8 +53,14 @@ class Context:
     def synchronize(self):
         capi.check(capi.lib().mirorr_ctx_synchronize(self._h), self._h)
 
-    def set_profiling(self, on: bool):
-        capi.check(capi.lib().mirorr_ctx_set_profiling(self._h, 1 if on else 0), self._h)
+    PROF_FAST = 1    # MIRORR_PROF_FAST: the all-masks-on search kernel
+    PROF_MASKED = 2  # MIRORR_PROF_MASKED: the masked search kernel
+
+    def set_profiling(self, on):
+        """True instruments both search kernels, False none; an int selects
+        them by bit (PROF_FAST | PROF_MASKED)."""
+        mask = (self.PROF_FAST | self.PROF_MASKED) if on is True else int(on)
+        capi.check(capi.lib().mirorr_ctx_set_profiling(self._h, mask), self._h)
 
     def profile(self, kernel: str):
         """(launches, total ms, work units) of an instrumented kernel."""

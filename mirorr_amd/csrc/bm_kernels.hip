@@ -1476,10 +1476,10 @@ void run_tiled(const BMGrid& g, TileArgs a, BMScratch& scr, hipStream_t s) {
   const int WYf = (T::TY - 1) * g.ng + T::WX + T::YPAD;
   const int WZ = (T::TZ - 1) * g.ng + T::WX;
   const size_t lds_fast = sizeof(float) * (size_t)T::PX * WYf * WZ;
-  if (scr.prof) a.work = scr.prof->begin("ncc_tile_fast", s);
+  if (scr.prof) a.work = scr.prof->begin("ncc_tile_fast", kProfFast, s);
   hipLaunchKernelGGL(ncc_tile_fast<T>, dim3((unsigned)nt), dim3(kThreads), lds_fast, s, a);
   MIRORR_CHECK_LAUNCH();
-  if (scr.prof) scr.prof->end(s);
+  if (scr.prof) scr.prof->end(kProfFast, s);
   const int WYm = (T::TY - 1) * g.ng + T::WX;
   // The search mask is staged as floats (no conversions in the inner loop)
   // when that window fits, else as bytes.
@@ -1488,7 +1488,7 @@ void run_tiled(const BMGrid& g, TileArgs a, BMScratch& scr, hipStream_t s) {
   const size_t lds_masked = fmask ? lds_fmask : (sizeof(float) + 1) * (size_t)T::PX * WYm * WZ;
   // Few tiles (a coarse level): split each queued tile over four workgroups.
   a.msplit = nt < 2048 ? 4 : 1;
-  if (scr.prof) a.work = scr.prof->begin("ncc_tile_masked", s);
+  if (scr.prof) a.work = scr.prof->begin("ncc_tile_masked", kProfMasked, s);
   if (fmask)
     hipLaunchKernelGGL((ncc_tile_masked<T, true>), dim3((unsigned)(nt * a.msplit)), dim3(kThreads),
                        lds_masked, s, a);
@@ -1496,7 +1496,7 @@ void run_tiled(const BMGrid& g, TileArgs a, BMScratch& scr, hipStream_t s) {
     hipLaunchKernelGGL((ncc_tile_masked<T, false>), dim3((unsigned)(nt * a.msplit)),
                        dim3(kThreads), lds_masked, s, a);
   MIRORR_CHECK_LAUNCH();
-  if (scr.prof) scr.prof->end(s);
+  if (scr.prof) scr.prof->end(kProfMasked, s);
 }
 
 }  // namespace

@@ -191,17 +191,20 @@ __global__ __launch_bounds__(kT) void final_select_kernel(const unsigned long lo
 }
 
 // ---- trimmed moments ---------------------------------------------------------
+// Butterfly over the wave with the step outermost: the N shuffles of a step
+// are independent, so they issue together and wait on LDS once per step
+// (the sums per value are the same as one value at a time).
 template <int N>
 __device__ void block_reduce(double (&v)[N], double* out) {
   __shared__ double sh[kT / 64][N];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
-  for (int j = 0; j < N; ++j) {
-    double x = v[j];
+  for (int o = 32; o > 0; o >>= 1)
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-    if (lane == 0) sh[w][j] = x;
-  }
+    for (int j = 0; j < N; ++j) v[j] += __shfl_xor(v[j], o, 64);
+  if (lane == 0)
+#pragma unroll
+    for (int j = 0; j < N; ++j) sh[w][j] = v[j];
   __syncthreads();
   if (threadIdx.x < N) {
     double s = 0.0;
@@ -613,6 +616,7 @@ __global__ void init_kernel(LtsState* st, const OuterState* outer, int kind, int
   st->k = k;
   st->ncand = 0;
   st->thr = DBL_MAX;
+  st->finished = 0;
   st->active = n > 0 ? 1 : 0;
 }
 
@@ -639,7 +643,18 @@ __device__ void prepare_maps(OuterState* o, int symmetric) {
   }
 }
 
-__global__ __launch_bounds__(64) void finish_kernel(LtsState* st, OuterState* o) {
+__device__ void finish_update(LtsState* st, OuterState* o);
+
+__global__ __launch_bounds__(64) void finish_kernel(LtsState* st, OuterState* o, int* flags) {
+  if (!st->active && !st->finished) {
+    st->finished = 1;
+    finish_update(st, o);
+  }
+  flags[0] = st->active;
+  flags[1] = o->stop;
+}
+
+__device__ void finish_update(LtsState* st, OuterState* o) {
   const int kind = st->kind;
   if (st->symmetric) {
     compose(o->t12F, st->F, true);
@@ -677,11 +692,13 @@ void lts_enqueue_iteration(const double* X, const double* Y, const uint8_t* kept
   if (nparts > b.nparts_cap) throw Error(kErrState, "LTS partial buffer too small");
   const int grid = (int)std::min<int64_t>((npts + kT - 1) / kT, 2048);
   const bool select = iteration > 0;  // the first pass keeps every pair
-  hipLaunchKernelGGL(residual_kernel, dim3(grid), dim3(kT), 0, s, X, Y, kept, npts, st, b.r,
-                     b.hist1, select ? 1 : 0);
+  // With the histogram, at most 256 workgroups: the residuals crowd into a
+  // few bins, and each workgroup merges its bins with global atomics.
+  const int g2 = std::min(grid, 256);
+  hipLaunchKernelGGL(residual_kernel, dim3(select ? g2 : grid), dim3(kT), 0, s, X, Y, kept, npts,
+                     st, b.r, b.hist1, select ? 1 : 0);
   if (select) {
     // At most 256 workgroups: each re-reads a 16 KB histogram to find its bin.
-    const int g2 = std::min(grid, 256);
     hipLaunchKernelGGL(hist2_kernel, dim3(g2), dim3(kT), 0, s, b.r, npts, st, b.hist1, b.hist2);
     hipLaunchKernelGGL(compact_kernel, dim3(g2), dim3(kT), 0, s, b.r, npts, st, b.hist2, b.cand);
     hipLaunchKernelGGL(final_select_kernel, dim3(1), dim3(kT), 0, s, b.cand, st);
@@ -699,8 +716,8 @@ void lts_start(LtsState* st, const OuterState* outer, int kind, int symmetric, d
   MIRORR_CHECK_LAUNCH();
 }
 
-void lts_finish(LtsState* st, OuterState* outer, hipStream_t s) {
-  hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(1), 0, s, st, outer);
+void lts_finish(LtsState* st, OuterState* outer, int* flags, hipStream_t s) {
+  hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(1), 0, s, st, outer, flags);
   MIRORR_CHECK_LAUNCH();
 }
 

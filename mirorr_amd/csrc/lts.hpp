@@ -15,6 +15,7 @@ struct LtsState {
   double fraction, delta, last_delta;
   double tol;        // inner-loop threshold (1e-3 in the reference)
   int iter, active;
+  int finished;      // the transform update of this run is done (finish_kernel)
   int kind, symmetric;
   int status;        // bit0: a rigid square root needed re-orthogonalising
   unsigned long long n_kept;  // kept pairs (points with a finite residual)
@@ -65,7 +66,10 @@ void lts_enqueue_iteration(const double* X, const double* Y, const uint8_t* kept
 // Composes the fits into the outer transforms (symmetric: t12F/t12B and
 // T = t12F o t12B^-1; classic: T o F), updates the stop flag and the
 // resampling maps of the next iteration.
-void lts_finish(LtsState* st, OuterState* outer, hipStream_t s);
+// Queued after each chunk of inner iterations: once the LTS is inactive it
+// composes the fit into the outer state (exactly once per run). Every launch
+// then writes flags[0] = active and flags[1] = the outer stop flag.
+void lts_finish(LtsState* st, OuterState* outer, int* flags, hipStream_t s);
 // Computes the resampling maps of the current outer state (level start).
 void outer_prepare_maps(OuterState* outer, int symmetric, hipStream_t s);
 // Zeroes the kept-pair counter (before the block-matching passes).

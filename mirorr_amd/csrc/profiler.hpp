@@ -10,6 +10,9 @@
 
 namespace mirorr {
 
+constexpr int kProfFast = 1;    // MIRORR_PROF_FAST
+constexpr int kProfMasked = 2;  // MIRORR_PROF_MASKED
+
 struct ProfAgg {
   int64_t launches = 0;
   double ms = 0.0;
@@ -19,18 +22,19 @@ struct ProfAgg {
 class Profiler {
  public:
   ~Profiler() { release(); }
-  bool on() const { return on_; }
-  void enable(bool v, Budget* bud) {
-    if (v && !counters_.bytes()) {
+  bool on() const { return mask_ != 0; }
+  // `mask`: the kernels (bits) whose launches are instrumented; 0 = off.
+  void enable(int mask, Budget* bud) {
+    if (mask && !counters_.bytes()) {
       counters_.ensure(sizeof(unsigned long long) * kSlots, bud);
       MIRORR_HIP(hipMemset(counters_.as<void>(), 0, sizeof(unsigned long long) * kSlots));
     }
-    on_ = v;
+    mask_ = mask;
   }
   // Records the start event; returns the launch's work counter (device) or
   // nullptr when profiling is off.
-  unsigned long long* begin(const char* name, hipStream_t s) {
-    if (!on_) return nullptr;
+  unsigned long long* begin(const char* name, int bit, hipStream_t s) {
+    if (!(mask_ & bit)) return nullptr;
     if (used_ >= kSlots) collect(s);
     Slot sl;
     sl.name = name;
@@ -41,8 +45,8 @@ class Profiler {
     pending_.push_back(sl);
     return counters_.as<unsigned long long>() + sl.counter;
   }
-  void end(hipStream_t s) {
-    if (!on_ || pending_.empty()) return;
+  void end(int bit, hipStream_t s) {
+    if (!(mask_ & bit) || pending_.empty()) return;
     MIRORR_HIP(hipEventRecord(pending_.back().stop, s));
   }
   // Waits for pending launches and folds them into the aggregates.
@@ -95,7 +99,7 @@ class Profiler {
     MIRORR_HIP(hipEventCreate(&e));
     return e;
   }
-  bool on_ = false;
+  int mask_ = 0;
   int used_ = 0;
   DeviceBuffer counters_;
   std::vector<Slot> pending_;

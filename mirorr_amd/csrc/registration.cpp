@@ -123,7 +123,7 @@ struct Workspace {
   DeviceBuffer queue, qbits, qcount;
   DeviceBuffer X, Y, kept;                    // LTS pairs
   DeviceBuffer r, hist1, hist2, cand, part;
-  DeviceBuffer lts, outer, mapI;
+  DeviceBuffer lts, outer, mapI, flagsD;  // flagsD: {active, stop} for one read-back
   DeviceBuffer pyr_tmp, pyr_map;
   DeviceBuffer vsel_st, vsel_hist, vsel_evals;  // block-variance selection
 };
@@ -256,17 +256,22 @@ struct RegRun {
     lb.part = w.part.as<double>();
     lb.nparts_cap = lts_parts(npts);
     lts_start(st, outer, kind, symmetric ? 1 : 0, p->lts_tolerance, lb, s);
+    // Each chunk is followed by the (predicated) finish and one read-back of
+    // {active, stop}: when the run has converged, the outer update is already
+    // done and its stop flag known.
+    w.flagsD.ensure(2 * sizeof(int), bud);
+    static_assert(sizeof(Flags) == 2 * sizeof(int), "Flags is {active, stop}");
     const int chunk = 4;
     for (int it = 0; it < 101; it += chunk) {
       for (int j = it; j < std::min(it + chunk, 101); ++j)
         lts_enqueue_iteration(w.X.as<double>(), w.Y.as<double>(), w.kept.as<uint8_t>(), npts, st,
                               outer, lb, j, s);
-      MIRORR_HIP(hipMemcpyAsync(&h_flags->active, &st->active, sizeof(int),
-                                hipMemcpyDeviceToHost, s));
+      lts_finish(st, outer, w.flagsD.as<int>(), s);
+      MIRORR_HIP(hipMemcpyAsync(h_flags, w.flagsD.as<int>(), sizeof(Flags), hipMemcpyDeviceToHost,
+                                s));
       MIRORR_HIP(hipStreamSynchronize(s));
       if (!h_flags->active) break;
     }
-    lts_finish(st, outer, s);
   }
 };
 
@@ -454,9 +459,7 @@ void run_registration(mirorr_ctx* ctx, const mirorr_reg_params* p, const mirorr_
                        w.Mmh.as<uint8_t>(), o.gHF, w.xtF, w.sxxF, w.sumF, w.flF, w.statsM, 1,
                        selF, st);
       // 3. LTS and transform update on the device
-      run.run_lts(st, outer, npts, flags);
-      MIRORR_HIP(hipMemcpyAsync(&flags->stop, &outer->stop, sizeof(int), hipMemcpyDeviceToHost, s));
-      MIRORR_HIP(hipStreamSynchronize(s));
+      run.run_lts(st, outer, npts, flags);  // leaves flags->stop read back
       if (flags->stop) {
         ++it;
         break;
