@@ -86,12 +86,15 @@ void* mirorr_ctx_stream(mirorr_ctx* ctx);
 /* Per-kernel timing (no counterpart in the reference; the OpenCL host timed
  * its kernel with CL_QUEUE_PROFILING_ENABLE, itkBlockMatcherGPU.txx:380-391).
  * `on` selects the instrumented kernels (0 switches profiling off):
- * MIRORR_PROF_FAST ("ncc_tile_fast") and MIRORR_PROF_MASKED ("ncc_tile_masked").
- * Each selected launch is bracketed by HIP events on the context stream and
- * reports its work in voxel-evals; mirorr_ctx_profile_get sums launches,
- * milliseconds and work. The events add a few microseconds per launch. */
+ * MIRORR_PROF_FAST ("ncc_tile_fast") and MIRORR_PROF_MASKED ("ncc_tile_masked");
+ * with MIRORR_PROF_FINEST_ONLY a registration instruments only the launches
+ * of its last (full-resolution) level. Each selected launch is bracketed by
+ * HIP events on the context stream and reports its work in voxel-evals;
+ * mirorr_ctx_profile_get sums launches, milliseconds and work. The events
+ * add a few microseconds per launch. */
 #define MIRORR_PROF_FAST 1
 #define MIRORR_PROF_MASKED 2
+#define MIRORR_PROF_FINEST_ONLY 4
 mirorr_status mirorr_ctx_set_profiling(mirorr_ctx* ctx, int on);
 mirorr_status mirorr_ctx_profile_get(mirorr_ctx* ctx, const char* kernel, int64_t* launches,
                                      double* total_ms, double* work);

@@ -55,6 +55,7 @@ class Context:
 
     PROF_FAST = 1    # MIRORR_PROF_FAST: the all-masks-on search kernel
     PROF_MASKED = 2  # MIRORR_PROF_MASKED: the masked search kernel
+    PROF_FINEST_ONLY = 4  # registrations: instrument the full-resolution level only
 
     def set_profiling(self, on):
         """True instruments both search kernels, False none; an int selects

@@ -735,7 +735,8 @@ __global__ __launch_bounds__(kThreads, 4) void ncc_tile_fast(TileArgs a) {
     if (tid < T::NB && ((todo >> tid) & 1ull)) bvalid[tid] = 3;
     __syncthreads();
   }
-  if (a.work && tid == 0) atomicAdd(a.work, tile_evals<T>(a, px, py0, pz0, bvalid));
+  if (a.work && tid == 0)
+    atomicAdd(a.work + blockIdx.x % kWorkStripe, tile_evals<T>(a, px, py0, pz0, bvalid));
 
   // Workgroup-uniform: every candidate window of the tile's blocks (those
   // inside the grid or not) lies inside the image.
@@ -984,7 +985,8 @@ __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
   }
   __syncthreads();
 
-  if (a.work && tid == 0) atomicAdd(a.work, tile_evals<T>(a, px, py0, pz0, bvalid));
+  if (a.work && tid == 0)
+    atomicAdd(a.work + blockIdx.x % kWorkStripe, tile_evals<T>(a, px, py0, pz0, bvalid));
   // Tasks (block, cz, cy) run over the queued blocks only. A thread owns rr
   // consecutive tasks (scan order within a block), so they span at most two
   // blocks: one running record each, in registers.

@@ -197,7 +197,8 @@ mirorr_status mirorr_ctx_set_profiling(mirorr_ctx* c, int on) {
   if (!c) return MIRORR_ERR_ARG;
   return guarded(c, [&] {
     c->prof.collect(c->stream);
-    c->prof.enable(on & (kProfFast | kProfMasked), &c->budget);
+    c->prof.enable((on & (kProfFast | kProfMasked)) ? on & (kProfFast | kProfMasked | kProfFinest) : 0,
+                   &c->budget);
   });
 }
 

@@ -12,6 +12,10 @@ namespace mirorr {
 
 constexpr int kProfFast = 1;    // MIRORR_PROF_FAST
 constexpr int kProfMasked = 2;  // MIRORR_PROF_MASKED
+constexpr int kProfFinest = 4;  // MIRORR_PROF_FINEST_ONLY
+// A launch's work counter is a stripe of kWorkStripe counters: workgroup b
+// adds to entry b % kWorkStripe, so the updates do not contend on one address.
+constexpr int kWorkStripe = 64;
 
 struct ProfAgg {
   int64_t launches = 0;
@@ -22,17 +26,20 @@ struct ProfAgg {
 class Profiler {
  public:
   ~Profiler() { release(); }
-  bool on() const { return mask_ != 0; }
+  bool on() const { return (mask_ & (kProfFast | kProfMasked)) != 0; }
+  // Registrations instrument only their last (full-resolution) level.
+  bool finest_only() const { return (mask_ & kProfFinest) != 0; }
   // `mask`: the kernels (bits) whose launches are instrumented; 0 = off.
   void enable(int mask, Budget* bud) {
     if (mask && !counters_.bytes()) {
-      counters_.ensure(sizeof(unsigned long long) * kSlots, bud);
-      MIRORR_HIP(hipMemset(counters_.as<void>(), 0, sizeof(unsigned long long) * kSlots));
+      counters_.ensure(sizeof(unsigned long long) * kSlots * kWorkStripe, bud);
+      MIRORR_HIP(hipMemset(counters_.as<void>(), 0,
+                           sizeof(unsigned long long) * kSlots * kWorkStripe));
     }
     mask_ = mask;
   }
-  // Records the start event; returns the launch's work counter (device) or
-  // nullptr when profiling is off.
+  // Records the start event; returns the launch's work counter stripe
+  // (device) or nullptr when this kernel is not instrumented.
   unsigned long long* begin(const char* name, int bit, hipStream_t s) {
     if (!(mask_ & bit)) return nullptr;
     if (used_ >= kSlots) collect(s);
@@ -43,7 +50,7 @@ class Profiler {
     sl.counter = used_++;  // zero: slots are cleared together after each collection
     MIRORR_HIP(hipEventRecord(sl.start, s));
     pending_.push_back(sl);
-    return counters_.as<unsigned long long>() + sl.counter;
+    return counters_.as<unsigned long long>() + (size_t)sl.counter * kWorkStripe;
   }
   void end(int bit, hipStream_t s) {
     if (!(mask_ & bit) || pending_.empty()) return;
@@ -53,21 +60,23 @@ class Profiler {
   void collect(hipStream_t s) {
     if (pending_.empty()) return;
     MIRORR_HIP(hipStreamSynchronize(s));
-    std::vector<unsigned long long> c(used_);
+    std::vector<unsigned long long> c((size_t)used_ * kWorkStripe);
     MIRORR_HIP(hipMemcpy(c.data(), counters_.as<unsigned long long>(),
-                         sizeof(unsigned long long) * used_, hipMemcpyDeviceToHost));
+                         sizeof(unsigned long long) * c.size(), hipMemcpyDeviceToHost));
     for (Slot& sl : pending_) {
       float ms = 0.f;
       MIRORR_HIP(hipEventElapsedTime(&ms, sl.start, sl.stop));
       ProfAgg& a = agg_[sl.name];
       a.launches += 1;
       a.ms += ms;
-      a.units += (double)c[sl.counter];
+      for (int k = 0; k < kWorkStripe; ++k)
+        a.units += (double)c[(size_t)sl.counter * kWorkStripe + k];
       free_.push_back(sl.start);
       free_.push_back(sl.stop);
     }
     pending_.clear();
-    MIRORR_HIP(hipMemsetAsync(counters_.as<void>(), 0, sizeof(unsigned long long) * used_, s));
+    MIRORR_HIP(hipMemsetAsync(counters_.as<void>(), 0,
+                              sizeof(unsigned long long) * kWorkStripe * used_, s));
     used_ = 0;
   }
   const std::map<std::string, ProfAgg>& aggregates() const { return agg_; }

@@ -162,6 +162,7 @@ struct RegRun {
   bool symmetric;
   hipStream_t s;
   Budget* bud;
+  bool finest = false;  // the current level is the last (full-resolution) one
 
   // Block preparation of a base image and, with portion < 1, the variance
   // selection over its blocks. Returns whether blocks were dropped.
@@ -189,7 +190,8 @@ struct RegRun {
     if (tiled_supported(g)) {
       BlockPrep bp{xt.as<float>(), sxx.as<float>(), sum.as<float>(), fl.as<uint32_t>()};
       BMScratch scr{w.queue.as<int32_t>(), w.qcount.as<int32_t>(),
-                    w.qbits.as<unsigned long long>(), tile_count(g), &ctx->prof};
+                    w.qbits.as<unsigned long long>(), tile_count(g),
+                    (ctx->prof.finest_only() && !finest) ? nullptr : &ctx->prof};
       launch_ncc_search(base, bmask, search, smask, g, bp, SearchStats{stats_search.as<float2>()},
                         scr, res, s);
     } else {
@@ -262,13 +264,17 @@ struct RegRun {
     w.flagsD.ensure(2 * sizeof(int), bud);
     static_assert(sizeof(Flags) == 2 * sizeof(int), "Flags is {active, stop}");
     const int chunk = 4;
-    for (int it = 0; it < 101; it += chunk) {
-      for (int j = it; j < std::min(it + chunk, 101); ++j)
+    auto enqueue_chunk = [&](int it0, int it1) {
+      for (int j = it0; j < it1; ++j)
         lts_enqueue_iteration(w.X.as<double>(), w.Y.as<double>(), w.kept.as<uint8_t>(), npts, st,
                               outer, lb, j, s);
       lts_finish(st, outer, w.flagsD.as<int>(), s);
       MIRORR_HIP(hipMemcpyAsync(h_flags, w.flagsD.as<int>(), sizeof(Flags), hipMemcpyDeviceToHost,
                                 s));
+    };
+    for (int it = 0; it < 101; it += chunk) {
+      const int it1 = std::min(it + chunk, 101);
+      enqueue_chunk(it, it1);
       MIRORR_HIP(hipStreamSynchronize(s));
       if (!h_flags->active) break;
     }
@@ -339,6 +345,7 @@ void run_registration(mirorr_ctx* ctx, const mirorr_reg_params* p, const mirorr_
   stats->levels = t_max;
   const int np = xf::nparams(T);
   for (int level = 0; level < t_max && level < 16; ++level) {
+    run.finest = level + 1 == std::min(t_max, 16);
     xf::set_params(T, cur);
     const Geom& gF = Fp[level].geom;
     const Geom& gM = Mp[level].geom;

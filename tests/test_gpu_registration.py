@@ -133,3 +133,4 @@ def test_registration_matches_oracle(oracle, symmetric, affine):
     ddeg = transforms.rotation_angle_deg(M1, M2)
     print(f"corner distance {dmm:.3e} mm, rotation {ddeg:.3e} deg")
     assert dmm < 0.01 and ddeg < 0.01
+
