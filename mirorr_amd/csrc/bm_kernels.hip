@@ -395,7 +395,8 @@ struct TileCfg {
 // Static LDS of the masked kernel (bytes, rounded up).
 template <class T>
 constexpr size_t masked_static_lds() {
-  return T::NB * (16 * 16 + 8 + 4 + 16 + 4) + 8 * kMaxWindow + 64;
+  return T::NB * (16 * 16 + 8 + 4 + 4) + T::NB * T::NW * T::NW * (4 + 4 + 4 + 2) +
+         8 * kMaxWindow + 256;
 }
 
 struct TileArgs {
@@ -487,11 +488,11 @@ __device__ __forceinline__ void write_skipped(const TileArgs& a, int64_t b, int 
 // the runtime height WY, done with a float reciprocal (exact here: rows are
 // small integers and the quotient's fractional part is at least 0.5 / WY
 // away from an integer).
-//  * MASKED: voxels whose search mask is off stage as 0, the mask goes to
-//    `wm` (one byte per voxel, or with FMASK one float 0/1, same layout),
-//    and a voxel whose search mask is on sets bit wy of zrow[wz].
+//  * MASKED: voxels whose search mask is off stage as 0, the mask bits go
+//    to `wm` (one byte per voxel, same layout), and a voxel whose search
+//    mask is on sets bit wy of zrow[wz].
 //  * otherwise: a voxel whose search mask is off sets bit wy of zrow[wz].
-template <class T, bool MASKED, bool FMASK = false>
+template <class T, bool MASKED>
 __device__ __forceinline__ void stage_window(const TileArgs& a, float* win, uint8_t* wm,
                                              unsigned long long* zrow, int WY, int WZ, int wx0,
                                              int wy0, int wz0, float ref) {
@@ -551,12 +552,7 @@ __device__ __forceinline__ void stage_window(const TileArgs& a, float* win, uint
       }
     }
     reinterpret_cast<float4*>(win)[row * T::NV + v] = make_float4(q[0], q[1], q[2], q[3]);
-    if (MASKED && FMASK)
-      reinterpret_cast<float4*>(wm)[row * T::NV + v] =
-          make_float4((float)(mbits & 1u), (float)((mbits >> 8) & 1u), (float)((mbits >> 16) & 1u),
-                      (float)((mbits >> 24) & 1u));
-    else if (MASKED)
-      reinterpret_cast<uint32_t*>(wm)[row * T::NV + v] = mbits;
+    if (MASKED) reinterpret_cast<uint32_t*>(wm)[row * T::NV + v] = mbits;
   }
 }
 
@@ -862,21 +858,13 @@ __global__ __launch_bounds__(kThreads, 4) void ncc_tile_fast(TileArgs a) {
 }
 
 // Search mask values 4v .. 4v+3 of the staged row at float offset ro.
-template <class T, bool FMASK>
+template <class T>
 __device__ __forceinline__ void load_mask(const uint8_t* wv, int ro, int v, float* vr) {
-  if (FMASK) {
-    const float4 m = reinterpret_cast<const float4*>(reinterpret_cast<const float*>(wv) + ro)[v];
-    vr[4 * v + 0] = m.x;
-    vr[4 * v + 1] = m.y;
-    vr[4 * v + 2] = m.z;
-    vr[4 * v + 3] = m.w;
-  } else {
-    const uint32_t m = reinterpret_cast<const uint32_t*>(wv + ro)[v];
-    vr[4 * v + 0] = (float)(m & 0xffu);
-    vr[4 * v + 1] = (float)((m >> 8) & 0xffu);
-    vr[4 * v + 2] = (float)((m >> 16) & 0xffu);
-    vr[4 * v + 3] = (float)(m >> 24);
-  }
+  const uint32_t m = reinterpret_cast<const uint32_t*>(wv + ro)[v];
+  vr[4 * v + 0] = (float)(m & 0xffu);
+  vr[4 * v + 1] = (float)((m >> 8) & 0xffu);
+  vr[4 * v + 2] = (float)((m >> 16) & 0xffu);
+  vr[4 * v + 3] = (float)(m >> 24);
 }
 
 // Masked path: sums over voxels where both masks are on. The window is
@@ -889,10 +877,14 @@ __device__ __forceinline__ void load_mask(const uint8_t* wv, int ro, int v, floa
 //    carry the same centring (one voxel on gives exactly 1).
 //  * Base block partly masked (base mask b): Sxy = Sum b x~ t,
 //    Syy = Sum b t^2, Sxx = Sum b x~^2 v (5 ops per voxel).
-template <class T, bool FMASK>
+template <class T>
 __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
-  constexpr int R = (T::GT + kThreads - 1) / kThreads;  // tasks per thread
+  constexpr int kRows = T::NW * T::NW;  // row tasks per block
   extern __shared__ float4 dyn4[];
+  __shared__ float rbest[T::NB * kRows], rsecond[T::NB * kRows];  // per-task records
+  __shared__ int rscan[T::NB * kRows];
+  __shared__ uint16_t live_list[T::NB * kRows];  // tasks that need arithmetic
+  __shared__ int s_nlive;
   __shared__ float4 xh4[T::NB * 16];  // base-masked centred blocks
   __shared__ unsigned long long bbits[T::NB];  // base mask bit per voxel
   __shared__ int todo_list[T::NB];
@@ -900,7 +892,6 @@ __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
   __shared__ unsigned long long s_fullbase;  // queued blocks whose base mask is fully on
   __shared__ unsigned long long zon[kMaxWindow];  // rows holding a search-mask-on voxel
   __shared__ int bvalid[T::NB];
-  __shared__ MergeSlot mrg[T::NB];
   __shared__ float s_ref;
 
   // Workgroup (q, part) searches every msplit-th queued block of tile q
@@ -930,20 +921,22 @@ __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
   const int wx0 = px - T::H, wy0 = py0 - T::H, wz0 = pz0 - T::H;
   const int total = T::PX * WY * WZ;
   float* wy_ = reinterpret_cast<float*>(dyn4);
-  uint8_t* wv_ = reinterpret_cast<uint8_t*>(wy_ + total);  // mask bytes, or floats with FMASK
+  uint8_t* wv_ = reinterpret_cast<uint8_t*>(wy_ + total);  // search mask bytes
 
   if (tid == 0) {
     const int ry = min(py0 + NG * (T::TY / 2), ny - T::BW);
     const int rz = min(pz0 + NG * (T::TZ / 2), nz - T::BW);
     s_ref = a.stats[lin3(nx, ny, px, ry, rz)].x;
   }
-  merge_init(mrg, T::NB);
   if (tid < T::NB) {
     bvalid[tid] = ((todo >> tid) & 1ull) ? 1 : 0;
     bbits[tid] = 0ull;
   }
   for (int i = tid; i < WZ; i += kThreads) zon[i] = 0ull;
-  if (tid == 0) s_fullbase = 0ull;
+  if (tid == 0) {
+    s_fullbase = 0ull;
+    s_nlive = 0;
+  }
   __syncthreads();
   if (tid < T::NB && ((todo >> tid) & 1ull)) {
     const int gy = by0 + tid % T::TY, gz = bz0 + tid / T::TY;
@@ -964,7 +957,7 @@ __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
   }
   __syncthreads();
   const float ref = s_ref;
-  stage_window<T, true, FMASK>(a, wy_, wv_, zon, WY, WZ, wx0, wy0, wz0, ref);
+  stage_window<T, true>(a, wy_, wv_, zon, WY, WZ, wx0, wy0, wz0, ref);
   for (int i = tid; i < T::NB * 64; i += kThreads) {
     const int bl = i / 64, k = i % 64;
     const int gy = by0 + bl % T::TY, gz = bz0 + bl / T::TY;
@@ -987,54 +980,69 @@ __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
 
   if (a.work && tid == 0)
     atomicAdd(a.work + blockIdx.x % kWorkStripe, tile_evals<T>(a, px, py0, pz0, bvalid));
-  // Tasks (block, cz, cy) run over the queued blocks only. A thread owns rr
-  // consecutive tasks (scan order within a block), so they span at most two
-  // blocks: one running record each, in registers.
-  static_assert(R <= T::NW * T::NW, "a thread's tasks span at most two blocks");
+  // Row tasks (block k, cz, cy) of the queued blocks, numbered
+  // k * NW^2 + cz * NW + cy. Each task's top-2 record goes to LDS: empty for
+  // rows outside the image, -2 at every candidate for rows whose window holds
+  // no search-mask-on voxel (Syy = 0), and otherwise computed by a thread.
+  // The rows to compute are compacted and dealt to the threads strided, so
+  // nearly every lane works; then one wave per block appends the block's
+  // records in scan order.
   const int ntodo = s_ntodo;
-  const int ntasks = ntodo * T::NW * T::NW;
-  const int rr = (ntasks + kThreads - 1) / kThreads;
-  const int t0 = tid * rr;
-  const int k0 = t0 / (T::NW * T::NW);
-  Top2 rec0, rec1;
-  top2_init(rec0);
-  top2_init(rec1);
+  const int ntasks = ntodo * kRows;
+  const int lox = max(px - T::H, 0), hix = min(px - T::H + T::NW - 1, nx - T::BW);
+  for (int base = 0; base < ntasks; base += kThreads) {  // workgroup-uniform
+    const int i = base + tid;
+    bool live = false;
+    if (i < ntasks) {
+      const int k = i / kRows, r = i % kRows;
+      const int bl = todo_list[k];
+      const int cz = r / T::NW, cy = r % T::NW;
+      const int bly = bl % T::TY, blz = bl / T::TY;
+      const int py = py0 + NG * bly, pz = pz0 + NG * blz;
+      const int czg = pz - T::H + cz, cyg = py - T::H + cy;
+      Top2 tr;
+      top2_init(tr);
+      if (czg >= max(pz - T::H, 0) && czg <= min(pz - T::H + T::NW - 1, nz - T::BW) &&
+          cyg >= max(py - T::H, 0) && cyg <= min(py - T::H + T::NW - 1, ny - T::BW)) {
+        unsigned long long rows_on = 0ull;
+#pragma unroll
+        for (int iz = 0; iz < 4; ++iz) rows_on |= zon[blz * NG + cz + iz];
+        if (((rows_on >> (bly * NG + cy)) & 0xfull) != 0ull) {
+          live = true;
+        } else {
+#pragma unroll
+          for (int cx = 0; cx < T::NW; ++cx) {
+            const int cxg = px - T::H + cx;
+            if (cxg < lox || cxg > hix) continue;
+            top2_step(tr, -2.0f, (cz * T::NW + cy) * T::NW + cx);
+          }
+        }
+      }
+      if (!live) {
+        rbest[i] = tr.best;
+        rsecond[i] = tr.second;
+        rscan[i] = tr.scan;
+      }
+    }
+    const unsigned long long bal = __ballot(live);
+    int wbase = 0;
+    if ((tid & 63) == 0 && bal != 0ull) wbase = atomicAdd(&s_nlive, (int)__popcll(bal));
+    wbase = __shfl(wbase, 0, 64);
+    if (live) live_list[wbase + __popcll(bal & ((1ull << (tid & 63)) - 1ull))] = (uint16_t)i;
+  }
+  __syncthreads();
+  const int nlive = s_nlive;
 #pragma unroll 1
-  for (int j = 0; j < rr; ++j) {
-    const int t = t0 + j;
-    if (t >= ntasks) break;
-    const int k = t / (T::NW * T::NW);
+  for (int j = tid; j < nlive; j += kThreads) {
+    const int i = live_list[j];
+    const int k = i / kRows, r = i % kRows;
     const int bl = todo_list[k];
-    const int r = t % (T::NW * T::NW);
-    const int cz = r / T::NW;
-    const int cy = r % T::NW;
-    Top2 tr;
-    top2_init(tr);
+    const int cz = r / T::NW, cy = r % T::NW;
     const int bly = bl % T::TY, blz = bl / T::TY;
     const int py = py0 + NG * bly, pz = pz0 + NG * blz;
-    const int lox = max(px - T::H, 0), hix = min(px - T::H + T::NW - 1, nx - T::BW);
-    const int loy = max(py - T::H, 0), hiy = min(py - T::H + T::NW - 1, ny - T::BW);
-    const int loz = max(pz - T::H, 0), hiz = min(pz - T::H + T::NW - 1, nz - T::BW);
     const int czg = pz - T::H + cz, cyg = py - T::H + cy;
-    if (czg < loz || czg > hiz || cyg < loy || cyg > hiy) continue;
-    // A task whose 4 planes x 4 rows of window hold no search-mask-on
-    // voxel: every candidate has Syy = 0 and scores -2.
-    unsigned long long rows_on = 0ull;
-#pragma unroll
-    for (int iz = 0; iz < 4; ++iz) rows_on |= zon[blz * NG + cz + iz];
-    if (((rows_on >> (bly * NG + cy)) & 0xfull) == 0ull) {
-#pragma unroll
-      for (int cx = 0; cx < T::NW; ++cx) {
-        const int cxg = px - T::H + cx;
-        if (cxg < lox || cxg > hix) continue;
-        top2_step(tr, -2.0f, (cz * T::NW + cy) * T::NW + cx);
-      }
-      if (k == k0)
-        rec0 = top2_append(rec0, tr);
-      else
-        rec1 = top2_append(rec1, tr);
-      continue;
-    }
+    Top2 tr;
+    top2_init(tr);
     float dmu[T::NW];
     const float2* srow = a.stats + lin3(nx, ny, 0, cyg, czg);
 #pragma unroll
@@ -1059,7 +1067,7 @@ __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
           for (int v = 0; v < T::NV; ++v) {
             const float4 q = yp[v];
             yr[4 * v + 0] = q.x; yr[4 * v + 1] = q.y; yr[4 * v + 2] = q.z; yr[4 * v + 3] = q.w;
-            load_mask<T, FMASK>(wv_, ro, v, vr);
+            load_mask<T>(wv_, ro, v, vr);
           }
           const float4 xq4 = xh4[bl * 16 + iz * 4 + iy];
           const float xh[4] = {xq4.x, xq4.y, xq4.z, xq4.w};
@@ -1096,7 +1104,7 @@ __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
           for (int v = 0; v < T::NV; ++v) {
             const float4 q = yp[v];
             yr[4 * v + 0] = q.x; yr[4 * v + 1] = q.y; yr[4 * v + 2] = q.z; yr[4 * v + 3] = q.w;
-            load_mask<T, FMASK>(wv_, ro, v, vr);
+            load_mask<T>(wv_, ro, v, vr);
           }
           const float4 xq4 = xh4[bl * 16 + iz * 4 + iy];
           const float xh[4] = {xq4.x, xq4.y, xq4.z, xq4.w};
@@ -1125,23 +1133,43 @@ __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
         top2_push(tr, ncc_score(sxy[cx], sxx[cx], s3[cx]), (cz * T::NW + cy) * T::NW + cx);
       }
     }
-    if (k == k0)
-      rec0 = top2_append(rec0, tr);
-    else
-      rec1 = top2_append(rec1, tr);
+    rbest[i] = tr.best;
+    rsecond[i] = tr.second;
+    rscan[i] = tr.scan;
   }
-  // (threads past the last task hold empty records: no merge)
-  if (k0 < ntodo) merge_phase1(mrg, todo_list[k0], rec0);
-  if (k0 + 1 < ntodo) merge_phase1(mrg, todo_list[k0 + 1], rec1);
   __syncthreads();
-  if (k0 < ntodo) merge_phase2(mrg, todo_list[k0], rec0);
-  if (k0 + 1 < ntodo) merge_phase2(mrg, todo_list[k0 + 1], rec1);
-  __syncthreads();
-  if (tid < T::NB && bvalid[tid] == 1) {
-    const int gy = by0 + tid % T::TY, gz = bz0 + tid / T::TY;
-    const int64_t b = bx + (int64_t)g.nb[0] * (gy + (int64_t)g.nb[1] * gz);
-    write_block_result<T>(a, b, px, py0 + NG * (tid % T::TY), pz0 + NG * (tid / T::TY),
-                          merge_result(mrg[tid]));
+  // Wave w merges blocks w, w + 4, ...: lane L appends rows
+  // [L*C, L*C + C) in order, then lanes combine pairwise in order.
+  constexpr int C = (kRows + 63) / 64;
+  const int lane = tid & 63;
+  for (int k = tid >> 6; k < ntodo; k += kThreads / 64) {  // wave-uniform
+    Top2 acc;
+    top2_init(acc);
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const int r = lane * C + c;
+      if (r < kRows) {
+        Top2 t;
+        t.best = rbest[k * kRows + r];
+        t.second = rsecond[k * kRows + r];
+        t.scan = rscan[k * kRows + r];
+        acc = top2_append(acc, t);
+      }
+    }
+#pragma unroll
+    for (int sh = 1; sh < 64; sh <<= 1) {
+      Top2 o;
+      o.best = __shfl_down(acc.best, sh, 64);
+      o.second = __shfl_down(acc.second, sh, 64);
+      o.scan = __shfl_down(acc.scan, sh, 64);
+      if ((lane & (2 * sh - 1)) == 0 && lane + sh < 64) acc = top2_append(acc, o);
+    }
+    if (lane == 0) {
+      const int bl = todo_list[k];
+      const int gy = by0 + bl % T::TY, gz = bz0 + bl / T::TY;
+      const int64_t b = bx + (int64_t)g.nb[0] * (gy + (int64_t)g.nb[1] * gz);
+      write_block_result<T>(a, b, px, py0 + NG * (bl % T::TY), pz0 + NG * (bl / T::TY), acc);
+    }
   }
 }
 
@@ -1483,20 +1511,12 @@ void run_tiled(const BMGrid& g, TileArgs a, BMScratch& scr, hipStream_t s) {
   MIRORR_CHECK_LAUNCH();
   if (scr.prof) scr.prof->end(kProfFast, s);
   const int WYm = (T::TY - 1) * g.ng + T::WX;
-  // The search mask is staged as floats (no conversions in the inner loop)
-  // when that window fits, else as bytes.
-  const size_t lds_fmask = 2 * sizeof(float) * (size_t)T::PX * WYm * WZ;
-  const bool fmask = lds_fmask + masked_static_lds<T>() <= 65536;
-  const size_t lds_masked = fmask ? lds_fmask : (sizeof(float) + 1) * (size_t)T::PX * WYm * WZ;
+  const size_t lds_masked = (sizeof(float) + 1) * (size_t)T::PX * WYm * WZ;
   // Few tiles (a coarse level): split each queued tile over four workgroups.
   a.msplit = nt < 2048 ? 4 : 1;
   if (scr.prof) a.work = scr.prof->begin("ncc_tile_masked", kProfMasked, s);
-  if (fmask)
-    hipLaunchKernelGGL((ncc_tile_masked<T, true>), dim3((unsigned)(nt * a.msplit)), dim3(kThreads),
-                       lds_masked, s, a);
-  else
-    hipLaunchKernelGGL((ncc_tile_masked<T, false>), dim3((unsigned)(nt * a.msplit)),
-                       dim3(kThreads), lds_masked, s, a);
+  hipLaunchKernelGGL(ncc_tile_masked<T>, dim3((unsigned)(nt * a.msplit)), dim3(kThreads),
+                     lds_masked, s, a);
   MIRORR_CHECK_LAUNCH();
   if (scr.prof) scr.prof->end(kProfMasked, s);
 }

@@ -93,8 +93,6 @@ def _fov_mask(shape, normal, offset):
     return (v > 0).astype(np.uint8) * 255
 
 
-# (5, 4) stages the search mask as bytes (the float-staged window would not
-# fit in 64 KB of LDS); the others stage it as floats.
 @pytest.mark.parametrize("nw,ng", [(5, 4), (7, 4), (9, 3)])
 def test_masked_matches_oracle(oracle, nw, ng):
     shape = (28, 32, 36)
