@@ -94,31 +94,43 @@ __global__ __launch_bounds__(kThreads) void search_stats4_kernel(
   if (x > nx - 4 || yy > ny - 4) return;
   const int zend = min(z0 + kStatsZ, nz - 3);  // exclusive end of output z
   if (z0 >= zend) return;
-  const double c0 = (double)y[lin3(nx, ny, x, yy, z0)];
+  const int64_t nxy = (int64_t)nx * ny;
+  const float* ycol = y + lin3(nx, ny, x, yy, 0);  // the column's (x, y); + z * nxy
+  const uint8_t* mcol = MASK ? sm + lin3(nx, ny, x, yy, 0) : nullptr;
+  const double c0 = (double)ycol[(int64_t)z0 * nxy];
   double ps[4], p1[4], p2[4];
   int pn[4];
   float pmin[4], pmax[4];
   auto load = [&](int slot, int z) {
+    const float* yp = ycol + (int64_t)z * nxy;
+    float v[16];
+    uint32_t mw[4] = {0x01010101u, 0x01010101u, 0x01010101u, 0x01010101u};
+#pragma unroll
+    for (int dy = 0; dy < 4; ++dy) {
+#pragma unroll
+      for (int dx = 0; dx < 4; ++dx) v[4 * dy + dx] = yp[dy * nx + dx];
+      if (MASK) {
+        const uint8_t* mr = mcol + (int64_t)z * nxy + dy * nx;
+        mw[dy] = (uint32_t)mr[0] | ((uint32_t)mr[1] << 8) | ((uint32_t)mr[2] << 16) |
+                 ((uint32_t)mr[3] << 24);
+      }
+    }
     double s = 0.0, s1 = 0.0, s2 = 0.0;
     int cnt = 0;
     float lo = __builtin_huge_valf(), hi = -__builtin_huge_valf();
 #pragma unroll
-    for (int dy = 0; dy < 4; ++dy) {
-      const int64_t o = lin3(nx, ny, x, yy + dy, z);
-#pragma unroll
-      for (int dx = 0; dx < 4; ++dx) {
-        const float v = y[o + dx];
-        lo = fminf(lo, v);
-        hi = fmaxf(hi, v);
-        const double dv = (double)v;
-        s += dv;
-        if (!MASK || sm[o + dx] != 0) {
-          const double d = dv - c0;
-          s1 += d;
-          s2 = fma(d, d, s2);
-          ++cnt;
-        }
-      }
+    for (int k = 0; k < 16; ++k) {
+      lo = fminf(lo, v[k]);
+      hi = fmaxf(hi, v[k]);
+      s += (double)v[k];
+      // An off voxel adds exact zeros, so the sums are those over the on
+      // voxels, in the same order, without a branch.
+      const double d = (double)v[k] - c0;
+      const bool on = !MASK || ((mw[k >> 2] >> (8 * (k & 3))) & 0xffu) != 0u;
+      const double dw = on ? d : 0.0;
+      s1 += dw;
+      s2 = fma(dw, d, s2);
+      cnt += on ? 1 : 0;
     }
     ps[slot] = s;
     p1[slot] = s1;
@@ -139,22 +151,24 @@ __global__ __launch_bounds__(kThreads) void search_stats4_kernel(
     if (lo == hi || n == 0.0) ss = 0.0;
     out[lin3(nx, ny, x, yy, z)] = make_float2((float)mu, ss > 0.0 ? (float)(1.0 / ss) : -__builtin_huge_valf());
   };
+  // Slots 0..3 hold the planes z .. z + 3 of the current window: one plane
+  // enters per step and the slots shift down.
   load(0, z0);
   load(1, z0 + 1);
   load(2, z0 + 2);
-  // Plane z + k + 3 goes to slot (k + 3) % 4; the slots are a ring.
-  for (int z = z0; z < zend; z += 4) {
+#pragma unroll 1
+  for (int z = z0; z < zend; ++z) {
     load(3, z + 3);
     emit(z);
-    if (z + 1 >= zend) break;
-    load(0, z + 4);
-    emit(z + 1);
-    if (z + 2 >= zend) break;
-    load(1, z + 5);
-    emit(z + 2);
-    if (z + 3 >= zend) break;
-    load(2, z + 6);
-    emit(z + 3);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      ps[k] = ps[k + 1];
+      p1[k] = p1[k + 1];
+      p2[k] = p2[k + 1];
+      pn[k] = pn[k + 1];
+      pmin[k] = pmin[k + 1];
+      pmax[k] = pmax[k + 1];
+    }
   }
 }
 
