@@ -1,7 +1,8 @@
 """Builds the HIP library mirorr_amd/lib/libmirorr_gpu.so for gfx950.
 
 hipcc compiles every source in mirorr_amd/csrc (device code for gfx950 only)
-and links one shared library exporting the C ABI of include/mirorr_gpu.h.
+and links one shared library exporting the C ABI of include/mirorr_gpu.h and
+include/mirorr_io.h (the file I/O part links zlib).
 """
 from __future__ import annotations
 
@@ -47,7 +48,7 @@ def build(verbose: bool = False) -> str:
     with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lz"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")

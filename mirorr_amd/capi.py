@@ -1,4 +1,4 @@
-"""ctypes binding of the C ABI (include/mirorr_gpu.h).
+"""ctypes binding of the C ABI (include/mirorr_gpu.h, include/mirorr_io.h).
 
 Loads mirorr_amd/lib/libmirorr_gpu.so, built by mirorr_amd/build.py. There is
 no fallback: if the library is missing, importing the device path raises.
@@ -104,6 +104,16 @@ _SIGS = {
     "mirorr_resample": (C.c_int, [_P, _P, C.POINTER(Geom), _P, _P, C.POINTER(Geom), _P, _P]),
     "mirorr_pyramid_level": (C.c_int, [_P, _P, C.POINTER(Geom), _P, C.c_int, C.c_int, _P,
                                        C.POINTER(Geom)]),
+    # include/mirorr_io.h
+    "mirorr_io_last_error": (C.c_char_p, []),
+    "mirorr_io_free": (None, [_P]),
+    "mirorr_nifti_read": (C.c_int, [C.c_char_p, C.POINTER(Geom), C.POINTER(_P)]),
+    "mirorr_nifti_read_mask": (C.c_int, [C.c_char_p, C.POINTER(Geom), C.POINTER(_P)]),
+    "mirorr_nifti_write": (C.c_int, [C.c_char_p, C.POINTER(Geom), _P]),
+    "mirorr_reorient_ari": (C.c_int, [C.POINTER(Geom), _P, _P]),
+    "mirorr_centered_init": (C.c_int, [C.POINTER(Geom), C.POINTER(Geom), C.c_int, _P, _P]),
+    "mirorr_tfm_write": (C.c_int, [C.c_char_p, C.c_int, _P, _P, C.c_int]),
+    "mirorr_tfm_read": (C.c_int, [C.c_char_p, C.c_int, _P, _P, C.c_int]),
 }
 
 
@@ -131,6 +141,15 @@ def check(status: int, ctx=None):
     if status == OK:
         return
     msg = lib().mirorr_last_error(ctx).decode() if ctx else "error"
+    cls = {ERR_PLATFORM: PlatformError, ERR_ALGO: AlgoError}.get(status, MirorrError)
+    raise cls(status, msg)
+
+
+def check_io(status: int):
+    """Raises for a failed call of the file I/O part (include/mirorr_io.h)."""
+    if status == OK:
+        return
+    msg = lib().mirorr_io_last_error().decode()
     cls = {ERR_PLATFORM: PlatformError, ERR_ALGO: AlgoError}.get(status, MirorrError)
     raise cls(status, msg)
 
