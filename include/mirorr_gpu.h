@@ -235,6 +235,24 @@ mirorr_status mirorr_resample(mirorr_ctx* ctx, const float* data, const mirorr_g
                               const double matrix[9], const double offset[3],
                               const mirorr_geom* gout, float* out, uint8_t* out_mask);
 
+/* Interpolators of the final output resampling (--final-interpolator). */
+#define MIRORR_INTERP_NN 0      /* "nn": NearestNeighborInterpolateImageFunction        */
+#define MIRORR_INTERP_LINEAR 1  /* "linear": LinearInterpolateImageFunction             */
+#define MIRORR_INTERP_BSPLINE 2 /* "bspline" (default): cubic BSplineInterpolateImage-  */
+                                /* Function, mirror boundaries                          */
+#define MIRORR_INTERP_SINC 3    /* "sinc": WindowedSincInterpolateImageFunction, radius */
+                                /* 5, Welch window, zero-flux Neumann edges             */
+
+/* Final output resampling of the command line's --save-resampled-fixed/-moving,
+ * --save-fixed and --save-moving (itkMirorrPyramidImplement.txx:364-428
+ * GetInterpolatorFromString, 436-470 GetResampledImage): ResampleImageFilter
+ * with default value 0, out(p) = in(matrix * p + offset) for the physical
+ * points p of gout, evaluated only inside [-0.5, n-0.5) of the input grid.
+ * Host arrays; dims of gout at most 65535 per axis. */
+mirorr_status mirorr_resample_final(mirorr_ctx* ctx, const float* data, const mirorr_geom* gin,
+                                    const double matrix[9], const double offset[3],
+                                    const mirorr_geom* gout, int interpolator, float* out);
+
 /* Level `level` (0 = coarsest) of the recursive Gaussian pyramid of `data`
  * for a schedule of rows x 3 shrink factors (coarsest first). out holds the
  * level image; out_geom its geometry. */

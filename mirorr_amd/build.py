@@ -2,7 +2,8 @@
 
 hipcc compiles every source in mirorr_amd/csrc (device code for gfx950 only)
 and links one shared library exporting the C ABI of include/mirorr_gpu.h and
-include/mirorr_io.h (the file I/O part links zlib).
+include/mirorr_io.h (the file I/O part links zlib), then the `mirorr` command-line
+executable mirorr_amd/bin/mirorr over that library.
 """
 from __future__ import annotations
 
@@ -15,6 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "lib", "obj")
 LIB = os.path.join(HERE, "lib", "libmirorr_gpu.so")
+CLI = os.path.join(HERE, "bin", "mirorr")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
@@ -53,9 +55,24 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_cli()
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", CLI)
     return LIB
+
+
+def build_cli() -> str:
+    """The `mirorr` command-line executable (host code over the C ABI),
+    linked to the library with a relative run path."""
+    os.makedirs(os.path.dirname(CLI), exist_ok=True)
+    tmp = CLI + ".tmp"
+    cmd = [HIPCC, *COMMON, *DEVICE, os.path.join(HERE, "cli", "mirorr.cpp"), "-o", tmp,
+           f"-L{os.path.dirname(LIB)}", "-lmirorr_gpu", "-Wl,-rpath,$ORIGIN/../lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed on the command line tool:\n{r.stderr}")
+    os.replace(tmp, CLI)
+    return CLI
 
 
 if __name__ == "__main__":

@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libmirorr_gpu.so")
 
 OK, ERR_ARG, ERR_PLATFORM, ERR_ALGO, ERR_NOMEM, ERR_STATE = range(6)
 ROLE_BASE, ROLE_SEARCH = 0, 1
+INTERP_NN, INTERP_LINEAR, INTERP_BSPLINE, INTERP_SINC = range(4)
 
 
 class MirorrError(RuntimeError):
@@ -104,6 +105,8 @@ _SIGS = {
     "mirorr_resample": (C.c_int, [_P, _P, C.POINTER(Geom), _P, _P, C.POINTER(Geom), _P, _P]),
     "mirorr_pyramid_level": (C.c_int, [_P, _P, C.POINTER(Geom), _P, C.c_int, C.c_int, _P,
                                        C.POINTER(Geom)]),
+    "mirorr_resample_final": (C.c_int, [_P, _P, C.POINTER(Geom), _P, _P, C.POINTER(Geom),
+                                        C.c_int, _P]),
     # include/mirorr_io.h
     "mirorr_io_last_error": (C.c_char_p, []),
     "mirorr_io_free": (None, [_P]),

@@ -23,35 +23,6 @@ namespace {
 
 constexpr int kThreads = 256;
 
-__device__ __forceinline__ int64_t lin3(int nx, int ny, int x, int y, int z) {
-  return (int64_t)x + (int64_t)nx * ((int64_t)y + (int64_t)ny * z);
-}
-
-template <class T>
-__device__ __forceinline__ double trilinear(const T* __restrict__ img, int nx, int ny, int nz,
-                                            double cx, double cy, double cz, bool* inside) {
-  if (!(cx >= -0.5 && cx < nx - 0.5 && cy >= -0.5 && cy < ny - 0.5 && cz >= -0.5 &&
-        cz < nz - 0.5)) {
-    *inside = false;
-    return 0.0;
-  }
-  *inside = true;
-  int bx = (int)floor(cx), by = (int)floor(cy), bz = (int)floor(cz);
-  double fx = cx - bx, fy = cy - by, fz = cz - bz;
-  if (bx < 0) { bx = 0; fx = 0.0; }
-  if (by < 0) { by = 0; fy = 0.0; }
-  if (bz < 0) { bz = 0; fz = 0.0; }
-  const int x1 = min(bx + 1, nx - 1), y1 = min(by + 1, ny - 1), z1 = min(bz + 1, nz - 1);
-  const double v000 = img[lin3(nx, ny, bx, by, bz)], v100 = img[lin3(nx, ny, x1, by, bz)];
-  const double v010 = img[lin3(nx, ny, bx, y1, bz)], v110 = img[lin3(nx, ny, x1, y1, bz)];
-  const double v001 = img[lin3(nx, ny, bx, by, z1)], v101 = img[lin3(nx, ny, x1, by, z1)];
-  const double v011 = img[lin3(nx, ny, bx, y1, z1)], v111 = img[lin3(nx, ny, x1, y1, z1)];
-  const double x00 = v000 + (v100 - v000) * fx, x10 = v010 + (v110 - v010) * fx;
-  const double x01 = v001 + (v101 - v001) * fx, x11 = v011 + (v111 - v011) * fx;
-  const double xx0 = x00 + (x10 - x00) * fy, xx1 = x01 + (x11 - x01) * fy;
-  return xx0 + (xx1 - xx0) * fz;
-}
-
 __global__ __launch_bounds__(kThreads) void resample_kernel(
     const float* __restrict__ in, int inx, int iny, int inz, const IndexMap* __restrict__ map,
     const uint8_t* __restrict__ mask, int mnx, int mny, int mnz,

@@ -316,7 +316,9 @@ void run_registration(mirorr_ctx* ctx, const mirorr_reg_params* p, const mirorr_
   build_pyramid(moving->data.as<float>(), MG, ms, nlev, Mp, w.pyr_tmp, w.pyr_map, bud, s);
   const int t_max = std::min(cropped_level(level_max, nlev, rows), nlev);
   // Initial transform: given, or centred (itkMirorrPyramidWrapper.txx:611-623:
-  // centre of the moving image, translation between the image centres).
+  // centre of the moving image, translation between the image centres), then
+  // composed into an identity transform with centre 0 as the command line
+  // does before RunMirorr (itkMirorrPyramidWrapper.txx:626-640).
   xf::Xf T = xf::identity(run.kind);
   if (init_params) {
     if (init_center) xf::set_center(T, init_center);
@@ -328,6 +330,7 @@ void run_registration(mirorr_ctx* ctx, const mirorr_reg_params* p, const mirorr_
     for (int k = 0; k < 3; ++k) t[k] = cF[k] - cM[k];
     xf::set_center(T, cM);
     xf::set_translation(T, t);
+    xf::set_center_to_origin(T);
   }
   double cur[12];
   xf::get_params(T, cur);

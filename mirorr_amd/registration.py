@@ -171,6 +171,26 @@ def resample(data, gin: capi.Geom, matrix, offset, gout: capi.Geom, want_mask=Fa
     return (out, m) if want_mask else out
 
 
+_INTERP = {"nn": capi.INTERP_NN, "linear": capi.INTERP_LINEAR, "bspline": capi.INTERP_BSPLINE,
+           "sinc": capi.INTERP_SINC}
+
+
+def resample_final(data, gin: capi.Geom, matrix, offset, gout: capi.Geom, interpolator="bspline",
+                   ctx=None):
+    """Final output resampling (GetResampledImage): out(p) = data(matrix p +
+    offset) on gout with the named interpolator (nn, linear, bspline, sinc)."""
+    ctx = ctx or default_context()
+    d = np.ascontiguousarray(data, np.float32)
+    out = np.zeros(tuple(gout.dims)[::-1], np.float32)
+    M = np.ascontiguousarray(matrix, np.float64).reshape(9)
+    o = np.ascontiguousarray(offset, np.float64).reshape(3)
+    capi.check(capi.lib().mirorr_resample_final(ctx.handle, capi.ptr(d), C.byref(gin),
+                                                capi.ptr(M), capi.ptr(o), C.byref(gout),
+                                                _INTERP[interpolator], capi.ptr(out)),
+               ctx.handle)
+    return out
+
+
 def pyramid_level(data, geom: capi.Geom, schedule, level, ctx=None):
     ctx = ctx or default_context()
     data = np.ascontiguousarray(data, np.float32)

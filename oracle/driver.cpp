@@ -370,6 +370,9 @@ static int register_impl(const float* fixed, const uint8_t* fmask, const orc_geo
     const double t[3] = {cF[0] - cM[0], cF[1] - cM[1], cF[2] - cM[2]};
     T.set_center(cM);
     T.set_translation(t);
+    // Composed into an identity transform with centre 0 before RunMirorr
+    // (itkMirorrPyramidWrapper.txx:626-640).
+    set_center_to_origin(T);
   }
   double cur[12];
   T.get_params(cur);

@@ -1,0 +1,61 @@
+"""End to end through the `mirorr` command line on the GPU: NIfTI in,
+transform file and resampled image out, compared with the CPU oracle run on
+the same (ARI-reoriented) volumes and with the oracle's final resampling."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from mirorr_amd import fileio, phantom, transforms
+from oracle import final_resample as fr
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "mirorr_amd", "bin", "mirorr")
+
+
+class _G:
+    def __init__(self, v):
+        self.dims, self.spacing, self.origin, self.direction = (v.dims, v.spacing, v.origin,
+                                                                v.direction)
+
+
+@pytest.mark.parametrize("affine", [0, 1])
+def test_cli_matches_oracle(tmp_path, oracle, affine):
+    pair = phantom.make_pair(dims=(64, 64, 32), spacing=(3.0, 3.0, 3.0), n_ellipsoids=20,
+                             seed=11, model="affine" if affine else "rigid")
+    fileio.write_nifti(str(tmp_path / "mov.nii.gz"), pair.moving)
+    fileio.write_nifti(str(tmp_path / "fix.nii"), pair.fixed)
+    args = [CLI, "-m", "mov.nii.gz", "-f", "fix.nii", "-l", "out.tfm", "-t",
+            "affine" if affine else "rigid", "-d", "2", "-c", "0", "--fresh",
+            "--save-moving", "res.nii.gz"]
+    print("cli", " ".join(args[1:]), flush=True)
+    r = subprocess.run(args, cwd=tmp_path, capture_output=True, text=True, timeout=180)
+    assert r.returncode == 0, r.stderr + r.stdout[-2000:]
+    params, c = fileio.read_tfm(str(tmp_path / "out.tfm"), affine=bool(affine))
+    assert np.allclose(c, 0)
+    # The oracle on the volumes as the command line sees them: read back
+    # from the files and reoriented to ARI.
+    mov = fileio.reorient_ari(fileio.read_nifti(str(tmp_path / "mov.nii.gz")))
+    fix = fileio.reorient_ari(fileio.read_nifti(str(tmp_path / "fix.nii")))
+    rp = oracle.reg_params(symmetric=1, affine=affine, nw=7, ng=3, pyr_num=2, nthreads=8)
+    ref_p, ref_c, _ = oracle.register(fix, mov, rp)
+    M1, o1 = transforms.matrix_offset(params, c, affine)
+    M2, o2 = transforms.matrix_offset(ref_p, ref_c, affine)
+    dmm = transforms.max_corner_distance(M1, o1, M2, o2, pair.fixed)
+    ddeg = transforms.rotation_angle_deg(M1, M2)
+    print(f"cli vs oracle: corner distance {dmm:.3e} mm, rotation {ddeg:.3e} deg")
+    assert dmm < 0.01 and ddeg < 0.01
+    # --save-moving: the moving image resampled on the fixed grid under the
+    # inverse transform with the default cubic B-spline.
+    res = fileio.read_nifti(str(tmp_path / "res.nii.gz"))
+    m_in = fileio.read_nifti(str(tmp_path / "mov.nii.gz"))
+    f_in = fileio.read_nifti(str(tmp_path / "fix.nii"))
+    assert res.dims == f_in.dims
+    assert np.allclose(res.origin, f_in.origin) and np.allclose(res.spacing, f_in.spacing)
+    Mi = np.linalg.inv(M1)
+    ref = fr.resample_final(m_in.data, _G(m_in), Mi.ravel(), -Mi @ o1, _G(f_in), "bspline")
+    bad = ~np.isclose(res.data, ref, rtol=1e-5, atol=1e-2)
+    assert bad.mean() < 2e-3, (bad.sum(), np.abs(res.data - ref).max())
