@@ -66,7 +66,15 @@ typedef struct mirorr_bm_params {
                             code is disabled at itkBlockMatcher.txx:664-719);
                             1: keep the top portion_kept blocks by variance */
   int verbosity;         /* 0..3                                             */
+  int block_metric;      /* MIRORR_METRIC_NC (default), _SSD or _CR          */
 } mirorr_bm_params;
+
+/* Block metrics (--blockmetric nc | sd | cr; BlockMatcher's metric map,
+ * itkBlockMatcher.h:316-323). The search keeps the largest score under every
+ * metric, as the reference does. */
+#define MIRORR_METRIC_NC 0  /* normalizedCorrelation, itkBlockMatcher.txx:46-101  */
+#define MIRORR_METRIC_SSD 1 /* sumOfSquareDifference, itkBlockMatcher.txx:195-227 */
+#define MIRORR_METRIC_CR 2  /* correlationRatio, itkBlockMatcher.txx:105-191      */
 
 enum { MIRORR_ROLE_BASE = 0, MIRORR_ROLE_SEARCH = 1 };
 
@@ -188,6 +196,7 @@ typedef struct mirorr_reg_params {
    * (itkMirorrRegistrationMethod.txx:785-786). 0 disables early stopping. */
   double stop_tolerance;
   double lts_tolerance;
+  int block_metric;   /* --blockmetric: MIRORR_METRIC_NC (default), _SSD, _CR    */
 } mirorr_reg_params;
 
 typedef struct mirorr_reg_stats {

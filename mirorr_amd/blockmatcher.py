@@ -147,6 +147,15 @@ class BlockMatcher:
     def GetPortionMatchesKept(self): return self._p.portion_kept
     def SetVarianceSelection(self, on: bool): self._set("variance_select", 1 if on else 0)
 
+    def SetBlockMetricType(self, name: str):
+        """normalized_correlation (default), sum_of_squared_differences or
+        correlation_ratio (BlockMatcher::SetBlockMetricType, itkBlockMatcher.h:143-175)."""
+        if name not in capi.METRIC_NAMES:
+            raise ValueError(f"{name} is not a valid blockmetric type")
+        self._set("block_metric", capi.METRIC_NAMES[name])
+
+    def GetBlockMetric(self): return self._p.block_metric
+
     # -- images (itkAbstractBlockMatcher.h:85-95) --------------------------
     def SetBaseImage(self, vol):
         self._img[0] = vol

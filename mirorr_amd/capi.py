@@ -14,6 +14,11 @@ LIB_PATH = os.path.join(HERE, "lib", "libmirorr_gpu.so")
 OK, ERR_ARG, ERR_PLATFORM, ERR_ALGO, ERR_NOMEM, ERR_STATE = range(6)
 ROLE_BASE, ROLE_SEARCH = 0, 1
 INTERP_NN, INTERP_LINEAR, INTERP_BSPLINE, INTERP_SINC = range(4)
+METRIC_NC, METRIC_SSD, METRIC_CR = range(3)
+# BlockMatcher's metric names (itkBlockMatcher.h:316-323) and the CLI's short ones.
+METRIC_NAMES = {"normalized_correlation": METRIC_NC, "nc": METRIC_NC,
+                "sum_of_squared_differences": METRIC_SSD, "sd": METRIC_SSD,
+                "correlation_ratio": METRIC_CR, "cr": METRIC_CR}
 
 
 class MirorrError(RuntimeError):
@@ -50,7 +55,8 @@ class Geom(C.Structure):
 class BMParams(C.Structure):
     _fields_ = [("block_width", C.c_int), ("block_gap", C.c_int), ("nhood_width", C.c_int),
                 ("nhood_gap", C.c_int), ("portion_kept", C.c_double),
-                ("variance_select", C.c_int), ("verbosity", C.c_int)]
+                ("variance_select", C.c_int), ("verbosity", C.c_int),
+                ("block_metric", C.c_int)]
 
 
 class RegParams(C.Structure):
@@ -59,7 +65,8 @@ class RegParams(C.Structure):
                 ("iterations", C.c_int), ("pyr_start", C.c_int), ("pyr_end", C.c_int),
                 ("pyr_num", C.c_int), ("pyr_min_size", C.c_int), ("portion_kept", C.c_double),
                 ("variance_select", C.c_int), ("verbosity", C.c_int), ("centre_init", C.c_int),
-                ("stop_tolerance", C.c_double), ("lts_tolerance", C.c_double)]
+                ("stop_tolerance", C.c_double), ("lts_tolerance", C.c_double),
+                ("block_metric", C.c_int)]
 
 
 class RegStats(C.Structure):

@@ -6,7 +6,7 @@
 // Same option names, short forms, defaults, positional order (moving, fixed,
 // last-tfm, start-tfm), output-transform naming, --fresh rule and output
 // images. Options that select work this build does not implement
-// (--reorient-*, --crop, --blockmetric other than nc, --tfm-type quat,
+// (--reorient-*, --crop, --blockmetric mi, --tfm-type quat,
 // --resampling-mode other than middle) fail with a message instead of being
 // ignored; --nthreads and --use-gpu-bm are accepted and have no effect.
 #include <sys/stat.h>
@@ -76,7 +76,7 @@ const Opt kOptions[] = {
     {"save-fixed", 0, true, nullptr, "Save the fixed image resampled in moving space"},
     {"save-moving", 0, true, nullptr, "Save the moving image resampled in fixed space"},
     {"no-centering-init", '0', false, nullptr, "Do not centre the images before registration"},
-    {"blockmetric", 0, true, "nc", "Block metric (nc)"},
+    {"blockmetric", 0, true, "nc", "Block metric: nc, sd or cr"},
     {"npw-bins", 0, true, "32", "Accepted for compatibility"},
     {"npw-scale-factor", 0, true, "2", "Accepted for compatibility"},
     {"npw-shape-expansion", 0, false, nullptr, "Accepted for compatibility"},
@@ -385,10 +385,18 @@ int run(int argc, char** argv) {
     throw std::runtime_error("--resampling-mode " + a.str("resampling-mode") +
                              " is not supported by this build (middle only)");
   const std::string metric = a.str("blockmetric");
-  if (metric != "nc") {
-    if (metric == "sd" || metric == "cr" || metric == "mi")
-      throw std::runtime_error("--blockmetric " + metric + " is not supported by this build (nc only)");
-    std::fprintf(stderr, "ERROR: %s is not a valid blockmetric type. Exiting!\n", metric.c_str());
+  int block_metric = MIRORR_METRIC_NC;
+  if (metric == "sd") {
+    block_metric = MIRORR_METRIC_SSD;
+  } else if (metric == "cr") {
+    block_metric = MIRORR_METRIC_CR;
+  } else if (metric == "mi") {
+    throw std::runtime_error("--blockmetric mi is not supported by this build (nc, sd or cr)");
+  } else if (metric != "nc") {
+    std::fprintf(stderr, "ERROR: %s is not a valid blockmetric type. Exiting!\n"
+                         "possible options are:\n  cr: correlation_ratio\n"
+                         "  mi: mutual_information\n  nc: normalized_correlation\n"
+                         "  sd: sum_of_squared_differences\n", metric.c_str());
     return -1;
   }
   const std::string tfm_type = a.str("tfm-type");
@@ -459,6 +467,7 @@ int run(int argc, char** argv) {
   rp.portion_kept = a.real("portion-kept");
   rp.verbosity = verbosity;
   rp.centre_init = !a.has("no-centering-init");
+  rp.block_metric = block_metric;
   if (a.integer("blockwidth") > 0) rp.block_width = a.integer("blockwidth");
   if (a.integer("blockgap") > 0) rp.block_gap = a.integer("blockgap");
   if (a.integer("nhoodwidth") > 0) rp.nhood_width = a.integer("nhoodwidth");

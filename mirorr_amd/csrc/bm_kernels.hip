@@ -22,6 +22,7 @@
 //  * Top-2 per block uses the float score buffer semantics of the reference
 //    with ties on the best broken by the earliest candidate in scan order.
 #include "bm_kernels.hpp"
+#include "../../include/mirorr_gpu.h"
 #include "select.hpp"
 
 #include <algorithm>
@@ -1264,6 +1265,215 @@ __global__ __launch_bounds__(kThreads) void ncc_generic_kernel(
 }
 
 // ---------------------------------------------------------------------------
+// The other block metrics (itkBlockMatcher.txx:105-227), searched like
+// ncc_generic_kernel: one workgroup per block, scan order, the same top-2
+// (the search keeps the LARGEST score whatever the metric).
+
+// Base block of workgroup `b` staged in LDS: values and the base-mask bit.
+__device__ __forceinline__ void stage_base_block(const float* __restrict__ x,
+                                                 const uint8_t* __restrict__ bm, const BMGrid& g,
+                                                 const int p[3], float* lx, uint8_t* lm) {
+  const int bw = g.bw, n3 = bw * bw * bw;
+  for (int i = threadIdx.x; i < n3; i += kThreads) {
+    const int dx = i % bw, dy = (i / bw) % bw, dz = i / (bw * bw);
+    const int64_t o = lin3(g.dims[0], g.dims[1], p[0] + dx, p[1] + dy, p[2] + dz);
+    lx[i] = x[o];
+    lm[i] = (!bm || bm[o] != 0) ? 1 : 0;
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void block_corner(const BMGrid& g, int64_t b, int p[3]) {
+  p[0] = g.start[0] + g.ng * (int)(b % g.nb[0]);
+  p[1] = g.start[1] + g.ng * (int)((b / g.nb[0]) % g.nb[1]);
+  p[2] = g.start[2] + g.ng * (int)(b / ((int64_t)g.nb[0] * g.nb[1]));
+}
+
+__device__ __forceinline__ void write_generic_result(const BMGrid& g, int64_t b,
+                                                     const int first[3], const int cnt[3],
+                                                     const Top2& r, float* out_best,
+                                                     float* out_second, int32_t* out_match) {
+  const int k = r.scan;
+  const int kx = k % cnt[0], ky = (k / cnt[0]) % cnt[1], kz = k / (cnt[0] * cnt[1]);
+  out_best[b] = r.best;
+  out_second[b] = r.second;
+  out_match[3 * b + 0] = first[0] + kx * g.bg;
+  out_match[3 * b + 1] = first[1] + ky * g.bg;
+  out_match[3 * b + 2] = first[2] + kz * g.bg;
+}
+
+// Sum of squared differences (itkBlockMatcher.txx:195-227): the mean of
+// (base - search)^2 over the voxels inside both masks, -2 when there are
+// none. One thread per candidate; the differences of two floats and their
+// squares are exact in double, so the sum matches the reference's to the
+// last rounding of the additions.
+__global__ __launch_bounds__(kThreads) void ssd_generic_kernel(
+    const float* __restrict__ x, const uint8_t* __restrict__ bm,
+    const float* __restrict__ y, const uint8_t* __restrict__ sm, BMGrid g,
+    float* out_best, float* out_second, int32_t* out_match, const uint32_t* flags) {
+  extern __shared__ float dyn[];
+  __shared__ float tb[kThreads], ts[kThreads];
+  __shared__ int ti[kThreads];
+  const int64_t b = blockIdx.x;
+  int p[3];
+  block_corner(g, b, p);
+  if (flags && (flags[b] & kBlockSkip)) {
+    if (threadIdx.x == 0) {
+      out_best[b] = -3.0f;
+      out_second[b] = -3.0f;
+      for (int k = 0; k < 3; ++k) out_match[3 * b + k] = p[k];
+    }
+    return;
+  }
+  const int bw = g.bw, n3 = bw * bw * bw, nx = g.dims[0], ny = g.dims[1];
+  float* lx = dyn;
+  uint8_t* lm = reinterpret_cast<uint8_t*>(dyn + n3);
+  stage_base_block(x, bm, g, p, lx, lm);
+  int first[3], cnt[3];
+  for (int k = 0; k < 3; ++k) cand_range(p[k], g.dims[k], g.bw, g.bg, g.nw, &first[k], &cnt[k]);
+  Top2 t2;
+  top2_init(t2);
+  const int total = cnt[0] * cnt[1] * cnt[2];
+  for (int k = threadIdx.x; k < total; k += kThreads) {
+    const int kx = k % cnt[0], ky = (k / cnt[0]) % cnt[1], kz = k / (cnt[0] * cnt[1]);
+    const int s0 = first[0] + kx * g.bg, s1 = first[1] + ky * g.bg, s2 = first[2] + kz * g.bg;
+    double sum = 0.0;
+    int count = 0;
+    int i = 0;
+    for (int dz = 0; dz < bw; ++dz)
+      for (int dy = 0; dy < bw; ++dy) {
+        const int64_t row = lin3(nx, ny, s0, s1 + dy, s2 + dz);
+        for (int dx = 0; dx < bw; ++dx, ++i) {
+          if (!lm[i] || (sm && sm[row + dx] == 0)) continue;
+          const double d = (double)lx[i] - (double)y[row + dx];
+          sum += d * d;
+          ++count;
+        }
+      }
+    top2_push(t2, count == 0 ? -2.0f : (float)(sum / count), k);
+  }
+  tb[threadIdx.x] = t2.best;
+  ti[threadIdx.x] = t2.scan;
+  ts[threadIdx.x] = t2.second;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    write_generic_result(g, b, first, cnt, top2_merge(tb, ti, ts, 0, kThreads), out_best,
+                         out_second, out_match);
+}
+
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Correlation ratio (itkBlockMatcher.txx:105-191). The voxels inside both
+// masks are grouped by the search intensity's bin floor(y * 0.1); a bin's
+// statistics use only base values above 10. The score is
+// 1 - (sum over bins with more than 2 such values of their variance) /
+// (variance of every such value), kept when above 0.4, else -2; -2 also
+// when fewer than 3 values count or the bin variances sum to 0 (and when no
+// voxel is inside both masks, where the reference reads an empty list).
+// One wave per candidate, lane l holding voxels l, l + 64, ...; the bins are
+// visited in increasing order (the order of the reference's sort), each by a
+// wave minimum and three wave sums in double. Sums within a bin are formed
+// in another order than the reference's sorted one, so scores agree to
+// rounding.
+__global__ __launch_bounds__(kThreads) void cr_generic_kernel(
+    const float* __restrict__ x, const uint8_t* __restrict__ bm,
+    const float* __restrict__ y, const uint8_t* __restrict__ sm, BMGrid g,
+    float* out_best, float* out_second, int32_t* out_match, const uint32_t* flags) {
+  extern __shared__ float dyn[];
+  constexpr int kWaves = kThreads / 64;
+  __shared__ float tb[kWaves], ts[kWaves];
+  __shared__ int ti[kWaves];
+  const int64_t b = blockIdx.x;
+  int p[3];
+  block_corner(g, b, p);
+  if (flags && (flags[b] & kBlockSkip)) {
+    if (threadIdx.x == 0) {
+      out_best[b] = -3.0f;
+      out_second[b] = -3.0f;
+      for (int k = 0; k < 3; ++k) out_match[3 * b + k] = p[k];
+    }
+    return;
+  }
+  const int bw = g.bw, n3 = bw * bw * bw, nx = g.dims[0], ny = g.dims[1];
+  float* lx = dyn;
+  uint8_t* lm = reinterpret_cast<uint8_t*>(dyn + n3);
+  stage_base_block(x, bm, g, p, lx, lm);
+  int first[3], cnt[3];
+  for (int k = 0; k < 3; ++k) cand_range(p[k], g.dims[k], g.bw, g.bg, g.nw, &first[k], &cnt[k]);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  Top2 t2;
+  top2_init(t2);
+  const int total = cnt[0] * cnt[1] * cnt[2];
+  for (int k = wave; k < total; k += kWaves) {  // wave-uniform
+    const int kx = k % cnt[0], ky = (k / cnt[0]) % cnt[1], kz = k / (cnt[0] * cnt[1]);
+    const int s0 = first[0] + kx * g.bg, s1 = first[1] + ky * g.bg, s2 = first[2] + kz * g.bg;
+    // Voxels of this lane: bin key (INT_MAX when outside a mask) and value.
+    uint64_t pending = 0;  // bit j: voxel lane + 64 j still to be binned
+    for (int i = lane, j = 0; i < n3; i += 64, ++j) {
+      const int dx = i % bw, dy = (i / bw) % bw, dz = i / (bw * bw);
+      const int64_t o = lin3(nx, ny, s0 + dx, s1 + dy, s2 + dz);
+      if (lm[i] && (!sm || sm[o] != 0)) pending |= 1ull << j;
+    }
+    double sum_var = 0.0, sx = 0.0, sx2 = 0.0, cx = 0.0;
+    while (true) {
+      int lane_min = INT_MAX;
+      for (int i = lane, j = 0; i < n3; i += 64, ++j) {
+        if (!((pending >> j) & 1)) continue;
+        const int dx = i % bw, dy = (i / bw) % bw, dz = i / (bw * bw);
+        const float yv = y[lin3(nx, ny, s0 + dx, s1 + dy, s2 + dz)];
+        lane_min = min(lane_min, (int)floor((double)yv * 0.1));
+      }
+      const int key = wave_min(lane_min);
+      if (key == INT_MAX) break;
+      double s = 0.0, s2v = 0.0, c = 0.0;
+      for (int i = lane, j = 0; i < n3; i += 64, ++j) {
+        if (!((pending >> j) & 1)) continue;
+        const int dx = i % bw, dy = (i / bw) % bw, dz = i / (bw * bw);
+        const float yv = y[lin3(nx, ny, s0 + dx, s1 + dy, s2 + dz)];
+        if ((int)floor((double)yv * 0.1) != key) continue;
+        pending &= ~(1ull << j);
+        const double xv = lx[i];
+        if (xv > 10.0) {
+          s += xv;
+          s2v += xv * xv;
+          c += 1.0;
+        }
+      }
+      s = wave_sum(s);
+      s2v = wave_sum(s2v);
+      c = wave_sum(c);
+      if (c > 2.0) {
+        const double m = s / c;
+        sum_var += s2v / c - m * m;
+      }
+      sx += s;
+      sx2 += s2v;
+      cx += c;
+    }
+    float score = -2.0f;
+    if (cx > 2.0 && sum_var > 0.0) {
+      const double m = sx / cx;
+      const double cr = 1.0 - sum_var / (sx2 / cx - m * m);
+      if (cr > 0.4) score = (float)cr;
+    }
+    top2_push(t2, score, k);  // every lane, identical up to rounding; lane 0 reports
+  }
+  if (lane == 0) {
+    tb[wave] = t2.best;
+    ti[wave] = t2.scan;
+    ts[wave] = t2.second;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    write_generic_result(g, b, first, cnt, top2_merge(tb, ti, ts, 0, kWaves), out_best,
+                         out_second, out_match);
+}
+
+// ---------------------------------------------------------------------------
 __global__ void postprocess_kernel(BMGrid g, Geom geom, const float* __restrict__ best,
                                    const float* __restrict__ second,
                                    const int32_t* __restrict__ match, uint8_t* kept,
@@ -1637,6 +1847,25 @@ void launch_ncc_generic(const float* x, const uint8_t* bmask, const float* y,
   if (g.nblocks == 0) return;
   hipLaunchKernelGGL(ncc_generic_kernel, dim3((unsigned)g.nblocks), dim3(kThreads), 0, s, x,
                      bmask, y, smask, g, res.best, res.second, res.match, flags);
+  MIRORR_CHECK_LAUNCH();
+}
+
+void launch_metric_search(int metric, const float* x, const uint8_t* bmask, const float* y,
+                          const uint8_t* smask, const BMGrid& g, BMResult res, hipStream_t s,
+                          const uint32_t* flags) {
+  if (g.nblocks == 0) return;
+  if (g.bw > kMaxMetricBW) throw Error(kErrArg, "block width above 16 for this block metric");
+  const size_t lds = (size_t)g.bw * g.bw * g.bw * (sizeof(float) + 1);
+  if (metric == MIRORR_METRIC_SSD) {
+    hipLaunchKernelGGL(ssd_generic_kernel, dim3((unsigned)g.nblocks), dim3(kThreads), lds, s, x,
+                       bmask, y, smask, g, res.best, res.second, res.match, flags);
+  } else if (metric == MIRORR_METRIC_CR) {
+    hipLaunchKernelGGL(cr_generic_kernel, dim3((unsigned)g.nblocks), dim3(kThreads), lds, s, x,
+                       bmask, y, smask, g, res.best, res.second, res.match, flags);
+  } else {
+    launch_ncc_generic(x, bmask, y, smask, g, res, s, flags);
+    return;
+  }
   MIRORR_CHECK_LAUNCH();
 }
 

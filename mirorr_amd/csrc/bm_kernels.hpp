@@ -78,6 +78,13 @@ void launch_ncc_search(const float* x, const uint8_t* bmask, const float* y,
 void launch_ncc_generic(const float* x, const uint8_t* bmask, const float* y,
                         const uint8_t* smask, const BMGrid& g, BMResult res,
                         hipStream_t s, const uint32_t* flags = nullptr);
+// Exhaustive search under another block metric (MIRORR_METRIC_SSD or
+// MIRORR_METRIC_CR; MIRORR_METRIC_NC runs launch_ncc_generic), any BW up to
+// kMaxMetricBW, BG and NW.
+constexpr int kMaxMetricBW = 16;
+void launch_metric_search(int metric, const float* x, const uint8_t* bmask, const float* y,
+                          const uint8_t* smask, const BMGrid& g, BMResult res, hipStream_t s,
+                          const uint32_t* flags = nullptr);
 int64_t tile_count(const BMGrid& g);
 bool tiled_supported(const BMGrid& g);
 

@@ -51,6 +51,8 @@ static void check_params(const mirorr_bm_params* p) {
   // SetPortionMatchesKept (itkAbstractBlockMatcher.txx:83-99): 0 means 0.5.
   if (p->portion_kept < 0.0 || p->portion_kept > 1.0)
     throw Error(kErrArg, "portion_kept should be between 0 and 1.0");
+  if (p->block_metric < MIRORR_METRIC_NC || p->block_metric > MIRORR_METRIC_CR)
+    throw Error(kErrArg, "unknown block metric");
 }
 
 static BMGrid current_grid(const mirorr_bm* bm) {
@@ -73,7 +75,8 @@ static void run_search(mirorr_bm* bm) {
   require_inputs(bm);
   const BMGrid g = current_grid(bm);
   const uint64_t key0 = bm->slot[0].version ^ ((uint64_t)g.bw << 40) ^ ((uint64_t)g.nw << 48) ^
-                        ((uint64_t)g.ng << 56) ^ ((uint64_t)g.bg << 32);
+                        ((uint64_t)g.ng << 56) ^ ((uint64_t)g.bg << 32) ^
+                        ((uint64_t)bm->p.block_metric << 24);
   const uint64_t key1 = bm->slot[1].version;
   if (bm->results_valid && bm->results_key[0] == key0 && bm->results_key[1] == key1 &&
       std::memcmp(&bm->grid, &g, sizeof(g)) == 0)
@@ -91,7 +94,8 @@ static void run_search(mirorr_bm* bm) {
   bm->match.ensure(sizeof(int32_t) * 3 * nb, bud);
   BMResult res{bm->best.as<float>(), bm->second.as<float>(), bm->match.as<int32_t>()};
   const int si = MIRORR_ROLE_SEARCH, bi = MIRORR_ROLE_BASE;
-  const bool tiled = tiled_supported(g);
+  const int metric = bm->p.block_metric;
+  const bool tiled = metric == MIRORR_METRIC_NC && tiled_supported(g);
   const double portion = bm->p.variance_select ? bm->p.portion_kept : 1.0;
   // Base-image preparation (tiled search, or sums for the variance selection).
   if (tiled || portion < 1.0) {
@@ -140,8 +144,8 @@ static void run_search(mirorr_bm* bm) {
     launch_ncc_search(B.data, B.mask, S.data, S.mask, g, bp,
                       SearchStats{bm->stats[si].as<float2>()}, scr, res, s);
   } else {
-    launch_ncc_generic(B.data, B.mask, S.data, S.mask, g, res, s,
-                       bm->prep_selected[bi] ? bp.flags : nullptr);
+    launch_metric_search(metric, B.data, B.mask, S.data, S.mask, g, res, s,
+                         bm->prep_selected[bi] ? bp.flags : nullptr);
   }
   bm->results_valid = true;
   bm->results_key[0] = key0;
@@ -235,6 +239,7 @@ void mirorr_bm_default_params(mirorr_bm_params* p) {
   p->portion_kept = 0.5;
   p->variance_select = 0;
   p->verbosity = 0;
+  p->block_metric = MIRORR_METRIC_NC;
 }
 
 mirorr_status mirorr_bm_create(mirorr_ctx* c, const mirorr_bm_params* p, mirorr_bm** out) {

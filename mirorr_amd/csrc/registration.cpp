@@ -187,7 +187,7 @@ struct RegRun {
                                p->nhood_gap);
     if (g.nblocks <= 0) return;
     BMResult res{w.best.as<float>(), w.second.as<float>(), w.match.as<int32_t>()};
-    if (tiled_supported(g)) {
+    if (p->block_metric == MIRORR_METRIC_NC && tiled_supported(g)) {
       BlockPrep bp{xt.as<float>(), sxx.as<float>(), sum.as<float>(), fl.as<uint32_t>()};
       BMScratch scr{w.queue.as<int32_t>(), w.qcount.as<int32_t>(),
                     w.qbits.as<unsigned long long>(), tile_count(g),
@@ -195,8 +195,8 @@ struct RegRun {
       launch_ncc_search(base, bmask, search, smask, g, bp, SearchStats{stats_search.as<float2>()},
                         scr, res, s);
     } else {
-      launch_ncc_generic(base, bmask, search, smask, g, res, s,
-                         selected ? fl.as<uint32_t>() : nullptr);
+      launch_metric_search(p->block_metric, base, bmask, search, smask, g, res, s,
+                           selected ? fl.as<uint32_t>() : nullptr);
     }
     const int64_t off = swap ? g.nblocks : 0;
     launch_postprocess_pairs(g, g_base, res, swap, w.kept.as<uint8_t>() + off,
@@ -414,7 +414,7 @@ void run_registration(mirorr_ctx* ctx, const mirorr_reg_params* p, const mirorr_
     const int mmode = moving->has_mask ? kMaskLinear : kMaskInside;
     const BMGrid gh = make_grid(o.gHF.dims, p->block_width, p->block_gap, p->nhood_width,
                                 p->nhood_gap);
-    const bool tiled = tiled_supported(gh);
+    const bool tiled = p->block_metric == MIRORR_METRIC_NC && tiled_supported(gh);
     // Portion of blocks searched: levels 0 and 1 use every block
     // (itkMirorrPyramidImplement.txx:116-119); the selection is opt-in.
     const double portion = (p->variance_select && level >= 2)
@@ -539,6 +539,7 @@ void mirorr_reg_default_params(mirorr_reg_params* p) {
   p->centre_init = 1;
   p->stop_tolerance = 1e-3;
   p->lts_tolerance = 1e-3;
+  p->block_metric = MIRORR_METRIC_NC;
 }
 
 mirorr_status mirorr_volume_create(mirorr_ctx* c, const float* data, const uint8_t* mask,
@@ -592,6 +593,10 @@ mirorr_status mirorr_register(mirorr_ctx* c, const mirorr_reg_params* p,
     // SetPortionMatchesKept (itkAbstractBlockMatcher.txx:83-99): 0 means 0.5.
     if (p->portion_kept < 0.0 || p->portion_kept > 1.0)
       throw Error(kErrArg, "portion_kept should be between 0 and 1.0");
+    if (p->block_metric < MIRORR_METRIC_NC || p->block_metric > MIRORR_METRIC_CR)
+      throw Error(kErrArg, "unknown block metric");
+    if (p->block_metric != MIRORR_METRIC_NC && p->block_width > kMaxMetricBW)
+      throw Error(kErrArg, "block width above 16 for this block metric");
     if (!(p->stop_tolerance >= 0.0) || !(p->lts_tolerance >= 0.0))
       throw Error(kErrArg, "stop tolerances must be non-negative");
     run_registration(c, p, fixed, moving, init_params, init_center, out_params, out_center,

@@ -98,6 +98,13 @@ class MirorrPyramid:
     def SetVarianceSelection(self, on: bool): self._p.variance_select = 1 if on else 0
     def SetVerbosity(self, v): self._p.verbosity = int(v)
     def SetCentreInitialisation(self, on: bool): self._p.centre_init = 1 if on else 0
+
+    def SetBlockMetricType(self, name: str):
+        """normalized_correlation (default), sum_of_squared_differences or
+        correlation_ratio (MirorrRegistrationMethod::SetBlockMetricType)."""
+        if name not in capi.METRIC_NAMES:
+            raise ValueError(f"{name} is not a valid blockmetric type")
+        self._p.block_metric = capi.METRIC_NAMES[name]
     def SetStopTolerances(self, outer: float, lts: float):
         """Early-stop thresholds (1e-3 each by default); 0 disables them."""
         self._p.stop_tolerance = float(outer)

@@ -11,6 +11,7 @@
 #include "oracle.h"
 
 #include <algorithm>
+#include <utility>
 #include <cmath>
 #include <thread>
 #include <vector>
@@ -88,6 +89,77 @@ extern "C" double orc_ncc_pair(const float* base, const uint8_t* bmask,
   return prod * prod / bss / sss;
 }
 
+// BlockMatcher::sumOfSquareDifference (itkBlockMatcher.txx:195-227): the
+// mean of (base - search)^2 over voxels inside both masks; -2 when none is.
+extern "C" double orc_ssd_pair(const float* base, const uint8_t* bmask, const float* search,
+                               const uint8_t* smask, const int dims[3], const int bi[3],
+                               const int si[3], int bw) {
+  double acc = 0.0, n = 0.0;
+  for (int z = 0; z < bw; ++z)
+    for (int y = 0; y < bw; ++y)
+      for (int x = 0; x < bw; ++x) {
+        const int64_t ob = lin(dims, bi[0] + x, bi[1] + y, bi[2] + z);
+        const int64_t os = lin(dims, si[0] + x, si[1] + y, si[2] + z);
+        if ((bmask && !bmask[ob]) || (smask && !smask[os])) continue;
+        const double d = (double)base[ob] - (double)search[os];
+        acc += d * d;
+        n += 1.0;
+      }
+  return n == 0.0 ? -2.0 : acc / n;
+}
+
+// BlockMatcher::correlationRatio (itkBlockMatcher.txx:105-191) with the
+// matcher's scale_factor 0.1 (itkBlockMatcher.h:314). The (bin, base value)
+// pairs of the voxels inside both masks, bin = floor(search * 0.1), are
+// sorted; per bin, base values above 10 give a count, sum and sum of
+// squares; bins with more than 2 such values add their variance
+// (E[x^2] - E[x]^2) to a sum of variances; every such value adds to the
+// totals. Score 1 - sum_var / var_total when the total count exceeds 2 and
+// sum_var > 0, kept when above 0.4, else -2. With no voxel inside both masks
+// the reference reads the front of an empty list and then returns -2 through
+// the same test; -2 is returned here directly.
+extern "C" double orc_cr_pair(const float* base, const uint8_t* bmask, const float* search,
+                              const uint8_t* smask, const int dims[3], const int bi[3],
+                              const int si[3], int bw) {
+  std::vector<std::pair<int, double>> v;
+  v.reserve((size_t)bw * bw * bw);
+  for (int z = 0; z < bw; ++z)
+    for (int y = 0; y < bw; ++y)
+      for (int x = 0; x < bw; ++x) {
+        const int64_t ob = lin(dims, bi[0] + x, bi[1] + y, bi[2] + z);
+        const int64_t os = lin(dims, si[0] + x, si[1] + y, si[2] + z);
+        if ((bmask && !bmask[ob]) || (smask && !smask[os])) continue;
+        v.emplace_back((int)std::floor((double)search[os] * 0.1), (double)base[ob]);
+      }
+  if (v.empty()) return -2.0;
+  std::sort(v.begin(), v.end());
+  double tot_s = 0.0, tot_s2 = 0.0, tot_n = 0.0, sum_var = 0.0;
+  size_t i = 0;
+  while (i < v.size()) {
+    const int bin = v[i].first;
+    double s = 0.0, s2 = 0.0, n = 0.0;
+    for (; i < v.size() && v[i].first == bin; ++i) {
+      const double xv = v[i].second;
+      if (xv > 10.0) {
+        s += xv;
+        s2 += xv * xv;
+        n += 1.0;
+      }
+    }
+    if (n > 2.0) {
+      const double m = s / n;
+      sum_var += s2 / n - m * m;
+    }
+    tot_s += s;
+    tot_s2 += s2;
+    tot_n += n;
+  }
+  if (!(tot_n > 2.0 && sum_var > 0.0)) return -2.0;
+  const double m = tot_s / tot_n;
+  const double cr = 1.0 - sum_var / (tot_s2 / tot_n - m * m);
+  return cr > 0.4 ? cr : -2.0;
+}
+
 extern "C" double orc_ncc_pair_unmasked(const float* base, const float* search,
                                         const int dims[3], const int bi[3],
                                         const int si[3], int bw) {
@@ -121,7 +193,15 @@ struct BMArgs {
   float* best;
   float* second;
   const uint8_t* keep;  // per block (raster order): 0 = not selected, skipped
+  int metric;           // 0 NCC, 1 SSD, 2 CR (BlockMatcher's metric map)
 };
+
+using PairMetric = double (*)(const float*, const uint8_t*, const float*, const uint8_t*,
+                              const int*, const int*, const int*, int);
+
+PairMetric metric_fn(int metric) {
+  return metric == 1 ? orc_ssd_pair : metric == 2 ? orc_cr_pair : orc_ncc_pair;
+}
 
 // blockPositionToPoint (itkAbstractBlockMatcher.h:271-287): the physical
 // point of the block corner plus spacing*BW/2, added WITHOUT the direction.
@@ -167,13 +247,13 @@ void match_region(const BMArgs& a, const Region& r, std::vector<double>& out_b,
         // initialised to -3 with the first candidate as match (:764-774).
         float best = -3.0f, second = -3.0f;
         int bidx[3] = {first[0], first[1], first[2]};
+        const PairMetric score = metric_fn(a.metric);
         for (int cz = 0; cz < ncand[2]; ++cz)
           for (int cy = 0; cy < ncand[1]; ++cy)
             for (int cx = 0; cx < ncand[0]; ++cx) {
               const int s[3] = {first[0] + cx * a.bg, first[1] + cy * a.bg,
                                 first[2] + cz * a.bg};
-              const double v = orc_ncc_pair(a.base, a.bmask, a.search, a.smask,
-                                            dims, p, s, a.bw);
+              const double v = score(a.base, a.bmask, a.search, a.smask, dims, p, s, a.bw);
               // itkBlockMatcher.txx:823-833: float buffer compared as double.
               if ((double)best < v) {
                 second = best;
@@ -325,8 +405,20 @@ extern "C" int orc_bm_get_offsets_sel(const float* base, const uint8_t* bmask,
                                       int32_t* block_idx, int32_t* match_idx, float* best,
                                       float* second, double* base_mm, double* match_mm,
                                       int64_t* n_kept) {
+  return orc_bm_get_offsets_metric(base, bmask, search, smask, g, bw, bg, nw, ng, nthreads,
+                                   block_keep, 0, block_idx, match_idx, best, second, base_mm,
+                                   match_mm, n_kept);
+}
+
+extern "C" int orc_bm_get_offsets_metric(const float* base, const uint8_t* bmask,
+                                         const float* search, const uint8_t* smask,
+                                         const orc_geom* g, int bw, int bg, int nw, int ng,
+                                         int nthreads, const uint8_t* block_keep, int metric,
+                                         int32_t* block_idx, int32_t* match_idx, float* best,
+                                         float* second, double* base_mm, double* match_mm,
+                                         int64_t* n_kept) {
   BMArgs a{base, bmask, search, smask, g, bw, bg, nw, ng, {0, 0, 0}, {0, 0, 0},
-           block_idx, match_idx, best, second, block_keep};
+           block_idx, match_idx, best, second, block_keep, metric};
   orc_block_grid(g->dims, bw, ng, a.nb, a.bstart);
   // ComputeRegionForMatching: region of block starts (itkBlockMatcher.txx:566-574).
   Region full;

@@ -105,9 +105,9 @@ void match(const std::vector<float>& base, const std::vector<uint8_t>& bmask,
   }
   const uint8_t* kp = keep.empty() ? nullptr : keep.data();
   int64_t k = 0;
-  if (orc_bm_get_offsets_sel(base.data(), bmask.data(), search.data(), smask.data(), &bg, p.bw,
-                             p.bg, p.nw, p.ng, p.nthreads, kp, nullptr, nullptr, nullptr,
-                             nullptr, b.data(), m.data(), &k) != 0) {
+  if (orc_bm_get_offsets_metric(base.data(), bmask.data(), search.data(), smask.data(), &bg,
+                                p.bw, p.bg, p.nw, p.ng, p.nthreads, kp, p.metric, nullptr,
+                                nullptr, nullptr, nullptr, b.data(), m.data(), &k) != 0) {
     std::fprintf(stderr, "oracle: no block matches occurred\n");
     k = 0;
   }
@@ -327,6 +327,7 @@ void orc_reg_default_params(orc_reg_params* p) {
   p->variance_select = 0;
   p->stop_tolerance = 1e-3;
   p->lts_tolerance = 1e-3;
+  p->metric = 0;
   p->nthreads = 0;
   p->verbosity = 0;
   p->centre_init = 1;

@@ -57,6 +57,23 @@ int orc_bm_get_offsets(const float* base, const uint8_t* base_mask,
 
 // GetOffsets restricted to the blocks with block_keep[b] != 0 (NULL: all).
 // Skipped blocks get best = second = -3 and their own position as match.
+// SSD and correlation ratio of one block pair (BlockMatcher's other metrics).
+double orc_ssd_pair(const float* base, const uint8_t* base_mask, const float* search,
+                    const uint8_t* search_mask, const int dims[3], const int base_idx[3],
+                    const int search_idx[3], int bw);
+double orc_cr_pair(const float* base, const uint8_t* base_mask, const float* search,
+                   const uint8_t* search_mask, const int dims[3], const int base_idx[3],
+                   const int search_idx[3], int bw);
+
+// orc_bm_get_offsets_sel under block metric `metric` (0 NCC, 1 SSD, 2 CR).
+int orc_bm_get_offsets_metric(const float* base, const uint8_t* base_mask,
+                              const float* search, const uint8_t* search_mask,
+                              const orc_geom* geom, int bw, int bg, int nw, int ng,
+                              int nthreads, const uint8_t* block_keep, int metric,
+                              int32_t* block_idx, int32_t* match_idx, float* best,
+                              float* second, double* base_mm, double* match_mm,
+                              int64_t* n_kept);
+
 int orc_bm_get_offsets_sel(const float* base, const uint8_t* base_mask,
                            const float* search, const uint8_t* search_mask,
                            const orc_geom* geom, int bw, int bg, int nw, int ng,

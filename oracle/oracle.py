@@ -80,6 +80,14 @@ def _declare(L):
     L.orc_bm_get_offsets_sel.argtypes = [_P, _P, _P, _P, C.POINTER(OracleGeom), C.c_int,
                                          C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P,
                                          _P, _P, _P, _P]
+    L.orc_bm_get_offsets_metric.restype = C.c_int
+    L.orc_bm_get_offsets_metric.argtypes = [_P, _P, _P, _P, C.POINTER(OracleGeom), C.c_int,
+                                            C.c_int, C.c_int, C.c_int, C.c_int, _P, C.c_int, _P,
+                                            _P, _P, _P, _P, _P, _P]
+    for name in ("orc_ssd_pair", "orc_cr_pair"):
+        f = getattr(L, name)
+        f.restype = C.c_double
+        f.argtypes = [_P, _P, _P, _P, _P, _P, _P, C.c_int]
     L.orc_variance_select.restype = C.c_int64
     L.orc_variance_select.argtypes = [_P, _P, C.c_int, C.c_int, C.c_double, _P, _P]
     for name, rt, at in _EXTRA_DECLS:
@@ -96,7 +104,7 @@ class RegParams(C.Structure):
                 ("pyr_min_size", C.c_int), ("portion_kept", C.c_double), ("nthreads", C.c_int),
                 ("verbosity", C.c_int), ("centre_init", C.c_int),
                 ("variance_select", C.c_int), ("stop_tolerance", C.c_double),
-                ("lts_tolerance", C.c_double)]
+                ("lts_tolerance", C.c_double), ("metric", C.c_int)]
 
 
 class RegStats(C.Structure):
@@ -185,8 +193,23 @@ def candidate_count_sel(dims, bw, bg, nw, ng, keep=None):
     return int(lib().orc_bm_candidate_count_sel(_ptr(_i3(dims)), bw, bg, nw, ng, _ptr(kp)))
 
 
+METRICS = {"nc": 0, "sd": 1, "cr": 2}
+
+
+def pair_score(metric, base, base_mask, search, search_mask, base_idx, search_idx, bw):
+    """One block pair under metric "nc", "sd" or "cr"."""
+    if metric == "nc":
+        return ncc_pair(base, base_mask, search, search_mask, base_idx, search_idx, bw)
+    base = _vol(base, np.float32)
+    search = _vol(search, np.float32)
+    f = lib().orc_ssd_pair if metric == "sd" else lib().orc_cr_pair
+    dims, bi, si = _i3(base.shape[::-1]), _i3(base_idx), _i3(search_idx)
+    return f(_ptr(base), _ptr(_vol(base_mask, np.uint8)), _ptr(search),
+             _ptr(_vol(search_mask, np.uint8)), _ptr(dims), _ptr(bi), _ptr(si), bw)
+
+
 def bm_get_offsets(base, base_mask, search, search_mask, geom: Geom, bw=4, bg=1, nw=7,
-                   ng=3, nthreads=0, keep=None):
+                   ng=3, nthreads=0, keep=None, metric="nc"):
     """Oracle GetOffsets. Returns a dict with raw per-block arrays (block raster
     order) and the kept point lists in the reference's thread order. `keep`
     (uint8 per block, optional) restricts the search to the selected blocks."""
@@ -207,10 +230,11 @@ def bm_get_offsets(base, base_mask, search, search_mask, geom: Geom, bw=4, bg=1,
     nk = C.c_int64(0)
     g = geom.to_c()
     kp = None if keep is None else _vol(keep, np.uint8)
-    rc = lib().orc_bm_get_offsets_sel(_ptr(base), _ptr(_vol(base_mask, np.uint8)),
-                                      _ptr(search), _ptr(_vol(search_mask, np.uint8)),
-                                      C.byref(g), bw, bg, nw, ng, nthreads, _ptr(kp),
-                                      _ptr(out["block_idx"]), _ptr(out["match_idx"]),
+    rc = lib().orc_bm_get_offsets_metric(_ptr(base), _ptr(_vol(base_mask, np.uint8)),
+                                         _ptr(search), _ptr(_vol(search_mask, np.uint8)),
+                                         C.byref(g), bw, bg, nw, ng, nthreads, _ptr(kp),
+                                         METRICS[metric],
+                                         _ptr(out["block_idx"]), _ptr(out["match_idx"]),
                                       _ptr(out["best"]), _ptr(out["second"]), _ptr(bmm),
                                       _ptr(mmm), C.byref(nk))
     if rc != 0:

@@ -37,6 +37,7 @@ typedef struct orc_reg_params {
   // 0 disables early stopping (fixed iteration counts), for comparisons.
   double stop_tolerance;
   double lts_tolerance;
+  int metric;        // --blockmetric: 0 nc (default), 1 sd, 2 cr
 } orc_reg_params;
 
 typedef struct orc_reg_stats {

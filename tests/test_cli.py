@@ -49,7 +49,7 @@ def test_help_and_missing_inputs(images):
 @pytest.mark.parametrize("args,msg", [
     (["--reg-mode", "other"], "Unknown reg-mode"),
     (["--blockmetric", "xx"], "not a valid blockmetric"),
-    (["--blockmetric", "cr"], "not supported"),
+    (["--blockmetric", "mi"], "not supported"),
     (["--reorient-moving"], "not supported"),
     (["--crop", "1,1,1,1"], "not supported"),
     (["-t", "quat"], "not supported"),
