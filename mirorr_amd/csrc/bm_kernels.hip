@@ -403,6 +403,9 @@ struct TileCfg {
   static constexpr int NV = PX / 4;         // float4 per row
   static constexpr int YPAD = NCY * KY - NW;
   static constexpr int GT = NB * NW * NW;  // masked-path tasks per tile
+  // Workgroups per CU the masked kernel's registers are budgeted for: four
+  // (128 VGPRs) up to NW 9, three for the wider rows of NW 11 and 13.
+  static constexpr int MWG = NW <= 9 ? 4 : 3;
   static_assert(NT <= kThreads, "tile too large");
   static_assert(NB <= 64, "block bitmasks are 64-bit");
 };
@@ -410,8 +413,8 @@ struct TileCfg {
 // Static LDS of the masked kernel (bytes, rounded up).
 template <class T>
 constexpr size_t masked_static_lds() {
-  return T::NB * (16 * 16 + 8 + 4 + 4) + T::NB * T::NW * T::NW * (4 + 4 + 4 + 2) +
-         8 * kMaxWindow + 256;
+  return T::NB * (16 * 16 + 8 + 4 + 4 + 8) + T::NB * T::NW * T::NW * (4 + 4 + 4 + 2) +
+         2 * 8 * kMaxWindow + 256;
 }
 
 struct TileArgs {
@@ -504,13 +507,15 @@ __device__ __forceinline__ void write_skipped(const TileArgs& a, int64_t b, int 
 // small integers and the quotient's fractional part is at least 0.5 / WY
 // away from an integer).
 //  * MASKED: voxels whose search mask is off stage as 0, the mask bits go
-//    to `wm` (one byte per voxel, same layout), and a voxel whose search
-//    mask is on sets bit wy of zrow[wz].
+//    to `wm` (one byte per voxel, same layout), a voxel whose search mask is
+//    on sets bit wy of zrow[wz], and a window voxel that is outside the image
+//    or whose search mask is off sets bit wy of zbad[wz].
 //  * otherwise: a voxel whose search mask is off sets bit wy of zrow[wz].
 template <class T, bool MASKED>
 __device__ __forceinline__ void stage_window(const TileArgs& a, float* win, uint8_t* wm,
                                              unsigned long long* zrow, int WY, int WZ, int wx0,
-                                             int wy0, int wz0, float ref) {
+                                             int wy0, int wz0, float ref,
+                                             unsigned long long* zbad = nullptr) {
   const int nx = a.g.dims[0], ny = a.g.dims[1], nz = a.g.dims[2];
   const int items = T::NV * WY * WZ;
   const float inv_wy = 1.0f / (float)WY;
@@ -562,9 +567,13 @@ __device__ __forceinline__ void stage_window(const TileArgs& a, float* win, uint
 #pragma unroll
         for (int k = 0; k < 4; ++k) mbits |= ((on >> k) & 1u) << (8 * k);
         if (on) atomicOr(&zrow[wz], 1ull << wy);
+        const uint32_t cols = 4 * v + 3 < T::WX ? 0xfu : (1u << (T::WX - 4 * v)) - 1u;
+        if (cols & ~on) atomicOr(&zbad[wz], 1ull << wy);
       } else if (inside & ~on) {
         atomicOr(&zrow[wz], 1ull << wy);
       }
+    } else if (MASKED && 4 * v < T::WX) {
+      atomicOr(&zbad[wz], 1ull << wy);  // a row outside the image
     }
     reinterpret_cast<float4*>(win)[row * T::NV + v] = make_float4(q[0], q[1], q[2], q[3]);
     if (MASKED) reinterpret_cast<uint32_t*>(wm)[row * T::NV + v] = mbits;
@@ -893,7 +902,7 @@ __device__ __forceinline__ void load_mask(const uint8_t* wv, int ro, int v, floa
 //  * Base block partly masked (base mask b): Sxy = Sum b x~ t,
 //    Syy = Sum b t^2, Sxx = Sum b x~^2 v (5 ops per voxel).
 template <class T>
-__global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
+__global__ __launch_bounds__(kThreads, T::MWG) void ncc_tile_masked(TileArgs a) {
   constexpr int kRows = T::NW * T::NW;  // row tasks per block
   extern __shared__ float4 dyn4[];
   __shared__ float rbest[T::NB * kRows], rsecond[T::NB * kRows];  // per-task records
@@ -905,7 +914,10 @@ __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
   __shared__ int todo_list[T::NB];
   __shared__ int s_ntodo;
   __shared__ unsigned long long s_fullbase;  // queued blocks whose base mask is fully on
-  __shared__ unsigned long long zon[kMaxWindow];  // rows holding a search-mask-on voxel
+  __shared__ unsigned long long zon[kMaxWindow];   // rows holding a search-mask-on voxel
+  __shared__ unsigned long long zbad[kMaxWindow];  // rows holding an off or outside voxel
+  __shared__ float bsxx[T::NB], bsum[T::NB];       // block sums of fully-on bases
+  __shared__ int s_nfull;
   __shared__ int bvalid[T::NB];
   __shared__ float s_ref;
 
@@ -947,16 +959,24 @@ __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
     bvalid[tid] = ((todo >> tid) & 1ull) ? 1 : 0;
     bbits[tid] = 0ull;
   }
-  for (int i = tid; i < WZ; i += kThreads) zon[i] = 0ull;
+  for (int i = tid; i < WZ; i += kThreads) {
+    zon[i] = 0ull;
+    zbad[i] = 0ull;
+  }
   if (tid == 0) {
     s_fullbase = 0ull;
     s_nlive = 0;
+    s_nfull = 0;
   }
   __syncthreads();
   if (tid < T::NB && ((todo >> tid) & 1ull)) {
     const int gy = by0 + tid % T::TY, gz = bz0 + tid / T::TY;
     const int64_t b = bx + (int64_t)g.nb[0] * (gy + (int64_t)g.nb[1] * gz);
-    if (a.bm == nullptr || (a.bflags[b] & 1u) != 0u) atomicOr(&s_fullbase, 1ull << tid);
+    if (a.bm == nullptr || (a.bflags[b] & 1u) != 0u) {
+      atomicOr(&s_fullbase, 1ull << tid);
+      bsxx[tid] = a.sxx[b];
+      bsum[tid] = a.sumxt[b];
+    }
   }
   __syncthreads();
   if (tid == 0) {
@@ -972,7 +992,7 @@ __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
   }
   __syncthreads();
   const float ref = s_ref;
-  stage_window<T, true>(a, wy_, wv_, zon, WY, WZ, wx0, wy0, wz0, ref);
+  stage_window<T, true>(a, wy_, wv_, zon, WY, WZ, wx0, wy0, wz0, ref, zbad);
   for (int i = tid; i < T::NB * 64; i += kThreads) {
     const int bl = i / 64, k = i % 64;
     const int gy = by0 + bl % T::TY, gz = bz0 + bl / T::TY;
@@ -999,15 +1019,20 @@ __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
   // k * NW^2 + cz * NW + cy. Each task's top-2 record goes to LDS: empty for
   // rows outside the image, -2 at every candidate for rows whose window holds
   // no search-mask-on voxel (Syy = 0), and otherwise computed by a thread.
-  // The rows to compute are compacted and dealt to the threads strided, so
-  // nearly every lane works; then one wave per block appends the block's
-  // records in scan order.
+  // Rows of a fully-on base block whose whole window is inside the image and
+  // search-mask on ("full" rows) are scored as the fast kernel scores them
+  // (one FMA per voxel-eval, block and window statistics); the others take
+  // the masked sums. The rows to compute are compacted, full rows from the
+  // front of the list and masked rows from the back, and dealt to the
+  // threads strided, so nearly every lane works and few waves run both
+  // paths; then one wave per block appends the block's records in scan
+  // order.
   const int ntodo = s_ntodo;
   const int ntasks = ntodo * kRows;
   const int lox = max(px - T::H, 0), hix = min(px - T::H + T::NW - 1, nx - T::BW);
   for (int base = 0; base < ntasks; base += kThreads) {  // workgroup-uniform
     const int i = base + tid;
-    bool live = false;
+    bool live = false, full = false;
     if (i < ntasks) {
       const int k = i / kRows, r = i % kRows;
       const int bl = todo_list[k];
@@ -1024,6 +1049,12 @@ __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
         for (int iz = 0; iz < 4; ++iz) rows_on |= zon[blz * NG + cz + iz];
         if (((rows_on >> (bly * NG + cy)) & 0xfull) != 0ull) {
           live = true;
+          if (((s_fullbase >> bl) & 1ull) && bbits[bl] == ~0ull) {
+            unsigned long long rows_bad = 0ull;
+#pragma unroll
+            for (int iz = 0; iz < 4; ++iz) rows_bad |= zbad[blz * NG + cz + iz];
+            full = ((rows_bad >> (bly * NG + cy)) & 0xfull) == 0ull;
+          }
         } else {
 #pragma unroll
           for (int cx = 0; cx < T::NW; ++cx) {
@@ -1039,17 +1070,26 @@ __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
         rscan[i] = tr.scan;
       }
     }
-    const unsigned long long bal = __ballot(live);
-    int wbase = 0;
-    if ((tid & 63) == 0 && bal != 0ull) wbase = atomicAdd(&s_nlive, (int)__popcll(bal));
-    wbase = __shfl(wbase, 0, 64);
-    if (live) live_list[wbase + __popcll(bal & ((1ull << (tid & 63)) - 1ull))] = (uint16_t)i;
+    const unsigned long long balf = __ballot(full);
+    const unsigned long long balm = __ballot(live && !full);
+    int fbase = 0, mbase = 0;
+    if ((tid & 63) == 0) {
+      if (balf != 0ull) fbase = atomicAdd(&s_nfull, (int)__popcll(balf));
+      if (balm != 0ull) mbase = atomicAdd(&s_nlive, (int)__popcll(balm));
+    }
+    fbase = __shfl(fbase, 0, 64);
+    mbase = __shfl(mbase, 0, 64);
+    const unsigned long long below = (1ull << (tid & 63)) - 1ull;
+    if (full) live_list[fbase + __popcll(balf & below)] = (uint16_t)i;
+    else if (live) live_list[T::NB * kRows - 1 - (mbase + __popcll(balm & below))] = (uint16_t)i;
   }
   __syncthreads();
-  const int nlive = s_nlive;
+  const int nfull = s_nfull, nmasked = s_nlive;
+  const int nlive = nfull + nmasked;
 #pragma unroll 1
   for (int j = tid; j < nlive; j += kThreads) {
-    const int i = live_list[j];
+    const bool full = j < nfull;
+    const int i = live_list[full ? j : T::NB * kRows - nmasked + (j - nfull)];
     const int k = i / kRows, r = i % kRows;
     const int bl = todo_list[k];
     const int cz = r / T::NW, cy = r % T::NW;
@@ -1058,15 +1098,57 @@ __global__ __launch_bounds__(kThreads, 3) void ncc_tile_masked(TileArgs a) {
     const int czg = pz - T::H + cz, cyg = py - T::H + cy;
     Top2 tr;
     top2_init(tr);
-    float dmu[T::NW];
     const float2* srow = a.stats + lin3(nx, ny, 0, cyg, czg);
+    const int rowoff = ((blz * NG + cz) * WY + (bly * NG + cy)) * T::PX;
+    if (full) {
+      // Every voxel of every candidate window of this row is on and the
+      // base block is fully on: Sxy = Sum x~ (y - ref) - (mu - ref) Sum x~,
+      // Sxx and Syy from the block and window statistics.
+      float acc[T::NW];
+#pragma unroll
+      for (int cx = 0; cx < T::NW; ++cx) acc[cx] = 0.0f;
+#pragma unroll 1
+      for (int iz = 0; iz < 4; ++iz) {
+#pragma unroll
+        for (int iy = 0; iy < 4; ++iy) {
+          const int ro = rowoff + (iz * WY + iy) * T::PX;
+          float yr[T::PX];
+          const float4* yp = reinterpret_cast<const float4*>(wy_ + ro);
+#pragma unroll
+          for (int v = 0; v < T::NV; ++v) {
+            const float4 q = yp[v];
+            yr[4 * v + 0] = q.x; yr[4 * v + 1] = q.y; yr[4 * v + 2] = q.z; yr[4 * v + 3] = q.w;
+          }
+          const float4 xq4 = xh4[bl * 16 + iz * 4 + iy];
+          const float xh[4] = {xq4.x, xq4.y, xq4.z, xq4.w};
+#pragma unroll
+          for (int cx = 0; cx < T::NW; ++cx)
+#pragma unroll
+            for (int ix = 0; ix < 4; ++ix) acc[cx] = fmaf(xh[ix], yr[cx + ix], acc[cx]);
+        }
+      }
+      const float sxx = bsxx[bl], sumx = bsum[bl];
+      const float isxx = sxx > 0.0f ? __builtin_amdgcn_rcpf(sxx) : __builtin_nanf("");
+#pragma unroll
+      for (int cx = 0; cx < T::NW; ++cx) {
+        const int cxg = px - T::H + cx;
+        if (cxg < lox || cxg > hix) continue;
+        const float2 st = srow[cxg];
+        const float sxy = fmaf(-(st.x - ref), sumx, acc[cx]);
+        top2_push(tr, fmaxf(sxy * sxy * isxx * st.y, -2.0f), (cz * T::NW + cy) * T::NW + cx);
+      }
+      rbest[i] = tr.best;
+      rsecond[i] = tr.second;
+      rscan[i] = tr.scan;
+      continue;
+    }
+    float dmu[T::NW];
 #pragma unroll
     for (int cx = 0; cx < T::NW; ++cx) {
       const int cxg = min(max(px - T::H + cx, lox), hix);
       dmu[cx] = srow[cxg].x - ref;
     }
     const unsigned long long bits = bbits[bl];
-    const int rowoff = ((blz * NG + cz) * WY + (bly * NG + cy)) * T::PX;
     float sxy[T::NW], sxx[T::NW], s3[T::NW];
 #pragma unroll
     for (int cx = 0; cx < T::NW; ++cx) sxy[cx] = sxx[cx] = s3[cx] = 0.0f;
