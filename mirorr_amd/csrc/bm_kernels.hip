@@ -403,6 +403,12 @@ struct TileCfg {
   static constexpr int NV = PX / 4;         // float4 per row
   static constexpr int YPAD = NCY * KY - NW;
   static constexpr int GT = NB * NW * NW;  // masked-path tasks per tile
+  // Rows per staged plane of the fast kernel's window: with 12-float rows an
+  // odd count makes the plane stride an odd number of 16-byte words, so the
+  // lanes of a wave (consecutive cz) read rows in different LDS banks.
+  __host__ __device__ static constexpr int fast_pitch(int wy) {
+    return PX % 8 == 4 ? (wy | 1) : wy;
+  }
   // Workgroups per CU the masked kernel's registers are budgeted for: four
   // (128 VGPRs) up to NW 9, three for the wider rows of NW 11 and 13.
   static constexpr int MWG = NW <= 9 ? 4 : 3;
@@ -687,7 +693,7 @@ __global__ __launch_bounds__(kThreads, 4) void ncc_tile_fast(TileArgs a) {
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   tile_origin<T>(a, tile, bx, by0, bz0);
   const int NG = g.ng;
-  const int WY = (T::TY - 1) * NG + T::WX + T::YPAD;
+  const int WY = T::fast_pitch((T::TY - 1) * NG + T::WX + T::YPAD);
   const int WZ = (T::TZ - 1) * NG + T::WX;
   const int px = g.start[0] + NG * bx;
   const int py0 = g.start[1] + NG * by0;
@@ -768,9 +774,9 @@ __global__ __launch_bounds__(kThreads, 4) void ncc_tile_fast(TileArgs a) {
   top2_init(t2);
   const int bl = tid / T::TPB;
   if (tid < T::NT && bvalid[bl] == 1) {
-    const int r = tid % T::TPB;
-    const int cz = r / T::NCY;
-    const int cyg = r % T::NCY;
+    const int r = tid % T::TPB;  // consecutive lanes: consecutive cz
+    const int cz = r % T::NW;
+    const int cyg = r / T::NW;
     const int bly = bl % T::TY, blz = bl / T::TY;
     float acc[T::KY][T::NW];
 #pragma unroll
@@ -1809,7 +1815,7 @@ void run_tiled(const BMGrid& g, TileArgs a, BMScratch& scr, hipStream_t s) {
   a.qbits = scr.qbits;
   a.qcount = scr.count;
   hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, s, scr.count);
-  const int WYf = (T::TY - 1) * g.ng + T::WX + T::YPAD;
+  const int WYf = T::fast_pitch((T::TY - 1) * g.ng + T::WX + T::YPAD);
   const int WZ = (T::TZ - 1) * g.ng + T::WX;
   const size_t lds_fast = sizeof(float) * (size_t)T::PX * WYf * WZ;
   if (scr.prof) a.work = scr.prof->begin("ncc_tile_fast", kProfFast, s);
@@ -1833,7 +1839,8 @@ template <class T>
 bool lds_fits(int ng) {
   // Dynamic windows plus the static arrays of each kernel, within 64 KB; the
   // window spans at most kMaxWindow rows and planes (64-bit row masks).
-  const size_t wyf = (T::TY - 1) * ng + T::WX + T::YPAD, wym = (T::TY - 1) * ng + T::WX;
+  const size_t wyf = T::fast_pitch((T::TY - 1) * ng + T::WX + T::YPAD),
+               wym = (T::TY - 1) * ng + T::WX;
   const size_t wz = (T::TZ - 1) * ng + T::WX;
   if (wyf > (size_t)kMaxWindow || wz > (size_t)kMaxWindow) return false;
   const size_t fast = 4 * T::PX * wyf * wz + T::NB * (16 * 16 + 12 + 16) + 16;
