@@ -6,9 +6,9 @@
 // Same option names, short forms, defaults, positional order (moving, fixed,
 // last-tfm, start-tfm), output-transform naming, --fresh rule and output
 // images. Options that select work this build does not implement
-// (--reorient-*, --crop, --blockmetric mi, --tfm-type quat,
-// --resampling-mode other than middle) fail with a message instead of being
-// ignored; --nthreads and --use-gpu-bm are accepted and have no effect.
+// (--blockmetric mi, --tfm-type quat, --resampling-mode other than middle)
+// fail with a message instead of being ignored; --nthreads and --use-gpu-bm
+// are accepted and have no effect.
 #include <sys/stat.h>
 
 #include <cmath>
@@ -67,12 +67,12 @@ const Opt kOptions[] = {
     {"pyr-num", 'd', true, "1024", "Maximum number of pyramid levels"},
     {"pyr-min-size", 'e', true, "32", "Minimum image size at the coarsest level"},
     {"resampling-mode", 0, true, "middle", "Resampling mode (middle)"},
-    {"crop", 0, true, "", "Not supported by this build"},
+    {"crop", 0, true, "", "Crop the moving and fixed images on both sides of axes 0 and 1: \"mx,my,fx,fy\" voxels"},
     {"iterations", 'n', true, "0", "Iterations per level (0: default, -ve halves per level)"},
     {"portion-kept", 0, true, "0.5", "Portion of the block matches kept"},
     {"use-gpu-bm", 0, false, nullptr, "Accepted for compatibility (always on)"},
-    {"reorient-fixed", 0, false, nullptr, "Not supported by this build"},
-    {"reorient-moving", 0, false, nullptr, "Not supported by this build"},
+    {"reorient-fixed", 0, false, nullptr, "Shift the fixed image to identity space (ARI, origin 0) before registering"},
+    {"reorient-moving", 0, false, nullptr, "Shift the moving image to identity space (ARI, origin 0) before registering"},
     {"save-fixed", 0, true, nullptr, "Save the fixed image resampled in moving space"},
     {"save-moving", 0, true, nullptr, "Save the moving image resampled in fixed space"},
     {"no-centering-init", '0', false, nullptr, "Do not centre the images before registration"},
@@ -306,6 +306,91 @@ mirorr_geom reoriented_geom(const mirorr_geom& g, const xf::Xf& T) {
   return o;
 }
 
+// GetTransformFromImageToIdentitySpace (itkMirorrUtilities.cxx:44-160,
+// with do_centre off as the wrapper calls it): each row of the direction
+// matrix snapped to its largest entry (+-1), divided by |det|, and the
+// affine map of four grid landmarks (index 0 and the unit steps) from the
+// image onto the same grid with that direction and origin 0. With the
+// landmarks exact the map is M = D' D^-1, offset = -M o.
+xf::Xf identity_space_tfm(const mirorr_geom& g) {
+  xf::M3 D, Dp;
+  for (int r = 0; r < 3; ++r) {
+    int pos = 0;
+    for (int c = 1; c < 3; ++c)
+      if (std::fabs(g.direction[3 * r + c]) > std::fabs(g.direction[3 * r + pos])) pos = c;
+    for (int c = 0; c < 3; ++c) {
+      D.a[r][c] = g.direction[3 * r + c];
+      Dp.a[r][c] = c == pos ? (g.direction[3 * r + c] > 0.0 ? 1.0 : -1.0) : 0.0;
+    }
+  }
+  const double det = std::fabs(xf::det(Dp));
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) Dp.a[r][c] /= det;
+  xf::Xf x = xf::identity(xf::kAffine);
+  x.M = xf::mul(Dp, xf::inverse(D));
+  double mo[3];
+  xf::mulv(x.M, g.origin, mo);
+  for (int i = 0; i < 3; ++i) x.off[i] = -mo[i];
+  xf::compute_translation(x);
+  return x;
+}
+
+// ShiftImageToIdSpaceInplace (itkMirorrPyramidWrapper.txx:94-118): the ARI
+// direction and origin 0.
+void shift_to_identity_space(mirorr_geom& g) {
+  const double ari[9] = {0, 1, 0, 1, 0, 0, 0, 0, 1};
+  for (int i = 0; i < 9; ++i) g.direction[i] = ari[i];
+  for (int i = 0; i < 3; ++i) g.origin[i] = 0.0;
+}
+
+// --crop "mx,my,fx,fy": voxels removed at both ends of axes 0 and 1 of the
+// moving and fixed images (mirorr.cxx:205-207,414-420; parseInputToList
+// splits on any of "x :,()=" and missing entries are 0).
+std::vector<unsigned> parse_crop(const std::string& s) {
+  std::vector<unsigned> out(4, 0u);
+  std::vector<std::string> parts(1);
+  for (char ch : s) {
+    if (std::strchr("x :,()=", ch)) parts.emplace_back();
+    else parts.back() += ch;
+  }
+  for (size_t i = 0; i < parts.size() && i < 4; ++i) {
+    if (parts[i].empty()) continue;
+    char* end = nullptr;
+    const long v = std::strtol(parts[i].c_str(), &end, 10);
+    if (*end || v < 0) throw std::runtime_error("Failed to parse input string: \"" + s + "\"");
+    out[i] = (unsigned)v;
+  }
+  return out;
+}
+
+// CropImageFilter with lower = upper = (cx, cy, 0): the kept voxels keep
+// their physical positions.
+void crop_xy(Image& im, unsigned cx, unsigned cy) {
+  const mirorr_geom g = im.g;
+  if (2 * (int64_t)cx >= g.dims[0] || 2 * (int64_t)cy >= g.dims[1])
+    throw std::runtime_error("--crop removes the whole image");
+  mirorr_geom o = g;
+  o.dims[0] = g.dims[0] - 2 * (int)cx;
+  o.dims[1] = g.dims[1] - 2 * (int)cy;
+  for (int r = 0; r < 3; ++r)
+    o.origin[r] = g.origin[r] + g.direction[3 * r + 0] * g.spacing[0] * cx +
+                  g.direction[3 * r + 1] * g.spacing[1] * cy;
+  const size_t n = (size_t)o.dims[0] * o.dims[1] * o.dims[2];
+  std::vector<float> d(n);
+  std::vector<uint8_t> m(im.mask.empty() ? 0 : n);
+  for (int z = 0; z < o.dims[2]; ++z)
+    for (int y = 0; y < o.dims[1]; ++y)
+      for (int x = 0; x < o.dims[0]; ++x) {
+        const size_t src = (size_t)(x + cx) + (size_t)g.dims[0] * ((size_t)(y + cy) + (size_t)g.dims[1] * z);
+        const size_t dst = (size_t)x + (size_t)o.dims[0] * ((size_t)y + (size_t)o.dims[1] * z);
+        d[dst] = im.data[src];
+        if (!m.empty()) m[dst] = im.mask[src];
+      }
+  im.g = o;
+  im.data.swap(d);
+  im.mask.swap(m);
+}
+
 int interpolator_code(const std::string& name) {
   if (name.find("nn") != std::string::npos) return MIRORR_INTERP_NN;
   if (name.find("linear") != std::string::npos) return MIRORR_INTERP_LINEAR;
@@ -378,9 +463,8 @@ int run(int argc, char** argv) {
                          "'symmetric' (default).\n");
     return -1;
   }
-  for (const char* k : {"reorient-fixed", "reorient-moving"})
-    if (a.has(k)) throw std::runtime_error(std::string("--") + k + " is not supported by this build");
-  if (!a.str("crop").empty()) throw std::runtime_error("--crop is not supported by this build");
+  const bool reorient_fixed = a.has("reorient-fixed"), reorient_moving = a.has("reorient-moving");
+  const std::vector<unsigned> crop = parse_crop(a.str("crop"));
   if (a.str("resampling-mode") != "middle")
     throw std::runtime_error("--resampling-mode " + a.str("resampling-mode") +
                              " is not supported by this build (middle only)");
@@ -480,6 +564,20 @@ int run(int argc, char** argv) {
   if (!fixed_mask.empty()) read_mask_into(fixed, fixed_mask);
   check_io(mirorr_reorient_ari(&fixed.g, fixed.data.data(), fixed.mask.empty() ? nullptr : fixed.mask.data()));
   check_io(mirorr_reorient_ari(&moving.g, moving.data.data(), moving.mask.empty() ? nullptr : moving.mask.data()));
+  // Transforms to identity space, taken before any crop
+  // (itkMirorrPyramidWrapper.txx:213-240), then the optional shifts and crops.
+  const xf::Xf fixed_to_id = identity_space_tfm(fixed.g);
+  const xf::Xf moving_to_id = identity_space_tfm(moving.g);
+  if (reorient_fixed) {
+    std::printf("Fixed: %s\n", fixed_name.c_str());
+    shift_to_identity_space(fixed.g);
+  }
+  if (reorient_moving) {
+    std::printf("Moving: %s\n", moving_name.c_str());
+    shift_to_identity_space(moving.g);
+  }
+  if (crop[0] > 0 || crop[1] > 0) crop_xy(moving, crop[0], crop[1]);
+  if (crop[2] > 0 || crop[3] > 0) crop_xy(fixed, crop[2], crop[3]);
   {
     double va[3], vb[3];
     rotation_vector(moving.g.direction, va);
@@ -495,9 +593,11 @@ int run(int argc, char** argv) {
   }
 
   double params[12] = {0}, center[3] = {0, 0, 0};
-  xf::Xf init = xf::identity(affine ? xf::kAffine : xf::kRigid);
+  const int kind = affine ? xf::kAffine : xf::kRigid;
+  xf::Xf init = xf::identity(kind);
   xf::get_params(init, params);
-  if (in_tfm.empty() && rp.centre_init) {
+  const bool did_init = in_tfm.empty() && rp.centre_init;
+  if (did_init) {
     check_io(mirorr_centered_init(&moving.g, &fixed.g, affine, params, center));
   } else if (!in_tfm.empty()) {
     const mirorr_status st = mirorr_tfm_read(in_tfm.c_str(), affine, params, center,
@@ -517,6 +617,18 @@ int run(int argc, char** argv) {
       for (double& c : center) c = 0.0;
       check_io(mirorr_tfm_write(example.c_str(), affine, params, center, 0));
     }
+  }
+  // The start transform, composed into an identity transform with centre 0
+  // between the identity-space shifts of the reoriented images
+  // (itkMirorrPyramidWrapper.txx:626-648).
+  {
+    const xf::Xf orig = make_xf(affine, params, center);
+    xf::Xf t = xf::identity(kind);
+    if (reorient_moving && !did_init) xf::compose(t, xf::inverse(moving_to_id), false);
+    xf::compose(t, orig, false);
+    if (reorient_fixed && !did_init) xf::compose(t, fixed_to_id, false);
+    xf::get_params(t, params);
+    for (int i = 0; i < 3; ++i) center[i] = t.c[i];
   }
   if (verbosity >= 1)
     std::printf("Transform Class:                 %s\n"
@@ -551,10 +663,13 @@ int run(int argc, char** argv) {
     std::memcpy(out_center, center, sizeof(center));
   }
 
-  // The written transform is the registration transform composed into an
-  // identity transform with centre 0 (itkMirorrPyramidWrapper.txx:680-700).
-  xf::Xf T = make_xf(affine, out_params, out_center);
-  xf::set_center_to_origin(T);
+  // The written transform: the identity-space shifts and the registration
+  // transform composed into an identity transform with centre 0
+  // (itkMirorrPyramidWrapper.txx:680-700).
+  xf::Xf T = xf::identity(kind);
+  if (reorient_moving) xf::compose(T, moving_to_id, false);
+  xf::compose(T, make_xf(affine, out_params, out_center), false);
+  if (reorient_fixed) xf::compose(T, xf::inverse(fixed_to_id), false);
   double full[12];
   xf::get_params(T, full);
   const double zero[3] = {0, 0, 0};

@@ -50,8 +50,6 @@ def test_help_and_missing_inputs(images):
     (["--reg-mode", "other"], "Unknown reg-mode"),
     (["--blockmetric", "xx"], "not a valid blockmetric"),
     (["--blockmetric", "mi"], "not supported"),
-    (["--reorient-moving"], "not supported"),
-    (["--crop", "1,1,1,1"], "not supported"),
     (["-t", "quat"], "not supported"),
     (["--resampling-mode", "basic"], "not supported"),
     (["--save-reoriented-moving", "a.nii", "--save-resampled-moving", "b.nii"],
@@ -137,3 +135,38 @@ def test_reoriented_moving_is_header_only(images):
     rf = fileio.read_nifti(str(images / "rf.nii"))
     Mi = M.T
     assert np.allclose(rf.origin, Mi @ (np.asarray(fix.origin) - off), atol=1e-4)
+
+
+def test_reorient_shifts_cancel_in_the_written_transform(images):
+    # With a start transform and no registration, the identity-space shifts
+    # enter the start transform and leave the written one: the output equals
+    # the start (itkMirorrPyramidWrapper.txx:626-700).
+    fileio.write_tfm(str(images / "s.tfm"), [0.05, -0.1, 0.2, 4, -5, 6], [1, 2, 3], affine=False)
+    for flags in (["--reorient-moving"], ["--reorient-fixed"],
+                  ["--reorient-moving", "--reorient-fixed"]):
+        rc, out, err = run(["mov.nii.gz", "fix.nii.gz", "o.tfm", "s.tfm", "-R", *flags], images)
+        assert rc == 0, err
+        a, ca = fileio.read_tfm(str(images / "s.tfm"), affine=False)
+        b, cb = fileio.read_tfm(str(images / "o.tfm"), affine=False)
+        Ma, oa = transforms.matrix_offset(a, ca, False)
+        Mb, ob = transforms.matrix_offset(b, cb, False)
+        assert np.allclose(Ma, Mb, atol=1e-12) and np.allclose(oa, ob, atol=1e-9), flags
+
+
+def test_reorient_moving_with_centred_start(images):
+    # The centred start on the shifted moving image, composed with the shift,
+    # is the centred start on the original grids: identity for equal grids.
+    rc, out, err = run(["mov.nii.gz", "fix.nii.gz", "o.tfm", "-R", "--reorient-moving"], images)
+    assert rc == 0, err
+    p, c = fileio.read_tfm(str(images / "o.tfm"), affine=False)
+    assert np.allclose(p, 0, atol=1e-9)
+
+
+def test_crop_option_parsing(images):
+    for arg in ("2,3,1,1", "2x3", "(1:1)", ""):
+        rc, _, err = run(["mov.nii.gz", "fix.nii.gz", "-R", "--crop", arg], images)
+        assert rc == 0, (arg, err)
+    rc, _, err = run(["mov.nii.gz", "fix.nii.gz", "-R", "--crop", "a,b"], images)
+    assert rc != 0 and "Failed to parse" in err
+    rc, _, err = run(["mov.nii.gz", "fix.nii.gz", "-R", "--crop", "10,0,0,0"], images)
+    assert rc != 0 and "whole image" in err

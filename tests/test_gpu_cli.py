@@ -81,3 +81,56 @@ def test_nipype_interface_command_runs(tmp_path):
     for key, grid in (("resampled_moving", pair.fixed), ("resampled_fixed", pair.moving)):
         v = fileio.read_nifti(outputs[key])
         assert v.dims == grid.dims and np.count_nonzero(v.data) > 0.5 * v.data.size
+
+
+def _crop_xy(v, cx, cy):
+    """The volume without cx voxels at both ends of axis 0 and cy of axis 1."""
+    d = np.ascontiguousarray(v.data[:, cy:v.data.shape[1] - cy, cx:v.data.shape[2] - cx])
+    D = np.reshape(v.direction, (3, 3))
+    org = np.asarray(v.origin) + D @ (np.asarray(v.spacing) * np.array([cx, cy, 0]))
+    from mirorr_amd.phantom import Volume
+    return Volume(d, v.spacing, tuple(org), v.direction)
+
+
+def test_cli_crop_matches_oracle_on_cropped_volumes(tmp_path, oracle):
+    pair = phantom.make_pair(dims=(64, 64, 32), spacing=(3.0, 3.0, 3.0), n_ellipsoids=20,
+                             seed=11, model="rigid")
+    fileio.write_nifti(str(tmp_path / "mov.nii.gz"), pair.moving)
+    fileio.write_nifti(str(tmp_path / "fix.nii.gz"), pair.fixed)
+    args = [CLI, "mov.nii.gz", "fix.nii.gz", "out.tfm", "-d", "2", "-c", "0", "--fresh",
+            "--crop", "4,2,3,5"]
+    print("cli", " ".join(args[1:]), flush=True)
+    r = subprocess.run(args, cwd=tmp_path, capture_output=True, text=True, timeout=180)
+    assert r.returncode == 0, r.stderr + r.stdout[-2000:]
+    params, c = fileio.read_tfm(str(tmp_path / "out.tfm"), affine=False)
+    mov = _crop_xy(fileio.reorient_ari(fileio.read_nifti(str(tmp_path / "mov.nii.gz"))), 4, 2)
+    fix = _crop_xy(fileio.reorient_ari(fileio.read_nifti(str(tmp_path / "fix.nii.gz"))), 3, 5)
+    rp = oracle.reg_params(symmetric=1, affine=0, nw=7, ng=3, pyr_num=2, nthreads=8)
+    ref_p, ref_c, _ = oracle.register(fix, mov, rp)
+    M1, o1 = transforms.matrix_offset(params, c, False)
+    M2, o2 = transforms.matrix_offset(ref_p, ref_c, False)
+    dmm = transforms.max_corner_distance(M1, o1, M2, o2, pair.fixed)
+    print(f"crop: cli vs oracle {dmm:.3e} mm")
+    assert dmm < 0.01
+
+
+def test_cli_reorient_flags_give_the_same_registration(tmp_path):
+    """Shifting both images to identity space before registering and
+    composing the shifts back gives the registration of the original images
+    (the shifts are translations of images already in ARI orientation)."""
+    pair = phantom.make_pair(dims=(64, 64, 32), spacing=(3.0, 3.0, 3.0), n_ellipsoids=20,
+                             seed=4, model="rigid")
+    fileio.write_nifti(str(tmp_path / "mov.nii.gz"), pair.moving)
+    fileio.write_nifti(str(tmp_path / "fix.nii.gz"), pair.fixed)
+    out = {}
+    for name, extra in (("plain", []), ("shifted", ["--reorient-moving", "--reorient-fixed"])):
+        args = [CLI, "mov.nii.gz", "fix.nii.gz", f"{name}.tfm", "-d", "2", "-c", "0", "--fresh",
+                *extra]
+        print("cli", " ".join(args[1:]), flush=True)
+        r = subprocess.run(args, cwd=tmp_path, capture_output=True, text=True, timeout=180)
+        assert r.returncode == 0, r.stderr + r.stdout[-2000:]
+        out[name] = transforms.matrix_offset(*fileio.read_tfm(str(tmp_path / f"{name}.tfm"),
+                                                              affine=False), False)
+    dmm = transforms.max_corner_distance(*out["plain"], *out["shifted"], pair.fixed)
+    print(f"reorient flags: {dmm:.3e} mm")
+    assert dmm < 0.05
