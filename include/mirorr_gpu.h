@@ -197,7 +197,20 @@ typedef struct mirorr_reg_params {
   double stop_tolerance;
   double lts_tolerance;
   int block_metric;   /* --blockmetric: MIRORR_METRIC_NC (default), _SSD, _CR    */
+  int resampling;     /* --resampling-mode: MIRORR_RESAMPLE_MIDDLE (default),    */
+                      /* _BASIC, _MOVING or _FIXED                               */
 } mirorr_reg_params;
+
+/* Resampling modes (MirorrRegistrationMethod's EResampling*): the grid each
+ * level's images are resampled onto. Symmetric mode: middle and basic use
+ * the half-way grid, moving and fixed that image's grid, all centred.
+ * Classic mode: middle the half-way grid about the moving centre, moving and
+ * basic the moving grid, fixed the fixed grid mapped back by the level's
+ * start transform (itkMirorrRegistrationMethod.txx:450-531). */
+#define MIRORR_RESAMPLE_MIDDLE 0
+#define MIRORR_RESAMPLE_BASIC 1
+#define MIRORR_RESAMPLE_MOVING 2
+#define MIRORR_RESAMPLE_FIXED 3
 
 typedef struct mirorr_reg_stats {
   int levels;

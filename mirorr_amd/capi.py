@@ -15,6 +15,9 @@ OK, ERR_ARG, ERR_PLATFORM, ERR_ALGO, ERR_NOMEM, ERR_STATE = range(6)
 ROLE_BASE, ROLE_SEARCH = 0, 1
 INTERP_NN, INTERP_LINEAR, INTERP_BSPLINE, INTERP_SINC = range(4)
 METRIC_NC, METRIC_SSD, METRIC_CR = range(3)
+RESAMPLE_MIDDLE, RESAMPLE_BASIC, RESAMPLE_MOVING, RESAMPLE_FIXED = range(4)
+RESAMPLING_NAMES = {"middle": RESAMPLE_MIDDLE, "basic": RESAMPLE_BASIC,
+                    "moving": RESAMPLE_MOVING, "fixed": RESAMPLE_FIXED}
 # BlockMatcher's metric names (itkBlockMatcher.h:316-323) and the CLI's short ones.
 METRIC_NAMES = {"normalized_correlation": METRIC_NC, "nc": METRIC_NC,
                 "sum_of_squared_differences": METRIC_SSD, "sd": METRIC_SSD,
@@ -66,7 +69,7 @@ class RegParams(C.Structure):
                 ("pyr_num", C.c_int), ("pyr_min_size", C.c_int), ("portion_kept", C.c_double),
                 ("variance_select", C.c_int), ("verbosity", C.c_int), ("centre_init", C.c_int),
                 ("stop_tolerance", C.c_double), ("lts_tolerance", C.c_double),
-                ("block_metric", C.c_int)]
+                ("block_metric", C.c_int), ("resampling", C.c_int)]
 
 
 class RegStats(C.Structure):

@@ -6,9 +6,8 @@
 // Same option names, short forms, defaults, positional order (moving, fixed,
 // last-tfm, start-tfm), output-transform naming, --fresh rule and output
 // images. Options that select work this build does not implement
-// (--blockmetric mi, --tfm-type quat, --resampling-mode other than middle)
-// fail with a message instead of being ignored; --nthreads and --use-gpu-bm
-// are accepted and have no effect.
+// (--blockmetric mi, --tfm-type quat) fail with a message instead of being
+// ignored; --nthreads and --use-gpu-bm are accepted and have no effect.
 #include <sys/stat.h>
 
 #include <cmath>
@@ -66,7 +65,7 @@ const Opt kOptions[] = {
     {"pyr-end", 'c', true, "1", "Last pyramid level to process (-ve counts back from max)"},
     {"pyr-num", 'd', true, "1024", "Maximum number of pyramid levels"},
     {"pyr-min-size", 'e', true, "32", "Minimum image size at the coarsest level"},
-    {"resampling-mode", 0, true, "middle", "Resampling mode (middle)"},
+    {"resampling-mode", 0, true, "middle", "Resampling mode: middle, basic, moving or fixed"},
     {"crop", 0, true, "", "Crop the moving and fixed images on both sides of axes 0 and 1: \"mx,my,fx,fy\" voxels"},
     {"iterations", 'n', true, "0", "Iterations per level (0: default, -ve halves per level)"},
     {"portion-kept", 0, true, "0.5", "Portion of the block matches kept"},
@@ -465,9 +464,23 @@ int run(int argc, char** argv) {
   }
   const bool reorient_fixed = a.has("reorient-fixed"), reorient_moving = a.has("reorient-moving");
   const std::vector<unsigned> crop = parse_crop(a.str("crop"));
-  if (a.str("resampling-mode") != "middle")
-    throw std::runtime_error("--resampling-mode " + a.str("resampling-mode") +
-                             " is not supported by this build (middle only)");
+  // mirorr.cxx:532-555. max-resolution and max-size pass the option parser
+  // but StartOptimization rejects them (itkMirorrRegistrationMethod.txx:573-579).
+  const std::string rmode = a.str("resampling-mode");
+  int resampling = MIRORR_RESAMPLE_MIDDLE;
+  if (rmode == "basic") {
+    resampling = MIRORR_RESAMPLE_BASIC;
+  } else if (rmode == "moving") {
+    resampling = MIRORR_RESAMPLE_MOVING;
+  } else if (rmode == "fixed") {
+    resampling = MIRORR_RESAMPLE_FIXED;
+  } else if (rmode == "max-resolution" || rmode == "max-size") {
+    throw std::runtime_error("Unknown resampling type (" + rmode + "). This should not happen...");
+  } else if (rmode != "middle") {
+    std::fprintf(stderr, "ERROR: Unknown resampling-mode. Available options are 'classic' and "
+                         "'symmetric' (default).\n");
+    return -1;
+  }
   const std::string metric = a.str("blockmetric");
   int block_metric = MIRORR_METRIC_NC;
   if (metric == "sd") {
@@ -552,6 +565,7 @@ int run(int argc, char** argv) {
   rp.verbosity = verbosity;
   rp.centre_init = !a.has("no-centering-init");
   rp.block_metric = block_metric;
+  rp.resampling = resampling;
   if (a.integer("blockwidth") > 0) rp.block_width = a.integer("blockwidth");
   if (a.integer("blockgap") > 0) rp.block_gap = a.integer("blockgap");
   if (a.integer("nhoodwidth") > 0) rp.nhood_width = a.integer("nhoodwidth");

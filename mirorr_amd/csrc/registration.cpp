@@ -80,6 +80,32 @@ Geom middle_grid(const Geom& F, const Geom& M, const double cF[3], const double 
   return h;
 }
 
+// Grid the images are resampled onto for one level under a resampling mode
+// (itkSymMirorrRegistrationMethod.txx:264-301, itkMirorrRegistrationMethod.txx:
+// 450-531). Symmetric: the grid is in centred coordinates (basic = middle).
+// Classic: middle adds the moving centre; fixed takes the fixed grid with its
+// origin mapped by the inverse of the level's start transform T; moving and
+// basic take the moving grid (basic with the moving direction for both
+// images, as its resampler uses the moving image as reference).
+Geom level_grid(int mode, bool classic, const Geom& F, const Geom& M, const double cF[3],
+                const double cM[3], const xf::Xf& T, const double* dir) {
+  if (mode == MIRORR_RESAMPLE_MIDDLE || (mode == MIRORR_RESAMPLE_BASIC && !classic))
+    return middle_grid(F, M, cF, cM, classic, dir);
+  const bool fixed = mode == MIRORR_RESAMPLE_FIXED;
+  Geom h = fixed ? F : M;
+  const double* c = fixed ? cF : cM;
+  if (!classic) {
+    for (int a = 0; a < 3; ++a) h.origin[a] -= c[a];
+  } else if (fixed) {
+    double o[3];
+    xf::apply(xf::inverse(T), F.origin, o);
+    for (int a = 0; a < 3; ++a) h.origin[a] = o[a];
+  }
+  const double* d = (classic && mode == MIRORR_RESAMPLE_BASIC) ? M.dir : dir;
+  for (int i = 0; i < 9; ++i) h.dir[i] = d[i];
+  return h;
+}
+
 struct Level {
   DeviceBuffer data;
   Geom geom;
@@ -362,8 +388,8 @@ void run_registration(mirorr_ctx* ctx, const mirorr_reg_params* p, const mirorr_
     o.gMfull = MG;
     const double c0[3] = {T.c[0], T.c[1], T.c[2]};
     if (run.symmetric) {
-      o.gHF = middle_grid(gF, gM, cF, cM, false, gF.dir);
-      o.gHM = middle_grid(gF, gM, cF, cM, false, gM.dir);
+      o.gHF = level_grid(p->resampling, false, gF, gM, cF, cM, T, gF.dir);
+      o.gHM = level_grid(p->resampling, false, gF, gM, cF, cM, T, gM.dir);
       // Centre m_Transform and express it between centred images.
       xf::set_center_to_origin(T);
       o.fixT = xf::translation(cF);
@@ -395,8 +421,8 @@ void run_registration(mirorr_ctx* ctx, const mirorr_reg_params* p, const mirorr_
       xf::set_translation(o.t12F, a);
       xf::set_translation(o.t12B, b);
     } else {
-      o.gHF = middle_grid(gF, gM, cF, cM, true, gF.dir);
-      o.gHM = middle_grid(gF, gM, cF, cM, true, gM.dir);
+      o.gHF = level_grid(p->resampling, true, gF, gM, cF, cM, T, gF.dir);
+      o.gHM = level_grid(p->resampling, true, gF, gM, cF, cM, T, gM.dir);
     }
     o.T = T;
     o.stop_tol = p->stop_tolerance;
@@ -540,6 +566,7 @@ void mirorr_reg_default_params(mirorr_reg_params* p) {
   p->stop_tolerance = 1e-3;
   p->lts_tolerance = 1e-3;
   p->block_metric = MIRORR_METRIC_NC;
+  p->resampling = MIRORR_RESAMPLE_MIDDLE;
 }
 
 mirorr_status mirorr_volume_create(mirorr_ctx* c, const float* data, const uint8_t* mask,
@@ -595,6 +622,8 @@ mirorr_status mirorr_register(mirorr_ctx* c, const mirorr_reg_params* p,
       throw Error(kErrArg, "portion_kept should be between 0 and 1.0");
     if (p->block_metric < MIRORR_METRIC_NC || p->block_metric > MIRORR_METRIC_CR)
       throw Error(kErrArg, "unknown block metric");
+    if (p->resampling < MIRORR_RESAMPLE_MIDDLE || p->resampling > MIRORR_RESAMPLE_FIXED)
+      throw Error(kErrArg, "Unknown resampling type");
     if (p->block_metric != MIRORR_METRIC_NC && p->block_width > kMaxMetricBW)
       throw Error(kErrArg, "block width above 16 for this block metric");
     if (!(p->stop_tolerance >= 0.0) || !(p->lts_tolerance >= 0.0))

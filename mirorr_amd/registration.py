@@ -99,6 +99,13 @@ class MirorrPyramid:
     def SetVerbosity(self, v): self._p.verbosity = int(v)
     def SetCentreInitialisation(self, on: bool): self._p.centre_init = 1 if on else 0
 
+    def SetResamplingType(self, name: str):
+        """middle (default), basic, moving or fixed (SetResamplingType,
+        itkMirorrPyramidImplement.txx:515-545)."""
+        if name not in capi.RESAMPLING_NAMES:
+            raise ValueError(f"Unknown resampling type {name}")
+        self._p.resampling = capi.RESAMPLING_NAMES[name]
+
     def SetBlockMetricType(self, name: str):
         """normalized_correlation (default), sum_of_squared_differences or
         correlation_ratio (MirorrRegistrationMethod::SetBlockMetricType)."""

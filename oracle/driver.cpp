@@ -130,6 +130,29 @@ orc_geom middle_grid(const orc_geom& F, const orc_geom& M, const double cF[3],
   return h;
 }
 
+// Level grid under a resampling mode (itkSymMirorrRegistrationMethod.txx:
+// 264-301; itkMirorrRegistrationMethod.txx:450-531). Symmetric grids are in
+// centred coordinates and basic equals middle there; classic fixed maps the
+// fixed origin back through T, classic basic uses the moving grid and
+// direction for both images.
+orc_geom mode_grid(int mode, bool classic, const orc_geom& F, const orc_geom& M,
+                   const double cF[3], const double cM[3], const Xf& T, const double* direction) {
+  if (mode == 0 || (mode == 1 && !classic)) return middle_grid(F, M, cF, cM, classic, direction);
+  const bool fixed = mode == 3;
+  orc_geom h = fixed ? F : M;
+  const double* c = fixed ? cF : cM;
+  if (!classic) {
+    for (int a = 0; a < 3; ++a) h.origin[a] -= c[a];
+  } else if (fixed) {
+    double o[3];
+    T.inverse().apply(F.origin, o);
+    for (int a = 0; a < 3; ++a) h.origin[a] = o[a];
+  }
+  const double* d = (classic && mode == 1) ? M.direction : direction;
+  for (int i = 0; i < 9; ++i) h.direction[i] = d[i];
+  return h;
+}
+
 Xf translation_xf(const double t[3]) {
   Xf x;
   x.kind = kEuler3D;
@@ -196,8 +219,9 @@ void start_opt_symmetric(Xf& T, const LevelIn& L, int max_iter, const orc_reg_pa
   double cF[3], cM[3];
   image_centre(L.F->g, cF);
   image_centre(L.M->g, cM);
-  const orc_geom gF = middle_grid(L.F->g, L.M->g, cF, cM, false, L.F->g.direction);
-  const orc_geom gM = middle_grid(L.F->g, L.M->g, cF, cM, false, L.M->g.direction);
+  const orc_geom gF = mode_grid(p.resampling, false, L.F->g, L.M->g, cF, cM, T, L.F->g.direction);
+  const orc_geom gM = mode_grid(p.resampling, false, L.F->g, L.M->g, cF, cM, T, L.M->g.direction);
+  for (int a = 0; a < 3; ++a) st->level_dims[level][a] = gF.dims[a];
   const double c0[3] = {T.c[0], T.c[1], T.c[2]};
   set_center_to_origin(T);
   const Xf fT = translation_xf(cF), mT = translation_xf(cM);
@@ -270,8 +294,9 @@ void start_opt_classic(Xf& T, const LevelIn& L, int max_iter, const orc_reg_para
   double cF[3], cM[3];
   image_centre(L.F->g, cF);
   image_centre(L.M->g, cM);
-  const orc_geom gF = middle_grid(L.F->g, L.M->g, cF, cM, true, L.F->g.direction);
-  const orc_geom gM = middle_grid(L.F->g, L.M->g, cF, cM, true, L.M->g.direction);
+  const orc_geom gF = mode_grid(p.resampling, true, L.F->g, L.M->g, cF, cM, T, L.F->g.direction);
+  const orc_geom gM = mode_grid(p.resampling, true, L.F->g, L.M->g, cF, cM, T, L.M->g.direction);
+  for (int a = 0; a < 3; ++a) st->level_dims[level][a] = gF.dims[a];
   Xf I;
   I.kind = T.kind;
   I.set_identity();
@@ -328,6 +353,7 @@ void orc_reg_default_params(orc_reg_params* p) {
   p->stop_tolerance = 1e-3;
   p->lts_tolerance = 1e-3;
   p->metric = 0;
+  p->resampling = 0;
   p->nthreads = 0;
   p->verbosity = 0;
   p->centre_init = 1;
@@ -383,7 +409,6 @@ static int register_impl(const float* fixed, const uint8_t* fmask, const orc_geo
   for (int level = 0; level < t_max && level < 16; ++level) {
     T.set_params(cur);
     LevelIn L{&Fp[level], &Mp[level], fixed, fmask, fg, moving, mmask, mg};
-    for (int a = 0; a < 3; ++a) st.level_dims[level][a] = (Fp[level].g.dims[a] + Mp[level].g.dims[a]) / 2;
     if (p->symmetric)
       start_opt_symmetric(T, L, max_it, *p, &st, level);
     else

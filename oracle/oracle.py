@@ -104,7 +104,7 @@ class RegParams(C.Structure):
                 ("pyr_min_size", C.c_int), ("portion_kept", C.c_double), ("nthreads", C.c_int),
                 ("verbosity", C.c_int), ("centre_init", C.c_int),
                 ("variance_select", C.c_int), ("stop_tolerance", C.c_double),
-                ("lts_tolerance", C.c_double), ("metric", C.c_int)]
+                ("lts_tolerance", C.c_double), ("metric", C.c_int), ("resampling", C.c_int)]
 
 
 class RegStats(C.Structure):

@@ -38,6 +38,7 @@ typedef struct orc_reg_params {
   double stop_tolerance;
   double lts_tolerance;
   int metric;        // --blockmetric: 0 nc (default), 1 sd, 2 cr
+  int resampling;    // --resampling-mode: 0 middle (default), 1 basic, 2 moving, 3 fixed
 } orc_reg_params;
 
 typedef struct orc_reg_stats {

@@ -51,7 +51,8 @@ def test_help_and_missing_inputs(images):
     (["--blockmetric", "xx"], "not a valid blockmetric"),
     (["--blockmetric", "mi"], "not supported"),
     (["-t", "quat"], "not supported"),
-    (["--resampling-mode", "basic"], "not supported"),
+    (["--resampling-mode", "max-size"], "Unknown resampling type"),
+    (["--resampling-mode", "other"], "Unknown resampling-mode"),
     (["--save-reoriented-moving", "a.nii", "--save-resampled-moving", "b.nii"],
      "Cannot use save-resampled-moving and save-reoriented-moving"),
     (["--bogus"], "unrecognised option"),

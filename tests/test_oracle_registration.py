@@ -52,3 +52,23 @@ def test_oracle_variance_selection_applies_from_level_two(oracle):
     init = transforms.max_corner_distance(np.eye(3), np.zeros(3), pair.matrix,
                                           pair.translation, pair.fixed)
     assert err < 4.0 and err < init / 5.0, (err, init)
+
+
+@pytest.mark.parametrize("symmetric", [1, 0])
+@pytest.mark.parametrize("mode,expect", [(0, "middle"), (1, "basic"), (2, "moving"), (3, "fixed")])
+def test_resampling_mode_grids(oracle, symmetric, mode, expect):
+    """The level grid of each resampling mode (itkSymMirorrRegistrationMethod.txx:264-301,
+    itkMirorrRegistrationMethod.txx:450-531): one level, fixed 40x36x24 and
+    moving 32x44x20 voxels of 2 mm."""
+    from mirorr_amd import phantom
+    from mirorr_amd.phantom import Volume
+    a = phantom.make_pair(dims=(40, 36, 24), spacing=(2.0, 2.0, 2.0), n_ellipsoids=8, seed=1)
+    b = phantom.make_pair(dims=(32, 44, 20), spacing=(2.0, 2.0, 2.0), n_ellipsoids=8, seed=1)
+    fixed, moving = a.fixed, b.moving
+    rp = oracle.reg_params(symmetric=symmetric, affine=0, nw=5, ng=4, pyr_num=1, iterations=1,
+                           resampling=mode)
+    _, _, st = oracle.register(fixed, moving, rp)
+    got = st["level_dims"][0]
+    want = {"middle": (36, 40, 22), "moving": (32, 44, 20), "fixed": (40, 36, 24),
+            "basic": (36, 40, 22) if symmetric else (32, 44, 20)}[expect]
+    assert tuple(got) == want
