@@ -44,6 +44,28 @@ WORKLOADS = {
 }
 
 
+# Served workloads: many registrations through the registration server
+# (mirorr_submit / mirorr_poll), against the plain one-pair-at-a-time path.
+SERVED = {
+    # 64 pairs of uneven size; even-numbered pairs rigid with the 128x128x64
+    # workload's parameters, odd-numbered affine with the 256x256x128 one's.
+    "batch64_uneven": dict(
+        workload="batch64_uneven", pairs=64, axis_lengths=[96, 320, 16],
+        spacing_mm=[0.8, 2.0], slots=4, seed=7,
+        rigid=dict(ellipsoids=20, nhood_width=7, nhood_gap=4, pyramid_levels=2,
+                   portion_kept=0.5, variance_select=1),
+        affine=dict(ellipsoids=60, nhood_width=9, nhood_gap=3, pyramid_levels=3,
+                    iterations_per_level=5)),
+    # Poisson arrivals of CT/MR-like rigid requests, 30% at 512x512x256 and
+    # the rest at 256x256x128, drawn from a pool of distinct pairs.
+    "serve_poisson_ctmr": dict(
+        workload="serve_poisson_ctmr", rates_per_s=[0.5, 1.0, 2.0, 4.0],
+        requests_per_rate=12, large_fraction=0.3, large_dims=[512, 512, 256],
+        small_dims=[256, 256, 128], pool_large=2, pool_small=3, slots=4, seed=11,
+        nhood_width=13, nhood_gap=3, pyramid_levels=4, ellipsoids=24),
+}
+
+
 def workload(name="pair256_affine_symmetric"):
     return dict(WORKLOADS[name])
 
@@ -112,19 +134,228 @@ def cpu_baseline(pair, cfg, threads, max_evals=20e9):
                       f"({evals / 1e9:.2f} G voxel-evals, {dt:.1f} s)"}
 
 
+def _make_batch_pair(job):
+    from mirorr_amd import phantom
+    i, dims, spacing, model, n_ell, seed = job
+    return phantom.make_pair(dims=dims, spacing=spacing, n_ellipsoids=n_ell, seed=seed,
+                             model=model)
+
+
+def _make_ctmr_pair(job):
+    from mirorr_amd import phantom
+    dims, seed, n_ell = job
+    return phantom.make_ct_mr_pair(dims=dims, spacing=(1.0, 1.0, 1.0), n_ellipsoids=n_ell,
+                                   seed=seed)
+
+
+def _pool_map(fn, jobs):
+    """Generates inputs in worker processes (before the GPU is touched)."""
+    import multiprocessing as mp
+    n = max(1, min(len(jobs), 16, len(os.sched_getaffinity(0))))
+    with mp.get_context("fork").Pool(n) as pool:
+        return pool.map(fn, jobs)
+
+
+def _params(model, c):
+    import ctypes as C
+    from mirorr_amd import capi
+    p = capi.RegParams()
+    capi.lib().mirorr_reg_default_params(C.byref(p))
+    p.symmetric = 1
+    p.affine = 1 if model == "affine" else 0
+    p.block_width, p.block_gap = 4, 1
+    p.nhood_width, p.nhood_gap = c["nhood_width"], c["nhood_gap"]
+    p.pyr_num, p.pyr_min_size, p.pyr_end = c["pyramid_levels"], 32, 0
+    p.iterations = c.get("iterations_per_level", 0)
+    if c.get("variance_select"):
+        p.variance_select, p.portion_kept = 1, c["portion_kept"]
+    return p
+
+
+def _plain_run(ctx, pair, p):
+    """The plain path: upload both volumes, register, on one context."""
+    import ctypes as C
+    from mirorr_amd import capi
+    from mirorr_amd.registration import DeviceVolume
+    fd, md = DeviceVolume(pair.fixed, ctx), DeviceVolume(pair.moving, ctx)
+    out, oc, st = np.zeros(12), np.zeros(3), capi.RegStats()
+    capi.check(capi.lib().mirorr_register(ctx.handle, C.byref(p), fd.handle, md.handle, None, None,
+                                          capi.ptr(out), capi.ptr(oc), C.byref(st)), ctx.handle)
+    fd.close()
+    md.close()
+    return out[: 12 if p.affine else 6], oc, st.voxel_evals
+
+
+def _served_line(cfg, args, extra):
+    out = {
+        "metric": "NCC voxel-evals/s + registrations/s (p50/p99 latency) vs plain GPU, 1 GPU",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic", "config": cfg, "roofline": None, "cpu_baseline": None,
+    }
+    out.update(extra)
+    return out
+
+
+def run_batch(args, cfg):
+    """One step = every pair of the batch registered once, through the server
+    (all submitted at once) and, for comparison, through the plain path."""
+    from mirorr_amd.blockmatcher import default_context
+    from mirorr_amd.server import RegistrationServer
+    rng = np.random.default_rng(cfg["seed"])
+    lo, hi, st = cfg["axis_lengths"]
+    jobs, params = [], []
+    for i in range(cfg["pairs"]):
+        model = "rigid" if i % 2 == 0 else "affine"
+        dims = tuple(int(v) for v in rng.choice(np.arange(lo, hi + 1, st), 3))
+        sp = tuple(float(v) for v in rng.uniform(cfg["spacing_mm"][0], cfg["spacing_mm"][1], 3))
+        jobs.append((i, dims, sp, model, cfg[model]["ellipsoids"], cfg["seed"] * 1000 + i))
+        params.append(_params(model, cfg[model]))
+    print(f"bench: generating {len(jobs)} pairs", file=sys.stderr, flush=True)
+    pairs = _pool_map(_make_batch_pair, jobs)
+    ctx = default_context()
+    print("bench: plain path", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        for pr, p in zip(pairs, params):
+            _plain_run(ctx, pr, p)
+    t = time.perf_counter()
+    plain = []
+    for _ in range(args.steps):
+        plain = [_plain_run(ctx, pr, p) for pr, p in zip(pairs, params)]
+    t_plain = time.perf_counter() - t
+    evals = sum(e for _, _, e in plain)
+    print(f"bench: plain {t_plain / args.steps * 1e3:.0f} ms per batch", file=sys.stderr,
+          flush=True)
+    with RegistrationServer(slots=cfg["slots"]) as srv:
+        def batch():
+            ts = [srv.submit(pr.fixed, pr.moving, p, borrow=True) for pr, p in zip(pairs, params)]
+            return [srv.poll(x) for x in ts]
+        for _ in range(args.warmup):
+            batch()
+        lat, res = [], []
+        t = time.perf_counter()
+        for k in range(args.steps):
+            res = batch()
+            lat += [r["latency"] for r in res]
+            print(f"bench: served step {k + 1}/{args.steps}", file=sys.stderr, flush=True)
+        t_served = time.perf_counter() - t
+        info = srv.info()
+    bad = [r["error"] for r in res if r["status"] != 0]
+    if bad:
+        raise RuntimeError(f"served registrations failed: {bad[:3]}")
+    same = all(np.array_equal(r["params"], q) and np.array_equal(r["center"], c)
+               for r, (q, c, _) in zip(res, plain))
+    n = cfg["pairs"] * args.steps
+    lat_ms = np.array(lat) * 1e3
+    return _served_line(cfg, args, {
+        "value": evals * args.steps / t_served,
+        "unit": "voxel-evals/s",
+        "ms_per_step": t_served / args.steps * 1e3,
+        "registrations_per_s": n / t_served,
+        "plain_registrations_per_s": n / t_plain,
+        "vs_plain": t_plain / t_served,
+        "latency_ms_p50": float(np.percentile(lat_ms, 50)),
+        "latency_ms_p99": float(np.percentile(lat_ms, 99)),
+        "latency_note": "all pairs of a step are submitted at once",
+        "voxel_evals_per_step": evals,
+        "results_identical_to_plain": bool(same),
+        "max_running": info["max_running"],
+        "hbm_peak_bytes": info["hbm_peak"],
+    })
+
+
+def run_serve(args, cfg):
+    """Poisson arrivals at each rate, served with cfg["slots"] workers; at the
+    highest rate the same stream also through one worker (one request at a
+    time, the plain path's service order)."""
+    from mirorr_amd.server import RegistrationServer
+    rng = np.random.default_rng(cfg["seed"])
+    jobs = ([(tuple(cfg["large_dims"]), cfg["seed"] + k, cfg["ellipsoids"])
+             for k in range(cfg["pool_large"])] +
+            [(tuple(cfg["small_dims"]), cfg["seed"] + 100 + k, cfg["ellipsoids"])
+             for k in range(cfg["pool_small"])])
+    print(f"bench: generating {len(jobs)} CT/MR pairs", file=sys.stderr, flush=True)
+    pool = _pool_map(_make_ctmr_pair, jobs)
+    large, small = pool[: cfg["pool_large"]], pool[cfg["pool_large"]:]
+    p = _params("rigid", cfg)
+
+    def stream(rate, n):
+        gaps = rng.exponential(1.0 / rate, n)
+        picks = [(large if rng.random() < cfg["large_fraction"] else small) for _ in range(n)]
+        return np.cumsum(gaps) - gaps[0], [c[rng.integers(len(c))] for c in picks]
+
+    def serve(srv, arrivals, reqs):
+        t0 = time.perf_counter()
+        ts = []
+        for a, pr in zip(arrivals, reqs):
+            d = t0 + a - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            ts.append(srv.submit(pr.fixed, pr.moving, p, borrow=True))
+        res = [srv.poll(x) for x in ts]
+        bad = [r["error"] for r in res if r["status"] != 0]
+        if bad:
+            raise RuntimeError(f"served registrations failed: {bad[:3]}")
+        lat = np.array([r["latency"] for r in res]) * 1e3
+        span = max(r["t_end"] for r in res) - min(r["t_submit"] for r in res)
+        return {"latency_ms_p50": float(np.percentile(lat, 50)),
+                "latency_ms_p99": float(np.percentile(lat, 99)),
+                "registrations_per_s": len(res) / span}, res
+
+    per_rate = {}
+    with RegistrationServer(slots=cfg["slots"]) as srv:
+        for pr in large[:1] + small[:1]:  # warm the workers' workspaces
+            srv.poll(srv.submit(pr.fixed, pr.moving, p, borrow=True))
+        streams = {}
+        for rate in cfg["rates_per_s"]:
+            streams[rate] = stream(rate, cfg["requests_per_rate"])
+            per_rate[str(rate)], res = serve(srv, *streams[rate])
+            print(f"bench: rate {rate}/s {per_rate[str(rate)]}", file=sys.stderr, flush=True)
+        evals = sum(r["stats"]["voxel_evals"] for r in res)
+        busy = sum(r["t_end"] - r["t_start"] for r in res)
+        info = srv.info()
+    top = max(cfg["rates_per_s"])
+    with RegistrationServer(slots=1) as one:
+        one.poll(one.submit(large[0].fixed, large[0].moving, p, borrow=True))
+        plain_top, _ = serve(one, *streams[top])
+    print(f"bench: one at a time at {top}/s {plain_top}", file=sys.stderr, flush=True)
+    at = per_rate[str(top)]
+    return _served_line(cfg, args, {
+        "value": evals / busy,
+        "unit": "voxel-evals/s",
+        "value_note": "voxel-evals per second of request service time at the highest rate",
+        "ms_per_step": busy / len(res) * 1e3,
+        "registrations_per_s": at["registrations_per_s"],
+        "latency_ms_p50": at["latency_ms_p50"],
+        "latency_ms_p99": at["latency_ms_p99"],
+        "per_rate": per_rate,
+        "one_at_a_time_at_highest_rate": plain_top,
+        "vs_plain_p99": plain_top["latency_ms_p99"] / at["latency_ms_p99"],
+        "max_running": info["max_running"],
+        "hbm_peak_bytes": info["hbm_peak"],
+        "data": "synthetic; requests drawn from a pool of distinct CT/MR-like pairs",
+    })
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="pair256_affine_symmetric", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="pair256_affine_symmetric",
+                    choices=sorted(WORKLOADS) + sorted(SERVED))
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported "
               "(independent registrations shard across GPUs with no communication).",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload in SERVED:
+        cfg = dict(SERVED[args.workload])
+        run = run_batch if args.workload.startswith("batch") else run_serve
+        print(json.dumps(run(args, cfg)), flush=True)
+        return
 
     from mirorr_amd.blockmatcher import default_context
     from mirorr_amd.registration import DeviceVolume, MirorrPyramid

@@ -39,7 +39,8 @@ typedef enum mirorr_status {
   MIRORR_ERR_PLATFORM = 2,
   MIRORR_ERR_ALGO = 3,
   MIRORR_ERR_NOMEM = 4,
-  MIRORR_ERR_STATE = 5
+  MIRORR_ERR_STATE = 5,
+  MIRORR_PENDING = 6 /* mirorr_poll: the request has not finished yet */
 } mirorr_status;
 
 typedef struct mirorr_ctx mirorr_ctx;
@@ -240,6 +241,85 @@ mirorr_status mirorr_register(mirorr_ctx* ctx, const mirorr_reg_params* p,
                               const mirorr_volume* fixed, const mirorr_volume* moving,
                               const double* init_params, const double* init_center,
                               double* out_params, double* out_center, mirorr_reg_stats* stats);
+
+/* ---- serving loop ---------------------------------------------------- */
+
+/* A registration server on one GPU. It has no counterpart in the reference,
+ * whose command line runs one pair per process (mirorr.cxx:309-619); it is
+ * the request loop a service puts in front of mirorr_register. Requests are
+ * admitted in arrival order onto `slots` workers, each with its own context
+ * and HIP stream, whenever a worker is free and the request's device memory
+ * bound (volumes, pyramids and workspace) fits in what the running requests
+ * and the workers' kept workspaces leave of `hbm_budget`; workspaces of idle
+ * workers are released when a request would not fit otherwise. The work of
+ * admitted requests overlaps on the device. A request that cannot fit even
+ * alone completes with MIRORR_ERR_NOMEM. Every worker's context charges the
+ * server's budget, so the limit also holds if a bound were exceeded.
+ * hbm_budget = 0: the device's free memory at creation minus a reserve of
+ * min(8 GiB, 10%). slots <= 0: 4. */
+typedef struct mirorr_server mirorr_server;
+
+/* One registration: mirorr_volume_create of both images and mirorr_register
+ * with these parameters. Masks and init_params/init_center may be NULL (as
+ * for mirorr_register). borrow = 0: mirorr_submit copies the host arrays;
+ * borrow = 1: the caller keeps the volume and mask arrays unchanged until
+ * mirorr_poll has returned the request's result, and the worker uploads
+ * from them directly. */
+typedef struct mirorr_request {
+  mirorr_reg_params params;
+  const float* fixed;
+  const uint8_t* fixed_mask;
+  mirorr_geom fixed_geom;
+  const float* moving;
+  const uint8_t* moving_mask;
+  mirorr_geom moving_geom;
+  const double* init_params;
+  const double* init_center;
+  int borrow;
+} mirorr_request;
+
+typedef struct mirorr_request_result {
+  mirorr_status status;   /* mirorr_register's status                       */
+  double params[12];      /* as mirorr_register's out_params / out_center    */
+  double center[3];
+  mirorr_reg_stats stats;
+  double t_submit;        /* seconds since the server's creation             */
+  double t_start;         /* admitted onto a worker                          */
+  double t_end;           /* result ready                                    */
+  int slot;               /* worker that ran it (-1: never admitted)         */
+  size_t bytes_bound;     /* the device memory bound used for admission      */
+  char error[256];        /* message of a failed request                     */
+} mirorr_request_result;
+
+typedef struct mirorr_server_info {
+  int slots;
+  int pending;            /* submitted, not admitted                         */
+  int running;
+  int64_t completed;      /* results produced (collected or not)             */
+  size_t hbm_budget;
+  size_t hbm_used;        /* bytes held by the workers' contexts now         */
+  size_t hbm_peak;
+  int max_running;        /* most requests running at once so far            */
+} mirorr_server_info;
+
+mirorr_status mirorr_server_create(int device, size_t hbm_budget, int slots,
+                                   mirorr_server** out);
+/* Queues a request; *ticket identifies it to mirorr_poll. */
+mirorr_status mirorr_submit(mirorr_server* s, const mirorr_request* req, int64_t* ticket);
+/* The result of a ticket. timeout_s = 0: returns MIRORR_PENDING at once if
+ * it has not finished; < 0: waits for it; > 0: waits at most that long.
+ * MIRORR_OK means *out holds the result (its own status in out->status); a
+ * result is handed out once, after which the ticket is unknown
+ * (MIRORR_ERR_ARG). */
+mirorr_status mirorr_poll(mirorr_server* s, int64_t ticket, double timeout_s,
+                          mirorr_request_result* out);
+mirorr_status mirorr_server_info_get(mirorr_server* s, mirorr_server_info* out);
+/* Message of the last failed mirorr_server_create / mirorr_submit /
+ * mirorr_poll call on this host thread. */
+const char* mirorr_server_last_error(void);
+/* Drops requests not yet admitted, waits for the running ones, then frees
+ * the workers. */
+void mirorr_server_destroy(mirorr_server* s);
 
 /* OptimizeTransform on given pairs (x_i block centres, y_i matches, n x 3 mm):
  * classic (itkMirorrRegistrationMethod.txx:758-1003) composes the fit into

@@ -11,7 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libmirorr_gpu.so")
 
-OK, ERR_ARG, ERR_PLATFORM, ERR_ALGO, ERR_NOMEM, ERR_STATE = range(6)
+OK, ERR_ARG, ERR_PLATFORM, ERR_ALGO, ERR_NOMEM, ERR_STATE, PENDING = range(7)
 ROLE_BASE, ROLE_SEARCH = 0, 1
 INTERP_NN, INTERP_LINEAR, INTERP_BSPLINE, INTERP_SINC = range(4)
 METRIC_NC, METRIC_SSD, METRIC_CR = range(3)
@@ -78,6 +78,26 @@ class RegStats(C.Structure):
                 ("level_dims", (C.c_int * 3) * 16), ("seconds", C.c_double)]
 
 
+class Request(C.Structure):
+    _fields_ = [("params", RegParams), ("fixed", C.c_void_p), ("fixed_mask", C.c_void_p),
+                ("fixed_geom", Geom), ("moving", C.c_void_p), ("moving_mask", C.c_void_p),
+                ("moving_geom", Geom), ("init_params", C.c_void_p), ("init_center", C.c_void_p),
+                ("borrow", C.c_int)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("params", C.c_double * 12), ("center", C.c_double * 3),
+                ("stats", RegStats), ("t_submit", C.c_double), ("t_start", C.c_double),
+                ("t_end", C.c_double), ("slot", C.c_int), ("bytes_bound", C.c_size_t),
+                ("error", C.c_char * 256)]
+
+
+class ServerInfo(C.Structure):
+    _fields_ = [("slots", C.c_int), ("pending", C.c_int), ("running", C.c_int),
+                ("completed", C.c_int64), ("hbm_budget", C.c_size_t), ("hbm_used", C.c_size_t),
+                ("hbm_peak", C.c_size_t), ("max_running", C.c_int)]
+
+
 _P = C.c_void_p
 _lib = None
 
@@ -117,6 +137,12 @@ _SIGS = {
                                        C.POINTER(Geom)]),
     "mirorr_resample_final": (C.c_int, [_P, _P, C.POINTER(Geom), _P, _P, C.POINTER(Geom),
                                         C.c_int, _P]),
+    "mirorr_server_create": (C.c_int, [C.c_int, C.c_size_t, C.c_int, C.POINTER(_P)]),
+    "mirorr_submit": (C.c_int, [_P, C.POINTER(Request), C.POINTER(C.c_int64)]),
+    "mirorr_poll": (C.c_int, [_P, C.c_int64, C.c_double, C.POINTER(RequestResult)]),
+    "mirorr_server_info_get": (C.c_int, [_P, C.POINTER(ServerInfo)]),
+    "mirorr_server_last_error": (C.c_char_p, []),
+    "mirorr_server_destroy": (None, [_P]),
     # include/mirorr_io.h
     "mirorr_io_last_error": (C.c_char_p, []),
     "mirorr_io_free": (None, [_P]),
@@ -154,6 +180,15 @@ def check(status: int, ctx=None):
     if status == OK:
         return
     msg = lib().mirorr_last_error(ctx).decode() if ctx else "error"
+    cls = {ERR_PLATFORM: PlatformError, ERR_ALGO: AlgoError}.get(status, MirorrError)
+    raise cls(status, msg)
+
+
+def check_server(status: int):
+    """Raises for a failed server call (mirorr_server_create / _submit / _poll)."""
+    if status == OK:
+        return
+    msg = lib().mirorr_server_last_error().decode()
     cls = {ERR_PLATFORM: PlatformError, ERR_ALGO: AlgoError}.get(status, MirorrError)
     raise cls(status, msg)
 

@@ -40,7 +40,7 @@ struct mirorr_bm {
   BMGrid grid{};
 };
 
-static uint64_t g_version = 1;
+static std::atomic<uint64_t> g_version{1};
 
 static void check_params(const mirorr_bm_params* p) {
   if (!p) throw Error(kErrArg, "null parameters");

@@ -9,17 +9,29 @@
 
 namespace mirorr {
 
+// Bytes charged against a limit. A budget with a parent charges the parent
+// too, so several contexts can share one limit (a server's worker slots).
 struct Budget {
   size_t limit = 0;  // 0 = unlimited
   std::atomic<size_t> used{0};
+  std::atomic<size_t> peak{0};
+  Budget* parent = nullptr;
   void charge(size_t n) {
+    if (parent) parent->charge(n);
     const size_t now = used.fetch_add(n) + n;
     if (limit != 0 && now > limit) {
       used.fetch_sub(n);
+      if (parent) parent->release(n);
       throw Error(kErrNoMem, "HBM budget exceeded");
     }
+    size_t p = peak.load();
+    while (now > p && !peak.compare_exchange_weak(p, now)) {
+    }
   }
-  void release(size_t n) { used.fetch_sub(n); }
+  void release(size_t n) {
+    used.fetch_sub(n);
+    if (parent) parent->release(n);
+  }
 };
 
 class DeviceBuffer {

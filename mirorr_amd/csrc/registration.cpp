@@ -140,6 +140,30 @@ void build_pyramid(const float* d_in, const Geom& g, const std::vector<int>& sch
   }
 }
 
+// Shrink factors of the levels a registration runs (coarsest first): the
+// schedule cropped to [pyr_start, pyr_end] (itkPyramidScheduleTuner.h:148-161).
+struct LevelPlan {
+  int nlev = 0, t_max = 0;
+  std::vector<int> fs, ms;
+};
+
+LevelPlan plan_levels(const mirorr_reg_params* p, const Geom& FG, const Geom& MG) {
+  const Schedule sch = schedule_symmetric(MG.dims, MG.spacing, FG.dims, FG.spacing,
+                                          (unsigned)p->pyr_min_size, (unsigned)p->pyr_num);
+  if (sch.rows <= 0) throw Error(kErrArg, "empty pyramid schedule");
+  const int rows = sch.rows;
+  const int level_min = p->pyr_start == 0 ? 1 : p->pyr_start;
+  const int level_max = p->pyr_end == 0 ? rows : p->pyr_end;
+  int mn = cropped_level(level_min, 1, rows), mx = cropped_level(level_max, rows, rows);
+  if (mn > mx) mx = mn;
+  LevelPlan pl;
+  pl.nlev = mx - mn + 1;
+  pl.ms.assign(sch.moving.begin() + 3 * (mn - 1), sch.moving.begin() + 3 * mx);
+  pl.fs.assign(sch.fixed.begin() + 3 * (mn - 1), sch.fixed.begin() + 3 * mx);
+  pl.t_max = std::min(cropped_level(level_max, pl.nlev, rows), pl.nlev);
+  return pl;
+}
+
 // Per-registration device workspace, grown to the largest level.
 struct Workspace {
   DeviceBuffer Fh, Mh, Fmh, Mmh;              // resampled images and masks
@@ -307,6 +331,43 @@ struct RegRun {
   }
 };
 
+size_t registration_bytes_bound(const mirorr_reg_params* p, const Geom& FG, bool fmask,
+                                const Geom& MG, bool mmask) {
+  const size_t nF = nvox(FG), nM = nvox(MG);
+  // The two uploaded volumes and masks.
+  size_t total = sizeof(float) * (nF + nM) + (fmask ? nF : 0) + (mmask ? nM : 0);
+  // Pyramid levels (build_pyramid), its double scratch, sized by the finest
+  // input, and index map.
+  const LevelPlan plan = plan_levels(p, FG, MG);
+  for (int l = 0; l < plan.nlev; ++l) {
+    total += sizeof(float) * nvox(level_geom(FG, &plan.fs[3 * l]));
+    total += sizeof(float) * nvox(level_geom(MG, &plan.ms[3 * l]));
+  }
+  total += sizeof(double) * (2 * std::max(nF, nM) + 3 * 64) + sizeof(IndexMap);
+  // Level buffers (RegRun::ensure_level_buffers) at a grid no larger than
+  // the per-axis maxima of the two images: every resampling mode's grid is.
+  Geom gh = FG;
+  for (int a = 0; a < 3; ++a) gh.dims[a] = std::max(FG.dims[a], MG.dims[a]);
+  const size_t n = nvox(gh);
+  const BMGrid g = make_grid(gh.dims, p->block_width, p->block_gap, p->nhood_width,
+                             p->nhood_gap);
+  const size_t nb = (size_t)std::max<int64_t>(g.nblocks, 1);
+  const size_t n3 = (size_t)g.bw * g.bw * g.bw;
+  const size_t tiles = (size_t)std::max<int64_t>(tile_count(g), 1);
+  const size_t np = 2 * nb;
+  total += 2 * sizeof(float) * n + 2 * n + 2 * sizeof(float2) * n;  // Fh, Mh, masks, stats
+  total += 2 * sizeof(float) * nb * n3 + 4 * sizeof(float) * nb + 2 * sizeof(uint32_t) * nb;
+  total += 2 * sizeof(float) * nb + 3 * sizeof(int32_t) * nb;  // best, second, match
+  total += tiles * (sizeof(int32_t) + sizeof(unsigned long long)) + sizeof(int32_t);
+  total += 2 * 3 * sizeof(double) * np + np + sizeof(double) * np +
+           sizeof(unsigned long long) * np;  // X, Y, kept, r, cand
+  total += sizeof(double) * 32 * (size_t)lts_parts((int64_t)np);
+  total += 2 * sizeof(unsigned int) * 4096 + sizeof(VarSelState) +
+           sizeof(unsigned int) * kVarSelHistBins + sizeof(unsigned long long);
+  total += sizeof(LtsState) + sizeof(OuterState) + 2 * sizeof(IndexMap) + 2 * sizeof(int);
+  return total;
+}
+
 }  // namespace mirorr
 
 namespace {
@@ -324,23 +385,13 @@ void run_registration(mirorr_ctx* ctx, const mirorr_reg_params* p, const mirorr_
   Workspace& w = run.w;
   const Geom& FG = fixed->geom;
   const Geom& MG = moving->geom;
-  // Schedule and cropping (itkPyramidScheduleTuner.h:148-161).
-  const Schedule sch = schedule_symmetric(MG.dims, MG.spacing, FG.dims, FG.spacing,
-                                          (unsigned)p->pyr_min_size, (unsigned)p->pyr_num);
-  if (sch.rows <= 0) throw Error(kErrArg, "empty pyramid schedule");
-  const int rows = sch.rows;
-  const int level_min = p->pyr_start == 0 ? 1 : p->pyr_start;
-  const int level_max = p->pyr_end == 0 ? rows : p->pyr_end;
-  int mn = cropped_level(level_min, 1, rows), mx = cropped_level(level_max, rows, rows);
-  if (mn > mx) mx = mn;
-  const int nlev = mx - mn + 1;
-  std::vector<int> ms(sch.moving.begin() + 3 * (mn - 1), sch.moving.begin() + 3 * mx);
-  std::vector<int> fs(sch.fixed.begin() + 3 * (mn - 1), sch.fixed.begin() + 3 * mx);
+  const LevelPlan plan = plan_levels(p, FG, MG);
+  const int nlev = plan.nlev;
   std::vector<Level>& Fp = cache.F;
   std::vector<Level>& Mp = cache.M;
-  build_pyramid(fixed->data.as<float>(), FG, fs, nlev, Fp, w.pyr_tmp, w.pyr_map, bud, s);
-  build_pyramid(moving->data.as<float>(), MG, ms, nlev, Mp, w.pyr_tmp, w.pyr_map, bud, s);
-  const int t_max = std::min(cropped_level(level_max, nlev, rows), nlev);
+  build_pyramid(fixed->data.as<float>(), FG, plan.fs, nlev, Fp, w.pyr_tmp, w.pyr_map, bud, s);
+  build_pyramid(moving->data.as<float>(), MG, plan.ms, nlev, Mp, w.pyr_tmp, w.pyr_map, bud, s);
+  const int t_max = plan.t_max;
   // Initial transform: given, or centred (itkMirorrPyramidWrapper.txx:611-623:
   // centre of the moving image, translation between the image centres), then
   // composed into an identity transform with centre 0 as the command line

@@ -1,0 +1,363 @@
+// Registration server: requests admitted onto worker slots of one GPU by an
+// HBM budget, their work overlapping on the device.
+//
+// Each slot owns a mirorr_ctx (its own HIP stream and registration cache)
+// and a host thread that runs mirorr_register for the request it is given.
+// Kernels of different slots' streams run concurrently, so a request that
+// arrives while others run starts at once if a slot is free and its memory
+// fits; the host round trips of one request (LTS flag reads, uploads) are
+// covered by the others' kernels. A slot's context keeps its workspace
+// between requests (grown to the largest it has run), so repeated requests
+// of similar size allocate nothing.
+//
+// Admission (under the server lock, in arrival order): a slot running a
+// request may hold up to the bytes it held when admitted plus the request's
+// bound (each workspace buffer only grows to the new size), an idle slot
+// holds its kept workspace. The head request is admitted onto an idle slot
+// when the sum stays within the budget; if not, idle slots' workspaces are
+// released first; if it still does not fit, it waits for a running request
+// to finish, or fails with MIRORR_ERR_NOMEM when none is running.
+#include <chrono>
+#include <condition_variable>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "context.hpp"
+
+namespace {
+
+using mirorr::Error;
+
+thread_local std::string g_server_error;
+
+struct Request {
+  int64_t ticket = 0;
+  mirorr_reg_params p{};
+  mirorr_geom gF{}, gM{};
+  std::vector<float> F, M;  // copies (empty when borrowed)
+  std::vector<uint8_t> Fm, Mm;
+  const float* pF = nullptr;  // what the worker uploads: the copies or the caller's arrays
+  const float* pM = nullptr;
+  const uint8_t* pFm = nullptr;
+  const uint8_t* pMm = nullptr;
+  bool has_init = false, has_center = false;
+  double init[12] = {0}, center[3] = {0};
+  size_t bound = 0;
+  double t_submit = 0.0;
+};
+
+struct Slot {
+  mirorr_ctx* ctx = nullptr;
+  std::thread th;
+  std::unique_ptr<Request> job;  // handed over by the scheduler
+  bool busy = false;
+  size_t reserve = 0;  // bytes the slot may hold while its request runs
+};
+
+size_t checked_nvox(const mirorr_geom& g) {
+  size_t n = 1;
+  for (int a = 0; a < 3; ++a) {
+    if (g.dims[a] < 1 || !(g.spacing[a] > 0.0)) throw Error(mirorr::kErrArg, "bad geometry");
+    n *= (size_t)g.dims[a];
+  }
+  return n;
+}
+
+}  // namespace
+
+struct mirorr_server {
+  int device = 0;
+  mirorr::Budget budget;  // every slot's context charges it
+  std::mutex mu;
+  std::condition_variable cv_work;  // slots wait for a request
+  std::condition_variable cv_done;  // pollers wait for results
+  std::deque<std::unique_ptr<Request>> pending;
+  std::map<int64_t, mirorr_request_result> done;
+  std::map<int64_t, bool> live;  // tickets submitted and not yet handed out
+  std::vector<Slot> slots;
+  int64_t next_ticket = 1;
+  int64_t completed = 0;
+  int max_running = 0;
+  bool stopping = false;
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+
+  double now() const {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  size_t held(const Slot& s) const { return s.busy ? s.reserve : s.ctx->budget.used.load(); }
+
+  void finish_unadmitted(std::unique_ptr<Request> r, mirorr_status st, const std::string& msg) {
+    mirorr_request_result res{};
+    res.status = st;
+    res.t_submit = r->t_submit;
+    res.t_start = res.t_end = now();
+    res.slot = -1;
+    res.bytes_bound = r->bound;
+    std::snprintf(res.error, sizeof(res.error), "%s", msg.c_str());
+    done[r->ticket] = res;
+    ++completed;
+    cv_done.notify_all();
+  }
+
+  // Admits pending requests in arrival order (caller holds mu).
+  void schedule() {
+    while (!pending.empty() && !stopping) {
+      int w = -1;
+      for (size_t i = 0; i < slots.size(); ++i)
+        if (!slots[i].busy && !slots[i].job) {
+          // prefer the idle slot keeping the largest workspace
+          if (w < 0 || slots[i].ctx->budget.used.load() > slots[w].ctx->budget.used.load())
+            w = (int)i;
+        }
+      if (w < 0) return;
+      Request& r = *pending.front();
+      const size_t limit = budget.limit;
+      auto fits = [&]() {
+        if (limit == 0) return true;
+        size_t sum = 0;
+        for (size_t i = 0; i < slots.size(); ++i) sum += held(slots[i]);
+        return sum + r.bound <= limit;  // slot w: its kept workspace plus the bound
+      };
+      if (!fits()) {
+        // Release the workspaces of idle slots, slot w's included (on this
+        // server's device; the calling thread's device is restored).
+        int prev = device;
+        (void)hipGetDevice(&prev);
+        (void)hipSetDevice(device);
+        for (Slot& s : slots)
+          if (!s.busy && !s.job && s.ctx->reg) {
+            std::lock_guard<std::mutex> lk(s.ctx->mu);
+            s.ctx->reg.reset();
+          }
+        (void)hipSetDevice(prev);
+      }
+      if (!fits()) {
+        bool running = false;
+        for (const Slot& s : slots) running = running || s.busy;
+        if (running) return;  // wait for a request to finish
+        const std::string msg = "request needs up to " + std::to_string(r.bound) +
+                                " device bytes, beyond the server's HBM budget of " +
+                                std::to_string(limit);
+        std::unique_ptr<Request> rr = std::move(pending.front());
+        pending.pop_front();
+        finish_unadmitted(std::move(rr), MIRORR_ERR_NOMEM, msg);
+        continue;
+      }
+      Slot& s = slots[w];
+      s.reserve = s.ctx->budget.used.load() + r.bound;
+      s.busy = true;
+      s.job = std::move(pending.front());
+      pending.pop_front();
+      int running = 0;
+      for (const Slot& x : slots) running += x.busy ? 1 : 0;
+      if (running > max_running) max_running = running;
+      cv_work.notify_all();
+    }
+  }
+
+  void run(Slot& slot, int w, Request& r, mirorr_request_result& res) {
+    res.t_submit = r.t_submit;
+    res.t_start = now();
+    res.slot = w;
+    res.bytes_bound = r.bound;
+    mirorr_ctx* c = slot.ctx;
+    mirorr_volume* vf = nullptr;
+    mirorr_volume* vm = nullptr;
+    mirorr_status st = mirorr_volume_create(c, r.pF, r.pFm, &r.gF, &vf);
+    if (st == MIRORR_OK) st = mirorr_volume_create(c, r.pM, r.pMm, &r.gM, &vm);
+    // The host copies are not needed once uploaded.
+    std::vector<float>().swap(r.F);
+    std::vector<float>().swap(r.M);
+    std::vector<uint8_t>().swap(r.Fm);
+    std::vector<uint8_t>().swap(r.Mm);
+    if (st == MIRORR_OK)
+      st = mirorr_register(c, &r.p, vf, vm, r.has_init ? r.init : nullptr,
+                           r.has_center ? r.center : nullptr, res.params, res.center, &res.stats);
+    if (st != MIRORR_OK) std::snprintf(res.error, sizeof(res.error), "%s", mirorr_last_error(c));
+    mirorr_volume_destroy(vf);
+    mirorr_volume_destroy(vm);
+    res.status = st;
+    res.t_end = now();
+  }
+
+  void worker(int w) {
+    (void)hipSetDevice(device);
+    for (;;) {
+      std::unique_ptr<Request> job;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv_work.wait(lk, [&] { return stopping || slots[w].job != nullptr; });
+        if (!slots[w].job) return;  // stopping with nothing to run
+        job = std::move(slots[w].job);
+      }
+      mirorr_request_result res{};
+      run(slots[w], w, *job, res);
+      std::lock_guard<std::mutex> lk(mu);
+      slots[w].busy = false;
+      slots[w].reserve = 0;
+      done[job->ticket] = res;
+      ++completed;
+      schedule();
+      cv_done.notify_all();
+    }
+  }
+};
+
+extern "C" {
+
+const char* mirorr_server_last_error(void) { return g_server_error.c_str(); }
+
+mirorr_status mirorr_server_create(int device, size_t hbm_budget, int nslots,
+                                   mirorr_server** out) {
+  if (!out) return MIRORR_ERR_ARG;
+  *out = nullptr;
+  auto S = std::make_unique<mirorr_server>();
+  S->device = device;
+  const int n = nslots > 0 ? nslots : 4;
+  S->slots.resize(n);
+  for (int i = 0; i < n; ++i) {
+    mirorr_status st = mirorr_ctx_create(device, 0, &S->slots[i].ctx);
+    if (st != MIRORR_OK) {
+      g_server_error = "worker context: HIP device unavailable";
+      for (Slot& s : S->slots) mirorr_ctx_destroy(s.ctx);
+      return st;
+    }
+    S->slots[i].ctx->budget.parent = &S->budget;
+  }
+  if (hbm_budget == 0) {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+      g_server_error = "hipMemGetInfo failed";
+      for (Slot& s : S->slots) mirorr_ctx_destroy(s.ctx);
+      return MIRORR_ERR_PLATFORM;
+    }
+    const size_t reserve = std::min<size_t>(size_t(8) << 30, free_b / 10);
+    hbm_budget = free_b - reserve;
+  }
+  S->budget.limit = hbm_budget;
+  mirorr_server* raw = S.release();
+  for (int i = 0; i < n; ++i) raw->slots[i].th = std::thread([raw, i] { raw->worker(i); });
+  *out = raw;
+  return MIRORR_OK;
+}
+
+mirorr_status mirorr_submit(mirorr_server* S, const mirorr_request* q, int64_t* ticket) {
+  if (!S || !q || !ticket) {
+    g_server_error = "null argument";
+    return MIRORR_ERR_ARG;
+  }
+  try {
+    if (!q->fixed || !q->moving) throw Error(mirorr::kErrArg, "null volume data");
+    auto r = std::make_unique<Request>();
+    r->p = q->params;
+    r->gF = q->fixed_geom;
+    r->gM = q->moving_geom;
+    const size_t nF = checked_nvox(r->gF), nM = checked_nvox(r->gM);
+    r->bound = mirorr::registration_bytes_bound(&r->p, mirorr::to_geom(r->gF),
+                                                q->fixed_mask != nullptr, mirorr::to_geom(r->gM),
+                                                q->moving_mask != nullptr);
+    if (q->borrow) {
+      r->pF = q->fixed;
+      r->pM = q->moving;
+      r->pFm = q->fixed_mask;
+      r->pMm = q->moving_mask;
+    } else {
+      r->F.assign(q->fixed, q->fixed + nF);
+      r->M.assign(q->moving, q->moving + nM);
+      if (q->fixed_mask) r->Fm.assign(q->fixed_mask, q->fixed_mask + nF);
+      if (q->moving_mask) r->Mm.assign(q->moving_mask, q->moving_mask + nM);
+      r->pF = r->F.data();
+      r->pM = r->M.data();
+      r->pFm = q->fixed_mask ? r->Fm.data() : nullptr;
+      r->pMm = q->moving_mask ? r->Mm.data() : nullptr;
+    }
+    if (q->init_params) {
+      r->has_init = true;
+      std::memcpy(r->init, q->init_params, sizeof(double) * (q->params.affine ? 12 : 6));
+    }
+    if (q->init_center) {
+      r->has_center = true;
+      std::memcpy(r->center, q->init_center, sizeof(r->center));
+    }
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (S->stopping) throw Error(mirorr::kErrState, "server is shutting down");
+    r->ticket = S->next_ticket++;
+    r->t_submit = S->now();
+    *ticket = r->ticket;
+    S->live[r->ticket] = true;
+    S->pending.push_back(std::move(r));
+    S->schedule();
+    return MIRORR_OK;
+  } catch (const Error& e) {
+    g_server_error = e.what();
+    return static_cast<mirorr_status>(e.code);
+  } catch (const std::bad_alloc&) {
+    g_server_error = "host allocation failed";
+    return MIRORR_ERR_NOMEM;
+  }
+}
+
+mirorr_status mirorr_poll(mirorr_server* S, int64_t ticket, double timeout_s,
+                          mirorr_request_result* out) {
+  if (!S || !out) {
+    g_server_error = "null argument";
+    return MIRORR_ERR_ARG;
+  }
+  std::unique_lock<std::mutex> lk(S->mu);
+  if (!S->live.count(ticket)) {
+    g_server_error = "unknown ticket " + std::to_string(ticket);
+    return MIRORR_ERR_ARG;
+  }
+  auto ready = [&] { return S->done.count(ticket) != 0; };
+  if (!ready()) {
+    if (timeout_s == 0.0) return MIRORR_PENDING;
+    if (timeout_s < 0.0) {
+      S->cv_done.wait(lk, ready);
+    } else if (!S->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), ready)) {
+      return MIRORR_PENDING;
+    }
+  }
+  *out = S->done[ticket];
+  S->done.erase(ticket);
+  S->live.erase(ticket);
+  return MIRORR_OK;
+}
+
+mirorr_status mirorr_server_info_get(mirorr_server* S, mirorr_server_info* out) {
+  if (!S || !out) return MIRORR_ERR_ARG;
+  std::lock_guard<std::mutex> lk(S->mu);
+  std::memset(out, 0, sizeof(*out));
+  out->slots = (int)S->slots.size();
+  out->pending = (int)S->pending.size();
+  for (const Slot& s : S->slots) out->running += s.busy ? 1 : 0;
+  out->completed = S->completed;
+  out->hbm_budget = S->budget.limit;
+  out->hbm_used = S->budget.used.load();
+  out->hbm_peak = S->budget.peak.load();
+  out->max_running = S->max_running;
+  return MIRORR_OK;
+}
+
+void mirorr_server_destroy(mirorr_server* S) {
+  if (!S) return;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->stopping = true;
+    S->pending.clear();
+    S->cv_work.notify_all();
+  }
+  // Workers finish the request they were handed, then exit.
+  for (Slot& s : S->slots)
+    if (s.th.joinable()) s.th.join();
+  for (Slot& s : S->slots) mirorr_ctx_destroy(s.ctx);
+  delete S;
+}
+
+}  // extern "C"

@@ -1,0 +1,137 @@
+"""The registration server (mirorr_server_* / mirorr_submit / mirorr_poll):
+each result is the same alone and in the mix, admission keeps the device
+memory within the HBM budget, the admission bound covers what a registration
+allocates, and failures come back as results."""
+import numpy as np
+import pytest
+
+from mirorr_amd import capi, phantom
+from mirorr_amd.registration import MirorrPyramid
+from mirorr_amd.server import RegistrationServer
+
+pytestmark = pytest.mark.gpu
+
+# dims, spacing, model, nhood width, nhood gap, levels
+CASES = [
+    ((64, 64, 32), (3.0, 3.0, 3.0), "rigid", 7, 3, 2),
+    ((56, 48, 40), (2.5, 3.0, 2.0), "affine", 5, 2, 2),
+    ((72, 64, 36), (2.0, 2.0, 3.0), "rigid", 9, 3, 2),
+    ((48, 56, 44), (3.0, 2.5, 2.5), "affine", 7, 4, 2),
+    ((64, 48, 48), (2.0, 3.0, 2.0), "affine", 9, 3, 3),
+    ((40, 40, 40), (3.0, 3.0, 3.0), "rigid", 5, 2, 2),
+]
+
+
+def _pair(i):
+    dims, sp, model, _, _, _ = CASES[i]
+    return phantom.make_pair(dims=dims, spacing=sp, n_ellipsoids=16, seed=100 + i, model=model)
+
+
+def _reg(i):
+    _, _, model, nw, ng, lev = CASES[i]
+    r = MirorrPyramid()
+    r.SetTransformType(model)
+    r.SetNhoodWidth(nw)
+    r.SetNhoodGap(ng)
+    r.GetPyramidScheduleTuner().SetMaxLevelNumber(lev)
+    return r
+
+
+def _alone(i, pair):
+    r = _reg(i)
+    r.SetFixedImage(pair.fixed)
+    r.SetMovingImage(pair.moving)
+    return r.RunMirorr()
+
+
+def test_results_alone_and_in_the_mix_are_identical():
+    pairs = [_pair(i) for i in range(len(CASES))]
+    alone = [_alone(i, p) for i, p in enumerate(pairs)]
+    with RegistrationServer(slots=4) as srv:
+        print("server: submitting", len(pairs), "requests", flush=True)
+        tickets = [srv.submit(p.fixed, p.moving, _reg(i).params, borrow=i % 2 == 1)
+                   for i, p in enumerate(pairs)]
+        res = [srv.poll(t) for t in tickets]
+        info = srv.info()
+    for i, (r, (p, c)) in enumerate(zip(res, alone)):
+        assert r["status"] == capi.OK, r["error"]
+        print(f"request {i}: slot {r['slot']} latency {1e3 * r['latency']:.1f} ms")
+        # The pipeline is deterministic (integer atomics only, fixed-order
+        # reductions), so a request's result does not depend on what runs
+        # beside it.
+        np.testing.assert_array_equal(r["params"], p)
+        np.testing.assert_array_equal(r["center"], c)
+    # The requests ran at the same time: admitted together, and at least two
+    # of them overlapped in time.
+    assert info["max_running"] >= 2 and info["completed"] == len(pairs)
+    spans = sorted((r["t_start"], r["t_end"]) for r in res)
+    assert any(spans[k + 1][0] < spans[k][1] for k in range(len(spans) - 1))
+    assert len({r["slot"] for r in res}) >= 2
+
+
+def test_budget_admits_one_request_at_a_time():
+    pairs = [_pair(0), _pair(0), _pair(0)]
+    with RegistrationServer(slots=1) as srv:
+        r0 = srv.poll(srv.submit(pairs[0].fixed, pairs[0].moving, _reg(0).params))
+    assert r0["status"] == capi.OK, r0["error"]
+    bound = r0["bytes_bound"]
+    budget = int(1.5 * bound)  # room for one request, not two
+    with RegistrationServer(slots=4, hbm_budget=budget) as srv:
+        ts = [srv.submit(p.fixed, p.moving, _reg(0).params) for p in pairs]
+        res = [srv.poll(t) for t in ts]
+        info = srv.info()
+    print("budget", budget, "peak", info["hbm_peak"], "max running", info["max_running"])
+    assert info["max_running"] == 1
+    assert info["hbm_peak"] <= budget
+    spans = sorted((r["t_start"], r["t_end"]) for r in res)
+    assert all(spans[k + 1][0] >= spans[k][1] for k in range(len(spans) - 1))
+    for r in res:
+        assert r["status"] == capi.OK, r["error"]
+        np.testing.assert_array_equal(r["params"], r0["params"])
+
+
+@pytest.mark.parametrize("variant", ["symmetric", "classic", "masked", "nw13", "fixed_grid"])
+def test_memory_bound_covers_the_registration(variant):
+    """The admission bound holds what one registration charges on a fresh
+    worker, and is not loose by more than 2x."""
+    i = 4 if variant != "nw13" else 2
+    pair = _pair(i)
+    reg = _reg(i)
+    p = reg.params
+    fm = mm = None
+    if variant == "classic":
+        p.symmetric = 0
+    if variant == "masked":
+        fm = (pair.fixed.data > 100).astype(np.uint8) * 255
+        mm = np.full(pair.moving.data.shape, 255, np.uint8)
+    if variant == "nw13":
+        p.nhood_width = 13
+    if variant == "fixed_grid":
+        p.resampling = capi.RESAMPLE_FIXED
+    with RegistrationServer(slots=1) as srv:
+        r = srv.poll(srv.submit(pair.fixed, pair.moving, p, fixed_mask=fm, moving_mask=mm))
+        info = srv.info()
+    assert r["status"] == capi.OK, r["error"]
+    print(f"{variant}: bound {r['bytes_bound']} peak {info['hbm_peak']}")
+    assert info["hbm_peak"] <= r["bytes_bound"] <= 2 * info["hbm_peak"]
+
+
+def test_failures_come_back_as_results():
+    pair = _pair(5)
+    with RegistrationServer(slots=2, hbm_budget=1 << 20) as srv:
+        r = srv.poll(srv.submit(pair.fixed, pair.moving, _reg(5).params))
+        assert r["status"] == capi.ERR_NOMEM and r["slot"] == -1
+        assert "budget" in r["error"]
+    with RegistrationServer(slots=2) as srv:
+        bad = _reg(5).params
+        bad.resampling = 9
+        t_bad = srv.submit(pair.fixed, pair.moving, bad)
+        t_ok = srv.submit(pair.fixed, pair.moving, _reg(5).params)
+        r_bad, r_ok = srv.poll(t_bad), srv.poll(t_ok)
+        assert r_bad["status"] == capi.ERR_ARG and "resampling" in r_bad["error"]
+        assert r_ok["status"] == capi.OK, r_ok["error"]
+        # a ticket is handed out once
+        with pytest.raises(capi.MirorrError):
+            srv.poll(t_ok)
+        with pytest.raises(capi.MirorrError):
+            srv.poll(12345)
