@@ -51,7 +51,7 @@ SERVED = {
     # workload's parameters, odd-numbered affine with the 256x256x128 one's.
     "batch64_uneven": dict(
         workload="batch64_uneven", pairs=64, axis_lengths=[96, 320, 16],
-        spacing_mm=[0.8, 2.0], slots=4, seed=7,
+        spacing_mm=[0.8, 2.0], slots=8, seed=7,
         rigid=dict(ellipsoids=20, nhood_width=7, nhood_gap=4, pyramid_levels=2,
                    portion_kept=0.5, variance_select=1),
         affine=dict(ellipsoids=60, nhood_width=9, nhood_gap=3, pyramid_levels=3,
@@ -345,6 +345,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="pair256_affine_symmetric",
                     choices=sorted(WORKLOADS) + sorted(SERVED))
+    ap.add_argument("--slots", type=int, default=0,
+                    help="served workloads: worker slots (0: the workload's own)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported "
@@ -353,6 +355,8 @@ def main():
         sys.exit(2)
     if args.workload in SERVED:
         cfg = dict(SERVED[args.workload])
+        if args.slots > 0:
+            cfg["slots"] = args.slots
         run = run_batch if args.workload.startswith("batch") else run_serve
         print(json.dumps(run(args, cfg)), flush=True)
         return
