@@ -56,6 +56,13 @@ SERVED = {
                    portion_kept=0.5, variance_select=1),
         affine=dict(ellipsoids=60, nhood_width=9, nhood_gap=3, pyramid_levels=3,
                     iterations_per_level=5)),
+    # 16 pairs of the 256x256x128 affine workload (distinct seeds), all
+    # submitted at once.
+    "batch16_pair256_affine": dict(
+        workload="batch16_pair256_affine", pairs=16, dims=[256, 256, 128], spacing_mm=1.0,
+        slots=8, seed=2,
+        affine=dict(ellipsoids=60, nhood_width=9, nhood_gap=3, pyramid_levels=3,
+                    iterations_per_level=5)),
     # Poisson arrivals of CT/MR-like rigid requests, 30% at 512x512x256 and
     # the rest at 256x256x128, drawn from a pool of distinct pairs.
     "serve_poisson_ctmr": dict(
@@ -203,12 +210,16 @@ def run_batch(args, cfg):
     from mirorr_amd.blockmatcher import default_context
     from mirorr_amd.server import RegistrationServer
     rng = np.random.default_rng(cfg["seed"])
-    lo, hi, st = cfg["axis_lengths"]
     jobs, params = [], []
     for i in range(cfg["pairs"]):
-        model = "rigid" if i % 2 == 0 else "affine"
-        dims = tuple(int(v) for v in rng.choice(np.arange(lo, hi + 1, st), 3))
-        sp = tuple(float(v) for v in rng.uniform(cfg["spacing_mm"][0], cfg["spacing_mm"][1], 3))
+        if "dims" in cfg:  # pairs of one size and model
+            model, dims, sp = "affine", tuple(cfg["dims"]), (cfg["spacing_mm"],) * 3
+        else:
+            lo, hi, st = cfg["axis_lengths"]
+            model = "rigid" if i % 2 == 0 else "affine"
+            dims = tuple(int(v) for v in rng.choice(np.arange(lo, hi + 1, st), 3))
+            sp = tuple(float(v) for v in rng.uniform(cfg["spacing_mm"][0], cfg["spacing_mm"][1],
+                                                     3))
         jobs.append((i, dims, sp, model, cfg[model]["ellipsoids"], cfg["seed"] * 1000 + i))
         params.append(_params(model, cfg[model]))
     print(f"bench: generating {len(jobs)} pairs", file=sys.stderr, flush=True)
