@@ -256,7 +256,7 @@ mirorr_status mirorr_register(mirorr_ctx* ctx, const mirorr_reg_params* p,
  * alone completes with MIRORR_ERR_NOMEM. Every worker's context charges the
  * server's budget, so the limit also holds if a bound were exceeded.
  * hbm_budget = 0: the device's free memory at creation minus a reserve of
- * min(8 GiB, 10%). slots <= 0: 4. */
+ * min(8 GiB, 10%). slots <= 0: 4; at most 64. */
 typedef struct mirorr_server mirorr_server;
 
 /* One registration: mirorr_volume_create of both images and mirorr_register

@@ -37,6 +37,8 @@ using mirorr::Error;
 
 thread_local std::string g_server_error;
 
+constexpr int kMaxSlots = 64;
+
 struct Request {
   int64_t ticket = 0;
   mirorr_reg_params p{};
@@ -216,8 +218,15 @@ const char* mirorr_server_last_error(void) { return g_server_error.c_str(); }
 
 mirorr_status mirorr_server_create(int device, size_t hbm_budget, int nslots,
                                    mirorr_server** out) {
-  if (!out) return MIRORR_ERR_ARG;
+  if (!out) {
+    g_server_error = "null argument";
+    return MIRORR_ERR_ARG;
+  }
   *out = nullptr;
+  if (nslots > kMaxSlots) {
+    g_server_error = "at most " + std::to_string(kMaxSlots) + " slots";
+    return MIRORR_ERR_ARG;
+  }
   auto S = std::make_unique<mirorr_server>();
   S->device = device;
   const int n = nslots > 0 ? nslots : 4;
@@ -243,7 +252,13 @@ mirorr_status mirorr_server_create(int device, size_t hbm_budget, int nslots,
   }
   S->budget.limit = hbm_budget;
   mirorr_server* raw = S.release();
-  for (int i = 0; i < n; ++i) raw->slots[i].th = std::thread([raw, i] { raw->worker(i); });
+  try {
+    for (int i = 0; i < n; ++i) raw->slots[i].th = std::thread([raw, i] { raw->worker(i); });
+  } catch (const std::exception& e) {
+    g_server_error = std::string("worker thread: ") + e.what();
+    mirorr_server_destroy(raw);  // joins the threads already started
+    return MIRORR_ERR_PLATFORM;
+  }
   *out = raw;
   return MIRORR_OK;
 }
@@ -318,7 +333,7 @@ mirorr_status mirorr_poll(mirorr_server* S, int64_t ticket, double timeout_s,
   auto ready = [&] { return S->done.count(ticket) != 0; };
   if (!ready()) {
     if (timeout_s == 0.0) return MIRORR_PENDING;
-    if (timeout_s < 0.0) {
+    if (timeout_s < 0.0 || !(timeout_s < 1e6)) {  // negative, NaN or past any deadline: wait
       S->cv_done.wait(lk, ready);
     } else if (!S->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), ready)) {
       return MIRORR_PENDING;

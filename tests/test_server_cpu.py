@@ -23,6 +23,9 @@ def test_null_arguments_are_rejected(lib):
     assert lib.mirorr_poll(None, 1, 0.0, None) == capi.ERR_ARG
     assert lib.mirorr_server_info_get(None, None) == capi.ERR_ARG
     assert lib.mirorr_server_create(0, 0, 1, None) == capi.ERR_ARG
+    h = C.c_void_p()
+    assert lib.mirorr_server_create(0, 0, 65, C.byref(h)) == capi.ERR_ARG and not h
+    assert b"64" in lib.mirorr_server_last_error()
     lib.mirorr_server_destroy(None)
 
 
