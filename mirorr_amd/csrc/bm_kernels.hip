@@ -84,108 +84,61 @@ __global__ __launch_bounds__(kThreads) void search_stats_kernel(
 // values are all equal has Syy = 0 exactly, as the reference's two-pass sum
 // gives, so the per-plane minimum and maximum detect that case.
 constexpr int kStatsZ = 16;  // z positions per thread
-// A workgroup owns 64 x 4 columns. Each plane it needs (x0 .. x0 + 66,
-// y0 .. y0 + 6) is loaded once, coalesced, into LDS (two buffers, one
-// barrier per plane, the next plane prefetched into registers meanwhile);
-// each thread then reads its 4x4 patch from LDS. The per-thread arithmetic
-// is the same whichever way the values arrive.
-constexpr int kStatsTX = 64 + 3, kStatsTY = 4 + 3;
-constexpr int kStatsPitch = 68;  // LDS row pitch
-constexpr int kStatsTile = kStatsTX * kStatsTY;
-static_assert(kThreads == 256 && kStatsTile <= 2 * kThreads, "two staged values per thread");
 
 template <bool MASK>
 __global__ __launch_bounds__(kThreads) void search_stats4_kernel(
     const float* __restrict__ y, const uint8_t* __restrict__ sm, int nx, int ny, int nz,
     float2* __restrict__ out) {
-  __shared__ float tile[2][kStatsTY * kStatsPitch];
-  __shared__ uint8_t mtile[2][MASK ? kStatsTY * kStatsPitch : 1];
-  const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
-  const int x0 = blockIdx.x * 64, y0 = blockIdx.y * 4;
-  const int x = x0 + lx, yy = y0 + ly;
+  const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int yy = blockIdx.y * 4 + (threadIdx.x >> 6);
   const int z0 = blockIdx.z * kStatsZ;
+  if (x > nx - 4 || yy > ny - 4) return;
   const int zend = min(z0 + kStatsZ, nz - 3);  // exclusive end of output z
-  if (z0 >= zend) return;                      // workgroup-uniform
-  // Columns past the last window position stage values and take part in the
-  // barriers but compute nothing.
-  const bool valid = x <= nx - 4 && yy <= ny - 4;
+  if (z0 >= zend) return;
   const int64_t nxy = (int64_t)nx * ny;
-  // The (up to) two tile elements this thread stages: in-plane offset of the
-  // voxel (or -1 outside the image) and LDS offset (or -1 past the tile).
-  int goff[2], loff[2];
-#pragma unroll
-  for (int j = 0; j < 2; ++j) {
-    const int e = threadIdx.x + kThreads * j;
-    const int r = e / kStatsTX, c = e - r * kStatsTX;
-    const int gx = x0 + c, gy = y0 + r;
-    loff[j] = e < kStatsTile ? r * kStatsPitch + c : -1;
-    goff[j] = (e < kStatsTile && gx < nx && gy < ny) ? gy * nx + gx : -1;
-  }
-  float pv[2];
-  uint8_t pm[2];
-  auto fetch = [&](int z) {
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      pv[j] = goff[j] >= 0 ? y[(int64_t)z * nxy + goff[j]] : 0.0f;
-      if (MASK) pm[j] = goff[j] >= 0 ? sm[(int64_t)z * nxy + goff[j]] : (uint8_t)0;
-    }
-  };
-  const int zlast = zend + 2;  // last plane read
-  const int pbase = ly * kStatsPitch + lx;
-  double c0 = 0.0;
+  const float* ycol = y + lin3(nx, ny, x, yy, 0);  // the column's (x, y); + z * nxy
+  const uint8_t* mcol = MASK ? sm + lin3(nx, ny, x, yy, 0) : nullptr;
+  const double c0 = (double)ycol[(int64_t)z0 * nxy];
   double ps[4], p1[4], p2[4];
   int pn[4];
   float pmin[4], pmax[4];
-  int buf = 0;
   auto load = [&](int slot, int z) {
+    const float* yp = ycol + (int64_t)z * nxy;
+    float v[16];
+    uint32_t mw[4] = {0x01010101u, 0x01010101u, 0x01010101u, 0x01010101u};
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
-      if (loff[j] >= 0) {
-        tile[buf][loff[j]] = pv[j];
-        if (MASK) mtile[buf][loff[j]] = pm[j];
+    for (int dy = 0; dy < 4; ++dy) {
+#pragma unroll
+      for (int dx = 0; dx < 4; ++dx) v[4 * dy + dx] = yp[dy * nx + dx];
+      if (MASK) {
+        const uint8_t* mr = mcol + (int64_t)z * nxy + dy * nx;
+        mw[dy] = (uint32_t)mr[0] | ((uint32_t)mr[1] << 8) | ((uint32_t)mr[2] << 16) |
+                 ((uint32_t)mr[3] << 24);
       }
-    if (z < zlast) fetch(z + 1);  // workgroup-uniform
-    __syncthreads();
-    if (valid) {
-      const float* tp = tile[buf] + pbase;
-      if (z == z0) c0 = (double)tp[0];
-      float v[16];
-      uint32_t mw[4] = {0x01010101u, 0x01010101u, 0x01010101u, 0x01010101u};
-#pragma unroll
-      for (int dy = 0; dy < 4; ++dy) {
-#pragma unroll
-        for (int dx = 0; dx < 4; ++dx) v[4 * dy + dx] = tp[dy * kStatsPitch + dx];
-        if (MASK) {
-          const uint8_t* mr = mtile[buf] + pbase + dy * kStatsPitch;
-          mw[dy] = (uint32_t)mr[0] | ((uint32_t)mr[1] << 8) | ((uint32_t)mr[2] << 16) |
-                   ((uint32_t)mr[3] << 24);
-        }
-      }
-      double s = 0.0, s1 = 0.0, s2 = 0.0;
-      int cnt = 0;
-      float lo = __builtin_huge_valf(), hi = -__builtin_huge_valf();
-#pragma unroll
-      for (int k = 0; k < 16; ++k) {
-        lo = fminf(lo, v[k]);
-        hi = fmaxf(hi, v[k]);
-        s += (double)v[k];
-        // An off voxel adds exact zeros, so the sums are those over the on
-        // voxels, in the same order, without a branch.
-        const double d = (double)v[k] - c0;
-        const bool on = !MASK || ((mw[k >> 2] >> (8 * (k & 3))) & 0xffu) != 0u;
-        const double dw = on ? d : 0.0;
-        s1 += dw;
-        s2 = fma(dw, d, s2);
-        cnt += on ? 1 : 0;
-      }
-      ps[slot] = s;
-      p1[slot] = s1;
-      p2[slot] = s2;
-      pn[slot] = cnt;
-      pmin[slot] = lo;
-      pmax[slot] = hi;
     }
-    buf ^= 1;
+    double s = 0.0, s1 = 0.0, s2 = 0.0;
+    int cnt = 0;
+    float lo = __builtin_huge_valf(), hi = -__builtin_huge_valf();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      lo = fminf(lo, v[k]);
+      hi = fmaxf(hi, v[k]);
+      s += (double)v[k];
+      // An off voxel adds exact zeros, so the sums are those over the on
+      // voxels, in the same order, without a branch.
+      const double d = (double)v[k] - c0;
+      const bool on = !MASK || ((mw[k >> 2] >> (8 * (k & 3))) & 0xffu) != 0u;
+      const double dw = on ? d : 0.0;
+      s1 += dw;
+      s2 = fma(dw, d, s2);
+      cnt += on ? 1 : 0;
+    }
+    ps[slot] = s;
+    p1[slot] = s1;
+    p2[slot] = s2;
+    pn[slot] = cnt;
+    pmin[slot] = lo;
+    pmax[slot] = hi;
   };
   auto emit = [&](int z) {
     const double mu = (ps[0] + ps[1] + ps[2] + ps[3]) * (1.0 / 64.0);
@@ -201,14 +154,13 @@ __global__ __launch_bounds__(kThreads) void search_stats4_kernel(
   };
   // Slots 0..3 hold the planes z .. z + 3 of the current window: one plane
   // enters per step and the slots shift down.
-  fetch(z0);
   load(0, z0);
   load(1, z0 + 1);
   load(2, z0 + 2);
 #pragma unroll 1
   for (int z = z0; z < zend; ++z) {
     load(3, z + 3);
-    if (valid) emit(z);
+    emit(z);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       ps[k] = ps[k + 1];
