@@ -23,6 +23,15 @@ size_t registration_bytes_bound(const mirorr_reg_params* p, const Geom& fixed, b
 struct mirorr_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // Second stream and fork/join events of a registration: the two
+  // block-matching passes of a symmetric iteration run on one each.
+  // Created on first use, default priority.
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // Whether registrations fork the second pass onto stream2. A server's
+  // worker contexts do not: their requests already overlap, and more
+  // streams than hardware queues measured slower.
+  bool overlap_passes = true;
   mirorr::Budget budget;
   std::string last_error;
   std::mutex mu;

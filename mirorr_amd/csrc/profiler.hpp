@@ -2,6 +2,7 @@
 // counter per launch for the algorithmic work the kernel reports.
 #pragma once
 
+#include <algorithm>
 #include <map>
 #include <string>
 #include <vector>
@@ -50,16 +51,18 @@ class Profiler {
     sl.counter = used_++;  // zero: slots are cleared together after each collection
     MIRORR_HIP(hipEventRecord(sl.start, s));
     pending_.push_back(sl);
+    if (std::find(streams_.begin(), streams_.end(), s) == streams_.end()) streams_.push_back(s);
     return counters_.as<unsigned long long>() + (size_t)sl.counter * kWorkStripe;
   }
   void end(int bit, hipStream_t s) {
     if (!(mask_ & bit) || pending_.empty()) return;
     MIRORR_HIP(hipEventRecord(pending_.back().stop, s));
   }
-  // Waits for pending launches and folds them into the aggregates.
+  // Waits for pending launches (on every stream they were recorded on) and
+  // folds them into the aggregates.
   void collect(hipStream_t s) {
     if (pending_.empty()) return;
-    MIRORR_HIP(hipStreamSynchronize(s));
+    for (hipStream_t x : streams_) MIRORR_HIP(hipStreamSynchronize(x));
     std::vector<unsigned long long> c((size_t)used_ * kWorkStripe);
     MIRORR_HIP(hipMemcpy(c.data(), counters_.as<unsigned long long>(),
                          sizeof(unsigned long long) * c.size(), hipMemcpyDeviceToHost));
@@ -75,9 +78,12 @@ class Profiler {
       free_.push_back(sl.stop);
     }
     pending_.clear();
+    // Cleared before any stream launches with these counters again.
     MIRORR_HIP(hipMemsetAsync(counters_.as<void>(), 0,
                               sizeof(unsigned long long) * kWorkStripe * used_, s));
+    MIRORR_HIP(hipStreamSynchronize(s));
     used_ = 0;
+    streams_.clear();
   }
   const std::map<std::string, ProfAgg>& aggregates() const { return agg_; }
   void reset() { agg_.clear(); }
@@ -92,6 +98,7 @@ class Profiler {
   }
 
  private:
+  std::vector<hipStream_t> streams_;  // streams with pending events
   static constexpr int kSlots = 4096;
   struct Slot {
     std::string name;

@@ -171,6 +171,7 @@ struct Workspace {
   DeviceBuffer xtF, xtM, sxxF, sxxM, sumF, sumM, flF, flM;  // block preparation
   DeviceBuffer best, second, match;           // raw BM results (one pass)
   DeviceBuffer queue, qbits, qcount;
+  DeviceBuffer best2, second2, match2, queue2, qbits2, qcount2;  // the second pass's
   DeviceBuffer X, Y, kept;                    // LTS pairs
   DeviceBuffer r, hist1, hist2, cand, part;
   DeviceBuffer lts, outer, mapI, flagsD;  // flagsD: {active, stop} for one read-back
@@ -214,10 +215,21 @@ struct RegRun {
   Budget* bud;
   bool finest = false;  // the current level is the last (full-resolution) one
 
+  // The context's second stream (and fork/join events), created on first use.
+  hipStream_t second_stream() {
+    if (!ctx->stream2) {
+      MIRORR_HIP(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
+      MIRORR_HIP(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+      MIRORR_HIP(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
+    }
+    return ctx->stream2;
+  }
+
   // Block preparation of a base image and, with portion < 1, the variance
   // selection over its blocks. Returns whether blocks were dropped.
   bool prepare_base(const float* base, const uint8_t* bmask, const BMGrid& g, DeviceBuffer& xt,
-                    DeviceBuffer& sxx, DeviceBuffer& sum, DeviceBuffer& fl, double portion) {
+                    DeviceBuffer& sxx, DeviceBuffer& sum, DeviceBuffer& fl, double portion,
+                    hipStream_t s) {
     const BlockPrep bp{xt.as<float>(), sxx.as<float>(), sum.as<float>(), fl.as<uint32_t>()};
     launch_block_prep(base, bmask, g, bp, s);
     if (!(portion < 1.0)) return false;
@@ -232,15 +244,19 @@ struct RegRun {
   void match_pass(const float* base, const uint8_t* bmask, const float* search,
                   const uint8_t* smask, const Geom& g_base, DeviceBuffer& xt, DeviceBuffer& sxx,
                   DeviceBuffer& sum, DeviceBuffer& fl, DeviceBuffer& stats_search, int swap,
-                  bool selected, LtsState* st) {
+                  bool selected, LtsState* st, hipStream_t s, bool second_set) {
     const BMGrid g = make_grid(g_base.dims, p->block_width, p->block_gap, p->nhood_width,
                                p->nhood_gap);
     if (g.nblocks <= 0) return;
-    BMResult res{w.best.as<float>(), w.second.as<float>(), w.match.as<int32_t>()};
+    BMResult res = second_set
+                       ? BMResult{w.best2.as<float>(), w.second2.as<float>(), w.match2.as<int32_t>()}
+                       : BMResult{w.best.as<float>(), w.second.as<float>(), w.match.as<int32_t>()};
     if (p->block_metric == MIRORR_METRIC_NC && tiled_supported(g)) {
       BlockPrep bp{xt.as<float>(), sxx.as<float>(), sum.as<float>(), fl.as<uint32_t>()};
-      BMScratch scr{w.queue.as<int32_t>(), w.qcount.as<int32_t>(),
-                    w.qbits.as<unsigned long long>(), tile_count(g),
+      DeviceBuffer& qu = second_set ? w.queue2 : w.queue;
+      DeviceBuffer& qc = second_set ? w.qcount2 : w.qcount;
+      DeviceBuffer& qb = second_set ? w.qbits2 : w.qbits;
+      BMScratch scr{qu.as<int32_t>(), qc.as<int32_t>(), qb.as<unsigned long long>(), tile_count(g),
                     (ctx->prof.finest_only() && !finest) ? nullptr : &ctx->prof};
       launch_ncc_search(base, bmask, search, smask, g, bp, SearchStats{stats_search.as<float2>()},
                         scr, res, s);
@@ -278,12 +294,14 @@ struct RegRun {
     for (DeviceBuffer* b : {&w.xtF, &w.xtM}) b->ensure(sizeof(float) * nb * n3, bud);
     for (DeviceBuffer* b : {&w.sxxF, &w.sxxM, &w.sumF, &w.sumM}) b->ensure(sizeof(float) * nb, bud);
     for (DeviceBuffer* b : {&w.flF, &w.flM}) b->ensure(sizeof(uint32_t) * nb, bud);
-    w.best.ensure(sizeof(float) * nb, bud);
-    w.second.ensure(sizeof(float) * nb, bud);
-    w.match.ensure(sizeof(int32_t) * 3 * nb, bud);
-    w.queue.ensure(sizeof(int32_t) * std::max<int64_t>(tile_count(g), 1), bud);
-    w.qbits.ensure(sizeof(unsigned long long) * std::max<int64_t>(tile_count(g), 1), bud);
-    w.qcount.ensure(sizeof(int32_t), bud);
+    for (DeviceBuffer* b : {&w.best, &w.second, &w.best2, &w.second2})
+      b->ensure(sizeof(float) * nb, bud);
+    for (DeviceBuffer* b : {&w.match, &w.match2}) b->ensure(sizeof(int32_t) * 3 * nb, bud);
+    for (DeviceBuffer* b : {&w.queue, &w.queue2})
+      b->ensure(sizeof(int32_t) * std::max<int64_t>(tile_count(g), 1), bud);
+    for (DeviceBuffer* b : {&w.qbits, &w.qbits2})
+      b->ensure(sizeof(unsigned long long) * std::max<int64_t>(tile_count(g), 1), bud);
+    for (DeviceBuffer* b : {&w.qcount, &w.qcount2}) b->ensure(sizeof(int32_t), bud);
     const int64_t np = 2 * nb;
     w.X.ensure(sizeof(double) * 3 * np, bud);
     w.Y.ensure(sizeof(double) * 3 * np, bud);
@@ -357,8 +375,9 @@ size_t registration_bytes_bound(const mirorr_reg_params* p, const Geom& FG, bool
   const size_t np = 2 * nb;
   total += 2 * sizeof(float) * n + 2 * n + 2 * sizeof(float2) * n;  // Fh, Mh, masks, stats
   total += 2 * sizeof(float) * nb * n3 + 4 * sizeof(float) * nb + 2 * sizeof(uint32_t) * nb;
-  total += 2 * sizeof(float) * nb + 3 * sizeof(int32_t) * nb;  // best, second, match
-  total += tiles * (sizeof(int32_t) + sizeof(unsigned long long)) + sizeof(int32_t);
+  // best, second, match and the masked-tile queue, one set per pass
+  total += 2 * (2 * sizeof(float) * nb + 3 * sizeof(int32_t) * nb);
+  total += 2 * (tiles * (sizeof(int32_t) + sizeof(unsigned long long)) + sizeof(int32_t));
   total += 2 * 3 * sizeof(double) * np + np + sizeof(double) * np +
            sizeof(unsigned long long) * np;  // X, Y, kept, r, cand
   total += sizeof(double) * 32 * (size_t)lts_parts((int64_t)np);
@@ -511,7 +530,7 @@ void run_registration(mirorr_ctx* ctx, const mirorr_reg_params* p, const mirorr_
                       w.Mmh.as<uint8_t>(), s);
       if (prep)
         selM = run.prepare_base(w.Mh.as<float>(), w.Mmh.as<uint8_t>(), gh, w.xtM, w.sxxM, w.sumM,
-                                w.flM, portion);
+                                w.flM, portion, s);
       // (the half-grid stays fixed through the level; hipMemcpy is ordered)
       MIRORR_HIP(hipStreamSynchronize(s));
     }
@@ -524,27 +543,44 @@ void run_registration(mirorr_ctx* ctx, const mirorr_reg_params* p, const mirorr_
         launch_resample(Mp[level].data.as<float>(), gM.dims, &outer->mapM, mmask, MG.dims,
                         &outer->mmapM, mmode, o.gHM.dims, w.Mh.as<float>(), w.Mmh.as<uint8_t>(),
                         s);
-      // 2. block matching: moving half as base, then (symmetric) swapped
+      // 2. block matching: moving half as base, then (symmetric) swapped.
+      // Without variance selection (whose scratch the passes share) the two
+      // symmetric passes are independent until the LTS: the second runs on
+      // the context's second stream, forked after the resampling and joined
+      // before the LTS. While the search kernels are being timed the passes
+      // run one after the other, so that each kernel's time is its own.
       lts_reset_count(st, s);
+      const bool fork =
+          run.symmetric && !(portion < 1.0) && !ctx->prof.on() && ctx->overlap_passes;
+      hipStream_t s_b = s;
+      if (fork) {
+        s_b = run.second_stream();
+        MIRORR_HIP(hipEventRecord(ctx->ev_fork, s));
+        MIRORR_HIP(hipStreamWaitEvent(s_b, ctx->ev_fork, 0));
+      }
       if (tiled) {
         launch_search_stats(w.Fh.as<float>(), w.Fmh.as<uint8_t>(), gh,
                             SearchStats{w.statsF.as<float2>()}, s);
         if (run.symmetric)
           launch_search_stats(w.Mh.as<float>(), w.Mmh.as<uint8_t>(), gh,
-                              SearchStats{w.statsM.as<float2>()}, s);
+                              SearchStats{w.statsM.as<float2>()}, s_b);
       }
       if (run.symmetric && prep) {
         selM = run.prepare_base(w.Mh.as<float>(), w.Mmh.as<uint8_t>(), gh, w.xtM, w.sxxM, w.sumM,
-                                w.flM, portion);
+                                w.flM, portion, s);
         selF = run.prepare_base(w.Fh.as<float>(), w.Fmh.as<uint8_t>(), gh, w.xtF, w.sxxF, w.sumF,
-                                w.flF, portion);
+                                w.flF, portion, s_b);  // the second pass's base
       }
       run.match_pass(w.Mh.as<float>(), w.Mmh.as<uint8_t>(), w.Fh.as<float>(), w.Fmh.as<uint8_t>(),
-                     o.gHM, w.xtM, w.sxxM, w.sumM, w.flM, w.statsF, 0, selM, st);
+                     o.gHM, w.xtM, w.sxxM, w.sumM, w.flM, w.statsF, 0, selM, st, s, false);
       if (run.symmetric)
         run.match_pass(w.Fh.as<float>(), w.Fmh.as<uint8_t>(), w.Mh.as<float>(),
                        w.Mmh.as<uint8_t>(), o.gHF, w.xtF, w.sxxF, w.sumF, w.flF, w.statsM, 1,
-                       selF, st);
+                       selF, st, s_b, fork);
+      if (fork) {
+        MIRORR_HIP(hipEventRecord(ctx->ev_join, s_b));
+        MIRORR_HIP(hipStreamWaitEvent(s, ctx->ev_join, 0));
+      }
       // 3. LTS and transform update on the device
       run.run_lts(st, outer, npts, flags);  // leaves flags->stop read back
       if (flags->stop) {

@@ -239,6 +239,7 @@ mirorr_status mirorr_server_create(int device, size_t hbm_budget, int nslots,
       return st;
     }
     S->slots[i].ctx->budget.parent = &S->budget;
+    S->slots[i].ctx->overlap_passes = false;  // the slots overlap whole requests
   }
   if (hbm_budget == 0) {
     size_t free_b = 0, total_b = 0;
