@@ -180,3 +180,33 @@ def test_resampling_modes_match_oracle(oracle, symmetric, mode):
           f"{ref_st['points']} gpu {reg.stats['points']}, iterations {ref_st['iterations']} "
           f"{reg.stats['iterations']}")
     assert dmm < 0.01
+
+
+@pytest.mark.parametrize("affine", [0, 1])
+def test_overlapped_passes_match_serial_passes(affine):
+    """The symmetric iteration's two block-matching passes run on two streams
+    of the context, except while search kernels are timed, when they run one
+    after the other. Both orders give bit-identical registrations: the passes
+    write disjoint buffers and every reduction runs in a fixed order."""
+    from mirorr_amd.blockmatcher import Context
+    from mirorr_amd.registration import MirorrPyramid
+    pair = phantom.make_pair(dims=(96, 80, 64), spacing=(2.0, 2.0, 2.5), n_ellipsoids=20,
+                             seed=21 + affine, model="affine" if affine else "rigid")
+    ctx = Context()
+    out = {}
+    for name, timed in (("overlapped", False), ("serial", True)):
+        ctx.set_profiling(ctx.PROF_FAST if timed else False)
+        reg = MirorrPyramid(ctx)
+        reg.SetTransformType("affine" if affine else "rigid")
+        reg.SetNhoodWidth(7)
+        reg.GetPyramidScheduleTuner().SetMaxLevelNumber(3)
+        reg.SetFixedImage(pair.fixed)
+        reg.SetMovingImage(pair.moving)
+        print("registration,", name, flush=True)
+        out[name] = reg.RunMirorr() + (reg.stats["iterations"],)
+    ctx.set_profiling(False)
+    n_timed, _, _ = ctx.profile("ncc_tile_fast")
+    assert n_timed > 0  # the serial run did time its search launches
+    np.testing.assert_array_equal(out["overlapped"][0], out["serial"][0])
+    np.testing.assert_array_equal(out["overlapped"][1], out["serial"][1])
+    assert out["overlapped"][2] == out["serial"][2]
