@@ -247,13 +247,15 @@ mirorr_status mirorr_register(mirorr_ctx* ctx, const mirorr_reg_params* p,
 /* A registration server on one GPU. It has no counterpart in the reference,
  * whose command line runs one pair per process (mirorr.cxx:309-619); it is
  * the request loop a service puts in front of mirorr_register. Requests are
- * admitted in arrival order onto `slots` workers, each with its own context
- * and HIP stream, whenever a worker is free and the request's device memory
- * bound (volumes, pyramids and workspace) fits in what the running requests
- * and the workers' kept workspaces leave of `hbm_budget`; workspaces of idle
- * workers are released when a request would not fit otherwise. The work of
- * admitted requests overlaps on the device. A request that cannot fit even
- * alone completes with MIRORR_ERR_NOMEM. Every worker's context charges the
+ * admitted onto `slots` workers, each with its own context and HIP stream,
+ * whenever a worker is free and the request's device memory bound (volumes,
+ * pyramids and workspace) fits in what the running requests and the workers'
+ * kept workspaces leave of `hbm_budget`; workspaces of idle workers are
+ * released when a request would not fit otherwise. Admission follows arrival
+ * order, except that later requests that fit may overtake one waiting for
+ * memory, at most 8 times. The work of admitted requests overlaps on the
+ * device. A request that cannot fit even alone completes with
+ * MIRORR_ERR_NOMEM. Every worker's context charges the
  * server's budget, so the limit also holds if a bound were exceeded.
  * hbm_budget = 0: the device's free memory at creation minus a reserve of
  * min(8 GiB, 10%). slots <= 0: 4; at most 64. */

@@ -10,13 +10,15 @@
 // between requests (grown to the largest it has run), so repeated requests
 // of similar size allocate nothing.
 //
-// Admission (under the server lock, in arrival order): a slot running a
-// request may hold up to the bytes it held when admitted plus the request's
-// bound (each workspace buffer only grows to the new size), an idle slot
-// holds its kept workspace. The head request is admitted onto an idle slot
-// when the sum stays within the budget; if not, idle slots' workspaces are
-// released first; if it still does not fit, it waits for a running request
-// to finish, or fails with MIRORR_ERR_NOMEM when none is running.
+// Admission (under the server lock): a slot running a request may hold up to
+// the bytes it held when admitted plus the request's bound (each workspace
+// buffer only grows to the new size), an idle slot holds its kept workspace.
+// A request is admitted onto an idle slot when the sum stays within the
+// budget; if not, idle slots' workspaces are released first; if it still does
+// not fit, it waits for a running request to finish, or fails with
+// MIRORR_ERR_NOMEM when none is running and nothing waits before it. Requests
+// are taken in arrival order, but later requests that fit may overtake one
+// that waits for memory, at most kMaxOvertake times.
 #include <chrono>
 #include <condition_variable>
 #include <cstdio>
@@ -38,6 +40,7 @@ using mirorr::Error;
 thread_local std::string g_server_error;
 
 constexpr int kMaxSlots = 64;
+constexpr int kMaxOvertake = 8;  // admissions that may pass a request waiting for memory
 
 struct Request {
   int64_t ticket = 0;
@@ -53,6 +56,7 @@ struct Request {
   double init[12] = {0}, center[3] = {0};
   size_t bound = 0;
   double t_submit = 0.0;
+  int overtaken = 0;  // later requests admitted while this one waited for memory
 };
 
 struct Slot {
@@ -108,9 +112,41 @@ struct mirorr_server {
     cv_done.notify_all();
   }
 
-  // Admits pending requests in arrival order (caller holds mu).
+  // Admits pending requests (caller holds mu). The queue is scanned in
+  // arrival order; a request that does not fit in memory waits while later
+  // ones that fit are admitted, until it has been overtaken kMaxOvertake
+  // times: from then on nothing behind it is admitted before it.
   void schedule() {
-    while (!pending.empty() && !stopping) {
+    const size_t limit = budget.limit;
+    auto running = [&] {
+      for (const Slot& x : slots)
+        if (x.busy) return true;
+      return false;
+    };
+    auto fits = [&](const Request& r) {
+      if (limit == 0) return true;
+      size_t sum = 0;
+      for (const Slot& x : slots) sum += held(x);
+      return sum + r.bound <= limit;  // the chosen slot: its kept workspace plus the bound
+    };
+    bool released = false;
+    auto release_idle = [&] {
+      // Release the workspaces of idle slots (on this server's device; the
+      // calling thread's device is restored).
+      if (released) return;
+      released = true;
+      int prev = device;
+      (void)hipGetDevice(&prev);
+      (void)hipSetDevice(device);
+      for (Slot& x : slots)
+        if (!x.busy && !x.job && x.ctx->reg) {
+          std::lock_guard<std::mutex> lk(x.ctx->mu);
+          x.ctx->reg.reset();
+        }
+      (void)hipSetDevice(prev);
+    };
+    std::vector<Request*> waiting;  // requests passed over in this scan
+    for (auto it = pending.begin(); it != pending.end() && !stopping;) {
       int w = -1;
       for (size_t i = 0; i < slots.size(); ++i)
         if (!slots[i].busy && !slots[i].job) {
@@ -119,47 +155,32 @@ struct mirorr_server {
             w = (int)i;
         }
       if (w < 0) return;
-      Request& r = *pending.front();
-      const size_t limit = budget.limit;
-      auto fits = [&]() {
-        if (limit == 0) return true;
-        size_t sum = 0;
-        for (size_t i = 0; i < slots.size(); ++i) sum += held(slots[i]);
-        return sum + r.bound <= limit;  // slot w: its kept workspace plus the bound
-      };
-      if (!fits()) {
-        // Release the workspaces of idle slots, slot w's included (on this
-        // server's device; the calling thread's device is restored).
-        int prev = device;
-        (void)hipGetDevice(&prev);
-        (void)hipSetDevice(device);
-        for (Slot& s : slots)
-          if (!s.busy && !s.job && s.ctx->reg) {
-            std::lock_guard<std::mutex> lk(s.ctx->mu);
-            s.ctx->reg.reset();
-          }
-        (void)hipSetDevice(prev);
-      }
-      if (!fits()) {
-        bool running = false;
-        for (const Slot& s : slots) running = running || s.busy;
-        if (running) return;  // wait for a request to finish
-        const std::string msg = "request needs up to " + std::to_string(r.bound) +
-                                " device bytes, beyond the server's HBM budget of " +
-                                std::to_string(limit);
-        std::unique_ptr<Request> rr = std::move(pending.front());
-        pending.pop_front();
-        finish_unadmitted(std::move(rr), MIRORR_ERR_NOMEM, msg);
+      Request& r = **it;
+      if (!fits(r)) release_idle();
+      if (!fits(r)) {
+        if (!running() && waiting.empty()) {
+          const std::string msg = "request needs up to " + std::to_string(r.bound) +
+                                  " device bytes, beyond the server's HBM budget of " +
+                                  std::to_string(limit);
+          std::unique_ptr<Request> rr = std::move(*it);
+          it = pending.erase(it);
+          finish_unadmitted(std::move(rr), MIRORR_ERR_NOMEM, msg);
+          continue;
+        }
+        if (r.overtaken >= kMaxOvertake) return;  // keep its place: admit nothing behind it
+        waiting.push_back(&r);
+        ++it;
         continue;
       }
+      for (Request* q : waiting) ++q->overtaken;
       Slot& s = slots[w];
       s.reserve = s.ctx->budget.used.load() + r.bound;
       s.busy = true;
-      s.job = std::move(pending.front());
-      pending.pop_front();
-      int running = 0;
-      for (const Slot& x : slots) running += x.busy ? 1 : 0;
-      if (running > max_running) max_running = running;
+      s.job = std::move(*it);
+      it = pending.erase(it);
+      int n = 0;
+      for (const Slot& x : slots) n += x.busy ? 1 : 0;
+      if (n > max_running) max_running = n;
       cv_work.notify_all();
     }
   }

@@ -5,9 +5,11 @@ mirorr_poll in include/mirorr_gpu.h).
         t = srv.submit(fixed, moving, params)      # params: a MirorrPyramid's
         r = srv.poll(t)                            # waits; dict with params, center, ...
 
-Requests are admitted in arrival order onto worker slots whenever a slot is
-free and the request's device-memory bound fits in the server's HBM budget;
-admitted requests run at the same time, each on its own HIP stream. There is
+Requests are admitted onto worker slots whenever a slot is free and the
+request's device-memory bound fits in the server's HBM budget, in arrival
+order except that later requests that fit may overtake one waiting for
+memory (at most 8 times); admitted requests run at the same time, each on
+its own HIP stream. There is
 no counterpart in the reference, whose command line runs one pair per
 process: this is the loop a service puts in front of MirorrPyramid.RunMirorr.
 """

@@ -135,3 +135,31 @@ def test_failures_come_back_as_results():
             srv.poll(t_ok)
         with pytest.raises(capi.MirorrError):
             srv.poll(12345)
+
+
+def test_small_request_overtakes_one_waiting_for_memory():
+    """Budget for one large and one small request: the second large request
+    waits for memory while the small one, submitted after it, is admitted
+    beside the first; every result is still the request's own."""
+    big, small = _pair(2), _pair(5)
+    with RegistrationServer(slots=1) as srv:
+        rb = srv.poll(srv.submit(big.fixed, big.moving, _reg(2).params))
+        rs = srv.poll(srv.submit(small.fixed, small.moving, _reg(5).params))
+    assert rb["status"] == capi.OK and rs["status"] == capi.OK
+    b_big, b_small = rb["bytes_bound"], rs["bytes_bound"]
+    assert b_small < b_big
+    budget = b_big + b_small + (b_big - b_small) // 2  # one large and one small, not two large
+    with RegistrationServer(slots=4, hbm_budget=budget) as srv:
+        t1 = srv.submit(big.fixed, big.moving, _reg(2).params)
+        t2 = srv.submit(big.fixed, big.moving, _reg(2).params)
+        t3 = srv.submit(small.fixed, small.moving, _reg(5).params)
+        r1, r2, r3 = srv.poll(t1), srv.poll(t2), srv.poll(t3)
+        info = srv.info()
+    print(f"budget {budget}: starts {r1['t_start']:.4f} {r2['t_start']:.4f} {r3['t_start']:.4f}, "
+          f"peak {info['hbm_peak']}")
+    assert r3["t_start"] < r2["t_start"]  # the small request did not wait behind the large one
+    assert r2["t_start"] >= r1["t_end"]   # two large requests never ran together
+    assert info["hbm_peak"] <= budget
+    for r, ref in ((r1, rb), (r2, rb), (r3, rs)):
+        assert r["status"] == capi.OK, r["error"]
+        np.testing.assert_array_equal(r["params"], ref["params"])
