@@ -221,6 +221,9 @@ typedef struct mirorr_reg_stats {
   int64_t voxel_evals;     /* NCC voxel-evals, all levels and passes         */
   int level_dims[16][3];   /* half-way grid per level                        */
   double seconds;          /* wall time of mirorr_register                   */
+  int64_t rescored_blocks; /* NCC blocks re-searched in double because their
+                              float best or best-second gap lay next to the
+                              score filter's thresholds, all levels and passes */
 } mirorr_reg_stats;
 
 void mirorr_reg_default_params(mirorr_reg_params* p);

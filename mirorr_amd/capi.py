@@ -75,7 +75,8 @@ class RegParams(C.Structure):
 class RegStats(C.Structure):
     _fields_ = [("levels", C.c_int), ("iterations", C.c_int * 16), ("points", C.c_int64 * 16),
                 ("inner_iterations", C.c_int), ("voxel_evals", C.c_int64),
-                ("level_dims", (C.c_int * 3) * 16), ("seconds", C.c_double)]
+                ("level_dims", (C.c_int * 3) * 16), ("seconds", C.c_double),
+                ("rescored_blocks", C.c_int64)]
 
 
 class Request(C.Structure):

@@ -1562,11 +1562,189 @@ __global__ __launch_bounds__(kThreads) void cr_generic_kernel(
 }
 
 // ---------------------------------------------------------------------------
-__global__ void postprocess_kernel(BMGrid g, Geom geom, const float* __restrict__ best,
-                                   const float* __restrict__ second,
-                                   const int32_t* __restrict__ match, uint8_t* kept,
-                                   double* base_mm, double* match_mm) {
-  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Exact re-scoring of the blocks next to the score filter's thresholds.
+//
+// The tiled kernels accumulate in float, so their NCC^2 values differ from the
+// reference's double sums (itkBlockMatcher.txx:46-101) by about 1e-6. The
+// filter keeps a block when best > 1e-7 and |best - second| > 1e-4
+// (itkBlockMatcher.txx:886-916), so a block whose float best or gap lies within
+// kRescoreBand of a threshold could land on the other side of it. Such blocks,
+// and blocks whose best is an exact tie, are searched again here in double with the reference's operation order (the
+// means over all voxels summed in raster order, the masked centred sums in
+// raster order, then prod * prod / bss / sss), without fused multiply-adds,
+// and their record is rebuilt with the reference's float score buffer
+// (itkBlockMatcher.txx:764-774,823-833). For the sequence v_1..v_n of double
+// scores in scan order that buffer ends with
+//   best   = float(max_k v_k),
+//   match  = the last k with v_k > double(float(max(-3, v_1..v_{k-1}))),
+//   second = float(max over the other candidates), -3 with one candidate,
+// which is what one workgroup computes below: candidate scores in LDS, a
+// prefix maximum over per-thread runs, each thread replaying its own run.
+constexpr double kRescoreBand = 2e-5;
+constexpr int kRescoreMaxCand = 13 * 13 * 13;
+
+// A block is re-scored when its float best or best-second gap lies within
+// kRescoreBand of a threshold, or when its best is an exact tie (the float
+// buffer's record of a tie depends on how the double score rounds).
+__device__ __forceinline__ bool near_filter(float best, float second) {
+  const double b = best, s = second;
+  return fabs(b - 1e-7) < kRescoreBand || fabs(fabs(b - s) - 1e-4) < kRescoreBand ||
+         (best == second && b > 0.0);
+}
+
+__device__ __forceinline__ int cand_total(const BMGrid& g, int64_t b, int p[3], int first[3],
+                                          int cnt[3]) {
+  block_corner(g, b, p);
+  for (int k = 0; k < 3; ++k) cand_range(p[k], g.dims[k], g.bw, g.bg, g.nw, &first[k], &cnt[k]);
+  return cnt[0] * cnt[1] * cnt[2];
+}
+
+struct RescoreLds {
+  double score[kRescoreMaxCand];
+  double wave[kThreads / 64];
+  int iwave[kThreads / 64];
+  int list[kThreads];
+  int nlist;
+};
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Maximum of v over the workgroup (all threads get it).
+__device__ __forceinline__ double group_max(double v, RescoreLds& L) {
+  v = wave_max(v);
+  if ((threadIdx.x & 63) == 0) L.wave[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = L.wave[0];
+  for (int w = 1; w < kThreads / 64; ++w) r = fmax(r, L.wave[w]);
+  __syncthreads();
+  return r;
+}
+
+// Re-scores block b in double and rewrites its record. Called by every
+// thread of the workgroup with the same b; total <= kRescoreMaxCand.
+__device__ void rescore_block(const RescoreArgs& ra, const BMGrid& g, int64_t b, float* best,
+                              float* second, int32_t* match, RescoreLds& L) {
+#pragma clang fp contract(off)
+  const float* __restrict__ x = ra.x;
+  const float* __restrict__ y = ra.y;
+  const uint8_t* __restrict__ bm = ra.bm;
+  const uint8_t* __restrict__ sm = ra.sm;
+  const int nx = g.dims[0], ny = g.dims[1], bw = g.bw;
+  int p[3], first[3], cnt[3];
+  const int total = cand_total(g, b, p, first, cnt);
+  double count = 0.0, mb = 0.0;
+  for (int dz = 0; dz < bw; ++dz)
+    for (int dy = 0; dy < bw; ++dy)
+      for (int dx = 0; dx < bw; ++dx) {
+        mb += x[lin3(nx, ny, p[0] + dx, p[1] + dy, p[2] + dz)];
+        count += 1.0;
+      }
+  mb /= count;
+  for (int k = threadIdx.x; k < total; k += kThreads) {
+    const int s0 = first[0] + (k % cnt[0]) * g.bg;
+    const int s1 = first[1] + ((k / cnt[0]) % cnt[1]) * g.bg;
+    const int s2 = first[2] + (k / (cnt[0] * cnt[1])) * g.bg;
+    double ms = 0.0;
+    for (int dz = 0; dz < bw; ++dz)
+      for (int dy = 0; dy < bw; ++dy)
+        for (int dx = 0; dx < bw; ++dx) ms += y[lin3(nx, ny, s0 + dx, s1 + dy, s2 + dz)];
+    ms /= count;
+    double prod = 0.0, bss = 0.0, sss = 0.0;
+    for (int dz = 0; dz < bw; ++dz)
+      for (int dy = 0; dy < bw; ++dy)
+        for (int dx = 0; dx < bw; ++dx) {
+          const int64_t ob = lin3(nx, ny, p[0] + dx, p[1] + dy, p[2] + dz);
+          const int64_t os = lin3(nx, ny, s0 + dx, s1 + dy, s2 + dz);
+          if ((bm && bm[ob] == 0) || (sm && sm[os] == 0)) continue;
+          const double u = x[ob] - mb;
+          const double v = y[os] - ms;
+          prod += u * v;
+          bss += u * u;
+          sss += v * v;
+        }
+    L.score[k] = (bss <= 0.0 || sss <= 0.0) ? -2.0 : prod * prod / bss / sss;
+  }
+  __syncthreads();
+  // Each thread replays a contiguous run of candidates.
+  const int per = (total + kThreads - 1) / kThreads;
+  const int k0 = min((int)threadIdx.x * per, total), k1 = min(k0 + per, total);
+  double run_max = -3.0;
+  for (int k = k0; k < k1; ++k) run_max = fmax(run_max, L.score[k]);
+  // Exclusive prefix maximum of the runs (the buffer starts at -3).
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  double inc = run_max;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const double t = __shfl_up(inc, o, 64);
+    if (lane >= o) inc = fmax(inc, t);
+  }
+  double pre = __shfl_up(inc, 1, 64);
+  if (lane == 0) pre = -3.0;
+  if (lane == 63) L.wave[wv] = inc;
+  __syncthreads();
+  for (int w = 0; w < wv; ++w) pre = fmax(pre, L.wave[w]);
+  __syncthreads();
+  int last = -1;
+  for (int k = k0; k < k1; ++k) {
+    const double v = L.score[k];
+    if ((double)(float)pre < v) last = k;
+    pre = fmax(pre, v);
+  }
+  // The last replacement over the whole sequence.
+  int m = last;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
+  if (lane == 0) L.iwave[wv] = m;
+  __syncthreads();
+  int I = L.iwave[0];
+  for (int w = 1; w < kThreads / 64; ++w) I = max(I, L.iwave[w]);
+  double other = -3.0;
+  for (int k = k0; k < k1; ++k)
+    if (k != I) other = fmax(other, L.score[k]);
+  const double top = group_max(run_max, L);
+  const double sec = group_max(other, L);
+  if (threadIdx.x == 0) {
+    best[b] = (float)top;
+    second[b] = (float)sec;
+    match[3 * b + 0] = first[0] + (I % cnt[0]) * g.bg;
+    match[3 * b + 1] = first[1] + ((I / cnt[0]) % cnt[1]) * g.bg;
+    match[3 * b + 2] = first[2] + (I / (cnt[0] * cnt[1])) * g.bg;
+  }
+  __syncthreads();
+}
+
+// Re-scores the blocks of [b0, b0 + kThreads) next to a threshold. Uniform
+// over the workgroup; returns the number re-scored.
+__device__ int rescore_chunk(const RescoreArgs& ra, const BMGrid& g, int64_t b0, float* best,
+                             float* second, int32_t* match, RescoreLds& L) {
+  if (threadIdx.x == 0) L.nlist = 0;
+  __syncthreads();
+  const int64_t b = b0 + threadIdx.x;
+  if (b < g.nblocks && near_filter(best[b], second[b])) {
+    int p[3], first[3], cnt[3];
+    if (cand_total(g, b, p, first, cnt) <= kRescoreMaxCand) L.list[atomicAdd(&L.nlist, 1)] = b - b0;
+  }
+  __syncthreads();
+  const int n = L.nlist;
+  for (int i = 0; i < n; ++i) rescore_block(ra, g, b0 + L.list[i], best, second, match, L);
+  return n;
+}
+
+__global__ void postprocess_kernel(BMGrid g, Geom geom, float* __restrict__ best,
+                                   float* __restrict__ second, int32_t* __restrict__ match,
+                                   uint8_t* kept, double* base_mm, double* match_mm,
+                                   RescoreArgs ra) {
+  __shared__ RescoreLds L;
+  const int64_t b0 = (int64_t)blockIdx.x * blockDim.x;
+  if (ra.x) {
+    const int n = rescore_chunk(ra, g, b0, best, second, match, L);
+    if (threadIdx.x == 0 && n && ra.count) atomicAdd(ra.count, (unsigned long long)n);
+  }
+  const int64_t b = b0 + threadIdx.x;
   if (b >= g.nblocks) return;
   const double cb = best[b], c2 = second[b];
   kept[b] = (cb > 1e-7 && fabs(cb - c2) > 1e-4) ? 1 : 0;
@@ -1588,22 +1766,39 @@ __global__ void postprocess_kernel(BMGrid g, Geom geom, const float* __restrict_
   }
 }
 
+// Re-scoring alone (the raw records of a standalone search).
+__global__ __launch_bounds__(kThreads) void rescore_kernel(BMGrid g, float* best, float* second,
+                                                           int32_t* match, RescoreArgs ra) {
+  __shared__ RescoreLds L;
+  unsigned long long tot = 0;
+  for (int64_t b0 = (int64_t)blockIdx.x * kThreads; b0 < g.nblocks;
+       b0 += (int64_t)gridDim.x * kThreads)
+    tot += rescore_chunk(ra, g, b0, best, second, match, L);
+  if (threadIdx.x == 0 && tot && ra.count) atomicAdd(ra.count, tot);
+}
+
 // Pair output for the LTS: X gets the moving-side point and Y the fixed-side
 // point of every block; `swap` marks the second symmetric pass, whose base
-// image is the fixed one (itkSymMirorrRegistrationMethod.txx:595-602).
+// image is the fixed one (itkSymMirorrRegistrationMethod.txx:595-602). With
+// ra.x set, blocks next to a threshold are re-scored first.
 __global__ __launch_bounds__(kThreads) void postprocess_pairs_kernel(
-    BMGrid g, Geom geom, const float* __restrict__ best, const float* __restrict__ second,
-    const int32_t* __restrict__ match, int swap, uint8_t* kept, double* X, double* Y,
-    unsigned long long* count) {
+    BMGrid g, Geom geom, float* __restrict__ best, float* __restrict__ second,
+    int32_t* __restrict__ match, int swap, uint8_t* kept, double* X, double* Y,
+    unsigned long long* count, RescoreArgs ra) {
   // Grid-stride over the blocks with one count update per workgroup: the
   // counter is a single device-wide address, so updates are kept few.
   __shared__ unsigned int wsum[kThreads / 64];
+  __shared__ RescoreLds L;
   const double half = 0.5 * g.bw;
   double* pb = swap ? Y : X;
   double* pm = swap ? X : Y;
   unsigned int nk = 0;
-  for (int64_t b = (int64_t)blockIdx.x * kThreads + threadIdx.x; b < g.nblocks;
-       b += (int64_t)gridDim.x * kThreads) {
+  unsigned long long nres = 0;
+  for (int64_t b0 = (int64_t)blockIdx.x * kThreads; b0 < g.nblocks;
+       b0 += (int64_t)gridDim.x * kThreads) {
+    if (ra.x) nres += rescore_chunk(ra, g, b0, best, second, match, L);
+    const int64_t b = b0 + threadIdx.x;
+    if (b >= g.nblocks) continue;
     const double cb = (double)best[b];
     const double c2 = (double)second[b];
     const bool k = cb > 1e-7 && fabs(cb - c2) > 1e-4;
@@ -1634,6 +1829,7 @@ __global__ __launch_bounds__(kThreads) void postprocess_pairs_kernel(
     unsigned int tot = 0;
     for (int w = 0; w < kThreads / 64; ++w) tot += wsum[w];
     if (tot) atomicAdd(count, (unsigned long long)tot);
+    if (nres && ra.count) atomicAdd(ra.count, nres);
   }
 }
 
@@ -1989,23 +2185,31 @@ void launch_ncc_search(const float* x, const uint8_t* bmask, const float* y,
   }
 }
 
+void launch_rescore(const BMGrid& g, const BMResult& res, const RescoreArgs& ra, hipStream_t s) {
+  if (g.nblocks == 0 || !ra.x) return;
+  const int64_t grid = std::min<int64_t>((g.nblocks + kThreads - 1) / kThreads, 1024);
+  hipLaunchKernelGGL(rescore_kernel, dim3((unsigned)grid), dim3(kThreads), 0, s, g, res.best,
+                     res.second, res.match, ra);
+  MIRORR_CHECK_LAUNCH();
+}
+
 void launch_postprocess(const BMGrid& g, const Geom& geom, const BMResult& res,
                         uint8_t* kept, double* base_mm, double* match_mm,
-                        hipStream_t s) {
+                        const RescoreArgs& ra, hipStream_t s) {
   if (g.nblocks == 0) return;
   hipLaunchKernelGGL(postprocess_kernel, dim3((unsigned)((g.nblocks + kThreads - 1) / kThreads)),
                      dim3(kThreads), 0, s, g, geom, res.best, res.second, res.match, kept,
-                     base_mm, match_mm);
+                     base_mm, match_mm, ra);
   MIRORR_CHECK_LAUNCH();
 }
 
 void launch_postprocess_pairs(const BMGrid& g, const Geom& geom, const BMResult& res, int swap,
                               uint8_t* kept, double* X, double* Y, unsigned long long* count,
-                              hipStream_t s) {
+                              const RescoreArgs& ra, hipStream_t s) {
   if (g.nblocks == 0) return;
   const int64_t grid = std::min<int64_t>((g.nblocks + kThreads - 1) / kThreads, 512);
   hipLaunchKernelGGL(postprocess_pairs_kernel, dim3((unsigned)grid), dim3(kThreads), 0, s, g, geom,
-                     res.best, res.second, res.match, swap, kept, X, Y, count);
+                     res.best, res.second, res.match, swap, kept, X, Y, count, ra);
   MIRORR_CHECK_LAUNCH();
 }
 

@@ -88,17 +88,38 @@ void launch_metric_search(int metric, const float* x, const uint8_t* bmask, cons
 int64_t tile_count(const BMGrid& g);
 bool tiled_supported(const BMGrid& g);
 
+// Images of an NCC search for the exact re-scoring of the blocks whose float
+// best or best-second gap lies next to the score filter's thresholds
+// (1e-7 and 1e-4, itkBlockMatcher.txx:886-916): those blocks are searched
+// again in double with the reference's operation order and float score
+// buffer, so the filter decides them as the reference does. x = NULL
+// switches the re-scoring off (other metrics). count (optional) += blocks
+// re-scored.
+struct RescoreArgs {
+  const float* x = nullptr;    // base image
+  const uint8_t* bm = nullptr; // base mask (NULL: all on)
+  const float* y = nullptr;    // search image
+  const uint8_t* sm = nullptr; // search mask (NULL: all on)
+  unsigned long long* count = nullptr;
+};
+
+// Re-scoring alone: rewrites the raw records of the blocks next to a
+// threshold (res is updated in place).
+void launch_rescore(const BMGrid& g, const BMResult& res, const RescoreArgs& ra, hipStream_t s);
+
 // Score filter + mm conversion (itkBlockMatcher.txx:886-916,
 // itkAbstractBlockMatcher.h:271-287). kept[i] in {0,1}; pts are nblocks x 3.
+// Re-scores first when ra.x is set.
 void launch_postprocess(const BMGrid& g, const Geom& geom, const BMResult& res,
                         uint8_t* kept, double* base_mm, double* match_mm,
-                        hipStream_t s);
+                        const RescoreArgs& ra, hipStream_t s);
 
 // Score filter + mm conversion written as LTS pairs: the base point goes to
 // X (Y when swap), the matched point to Y (X when swap); count += kept.
+// Re-scores first when ra.x is set.
 void launch_postprocess_pairs(const BMGrid& g, const Geom& geom, const BMResult& res, int swap,
                               uint8_t* kept, double* X, double* Y, unsigned long long* count,
-                              hipStream_t s);
+                              const RescoreArgs& ra, hipStream_t s);
 
 // Ordered compaction of nblocks records by a 0/1 flag array.
 struct Compactor {

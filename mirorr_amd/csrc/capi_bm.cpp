@@ -147,6 +147,14 @@ static void run_search(mirorr_bm* bm) {
     launch_metric_search(metric, B.data, B.mask, S.data, S.mask, g, res, s,
                          bm->prep_selected[bi] ? bp.flags : nullptr);
   }
+  if (metric == MIRORR_METRIC_NC) {
+    RescoreArgs ra;
+    ra.x = B.data;
+    ra.bm = B.mask;
+    ra.y = S.data;
+    ra.sm = S.mask;
+    launch_rescore(g, res, ra, s);
+  }
   bm->results_valid = true;
   bm->results_key[0] = key0;
   bm->results_key[1] = key1;
@@ -461,7 +469,7 @@ mirorr_status mirorr_bm_get_offsets(mirorr_bm* bm, double* base_mm, double* matc
     BMResult res{bm->best.as<float>(), bm->second.as<float>(), bm->match.as<int32_t>()};
     const Geom geom = to_geom(bm->slot[MIRORR_ROLE_BASE].geom);
     launch_postprocess(g, geom, res, bm->kept.as<uint8_t>(), bm->base_mm.as<double>(),
-                       bm->match_mm.as<double>(), s);
+                       bm->match_mm.as<double>(), RescoreArgs{}, s);
     Compactor cp{bm->chunk_counts.as<int32_t>(), bm->chunk_offsets.as<int32_t>(), nc};
     launch_compact_index(bm->kept.as<uint8_t>(), nb, cp, bm->index.as<int32_t>(), s);
     int32_t nk = 0;

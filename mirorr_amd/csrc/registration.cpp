@@ -177,6 +177,7 @@ struct Workspace {
   DeviceBuffer lts, outer, mapI, flagsD;  // flagsD: {active, stop} for one read-back
   DeviceBuffer pyr_tmp, pyr_map;
   DeviceBuffer vsel_st, vsel_hist, vsel_evals;  // block-variance selection
+  DeviceBuffer rescored;                        // blocks re-scored in double
 };
 
 struct Flags {
@@ -265,9 +266,17 @@ struct RegRun {
                            selected ? fl.as<uint32_t>() : nullptr);
     }
     const int64_t off = swap ? g.nblocks : 0;
+    RescoreArgs ra;
+    if (p->block_metric == MIRORR_METRIC_NC) {
+      ra.x = base;
+      ra.bm = bmask;
+      ra.y = search;
+      ra.sm = smask;
+      ra.count = w.rescored.as<unsigned long long>();
+    }
     launch_postprocess_pairs(g, g_base, res, swap, w.kept.as<uint8_t>() + off,
                              w.X.as<double>() + 3 * off, w.Y.as<double>() + 3 * off,
-                             &st->n_kept, s);
+                             &st->n_kept, ra, s);
     if (!selected) stats->voxel_evals += voxel_evals(g);
   }
 
@@ -382,7 +391,7 @@ size_t registration_bytes_bound(const mirorr_reg_params* p, const Geom& FG, bool
            sizeof(unsigned long long) * np;  // X, Y, kept, r, cand
   total += sizeof(double) * 32 * (size_t)lts_parts((int64_t)np);
   total += 2 * sizeof(unsigned int) * 4096 + sizeof(VarSelState) +
-           sizeof(unsigned int) * kVarSelHistBins + sizeof(unsigned long long);
+           sizeof(unsigned int) * kVarSelHistBins + 2 * sizeof(unsigned long long);
   total += sizeof(LtsState) + sizeof(OuterState) + 2 * sizeof(IndexMap) + 2 * sizeof(int);
   return total;
 }
@@ -441,6 +450,8 @@ void run_registration(mirorr_ctx* ctx, const mirorr_reg_params* p, const mirorr_
   std::memset(stats, 0, sizeof(*stats));
   w.vsel_evals.ensure(sizeof(unsigned long long), bud);
   MIRORR_HIP(hipMemsetAsync(w.vsel_evals.as<void>(), 0, sizeof(unsigned long long), s));
+  w.rescored.ensure(sizeof(unsigned long long), bud);
+  MIRORR_HIP(hipMemsetAsync(w.rescored.as<void>(), 0, sizeof(unsigned long long), s));
   stats->levels = t_max;
   const int np = xf::nparams(T);
   for (int level = 0; level < t_max && level < 16; ++level) {
@@ -616,11 +627,14 @@ void run_registration(mirorr_ctx* ctx, const mirorr_reg_params* p, const mirorr_
     xf::get_params(T, cur);
     if (halve) max_it = std::max(1, max_it / 2);
   }
-  unsigned long long sel_evals = 0;
+  unsigned long long sel_evals = 0, rescored = 0;
   MIRORR_HIP(hipMemcpyAsync(&sel_evals, w.vsel_evals.as<void>(), sizeof(sel_evals),
+                            hipMemcpyDeviceToHost, s));
+  MIRORR_HIP(hipMemcpyAsync(&rescored, w.rescored.as<void>(), sizeof(rescored),
                             hipMemcpyDeviceToHost, s));
   MIRORR_HIP(hipStreamSynchronize(s));
   stats->voxel_evals += (int64_t)sel_evals;
+  stats->rescored_blocks = (int64_t)rescored;
   for (int i = 0; i < np; ++i) out_params[i] = cur[i];
   if (out_center)
     for (int k = 0; k < 3; ++k) out_center[k] = T.c[k];

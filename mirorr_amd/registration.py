@@ -146,7 +146,7 @@ class MirorrPyramid:
                       "points": list(st.points[: st.levels]),
                       "inner_iterations": st.inner_iterations, "voxel_evals": st.voxel_evals,
                       "level_dims": [tuple(st.level_dims[i]) for i in range(st.levels)],
-                      "seconds": st.seconds}
+                      "seconds": st.seconds, "rescored_blocks": st.rescored_blocks}
         return out[:n], oc
 
 

@@ -36,7 +36,7 @@ def _result_dict(r: capi.RequestResult, affine: bool) -> dict:
         "stats": {"levels": st.levels, "iterations": list(st.iterations[: st.levels]),
                   "points": list(st.points[: st.levels]),
                   "inner_iterations": st.inner_iterations, "voxel_evals": st.voxel_evals,
-                  "seconds": st.seconds},
+                  "seconds": st.seconds, "rescored_blocks": st.rescored_blocks},
     }
 
 

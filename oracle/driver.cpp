@@ -87,6 +87,45 @@ struct Matches {
   int64_t evals = 0;
 };
 
+// Exact-correspondence mode (a check of the level loop's transform algebra,
+// used only by orc_register_synthetic): with g_truth set, a block-matching
+// pass returns, for every block of the base grid, its centre x (the
+// blockPositionToPoint convention, itkAbstractBlockMatcher.h:271-287) and
+// the exact point y = to_search(to_phys(x)) where its content lies in the
+// search image, instead of searching the images. g_truth maps moving
+// physical points to fixed ones, as m_Transform does.
+thread_local const Xf* g_truth = nullptr;
+
+void exact_pairs(const orc_geom& gb, const orc_reg_params& p, const Xf& to_phys,
+                 const Xf& to_search, std::vector<double>& out_b, std::vector<double>& out_m) {
+  int nb[3], st[3];
+  orc_block_grid(gb.dims, p.bw, p.ng, nb, st);
+  out_b.clear();
+  out_m.clear();
+  for (int z = 0; z < nb[2]; ++z)
+    for (int y = 0; y < nb[1]; ++y)
+      for (int x = 0; x < nb[0]; ++x) {
+        const int idx[3] = {st[0] + p.ng * x, st[1] + p.ng * y, st[2] + p.ng * z};
+        double c[3], q[3], r[3];
+        for (int k = 0; k < 3; ++k) {
+          double v = gb.origin[k];
+          for (int j = 0; j < 3; ++j) v += gb.direction[3 * k + j] * gb.spacing[j] * idx[j];
+          c[k] = v + gb.spacing[k] * (0.5 * p.bw);
+        }
+        to_phys.apply(c, q);
+        to_search.apply(q, r);
+        out_b.insert(out_b.end(), c, c + 3);
+        out_m.insert(out_m.end(), r, r + 3);
+      }
+}
+
+// Composition a o b (b applied first).
+Xf after(const Xf& a, const Xf& b) {
+  Xf r = a;
+  r.compose(b, true);
+  return r;
+}
+
 // One block-matching pass. With portion < 1 (variance selection on, at a
 // level whose portion is not forced to 1.0) only the selected base blocks
 // are searched.
@@ -252,8 +291,14 @@ void start_opt_symmetric(Xf& T, const LevelIn& L, int max_iter, const orc_reg_pa
     resample(L.Ffull, L.Fmask, *L.Fg, cF12, gF, nullptr, &Fmh, L.Fmask == nullptr);
     resample(L.Mfull, L.Mmask, *L.Mg, cM12, gM, nullptr, &Mmh, L.Mmask == nullptr);
     std::vector<double> xb, ym, fb, fm;
-    match(Mh, Mmh, Fh, Fmh, gM, p, portion, xb, ym, &st->voxel_evals);  // base = moving
-    match(Fh, Fmh, Mh, Mmh, gF, p, portion, fb, fm, &st->voxel_evals);  // swapped
+    if (g_truth) {
+      // moving half -> moving -> fixed -> fixed half, and the swapped pass
+      exact_pairs(gM, p, after(*g_truth, cM12), cF12.inverse(), xb, ym);
+      exact_pairs(gF, p, after(g_truth->inverse(), cF12), cM12.inverse(), fb, fm);
+    } else {
+      match(Mh, Mmh, Fh, Fmh, gM, p, portion, xb, ym, &st->voxel_evals);  // base = moving
+      match(Fh, Fmh, Mh, Mmh, gF, p, portion, fb, fm, &st->voxel_evals);  // swapped
+    }
     xb.insert(xb.end(), fm.begin(), fm.end());
     ym.insert(ym.end(), fb.begin(), fb.end());
     st->points[level] = (int64_t)xb.size() / 3;
@@ -313,7 +358,10 @@ void start_opt_classic(Xf& T, const LevelIn& L, int max_iter, const orc_reg_para
     resample(L.F->v.data(), nullptr, L.F->g, T, gF, &Fh, nullptr, true);
     resample(L.Ffull, L.Fmask, *L.Fg, T, gF, nullptr, &Fmh, L.Fmask == nullptr);
     std::vector<double> xb, ym;
-    match(Mh, Mmh, Fh, Fmh, gM, p, portion, xb, ym, &st->voxel_evals);
+    if (g_truth)  // the moving grid holds moving points; Fh(h) = F(T(h))
+      exact_pairs(gM, p, *g_truth, T.inverse(), xb, ym);
+    else
+      match(Mh, Mmh, Fh, Fmh, gM, p, portion, xb, ym, &st->voxel_evals);
     st->points[level] = (int64_t)xb.size() / 3;
     st->inner_iterations += lts_classic(T, xb, ym, p.lts_tolerance);
     T.get_params(cur);
@@ -435,6 +483,30 @@ int orc_register(const float* fixed, const uint8_t* fmask, const orc_geom* fg,
     std::fprintf(stderr, "oracle registration failed: %s\n", e.what());
     return 2;
   }
+}
+
+// The level loop with exact correspondences from a truth transform
+// (y = M x + t, moving -> fixed physical points) in place of the block
+// search: images of the given geometries are all zero and only set the
+// grids. A check of the loop's transform algebra (centring, half-space
+// roots, composition, re-centring, level hand-over).
+int orc_register_synthetic(const orc_geom* fg, const orc_geom* mg, const orc_reg_params* p,
+                           const double truth_M[9], const double truth_t[3],
+                           const double* init_params, const double* init_center,
+                           double* out_params, double* out_center, orc_reg_stats* stats) {
+  Xf truth;
+  truth.kind = kAffine;
+  truth.set_identity();
+  M3 m;
+  for (int i = 0; i < 9; ++i) m.a[i / 3][i % 3] = truth_M[i];
+  truth.set_matrix(m, false);
+  truth.set_translation(truth_t);
+  std::vector<float> F(nvox(*fg), 0.0f), M(nvox(*mg), 0.0f);
+  g_truth = &truth;
+  const int rc = orc_register(F.data(), nullptr, fg, M.data(), nullptr, mg, p, init_params,
+                              init_center, out_params, out_center, stats);
+  g_truth = nullptr;
+  return rc;
 }
 
 }  // extern "C"

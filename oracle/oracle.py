@@ -132,6 +132,8 @@ _EXTRA_DECLS: list = [
     ("orc_reg_default_params", None, [C.POINTER(RegParams)]),
     ("orc_register", C.c_int, [_P, _P, _G, _P, _P, _G, C.POINTER(RegParams), _P, _P, _P, _P,
                                C.POINTER(RegStats)]),
+    ("orc_register_synthetic", C.c_int, [_G, _G, C.POINTER(RegParams), _P, _P, _P, _P, _P, _P,
+                                         C.POINTER(RegStats)]),
 ]
 
 
@@ -363,6 +365,28 @@ def reg_params(**kw) -> RegParams:
     for k, v in kw.items():
         setattr(p, k, v)
     return p
+
+
+def register_synthetic(fixed_geom: Geom, moving_geom: Geom, params: RegParams, matrix,
+                       translation, init_params=None, init_center=None):
+    """The oracle's level loop with exact correspondences from the truth
+    y = matrix x + translation (moving -> fixed physical points) in place of
+    the block search. Returns (params, center, stats)."""
+    fg, mg = fixed_geom.to_c(), moving_geom.to_c()
+    out = np.zeros(12)
+    oc = np.zeros(3)
+    st = RegStats()
+    ip = None if init_params is None else _f64(init_params)
+    ic = None if init_center is None else _f64(init_center, 3)
+    rc = lib().orc_register_synthetic(C.byref(fg), C.byref(mg), C.byref(params),
+                                      _ptr(_f64(matrix, 9)), _ptr(_f64(translation, 3)),
+                                      _ptr(ip), _ptr(ic), _ptr(out), _ptr(oc), C.byref(st))
+    if rc != 0:
+        raise RuntimeError(f"oracle registration failed ({rc})")
+    n = 12 if params.affine else 6
+    stats = {"levels": st.levels, "iterations": list(st.iterations[: st.levels]),
+             "points": list(st.points[: st.levels]), "inner_iterations": st.inner_iterations}
+    return out[:n], oc, stats
 
 
 def register(fixed, moving, params: RegParams, init_params=None, init_center=None,

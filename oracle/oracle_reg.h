@@ -56,6 +56,12 @@ int orc_register(const float* fixed, const uint8_t* fmask, const orc_geom* fg,
                  const orc_reg_params* p, const double* init_params,
                  const double* init_center, double* out_params, double* out_center,
                  orc_reg_stats* stats);
+// The level loop with exact correspondences from a truth transform (moving
+// -> fixed physical points) in place of the block search.
+int orc_register_synthetic(const orc_geom* fg, const orc_geom* mg, const orc_reg_params* p,
+                           const double truth_M[9], const double truth_t[3],
+                           const double* init_params, const double* init_center,
+                           double* out_params, double* out_center, orc_reg_stats* stats);
 
 #ifdef __cplusplus
 }

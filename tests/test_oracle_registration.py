@@ -72,3 +72,110 @@ def test_resampling_mode_grids(oracle, symmetric, mode, expect):
     want = {"middle": (36, 40, 22), "moving": (32, 44, 20), "fixed": (40, 36, 24),
             "basic": (36, 40, 22) if symmetric else (32, 44, 20)}[expect]
     assert tuple(got) == want
+
+
+# ---------------------------------------------------------------------------
+# Closed-form pins of the level loop (itkMirorrPyramidImplement.txx:107-145,
+# itkSymMirorrRegistrationMethod.txx:456-692, itkMirorrRegistrationMethod.txx:
+# 596-658,758-1003). A swapped Compose order, a wrong half-space root or
+# re-centring, or a lost level hand-over changes the recovered transform.
+
+def _texture(shape, seed):
+    rng = np.random.default_rng(seed)
+    a = rng.normal(0.0, 1.0, shape)
+    for ax in range(3):
+        a = (np.roll(a, 1, ax) + 2 * a + np.roll(a, -1, ax)) / 4
+    return (500 + 200 * a).astype(np.float32)
+
+
+@pytest.mark.parametrize("symmetric", [1, 0])
+def test_whole_voxel_translation_is_exact_after_one_iteration(oracle, symmetric):
+    # moving(i) = fixed(i + d) on one grid: m_Transform (moving -> fixed
+    # points) is the translation d * spacing. One level and one iteration:
+    # the half-way grids are the images' own, every interior block matches
+    # exactly (r^2 = 1) and the LTS trims the clamped border blocks.
+    from mirorr_amd.phantom import Volume
+    d = np.array([2, -1, 3])  # voxels (x, y, z)
+    m = 4
+    big = _texture((24 + 2 * m, 28 + 2 * m, 32 + 2 * m), 5)
+    fixed = big[m:m + 24, m:m + 28, m:m + 32]
+    moving = big[m + d[2]:m + d[2] + 24, m + d[1]:m + d[1] + 28, m + d[0]:m + d[0] + 32]
+    sp = (2.0, 1.5, 2.5)
+    org = (-31.0, -20.25, -28.75)
+    F = Volume(np.ascontiguousarray(fixed), sp, org)
+    M = Volume(np.ascontiguousarray(moving), sp, org)
+    rp = oracle.reg_params(symmetric=symmetric, affine=0, nw=7, ng=3, pyr_num=1, iterations=1)
+    par, c, st = oracle.register(F, M, rp)
+    assert st["iterations"] == [1]
+    Mx, off = transforms.matrix_offset(par, c, False)
+    np.testing.assert_allclose(Mx, np.eye(3), atol=1e-12)
+    np.testing.assert_allclose(off, d * np.array(sp), atol=1e-9)
+
+
+FG = dict(dims=(40, 36, 24), spacing=(2.0, 2.2, 2.5), origin=(-38.0, -41.0, -30.0))
+MG = dict(dims=(32, 44, 20), spacing=(2.4, 1.8, 3.0), origin=(-40.0, -35.0, -26.0))
+
+
+@pytest.mark.parametrize("levels", [1, 2])
+@pytest.mark.parametrize("affine", [0, 1])
+@pytest.mark.parametrize("symmetric", [1, 0])
+@pytest.mark.parametrize("start", ["centred", "given"])
+def test_level_loop_recovers_exact_correspondences(oracle, symmetric, affine, levels, start):
+    # The loop with exact correspondences of a truth transform (rotation,
+    # anisotropic scale for affine, translation) between grids of different
+    # size, spacing and centre. The result must map points as the truth does
+    # and carry the start transform's centre: the symmetric loop re-centres
+    # its composed result on it (updateCenter, itkSymMirorrRegistrationMethod.txx:
+    # 661-692); the classic loop composes into it (Compose(.., true), :1002).
+    R = phantom.euler_zxy(0.12, -0.07, 0.2)
+    A = R @ np.diag([1.05, 0.96, 1.02]) if affine else R
+    t = np.array([4.0, -6.5, 3.2])
+    fg, mg = oracle.Geom(**FG), oracle.Geom(**MG)
+    init = {}
+    c0 = np.zeros(3)
+    if start == "given":
+        c0 = np.array([3.0, -2.0, 1.5])
+        p0 = np.zeros(12 if affine else 6)
+        if affine:
+            p0[[0, 4, 8]] = 1.0
+        p0[-3:] = [1.0, 0.5, -0.5]
+        init = dict(init_params=p0, init_center=c0)
+    # classic rigid: the centre quirk of the LTS needs a few outer iterations
+    its = 6 if (not symmetric and not affine) else 3
+    rp = oracle.reg_params(symmetric=symmetric, affine=affine, nw=5, ng=4, pyr_num=levels,
+                           pyr_min_size=8, iterations=its)
+    par, c, st = oracle.register_synthetic(fg, mg, rp, A, t, **init)
+    assert st["levels"] == levels
+    Mx, off = transforms.matrix_offset(par, c, bool(affine))
+    np.testing.assert_allclose(Mx, A, atol=1e-12)
+    np.testing.assert_allclose(off, t, atol=1e-9)
+    np.testing.assert_array_equal(c, c0)
+    if symmetric and levels == 2:
+        # the second level starts at the truth: one iteration, no change
+        assert st["iterations"][1] == 1
+
+
+def test_classic_lts_translation_about_the_old_centre(oracle):
+    # CalculateRigid's translation is taken about the old centre rc while the
+    # centre is reset to the new barycentre mx (itkMirorrRegistrationMethod.txx:
+    # 931-933 against 996-998). Exact pairs y = R x + t with a small rotation
+    # and a small translation end the inner loop after one iteration (squared
+    # parameter change < 1e-3, :785-786), so the quirk stays in the result:
+    # y = R (x - mx) + mx + (my - rc) - R (mx - rc), off the exact fit by
+    # (I - R) mx for rc = 0.
+    rng = np.random.default_rng(3)
+    X = rng.uniform(-20, 20, (400, 3)) + np.array([60.0, -40.0, 30.0])
+    R = phantom.euler_zxy(0.0, 0.0, 0.01)
+    t = np.array([0.01, -0.02, 0.015])
+    Y = X @ R.T + t
+    p, c, it = oracle.lts_classic(0, X, Y, np.zeros(6), np.zeros(3))
+    assert it == 1
+    mx, my = X.mean(0), Y.mean(0)
+    want_M = R
+    want_off = -R @ mx + mx + my - R @ mx
+    Mx, off = transforms.matrix_offset(p, c, False)
+    np.testing.assert_allclose(Mx, want_M, atol=1e-12)
+    np.testing.assert_allclose(off, want_off, atol=1e-9)
+    np.testing.assert_array_equal(c, np.zeros(3))  # composed into the start transform
+    # the exact fit would be y = R x + t: the quirk moves it by (I - R) mx
+    assert np.linalg.norm(off - t) > 0.1
