@@ -68,6 +68,14 @@ typedef struct mirorr_bm_params {
                             1: keep the top portion_kept blocks by variance */
   int verbosity;         /* 0..3                                             */
   int block_metric;      /* MIRORR_METRIC_NC (default), _SSD or _CR          */
+  int exact_rescore;     /* 1 (default): blocks of the float NCC search
+                            (BW 4, BG 1, NW 5-13) whose float best or
+                            best-second gap lies next to the score filter's
+                            thresholds (1e-7, 1e-4), and exact ties, are
+                            searched again in double with the reference's
+                            operation order, so their records and the kept
+                            set are the reference's; 0: float scores as
+                            searched (for measuring their error)          */
 } mirorr_bm_params;
 
 /* Block metrics (--blockmetric nc | sd | cr; BlockMatcher's metric map,

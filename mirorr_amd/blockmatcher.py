@@ -147,6 +147,10 @@ class BlockMatcher:
     def GetPortionMatchesKept(self): return self._p.portion_kept
     def SetVarianceSelection(self, on: bool): self._set("variance_select", 1 if on else 0)
 
+    # Exact double re-scoring of the blocks next to the score filter's
+    # thresholds (on by default); off leaves the float scores as searched.
+    def SetExactRescore(self, on: bool): self._set("exact_rescore", 1 if on else 0)
+
     def SetBlockMetricType(self, name: str):
         """normalized_correlation (default), sum_of_squared_differences or
         correlation_ratio (BlockMatcher::SetBlockMetricType, itkBlockMatcher.h:143-175)."""

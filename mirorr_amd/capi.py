@@ -59,7 +59,7 @@ class BMParams(C.Structure):
     _fields_ = [("block_width", C.c_int), ("block_gap", C.c_int), ("nhood_width", C.c_int),
                 ("nhood_gap", C.c_int), ("portion_kept", C.c_double),
                 ("variance_select", C.c_int), ("verbosity", C.c_int),
-                ("block_metric", C.c_int)]
+                ("block_metric", C.c_int), ("exact_rescore", C.c_int)]
 
 
 class RegParams(C.Structure):

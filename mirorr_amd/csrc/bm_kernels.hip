@@ -442,7 +442,29 @@ struct TileArgs {
   float* out_second;
   int32_t* out_match;
   unsigned long long* work;  // optional: voxel-evals done by this launch
+  int32_t* rlist;            // optional: blocks to re-score in double
+  int32_t* rcount;
+  int rties;                 // list exact ties too
 };
+
+// Blocks of the tiled search listed for the exact re-scoring (below): the
+// float best or best-second gap lies within kRescoreBand of a score-filter
+// threshold (best > 1e-7, |best - second| > 1e-4), or (with ties) the best is
+// an exact tie, whose float-buffer record depends on how the double score
+// rounds although the filter culls it either way. The band is twice the
+// largest float error measured on the phantom, resampled-with-masks and
+// CT/MR workloads (2.3e-5, at high-contrast blocks whose best window lies
+// across an intensity step from the block's own position); a GPU test
+// checks that margin on those workloads.
+constexpr double kRescoreBand = 5e-5;
+constexpr int kRescoreMaxCand = 13 * 13 * 13;
+constexpr int kRescoreMaxWin = 16 * 16 * 16;  // (NW + BW - 1)^3 at NW 13
+
+__device__ __forceinline__ bool near_filter(float best, float second, int ties) {
+  const double b = best, s = second;
+  return fabs(b - 1e-7) < kRescoreBand || fabs(fabs(b - s) - 1e-4) < kRescoreBand ||
+         (ties && best == second && b > 0.0);
+}
 
 // Voxel-evals of the valid blocks of a tile (profiling only).
 template <class T>
@@ -493,6 +515,7 @@ __device__ __forceinline__ void write_block_result(const TileArgs& a, int64_t b,
   a.out_match[3 * b + 0] = px - T::H + cx;
   a.out_match[3 * b + 1] = py - T::H + cy;
   a.out_match[3 * b + 2] = pz - T::H + cz;
+  if (a.rlist && near_filter(r.best, r.second, a.rties)) a.rlist[atomicAdd(a.rcount, 1)] = (int32_t)b;
 }
 
 // Record of a block dropped by the variance selection: the initial score
@@ -684,7 +707,7 @@ __global__ __launch_bounds__(kThreads, 4) void ncc_tile_fast(TileArgs a) {
   __shared__ int s_any;
   __shared__ unsigned long long s_bad, s_valid;
   __shared__ unsigned long long zrow[kMaxWindow];  // off-mask rows per window plane
-  __shared__ float s_ref;
+  __shared__ float bref[T::NB];  // per-block shift: the window mean at the block's position
 
   const BMGrid& g = a.g;
   const int nx = g.dims[0], ny = g.dims[1], nz = g.dims[2];
@@ -704,10 +727,6 @@ __global__ __launch_bounds__(kThreads, 4) void ncc_tile_fast(TileArgs a) {
     s_any = 0;
     s_bad = 0ull;
     s_valid = 0ull;
-    // Reference shift: window mean at the tile's middle block (clamped).
-    const int ry = min(py0 + NG * (T::TY / 2), ny - T::BW);
-    const int rz = min(pz0 + NG * (T::TZ / 2), nz - T::BW);
-    s_ref = a.stats[lin3(nx, ny, px, ry, rz)].x;
   }
   merge_init(mrg, T::NB);
   for (int i = tid; i < WZ; i += kThreads) zrow[i] = 0ull;
@@ -726,6 +745,7 @@ __global__ __launch_bounds__(kThreads, 4) void ncc_tile_fast(TileArgs a) {
       atomicOr(&s_valid, 1ull << tid);
       bsxx[tid] = a.sxx[b];
       bsum[tid] = a.sumxt[b];
+      bref[tid] = a.stats[lin3(nx, ny, px, py0 + NG * (tid % T::TY), pz0 + NG * (tid / T::TY))].x;
       if ((a.bflags[b] & 1u) == 0u) atomicOr(&s_bad, 1ull << tid);  // base partly masked
     } else if (st == 2) {
       write_skipped(a, b, px, py0 + NG * (tid % T::TY), pz0 + NG * (tid / T::TY));
@@ -733,11 +753,15 @@ __global__ __launch_bounds__(kThreads, 4) void ncc_tile_fast(TileArgs a) {
   }
   __syncthreads();
   if (!s_any) return;  // every block is skipped or outside the grid (workgroup-uniform)
-  const float ref = s_ref;
-  // Stage the union search window (shifted by ref; zeros outside the image).
-  // A search-mask-off voxel sends the blocks whose own window holds it to the
+  // Stage the union search window as it is (zeros outside the image); each
+  // lane shifts the rows it loads by its block's own reference, so that the
+  // accumulated products stay of the size of the contrast around the block
+  // (a shift shared by the tile leaves products of the size of the
+  // intensity step between a block's region and the shift, whose float
+  // rounding is large next to the Sxy of a low-contrast block). A
+  // search-mask-off voxel sends the blocks whose own window holds it to the
   // masked kernel; the other blocks stay here.
-  stage_window<T, false>(a, win, nullptr, zrow, WY, WZ, wx0, wy0, wz0, ref);
+  stage_window<T, false>(a, win, nullptr, zrow, WY, WZ, wx0, wy0, wz0, 0.0f);
   for (int i = tid; i < T::NB * 16; i += kThreads) {
     const int bl = i / 16;
     const int gy = by0 + bl % T::TY, gz = bz0 + bl / T::TY;
@@ -749,6 +773,7 @@ __global__ __launch_bounds__(kThreads, 4) void ncc_tile_fast(TileArgs a) {
   __syncthreads();
   if (tid < T::NB && ((s_valid >> tid) & 1ull) && window_has_off<T>(zrow, tid, NG))
     atomicOr(&s_bad, 1ull << tid);
+
   __syncthreads();
   const unsigned long long todo = s_bad & s_valid;
   if (todo) {
@@ -784,6 +809,7 @@ __global__ __launch_bounds__(kThreads, 4) void ncc_tile_fast(TileArgs a) {
 #pragma unroll
       for (int cx = 0; cx < T::NW; ++cx) acc[ky][cx] = 0.0f;
     const float* rowbase = win + ((blz * NG + cz) * WY + (bly * NG + cyg * T::KY)) * T::PX;
+    const float rb = bref[bl];
 #pragma unroll 1
     for (int iz = 0; iz < 4; ++iz) {
       // One z-plane of the centred block (16 values) stays in VGPRs.
@@ -804,10 +830,10 @@ __global__ __launch_bounds__(kThreads, 4) void ncc_tile_fast(TileArgs a) {
 #pragma unroll
         for (int v = 0; v < T::NV; ++v) {
           const float4 q = rp[v];
-          row[4 * v + 0] = q.x;
-          row[4 * v + 1] = q.y;
-          row[4 * v + 2] = q.z;
-          row[4 * v + 3] = q.w;
+          row[4 * v + 0] = q.x - rb;
+          row[4 * v + 1] = q.y - rb;
+          row[4 * v + 2] = q.z - rb;
+          row[4 * v + 3] = q.w - rb;
         }
 #pragma unroll
         for (int ky = 0; ky < T::KY; ++ky) {
@@ -841,7 +867,7 @@ __global__ __launch_bounds__(kThreads, 4) void ncc_tile_fast(TileArgs a) {
 #pragma unroll
         for (int cx = 0; cx < T::NW; ++cx) {
           const float2 st = srow[cx];
-          const float sxy = fmaf(-(st.x - ref), sumx, acc[ky][cx]);
+          const float sxy = fmaf(-(st.x - rb), sumx, acc[ky][cx]);
           float sc = fmaxf(sxy * sxy * isxx * st.y, -2.0f);
           if (T::YPAD > 0 && cy >= T::NW) sc = -__builtin_huge_valf();
           top2_step(t2, sc, (cz * T::NW + cy) * T::NW + cx);
@@ -867,7 +893,7 @@ __global__ __launch_bounds__(kThreads, 4) void ncc_tile_fast(TileArgs a) {
         for (int cx = 0; cx < T::NW; ++cx) {
           const int cxg = px - T::H + cx;
           const float2 st = srow[min(max(cxg, 0), nx - T::BW)];
-          const float sxy = fmaf(-(st.x - ref), sumx, acc[ky][cx]);
+          const float sxy = fmaf(-(st.x - rb), sumx, acc[ky][cx]);
           float sc = fmaxf(sxy * sxy * isxx * st.y, -2.0f);
           sc = (yok && cxg >= lox && cxg <= hix) ? sc : -__builtin_huge_valf();
           top2_step(t2, sc, (cz * T::NW + cy) * T::NW + cx);
@@ -898,8 +924,8 @@ __device__ __forceinline__ void load_mask(const uint8_t* wv, int ro, int v, floa
 }
 
 // Masked path: sums over voxels where both masks are on. The window is
-// staged as v * (y - ref) (v = search mask) plus the mask bytes.
-// With t = v (y - mu) = v (y - ref) - v (mu - ref) (one FMA per voxel):
+// staged as v * y (v = search mask) plus the mask bytes.
+// With t = v (y - mu), one FMA per voxel (v * y - v * mu, rounded once):
 //  * Base block fully on (the usual case at image borders): Sxy = Sum x~ t,
 //    Syy = Sum t^2, Sxx = Sum v x~^2 (4 FMAs per voxel). Syy is summed from
 //    the same float t as Sxy rather than taken from the window statistic:
@@ -925,7 +951,7 @@ __global__ __launch_bounds__(kThreads, T::MWG) void ncc_tile_masked(TileArgs a) 
   __shared__ float bsxx[T::NB], bsum[T::NB];       // block sums of fully-on bases
   __shared__ int s_nfull;
   __shared__ int bvalid[T::NB];
-  __shared__ float s_ref;
+  __shared__ float bref[T::NB];  // per-block shift of the full rows
 
   // Workgroup (q, part) searches every msplit-th queued block of tile q
   // (more, shorter workgroups where the level has few tiles).
@@ -956,11 +982,6 @@ __global__ __launch_bounds__(kThreads, T::MWG) void ncc_tile_masked(TileArgs a) 
   float* wy_ = reinterpret_cast<float*>(dyn4);
   uint8_t* wv_ = reinterpret_cast<uint8_t*>(wy_ + total);  // search mask bytes
 
-  if (tid == 0) {
-    const int ry = min(py0 + NG * (T::TY / 2), ny - T::BW);
-    const int rz = min(pz0 + NG * (T::TZ / 2), nz - T::BW);
-    s_ref = a.stats[lin3(nx, ny, px, ry, rz)].x;
-  }
   if (tid < T::NB) {
     bvalid[tid] = ((todo >> tid) & 1ull) ? 1 : 0;
     bbits[tid] = 0ull;
@@ -982,6 +1003,7 @@ __global__ __launch_bounds__(kThreads, T::MWG) void ncc_tile_masked(TileArgs a) 
       atomicOr(&s_fullbase, 1ull << tid);
       bsxx[tid] = a.sxx[b];
       bsum[tid] = a.sumxt[b];
+      bref[tid] = a.stats[lin3(nx, ny, px, py0 + NG * (tid % T::TY), pz0 + NG * (tid / T::TY))].x;
     }
   }
   __syncthreads();
@@ -997,8 +1019,10 @@ __global__ __launch_bounds__(kThreads, T::MWG) void ncc_tile_masked(TileArgs a) 
     s_ntodo = n;
   }
   __syncthreads();
-  const float ref = s_ref;
-  stage_window<T, true>(a, wy_, wv_, zon, WY, WZ, wx0, wy0, wz0, ref, zbad);
+  // Staged as v * y; full rows shift their loads by the block's reference
+  // as the fast kernel does, masked rows centre each voxel on the
+  // candidate's own mean with one rounding.
+  stage_window<T, true>(a, wy_, wv_, zon, WY, WZ, wx0, wy0, wz0, 0.0f, zbad);
   for (int i = tid; i < T::NB * 64; i += kThreads) {
     const int bl = i / 64, k = i % 64;
     const int gy = by0 + bl % T::TY, gz = bz0 + bl / T::TY;
@@ -1108,8 +1132,9 @@ __global__ __launch_bounds__(kThreads, T::MWG) void ncc_tile_masked(TileArgs a) 
     const int rowoff = ((blz * NG + cz) * WY + (bly * NG + cy)) * T::PX;
     if (full) {
       // Every voxel of every candidate window of this row is on and the
-      // base block is fully on: Sxy = Sum x~ (y - ref) - (mu - ref) Sum x~,
+      // base block is fully on: Sxy = Sum x~ (y - rb) - (mu - rb) Sum x~,
       // Sxx and Syy from the block and window statistics.
+      const float rb = bref[bl];
       float acc[T::NW];
 #pragma unroll
       for (int cx = 0; cx < T::NW; ++cx) acc[cx] = 0.0f;
@@ -1123,7 +1148,8 @@ __global__ __launch_bounds__(kThreads, T::MWG) void ncc_tile_masked(TileArgs a) 
 #pragma unroll
           for (int v = 0; v < T::NV; ++v) {
             const float4 q = yp[v];
-            yr[4 * v + 0] = q.x; yr[4 * v + 1] = q.y; yr[4 * v + 2] = q.z; yr[4 * v + 3] = q.w;
+            yr[4 * v + 0] = q.x - rb; yr[4 * v + 1] = q.y - rb;
+            yr[4 * v + 2] = q.z - rb; yr[4 * v + 3] = q.w - rb;
           }
           const float4 xq4 = xh4[bl * 16 + iz * 4 + iy];
           const float xh[4] = {xq4.x, xq4.y, xq4.z, xq4.w};
@@ -1140,7 +1166,7 @@ __global__ __launch_bounds__(kThreads, T::MWG) void ncc_tile_masked(TileArgs a) 
         const int cxg = px - T::H + cx;
         if (cxg < lox || cxg > hix) continue;
         const float2 st = srow[cxg];
-        const float sxy = fmaf(-(st.x - ref), sumx, acc[cx]);
+        const float sxy = fmaf(-(st.x - rb), sumx, acc[cx]);
         top2_push(tr, fmaxf(sxy * sxy * isxx * st.y, -2.0f), (cz * T::NW + cy) * T::NW + cx);
       }
       rbest[i] = tr.best;
@@ -1152,7 +1178,7 @@ __global__ __launch_bounds__(kThreads, T::MWG) void ncc_tile_masked(TileArgs a) 
 #pragma unroll
     for (int cx = 0; cx < T::NW; ++cx) {
       const int cxg = min(max(px - T::H + cx, lox), hix);
-      dmu[cx] = srow[cxg].x - ref;
+      dmu[cx] = srow[cxg].x;
     }
     const unsigned long long bits = bbits[bl];
     float sxy[T::NW], sxx[T::NW], s3[T::NW];
@@ -1276,7 +1302,10 @@ __global__ __launch_bounds__(kThreads, T::MWG) void ncc_tile_masked(TileArgs a) 
   }
 }
 
-__global__ void reset_counter(int32_t* c) { *c = 0; }
+__global__ void reset_counters(int32_t* c, int32_t* d) {
+  *c = 0;
+  if (d) *d = 0;
+}
 
 // ---------------------------------------------------------------------------
 // Generic kernel: any BW, BG, NW. One workgroup per block; candidates in
@@ -1564,34 +1593,32 @@ __global__ __launch_bounds__(kThreads) void cr_generic_kernel(
 // ---------------------------------------------------------------------------
 // Exact re-scoring of the blocks next to the score filter's thresholds.
 //
-// The tiled kernels accumulate in float, so their NCC^2 values differ from the
-// reference's double sums (itkBlockMatcher.txx:46-101) by about 1e-6. The
-// filter keeps a block when best > 1e-7 and |best - second| > 1e-4
-// (itkBlockMatcher.txx:886-916), so a block whose float best or gap lies within
-// kRescoreBand of a threshold could land on the other side of it. Such blocks,
-// and blocks whose best is an exact tie, are searched again here in double with the reference's operation order (the
-// means over all voxels summed in raster order, the masked centred sums in
-// raster order, then prod * prod / bss / sss), without fused multiply-adds,
-// and their record is rebuilt with the reference's float score buffer
-// (itkBlockMatcher.txx:764-774,823-833). For the sequence v_1..v_n of double
-// scores in scan order that buffer ends with
-//   best   = float(max_k v_k),
-//   match  = the last k with v_k > double(float(max(-3, v_1..v_{k-1}))),
-//   second = float(max over the other candidates), -3 with one candidate,
-// which is what one workgroup computes below: candidate scores in LDS, a
-// prefix maximum over per-thread runs, each thread replaying its own run.
-constexpr double kRescoreBand = 2e-5;
-constexpr int kRescoreMaxCand = 13 * 13 * 13;
-
-// A block is re-scored when its float best or best-second gap lies within
-// kRescoreBand of a threshold, or when its best is an exact tie (the float
-// buffer's record of a tie depends on how the double score rounds).
-__device__ __forceinline__ bool near_filter(float best, float second) {
-  const double b = best, s = second;
-  return fabs(b - 1e-7) < kRescoreBand || fabs(fabs(b - s) - 1e-4) < kRescoreBand ||
-         (best == second && b > 0.0);
-}
-
+// The tiled kernels (BW 4, BG 1) accumulate in float, so their NCC^2 values
+// differ from the reference's double sums (itkBlockMatcher.txx:46-101) by up
+// to a few 1e-5. The filter keeps a block when best > 1e-7 and |best -
+// second| > 1e-4 (itkBlockMatcher.txx:886-916), so a block whose float best
+// or gap lies within kRescoreBand of a threshold could land on the other
+// side of it. The tiled kernels list such blocks (and on request exact
+// ties), and one workgroup per listed block rebuilds its record as the
+// reference computes it:
+//  1. every candidate is scored in float with its own window mean, together
+//     with a rigorous interval [lo, hi] holding the reference's double
+//     score (first-order rounding-error bounds of the float sums, padded);
+//  2. the candidates whose hi reaches the second largest lo (less 2^-20)
+//     are scored again in double with the reference's operation order (the
+//     means over all voxels summed in raster order, the masked centred sums
+//     in raster order, then prod * prod / bss / sss), without fused
+//     multiply-adds; the others lie below the best and the second and
+//     cannot change the record;
+//  3. the record follows the reference's float score buffer
+//     (itkBlockMatcher.txx:764-774,823-833): for the double scores v_1..v_n
+//     in scan order it ends with
+//       best   = float(max_k v_k),
+//       match  = the last k with v_k > double(float(max(-3, v_1..v_{k-1}))),
+//       second = float(max over the other candidates), -3 with one candidate,
+//     computed over the candidates of step 2 (the others never replace the
+//     best after a candidate within 2^-20 of the maximum, and never attain
+//     the second).
 __device__ __forceinline__ int cand_total(const BMGrid& g, int64_t b, int p[3], int first[3],
                                           int cnt[3]) {
   block_corner(g, b, p);
@@ -1599,12 +1626,33 @@ __device__ __forceinline__ int cand_total(const BMGrid& g, int64_t b, int p[3], 
   return cnt[0] * cnt[1] * cnt[2];
 }
 
+// Images of a re-scoring (base and search, masks NULL = all on) and the
+// search image's window statistics (mean over all voxels in .x).
+struct RescoreArgs {
+  const float* x;
+  const uint8_t* bm;
+  const float* y;
+  const uint8_t* sm;
+  const float2* stats;
+  unsigned long long* count;  // optional: += blocks re-scored
+  int exact_records;          // 1: the reference's record; 0: its filter decision only
+};
+
 struct RescoreLds {
-  double score[kRescoreMaxCand];
+  union {
+    float2 iv[kRescoreMaxCand];  // phase 1: (lo, hi) of the double score
+    double v[kRescoreMaxCand];   // phase 2: the double score, -inf off the exact set
+  };
+  float win[kRescoreMaxWin];    // the union of the candidate windows
+  uint8_t won[kRescoreMaxWin];  // its search mask (1 = on)
+  float base[64];               // the block
+  float xm[64];                 // float(x - mean), 0 where the base mask is off
+  uint8_t bon[64];
+  int16_t elist[kRescoreMaxCand];  // phase 2: the candidates scored in double
   double wave[kThreads / 64];
   int iwave[kThreads / 64];
-  int list[kThreads];
-  int nlist;
+  int anyoff;                   // a base or window voxel is masked off
+  int ne;
 };
 
 __device__ __forceinline__ double wave_max(double v) {
@@ -1624,56 +1672,304 @@ __device__ __forceinline__ double group_max(double v, RescoreLds& L) {
   return r;
 }
 
-// Re-scores block b in double and rewrites its record. Called by every
-// thread of the workgroup with the same b; total <= kRescoreMaxCand.
+// The two largest of v over the workgroup (m1 >= m2; -inf when absent).
+__device__ __forceinline__ void group_top2(float a, float b, RescoreLds& L, float* m1,
+                                           float* m2) {
+  // (a, b) is a thread's own top-2; merge by waves then across waves.
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const float c = __shfl_xor(a, o, 64), d = __shfl_xor(b, o, 64);
+    const float hi = fmaxf(a, c);
+    b = fmaxf(fminf(a, c), fmaxf(b, d));
+    a = hi;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    L.wave[threadIdx.x >> 6] = __hiloint2double(__float_as_int(a), __float_as_int(b));
+  }
+  __syncthreads();
+  float x = -__builtin_huge_valf(), y = -__builtin_huge_valf();
+  for (int w = 0; w < kThreads / 64; ++w) {
+    const double q = L.wave[w];
+    const float c = __int_as_float(__double2hiint(q)), d = __int_as_float(__double2loint(q));
+    const float hi = fmaxf(x, c);
+    y = fmaxf(fminf(x, c), fmaxf(y, d));
+    x = hi;
+  }
+  __syncthreads();
+  *m1 = x;
+  *m2 = y;
+}
+
+// Candidate k's voxel v (raster order in the block) in the staged window.
+__device__ __forceinline__ int win_index(int w0, int v, int ex, int ey) {
+  return w0 + ((v >> 4) * ey + ((v >> 2) & 3)) * ex + (v & 3);
+}
+
+constexpr float kU = 5.9604645e-8f;          // 2^-24
+constexpr float kGamma = 64.0f * kU * 1.01f;  // error factor of a 64-term float sum
+
+// Phase 1 when every base and window voxel is on: one thread per candidate
+// row (cz, cy) scores its NW candidates from shared row loads, with
+// Sxy = Sum x~ t and Syy = Sum t^2 in float (t = y - mu, mu the window mean
+// of the statistics) and Sxx = the block's double sum (the same for every
+// candidate).
+template <int NW>
+__device__ void lean_phase1(const RescoreArgs& ra, const BMGrid& g, const int first[3],
+                            const int cnt[3], int ex, int ey, double bss, float bx, float sx,
+                            RescoreLds& L, float* l1, float* l2, float* h1, float* h2) {
+  const int nx = g.dims[0], ny = g.dims[1];
+  const int rows = cnt[1] * cnt[2];
+  const int r = threadIdx.x;
+  if (r >= rows) return;
+  const int ky = r % cnt[1], kz = r / cnt[1];
+  float mu[NW], syy[NW], sxy[NW], A[NW];
+  const float2* srow = ra.stats + lin3(nx, ny, first[0], first[1] + ky, first[2] + kz);
+#pragma unroll
+  for (int cx = 0; cx < NW; ++cx) {
+    mu[cx] = cx < cnt[0] ? srow[cx].x : 0.0f;
+    syy[cx] = 0.0f;
+    sxy[cx] = 0.0f;
+    A[cx] = 0.0f;
+  }
+#pragma unroll 1
+  for (int q = 0; q < 16; ++q) {  // block row (iz, iy) = (q / 4, q % 4)
+    const float* wr = L.win + ((kz + (q >> 2)) * ey + ky + (q & 3)) * ex;
+    float w[NW + 3];
+#pragma unroll
+    for (int i = 0; i < NW + 3; ++i) w[i] = i < ex ? wr[i] : 0.0f;
+    float xr[4];
+#pragma unroll
+    for (int ix = 0; ix < 4; ++ix) xr[ix] = L.xm[4 * q + ix];
+#pragma unroll
+    for (int cx = 0; cx < NW; ++cx)
+#pragma unroll
+      for (int ix = 0; ix < 4; ++ix) {
+        const float t = w[cx + ix] - mu[cx];
+        sxy[cx] = fmaf(xr[ix], t, sxy[cx]);
+        syy[cx] = fmaf(t, t, syy[cx]);
+        A[cx] = fmaf(fabsf(xr[ix]), fabsf(t), A[cx]);
+      }
+  }
+  const float isxx = (float)(1.0 / bss);
+#pragma unroll
+  for (int cx = 0; cx < NW; ++cx) {
+    if (cx >= cnt[0]) continue;
+    // Error bounds as in the general phase 1; with every voxel on, the
+    // exact centred values sum to zero over the block and over the window,
+    // so the error e of the float mean (|e| <= 2u |mu|) enters Sxy only
+    // through the float block sum (|Sum x~| <= |sx| + gamma bx) and Syy only
+    // as 64 e^2.
+    const float amu = fabsf(mu[cx]);
+    const float d = kGamma * A[cx] + 3.0f * kU * A[cx] + 2.0f * kU * amu * (fabsf(sx) + kGamma * bx);
+    const float dyy = kGamma * syy[cx] + 5.0f * kU * syy[cx] +
+                      256.0f * (kU * amu) * (kU * amu);
+    float lo, hi;
+    if (syy[cx] - dyy > 0.0f) {
+      const float smax = fabsf(sxy[cx]) + d, smin = fmaxf(fabsf(sxy[cx]) - d, 0.0f);
+      hi = smax * smax * isxx / (syy[cx] - dyy) * (1.0f + 16.0f * kU) + 1e-9f;
+      lo = smin * smin * isxx / (syy[cx] + dyy) * (1.0f - 16.0f * kU) - 1e-9f;
+    } else {
+      lo = -2.0f;  // the window sum may be zero: scored in double
+      hi = __builtin_huge_valf();
+    }
+    L.iv[r * cnt[0] + cx] = make_float2(lo, hi);
+    const float nl1 = fmaxf(*l1, lo);
+    *l2 = fmaxf(fminf(*l1, lo), *l2);
+    *l1 = nl1;
+    const float nh1 = fmaxf(*h1, hi);
+    *h2 = fmaxf(fminf(*h1, hi), *h2);
+    *h1 = nh1;
+  }
+}
+
+// Re-scores block b (BW 4, BG 1) and rewrites its record. Called by every
+// thread of the workgroup with the same b.
 __device__ void rescore_block(const RescoreArgs& ra, const BMGrid& g, int64_t b, float* best,
                               float* second, int32_t* match, RescoreLds& L) {
 #pragma clang fp contract(off)
-  const float* __restrict__ x = ra.x;
-  const float* __restrict__ y = ra.y;
-  const uint8_t* __restrict__ bm = ra.bm;
-  const uint8_t* __restrict__ sm = ra.sm;
-  const int nx = g.dims[0], ny = g.dims[1], bw = g.bw;
+  const int nx = g.dims[0], ny = g.dims[1];
   int p[3], first[3], cnt[3];
   const int total = cand_total(g, b, p, first, cnt);
+  const int ex = cnt[0] + 3, ey = cnt[1] + 3, ez = cnt[2] + 3;
+  if (threadIdx.x == 0) L.anyoff = 0;
+  __syncthreads();
+  for (int i = threadIdx.x; i < ex * ey * ez; i += kThreads) {
+    const int wx = i % ex, wy = (i / ex) % ey, wz = i / (ex * ey);
+    const int64_t o = lin3(nx, ny, first[0] + wx, first[1] + wy, first[2] + wz);
+    L.win[i] = ra.y[o];
+    const bool on = ra.sm == nullptr || ra.sm[o] != 0;
+    L.won[i] = on ? 1 : 0;
+    if (!on) L.anyoff = 1;
+  }
+  if (threadIdx.x < 64) {
+    const int k = threadIdx.x;
+    const int64_t o = lin3(nx, ny, p[0] + (k & 3), p[1] + ((k >> 2) & 3), p[2] + (k >> 4));
+    L.base[k] = ra.x[o];
+    const bool on = ra.bm == nullptr || ra.bm[o] != 0;
+    L.bon[k] = on ? 1 : 0;
+    if (!on) L.anyoff = 1;
+  }
+  __syncthreads();
+  // The block mean over all voxels, summed in raster order as the
+  // reference does (itkBlockMatcher.txx:57-68).
   double count = 0.0, mb = 0.0;
-  for (int dz = 0; dz < bw; ++dz)
-    for (int dy = 0; dy < bw; ++dy)
-      for (int dx = 0; dx < bw; ++dx) {
-        mb += x[lin3(nx, ny, p[0] + dx, p[1] + dy, p[2] + dz)];
-        count += 1.0;
-      }
+  for (int k = 0; k < 64; ++k) {
+    mb += L.base[k];
+    count += 1.0;
+  }
   mb /= count;
-  for (int k = threadIdx.x; k < total; k += kThreads) {
-    const int s0 = first[0] + (k % cnt[0]) * g.bg;
-    const int s1 = first[1] + ((k / cnt[0]) % cnt[1]) * g.bg;
-    const int s2 = first[2] + (k / (cnt[0] * cnt[1])) * g.bg;
-    double ms = 0.0;
-    for (int dz = 0; dz < bw; ++dz)
-      for (int dy = 0; dy < bw; ++dy)
-        for (int dx = 0; dx < bw; ++dx) ms += y[lin3(nx, ny, s0 + dx, s1 + dy, s2 + dz)];
-    ms /= count;
-    double prod = 0.0, bss = 0.0, sss = 0.0;
-    for (int dz = 0; dz < bw; ++dz)
-      for (int dy = 0; dy < bw; ++dy)
-        for (int dx = 0; dx < bw; ++dx) {
-          const int64_t ob = lin3(nx, ny, p[0] + dx, p[1] + dy, p[2] + dz);
-          const int64_t os = lin3(nx, ny, s0 + dx, s1 + dy, s2 + dz);
-          if ((bm && bm[ob] == 0) || (sm && sm[os] == 0)) continue;
-          const double u = x[ob] - mb;
-          const double v = y[os] - ms;
-          prod += u * v;
-          bss += u * u;
-          sss += v * v;
+  if (threadIdx.x < 64)
+    L.xm[threadIdx.x] = L.bon[threadIdx.x] ? (float)(L.base[threadIdx.x] - mb) : 0.0f;
+  __syncthreads();
+
+  // Phase 1: float scores with intervals.
+  constexpr float u = kU;
+  constexpr float gam = kGamma;
+  // thread's top-2 of lo and of hi
+  float l1 = -__builtin_huge_valf(), l2 = -__builtin_huge_valf();
+  float h1 = -__builtin_huge_valf(), h2 = -__builtin_huge_valf();
+  const bool lean = !L.anyoff && cnt[1] * cnt[2] <= kThreads;
+  if (lean) {
+    double bss = 0.0;  // the reference's base sum of squares (every voxel on)
+    float bx = 0.0f, sx = 0.0f;
+    for (int k = 0; k < 64; ++k) {
+      const double d = L.base[k] - mb;
+      bss += d * d;
+      bx += fabsf(L.xm[k]);
+      sx += L.xm[k];
+    }
+    if (bss > 0.0) {
+#define MIRORR_LEAN(NWV) \
+  lean_phase1<NWV>(ra, g, first, cnt, ex, ey, bss, bx, sx, L, &l1, &l2, &h1, &h2)
+      switch (g.nw) {
+        case 5: MIRORR_LEAN(5); break;
+        case 7: MIRORR_LEAN(7); break;
+        case 9: MIRORR_LEAN(9); break;
+        case 11: MIRORR_LEAN(11); break;
+        default: MIRORR_LEAN(13); break;
+      }
+#undef MIRORR_LEAN
+    } else {
+      for (int k = threadIdx.x; k < total; k += kThreads) L.iv[k] = make_float2(-2.0f, -2.0f);
+      l1 = h1 = threadIdx.x < total ? -2.0f : l1;
+      l2 = h2 = threadIdx.x + kThreads < total ? -2.0f : l2;
+    }
+  }
+  for (int k = lean ? total : threadIdx.x; k < total; k += kThreads) {
+    const int kx = k % cnt[0], ky = (k / cnt[0]) % cnt[1], kz = k / (cnt[0] * cnt[1]);
+    const int w0 = (kz * ey + ky) * ex + kx;
+    const float mu = ra.stats[lin3(nx, ny, first[0] + kx, first[1] + ky, first[2] + kz)].x;
+    float sxy = 0.0f, syy = 0.0f, sxx = 0.0f, A = 0.0f, Bx = 0.0f, C = 0.0f;
+#pragma unroll 8
+    for (int v = 0; v < 64; ++v) {
+      const int wi = win_index(w0, v, ex, ey);
+      const float m = (L.won[wi] && L.bon[v]) ? 1.0f : 0.0f;
+      const float t = m * (L.win[wi] - mu);
+      const float xv = L.xm[v] * m;
+      sxy = fmaf(xv, t, sxy);
+      syy = fmaf(t, t, syy);
+      sxx = fmaf(xv, xv, sxx);
+      A = fmaf(fabsf(xv), fabsf(t), A);
+      Bx += fabsf(xv);
+      C += fabsf(t);
+    }
+    const float amu = fabsf(mu);
+    const float dxy = gam * A + 3.0f * u * (A + amu * Bx);
+    const float dyy = gam * syy + 5.0f * u * (syy + amu * C) + 64.0f * (u * amu) * (u * amu);
+    const float dxx = gam * sxx + 3.0f * u * sxx;
+    float lo, hi;
+    if (sxx == 0.0f) {
+      // No voxel inside both masks, or the block's centred values are all
+      // exactly zero there (float(x - mean) is 0 only when x - mean is):
+      // the reference's -2.
+      lo = hi = -2.0f;
+    } else if (sxx - dxx > 0.0f && syy - dyy > 0.0f) {
+      const float smax = fabsf(sxy) + dxy, smin = fmaxf(fabsf(sxy) - dxy, 0.0f);
+      hi = smax * smax / ((sxx - dxx) * (syy - dyy)) * (1.0f + 16.0f * u) + 1e-9f;
+      lo = smin * smin / ((sxx + dxx) * (syy + dyy)) * (1.0f - 16.0f * u) - 1e-9f;
+    } else {
+      lo = -2.0f;  // the window sum may be zero: scored in double
+      hi = __builtin_huge_valf();
+    }
+    L.iv[k] = make_float2(lo, hi);
+    const float nl1 = fmaxf(l1, lo);
+    l2 = fmaxf(fminf(l1, lo), l2);
+    l1 = nl1;
+    const float nh1 = fmaxf(h1, hi);
+    h2 = fmaxf(fminf(h1, hi), h2);
+    h1 = nh1;
+  }
+  float L1, L2, U1, U2;
+  group_top2(l1, l2, L, &L1, &L2);  // (its syncs also publish L.iv)
+  group_top2(h1, h2, L, &U1, &U2);
+  if (!ra.exact_records) {
+    // The filter's decision alone (the registration): the true best M and
+    // second S satisfy L1 <= M <= U1, S >= L2 and, when one candidate's lo
+    // exceeds every other hi, S <= U2. A decision certain by more than 1e-6
+    // (above the float rounding of the filter's comparisons) is written as a
+    // record the filter decides the same way; the rest is scored in double.
+    constexpr float mg = 1e-6f;
+    const bool cull = U1 < 1e-7f - mg || (total >= 2 && U1 - L2 < 1e-4f - mg);
+    const bool keep = L1 > 1e-7f + mg && (total < 2 || L1 - U2 > 1e-4f + mg);
+    if (cull || keep) {
+      for (int k = threadIdx.x; k < total && keep; k += kThreads)
+        if (L.iv[k].y == U1) {  // the unique best candidate
+          match[3 * b + 0] = first[0] + k % cnt[0];
+          match[3 * b + 1] = first[1] + (k / cnt[0]) % cnt[1];
+          match[3 * b + 2] = first[2] + k / (cnt[0] * cnt[1]);
         }
-    L.score[k] = (bss <= 0.0 || sss <= 0.0) ? -2.0 : prod * prod / bss / sss;
+      if (threadIdx.x == 0) {
+        best[b] = 0.5f * (L1 + U1);
+        second[b] = cull ? best[b] : (total >= 2 ? 0.5f * (L2 + U2) : -3.0f);
+      }
+      __syncthreads();
+      return;
+    }
+  }
+  const float cut = (total >= 2 ? L2 : L1) - 9.5367432e-7f;  // less 2^-20
+
+  // Phase 2: the candidates that can reach the top two, in double, one wave
+  // per candidate with a lane per voxel; each sum is accumulated in raster
+  // order from the lanes' terms, as the reference's loops add them (a voxel
+  // outside either mask adds an exact zero where the reference skips it).
+  if (threadIdx.x == 0) L.ne = 0;
+  __syncthreads();
+  for (int k = threadIdx.x; k < total; k += kThreads) {
+    if (L.iv[k].y >= cut) L.elist[atomicAdd(&L.ne, 1)] = (int16_t)k;
+    L.v[k] = -__builtin_huge_val();  // the thread's own slot: its interval was read above
+  }
+  __syncthreads();
+  const int ne = L.ne;
+  {
+    const int ln = threadIdx.x & 63;
+    const double uu = L.base[ln] - mb;
+    for (int e = threadIdx.x >> 6; e < ne; e += kThreads / 64) {  // wave-uniform
+      const int k = L.elist[e];
+      const int kx = k % cnt[0], ky = (k / cnt[0]) % cnt[1], kz = k / (cnt[0] * cnt[1]);
+      const int wi = win_index((kz * ey + ky) * ex + kx, ln, ex, ey);
+      const double yv = L.win[wi];
+      const bool on = L.bon[ln] && L.won[wi];
+      double ms = 0.0;
+      for (int q = 0; q < 64; ++q) ms += __shfl(yv, q, 64);
+      ms /= count;
+      // Sums over the voxels inside both masks (itkBlockMatcher.txx:70-95).
+      const double tt = yv - ms;
+      const double pv = on ? uu * tt : 0.0, bv = on ? uu * uu : 0.0, sv = on ? tt * tt : 0.0;
+      double prod = 0.0, bss = 0.0, sss = 0.0;
+      for (int q = 0; q < 64; ++q) {
+        prod += __shfl(pv, q, 64);
+        bss += __shfl(bv, q, 64);
+        sss += __shfl(sv, q, 64);
+      }
+      if (ln == 0) L.v[k] = (bss <= 0.0 || sss <= 0.0) ? -2.0 : prod * prod / bss / sss;
+    }
   }
   __syncthreads();
   // Each thread replays a contiguous run of candidates.
   const int per = (total + kThreads - 1) / kThreads;
   const int k0 = min((int)threadIdx.x * per, total), k1 = min(k0 + per, total);
   double run_max = -3.0;
-  for (int k = k0; k < k1; ++k) run_max = fmax(run_max, L.score[k]);
+  for (int k = k0; k < k1; ++k) run_max = fmax(run_max, L.v[k]);
   // Exclusive prefix maximum of the runs (the buffer starts at -3).
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   double inc = run_max;
@@ -1690,7 +1986,7 @@ __device__ void rescore_block(const RescoreArgs& ra, const BMGrid& g, int64_t b,
   __syncthreads();
   int last = -1;
   for (int k = k0; k < k1; ++k) {
-    const double v = L.score[k];
+    const double v = L.v[k];
     if ((double)(float)pre < v) last = k;
     pre = fmax(pre, v);
   }
@@ -1704,47 +2000,38 @@ __device__ void rescore_block(const RescoreArgs& ra, const BMGrid& g, int64_t b,
   for (int w = 1; w < kThreads / 64; ++w) I = max(I, L.iwave[w]);
   double other = -3.0;
   for (int k = k0; k < k1; ++k)
-    if (k != I) other = fmax(other, L.score[k]);
+    if (k != I) other = fmax(other, L.v[k]);
   const double top = group_max(run_max, L);
   const double sec = group_max(other, L);
   if (threadIdx.x == 0) {
     best[b] = (float)top;
     second[b] = (float)sec;
-    match[3 * b + 0] = first[0] + (I % cnt[0]) * g.bg;
-    match[3 * b + 1] = first[1] + ((I / cnt[0]) % cnt[1]) * g.bg;
-    match[3 * b + 2] = first[2] + (I / (cnt[0] * cnt[1])) * g.bg;
+    match[3 * b + 0] = first[0] + I % cnt[0];
+    match[3 * b + 1] = first[1] + (I / cnt[0]) % cnt[1];
+    match[3 * b + 2] = first[2] + I / (cnt[0] * cnt[1]);
   }
   __syncthreads();
 }
 
-// Re-scores the blocks of [b0, b0 + kThreads) next to a threshold. Uniform
-// over the workgroup; returns the number re-scored.
-__device__ int rescore_chunk(const RescoreArgs& ra, const BMGrid& g, int64_t b0, float* best,
-                             float* second, int32_t* match, RescoreLds& L) {
-  if (threadIdx.x == 0) L.nlist = 0;
-  __syncthreads();
-  const int64_t b = b0 + threadIdx.x;
-  if (b < g.nblocks && near_filter(best[b], second[b])) {
-    int p[3], first[3], cnt[3];
-    if (cand_total(g, b, p, first, cnt) <= kRescoreMaxCand) L.list[atomicAdd(&L.nlist, 1)] = b - b0;
-  }
-  __syncthreads();
-  const int n = L.nlist;
-  for (int i = 0; i < n; ++i) rescore_block(ra, g, b0 + L.list[i], best, second, match, L);
-  return n;
-}
-
-__global__ void postprocess_kernel(BMGrid g, Geom geom, float* __restrict__ best,
-                                   float* __restrict__ second, int32_t* __restrict__ match,
-                                   uint8_t* kept, double* base_mm, double* match_mm,
-                                   RescoreArgs ra) {
+// Re-scores the listed blocks, one workgroup per block; the grid strides
+// over the list.
+__global__ __launch_bounds__(kThreads) void rescore_list_kernel(BMGrid g, float* best,
+                                                                float* second, int32_t* match,
+                                                                RescoreArgs ra,
+                                                                const int32_t* __restrict__ list,
+                                                                const int32_t* __restrict__ count) {
   __shared__ RescoreLds L;
-  const int64_t b0 = (int64_t)blockIdx.x * blockDim.x;
-  if (ra.x) {
-    const int n = rescore_chunk(ra, g, b0, best, second, match, L);
-    if (threadIdx.x == 0 && n && ra.count) atomicAdd(ra.count, (unsigned long long)n);
-  }
-  const int64_t b = b0 + threadIdx.x;
+  const int n = *count;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && ra.count && n > 0)
+    atomicAdd(ra.count, (unsigned long long)n);
+  for (int i = blockIdx.x; i < n; i += gridDim.x) rescore_block(ra, g, list[i], best, second, match, L);
+}
+
+__global__ void postprocess_kernel(BMGrid g, Geom geom, const float* __restrict__ best,
+                                   const float* __restrict__ second,
+                                   const int32_t* __restrict__ match, uint8_t* kept,
+                                   double* base_mm, double* match_mm) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= g.nblocks) return;
   const double cb = best[b], c2 = second[b];
   kept[b] = (cb > 1e-7 && fabs(cb - c2) > 1e-4) ? 1 : 0;
@@ -1766,39 +2053,22 @@ __global__ void postprocess_kernel(BMGrid g, Geom geom, float* __restrict__ best
   }
 }
 
-// Re-scoring alone (the raw records of a standalone search).
-__global__ __launch_bounds__(kThreads) void rescore_kernel(BMGrid g, float* best, float* second,
-                                                           int32_t* match, RescoreArgs ra) {
-  __shared__ RescoreLds L;
-  unsigned long long tot = 0;
-  for (int64_t b0 = (int64_t)blockIdx.x * kThreads; b0 < g.nblocks;
-       b0 += (int64_t)gridDim.x * kThreads)
-    tot += rescore_chunk(ra, g, b0, best, second, match, L);
-  if (threadIdx.x == 0 && tot && ra.count) atomicAdd(ra.count, tot);
-}
-
 // Pair output for the LTS: X gets the moving-side point and Y the fixed-side
 // point of every block; `swap` marks the second symmetric pass, whose base
-// image is the fixed one (itkSymMirorrRegistrationMethod.txx:595-602). With
-// ra.x set, blocks next to a threshold are re-scored first.
+// image is the fixed one (itkSymMirorrRegistrationMethod.txx:595-602).
 __global__ __launch_bounds__(kThreads) void postprocess_pairs_kernel(
-    BMGrid g, Geom geom, float* __restrict__ best, float* __restrict__ second,
-    int32_t* __restrict__ match, int swap, uint8_t* kept, double* X, double* Y,
-    unsigned long long* count, RescoreArgs ra) {
+    BMGrid g, Geom geom, const float* __restrict__ best, const float* __restrict__ second,
+    const int32_t* __restrict__ match, int swap, uint8_t* kept, double* X, double* Y,
+    unsigned long long* count) {
   // Grid-stride over the blocks with one count update per workgroup: the
   // counter is a single device-wide address, so updates are kept few.
   __shared__ unsigned int wsum[kThreads / 64];
-  __shared__ RescoreLds L;
   const double half = 0.5 * g.bw;
   double* pb = swap ? Y : X;
   double* pm = swap ? X : Y;
   unsigned int nk = 0;
-  unsigned long long nres = 0;
-  for (int64_t b0 = (int64_t)blockIdx.x * kThreads; b0 < g.nblocks;
-       b0 += (int64_t)gridDim.x * kThreads) {
-    if (ra.x) nres += rescore_chunk(ra, g, b0, best, second, match, L);
-    const int64_t b = b0 + threadIdx.x;
-    if (b >= g.nblocks) continue;
+  for (int64_t b = (int64_t)blockIdx.x * kThreads + threadIdx.x; b < g.nblocks;
+       b += (int64_t)gridDim.x * kThreads) {
     const double cb = (double)best[b];
     const double c2 = (double)second[b];
     const bool k = cb > 1e-7 && fabs(cb - c2) > 1e-4;
@@ -1829,7 +2099,6 @@ __global__ __launch_bounds__(kThreads) void postprocess_pairs_kernel(
     unsigned int tot = 0;
     for (int w = 0; w < kThreads / 64; ++w) tot += wsum[w];
     if (tot) atomicAdd(count, (unsigned long long)tot);
-    if (nres && ra.count) atomicAdd(ra.count, nres);
   }
 }
 
@@ -2010,7 +2279,10 @@ void run_tiled(const BMGrid& g, TileArgs a, BMScratch& scr, hipStream_t s) {
   a.queue = scr.queue;
   a.qbits = scr.qbits;
   a.qcount = scr.count;
-  hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, s, scr.count);
+  a.rlist = scr.rlist;
+  a.rcount = scr.rcount;
+  a.rties = scr.rescore_ties;
+  hipLaunchKernelGGL(reset_counters, dim3(1), dim3(1), 0, s, scr.count, scr.rcount);
   const int WYf = T::fast_pitch((T::TY - 1) * g.ng + T::WX + T::YPAD);
   const int WZ = (T::TZ - 1) * g.ng + T::WX;
   const size_t lds_fast = sizeof(float) * (size_t)T::PX * WYf * WZ;
@@ -2183,33 +2455,33 @@ void launch_ncc_search(const float* x, const uint8_t* bmask, const float* y,
     case 11: run_tiled<Cfg11>(g, a, scr, s); break;
     case 13: run_tiled<Cfg13>(g, a, scr, s); break;
   }
-}
-
-void launch_rescore(const BMGrid& g, const BMResult& res, const RescoreArgs& ra, hipStream_t s) {
-  if (g.nblocks == 0 || !ra.x) return;
-  const int64_t grid = std::min<int64_t>((g.nblocks + kThreads - 1) / kThreads, 1024);
-  hipLaunchKernelGGL(rescore_kernel, dim3((unsigned)grid), dim3(kThreads), 0, s, g, res.best,
-                     res.second, res.match, ra);
-  MIRORR_CHECK_LAUNCH();
+  if (scr.rlist) {
+    // Blocks listed by the search kernels are searched again in double.
+    RescoreArgs ra{x, bmask, y, smask, st.ms, scr.rescored, scr.exact_records};
+    const int64_t grid = std::min<int64_t>(g.nblocks, 1024);
+    hipLaunchKernelGGL(rescore_list_kernel, dim3((unsigned)grid), dim3(kThreads), 0, s, g,
+                       res.best, res.second, res.match, ra, scr.rlist, scr.rcount);
+    MIRORR_CHECK_LAUNCH();
+  }
 }
 
 void launch_postprocess(const BMGrid& g, const Geom& geom, const BMResult& res,
                         uint8_t* kept, double* base_mm, double* match_mm,
-                        const RescoreArgs& ra, hipStream_t s) {
+                        hipStream_t s) {
   if (g.nblocks == 0) return;
   hipLaunchKernelGGL(postprocess_kernel, dim3((unsigned)((g.nblocks + kThreads - 1) / kThreads)),
                      dim3(kThreads), 0, s, g, geom, res.best, res.second, res.match, kept,
-                     base_mm, match_mm, ra);
+                     base_mm, match_mm);
   MIRORR_CHECK_LAUNCH();
 }
 
 void launch_postprocess_pairs(const BMGrid& g, const Geom& geom, const BMResult& res, int swap,
                               uint8_t* kept, double* X, double* Y, unsigned long long* count,
-                              const RescoreArgs& ra, hipStream_t s) {
+                              hipStream_t s) {
   if (g.nblocks == 0) return;
   const int64_t grid = std::min<int64_t>((g.nblocks + kThreads - 1) / kThreads, 512);
   hipLaunchKernelGGL(postprocess_pairs_kernel, dim3((unsigned)grid), dim3(kThreads), 0, s, g, geom,
-                     res.best, res.second, res.match, swap, kept, X, Y, count, ra);
+                     res.best, res.second, res.match, swap, kept, X, Y, count);
   MIRORR_CHECK_LAUNCH();
 }
 

@@ -46,12 +46,30 @@ struct BMResult {
 };
 
 // Working memory of one search (tile queue for the masked path).
+//
+// Exact re-scoring (rlist set): the tiled kernels accumulate in float, so
+// their NCC^2 values differ from the reference's double sums
+// (itkBlockMatcher.txx:46-101) by up to a few 1e-5. The score filter keeps a
+// block when best > 1e-7 and |best - second| > 1e-4 (itkBlockMatcher.txx:
+// 886-916); the tiled kernels list every block whose float best or gap lies
+// within 1e-4 of a threshold (and, with rescore_ties, every exact tie),
+// and those blocks are searched again in double with the reference's
+// operation order and float score buffer, so the filter decides every block
+// as the reference does.
 struct BMScratch {
   int32_t* queue = nullptr;  // ntiles entries
   int32_t* count = nullptr;  // 1 counter
   unsigned long long* qbits = nullptr;  // ntiles entries: blocks per queued tile
   int64_t ntiles_cap = 0;
   Profiler* prof = nullptr;  // optional: event timing + voxel-evals per launch
+  int32_t* rlist = nullptr;  // nblocks entries: blocks to re-score (NULL: none)
+  int32_t* rcount = nullptr; // 1 counter
+  int rescore_ties = 0;
+  // 1: the listed blocks get the reference's record (best, second, match);
+  // 0: only the filter's decision (and a kept block's match) is the
+  // reference's, which is all a registration uses.
+  int exact_records = 1;
+  unsigned long long* rescored = nullptr;  // optional: += blocks re-scored
 };
 
 void launch_search_stats(const float* y, const uint8_t* smask, const BMGrid& g,
@@ -88,38 +106,17 @@ void launch_metric_search(int metric, const float* x, const uint8_t* bmask, cons
 int64_t tile_count(const BMGrid& g);
 bool tiled_supported(const BMGrid& g);
 
-// Images of an NCC search for the exact re-scoring of the blocks whose float
-// best or best-second gap lies next to the score filter's thresholds
-// (1e-7 and 1e-4, itkBlockMatcher.txx:886-916): those blocks are searched
-// again in double with the reference's operation order and float score
-// buffer, so the filter decides them as the reference does. x = NULL
-// switches the re-scoring off (other metrics). count (optional) += blocks
-// re-scored.
-struct RescoreArgs {
-  const float* x = nullptr;    // base image
-  const uint8_t* bm = nullptr; // base mask (NULL: all on)
-  const float* y = nullptr;    // search image
-  const uint8_t* sm = nullptr; // search mask (NULL: all on)
-  unsigned long long* count = nullptr;
-};
-
-// Re-scoring alone: rewrites the raw records of the blocks next to a
-// threshold (res is updated in place).
-void launch_rescore(const BMGrid& g, const BMResult& res, const RescoreArgs& ra, hipStream_t s);
-
 // Score filter + mm conversion (itkBlockMatcher.txx:886-916,
 // itkAbstractBlockMatcher.h:271-287). kept[i] in {0,1}; pts are nblocks x 3.
-// Re-scores first when ra.x is set.
 void launch_postprocess(const BMGrid& g, const Geom& geom, const BMResult& res,
                         uint8_t* kept, double* base_mm, double* match_mm,
-                        const RescoreArgs& ra, hipStream_t s);
+                        hipStream_t s);
 
 // Score filter + mm conversion written as LTS pairs: the base point goes to
 // X (Y when swap), the matched point to Y (X when swap); count += kept.
-// Re-scores first when ra.x is set.
 void launch_postprocess_pairs(const BMGrid& g, const Geom& geom, const BMResult& res, int swap,
                               uint8_t* kept, double* X, double* Y, unsigned long long* count,
-                              const RescoreArgs& ra, hipStream_t s);
+                              hipStream_t s);
 
 // Ordered compaction of nblocks records by a 0/1 flag array.
 struct Compactor {

@@ -35,6 +35,7 @@ struct mirorr_bm {
   // Results of the last search.
   DeviceBuffer best, second, match, kept, base_mm, match_mm, index;
   DeviceBuffer queue, qbits, qcount, chunk_counts, chunk_offsets;
+  DeviceBuffer rlist, rcount;  // blocks listed for the exact re-scoring
   bool results_valid = false;
   uint64_t results_key[2] = {0, 0};
   BMGrid grid{};
@@ -76,7 +77,7 @@ static void run_search(mirorr_bm* bm) {
   const BMGrid g = current_grid(bm);
   const uint64_t key0 = bm->slot[0].version ^ ((uint64_t)g.bw << 40) ^ ((uint64_t)g.nw << 48) ^
                         ((uint64_t)g.ng << 56) ^ ((uint64_t)g.bg << 32) ^
-                        ((uint64_t)bm->p.block_metric << 24);
+                        ((uint64_t)bm->p.block_metric << 24) ^ ((uint64_t)bm->p.exact_rescore << 20);
   const uint64_t key1 = bm->slot[1].version;
   if (bm->results_valid && bm->results_key[0] == key0 && bm->results_key[1] == key1 &&
       std::memcmp(&bm->grid, &g, sizeof(g)) == 0)
@@ -141,19 +142,18 @@ static void run_search(mirorr_bm* bm) {
     bm->qcount.ensure(sizeof(int32_t), bud);
     BMScratch scr{bm->queue.as<int32_t>(), bm->qcount.as<int32_t>(),
                   bm->qbits.as<unsigned long long>(), nt, &bm->ctx->prof};
+    if (bm->p.exact_rescore) {
+      bm->rlist.ensure(sizeof(int32_t) * nb, bud);
+      bm->rcount.ensure(sizeof(int32_t), bud);
+      scr.rlist = bm->rlist.as<int32_t>();
+      scr.rcount = bm->rcount.as<int32_t>();
+      scr.rescore_ties = 1;  // raw records of exact ties are the reference's too
+    }
     launch_ncc_search(B.data, B.mask, S.data, S.mask, g, bp,
                       SearchStats{bm->stats[si].as<float2>()}, scr, res, s);
   } else {
     launch_metric_search(metric, B.data, B.mask, S.data, S.mask, g, res, s,
                          bm->prep_selected[bi] ? bp.flags : nullptr);
-  }
-  if (metric == MIRORR_METRIC_NC) {
-    RescoreArgs ra;
-    ra.x = B.data;
-    ra.bm = B.mask;
-    ra.y = S.data;
-    ra.sm = S.mask;
-    launch_rescore(g, res, ra, s);
   }
   bm->results_valid = true;
   bm->results_key[0] = key0;
@@ -254,6 +254,7 @@ void mirorr_bm_default_params(mirorr_bm_params* p) {
   p->variance_select = 0;
   p->verbosity = 0;
   p->block_metric = MIRORR_METRIC_NC;
+  p->exact_rescore = 1;
 }
 
 mirorr_status mirorr_bm_create(mirorr_ctx* c, const mirorr_bm_params* p, mirorr_bm** out) {
@@ -469,7 +470,7 @@ mirorr_status mirorr_bm_get_offsets(mirorr_bm* bm, double* base_mm, double* matc
     BMResult res{bm->best.as<float>(), bm->second.as<float>(), bm->match.as<int32_t>()};
     const Geom geom = to_geom(bm->slot[MIRORR_ROLE_BASE].geom);
     launch_postprocess(g, geom, res, bm->kept.as<uint8_t>(), bm->base_mm.as<double>(),
-                       bm->match_mm.as<double>(), RescoreArgs{}, s);
+                       bm->match_mm.as<double>(), s);
     Compactor cp{bm->chunk_counts.as<int32_t>(), bm->chunk_offsets.as<int32_t>(), nc};
     launch_compact_index(bm->kept.as<uint8_t>(), nb, cp, bm->index.as<int32_t>(), s);
     int32_t nk = 0;

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -177,7 +178,8 @@ struct Workspace {
   DeviceBuffer lts, outer, mapI, flagsD;  // flagsD: {active, stop} for one read-back
   DeviceBuffer pyr_tmp, pyr_map;
   DeviceBuffer vsel_st, vsel_hist, vsel_evals;  // block-variance selection
-  DeviceBuffer rescored;                        // blocks re-scored in double
+  DeviceBuffer rlist, rcount, rlist2, rcount2;  // blocks listed for the exact re-scoring
+  DeviceBuffer rescored, rescored2;             // blocks re-scored in double, per pass
 };
 
 struct Flags {
@@ -259,6 +261,10 @@ struct RegRun {
       DeviceBuffer& qb = second_set ? w.qbits2 : w.qbits;
       BMScratch scr{qu.as<int32_t>(), qc.as<int32_t>(), qb.as<unsigned long long>(), tile_count(g),
                     (ctx->prof.finest_only() && !finest) ? nullptr : &ctx->prof};
+      scr.rlist = (second_set ? w.rlist2 : w.rlist).as<int32_t>();
+      scr.rcount = (second_set ? w.rcount2 : w.rcount).as<int32_t>();
+      scr.rescored = (second_set ? w.rescored2 : w.rescored).as<unsigned long long>();
+      scr.exact_records = 0;  // the pairs need the filter's decision and the match only
       launch_ncc_search(base, bmask, search, smask, g, bp, SearchStats{stats_search.as<float2>()},
                         scr, res, s);
     } else {
@@ -266,17 +272,9 @@ struct RegRun {
                            selected ? fl.as<uint32_t>() : nullptr);
     }
     const int64_t off = swap ? g.nblocks : 0;
-    RescoreArgs ra;
-    if (p->block_metric == MIRORR_METRIC_NC) {
-      ra.x = base;
-      ra.bm = bmask;
-      ra.y = search;
-      ra.sm = smask;
-      ra.count = w.rescored.as<unsigned long long>();
-    }
     launch_postprocess_pairs(g, g_base, res, swap, w.kept.as<uint8_t>() + off,
                              w.X.as<double>() + 3 * off, w.Y.as<double>() + 3 * off,
-                             &st->n_kept, ra, s);
+                             &st->n_kept, s);
     if (!selected) stats->voxel_evals += voxel_evals(g);
   }
 
@@ -310,7 +308,9 @@ struct RegRun {
       b->ensure(sizeof(int32_t) * std::max<int64_t>(tile_count(g), 1), bud);
     for (DeviceBuffer* b : {&w.qbits, &w.qbits2})
       b->ensure(sizeof(unsigned long long) * std::max<int64_t>(tile_count(g), 1), bud);
-    for (DeviceBuffer* b : {&w.qcount, &w.qcount2}) b->ensure(sizeof(int32_t), bud);
+    for (DeviceBuffer* b : {&w.qcount, &w.qcount2, &w.rcount, &w.rcount2})
+      b->ensure(sizeof(int32_t), bud);
+    for (DeviceBuffer* b : {&w.rlist, &w.rlist2}) b->ensure(sizeof(int32_t) * nb, bud);
     const int64_t np = 2 * nb;
     w.X.ensure(sizeof(double) * 3 * np, bud);
     w.Y.ensure(sizeof(double) * 3 * np, bud);
@@ -391,7 +391,8 @@ size_t registration_bytes_bound(const mirorr_reg_params* p, const Geom& FG, bool
            sizeof(unsigned long long) * np;  // X, Y, kept, r, cand
   total += sizeof(double) * 32 * (size_t)lts_parts((int64_t)np);
   total += 2 * sizeof(unsigned int) * 4096 + sizeof(VarSelState) +
-           sizeof(unsigned int) * kVarSelHistBins + 2 * sizeof(unsigned long long);
+           sizeof(unsigned int) * kVarSelHistBins + 3 * sizeof(unsigned long long);
+  total += 2 * (sizeof(int32_t) * nb + sizeof(int32_t));  // re-scoring lists, one per pass
   total += sizeof(LtsState) + sizeof(OuterState) + 2 * sizeof(IndexMap) + 2 * sizeof(int);
   return total;
 }
@@ -450,8 +451,10 @@ void run_registration(mirorr_ctx* ctx, const mirorr_reg_params* p, const mirorr_
   std::memset(stats, 0, sizeof(*stats));
   w.vsel_evals.ensure(sizeof(unsigned long long), bud);
   MIRORR_HIP(hipMemsetAsync(w.vsel_evals.as<void>(), 0, sizeof(unsigned long long), s));
-  w.rescored.ensure(sizeof(unsigned long long), bud);
-  MIRORR_HIP(hipMemsetAsync(w.rescored.as<void>(), 0, sizeof(unsigned long long), s));
+  for (DeviceBuffer* b : {&w.rescored, &w.rescored2}) {
+    b->ensure(sizeof(unsigned long long), bud);
+    MIRORR_HIP(hipMemsetAsync(b->as<void>(), 0, sizeof(unsigned long long), s));
+  }
   stats->levels = t_max;
   const int np = xf::nparams(T);
   for (int level = 0; level < t_max && level < 16; ++level) {
@@ -594,6 +597,15 @@ void run_registration(mirorr_ctx* ctx, const mirorr_reg_params* p, const mirorr_
       }
       // 3. LTS and transform update on the device
       run.run_lts(st, outer, npts, flags);  // leaves flags->stop read back
+      if (p->verbosity >= 1) {  // as the reference's iteration report
+        unsigned long long nk = 0;
+        double d = 0.0;
+        MIRORR_HIP(hipMemcpyAsync(&nk, &st->n_kept, sizeof(nk), hipMemcpyDeviceToHost, s));
+        MIRORR_HIP(hipMemcpyAsync(&d, &outer->sqdiff, sizeof(d), hipMemcpyDeviceToHost, s));
+        MIRORR_HIP(hipStreamSynchronize(s));
+        std::printf("level %d iteration %d: %llu blocks, delta %.6g\n", level, it + 1, nk, d);
+        std::fflush(stdout);
+      }
       if (flags->stop) {
         ++it;
         break;
@@ -627,14 +639,16 @@ void run_registration(mirorr_ctx* ctx, const mirorr_reg_params* p, const mirorr_
     xf::get_params(T, cur);
     if (halve) max_it = std::max(1, max_it / 2);
   }
-  unsigned long long sel_evals = 0, rescored = 0;
+  unsigned long long sel_evals = 0, rescored[2] = {0, 0};
   MIRORR_HIP(hipMemcpyAsync(&sel_evals, w.vsel_evals.as<void>(), sizeof(sel_evals),
                             hipMemcpyDeviceToHost, s));
-  MIRORR_HIP(hipMemcpyAsync(&rescored, w.rescored.as<void>(), sizeof(rescored),
+  MIRORR_HIP(hipMemcpyAsync(&rescored[0], w.rescored.as<void>(), sizeof(rescored[0]),
+                            hipMemcpyDeviceToHost, s));
+  MIRORR_HIP(hipMemcpyAsync(&rescored[1], w.rescored2.as<void>(), sizeof(rescored[1]),
                             hipMemcpyDeviceToHost, s));
   MIRORR_HIP(hipStreamSynchronize(s));
   stats->voxel_evals += (int64_t)sel_evals;
-  stats->rescored_blocks = (int64_t)rescored;
+  stats->rescored_blocks = (int64_t)(rescored[0] + rescored[1]);
   for (int i = 0; i < np; ++i) out_params[i] = cur[i];
   if (out_center)
     for (int k = 0; k < 3; ++k) out_center[k] = T.c[k];

@@ -308,7 +308,7 @@ void start_opt_symmetric(Xf& T, const LevelIn& L, int max_iter, const orc_reg_pa
     params_sq_delta(lastp, cur, np, &d);
     if (p.verbosity >= 1)
       std::printf("oracle level %d iteration %d: %lld blocks, delta %.6g\n", level, it + 1,
-                  (long long)st->points[level], d);
+                  (long long)st->points[level], d), std::fflush(stdout);
     std::memcpy(lastp, cur, sizeof(double) * np);
     if (d < p.stop_tolerance) {
       ++it;
@@ -369,7 +369,7 @@ void start_opt_classic(Xf& T, const LevelIn& L, int max_iter, const orc_reg_para
     params_sq_delta(lastp, cur, np, &d);
     if (p.verbosity >= 1)
       std::printf("oracle level %d iteration %d: %lld blocks, delta %.6g\n", level, it + 1,
-                  (long long)st->points[level], d);
+                  (long long)st->points[level], d), std::fflush(stdout);
     std::memcpy(lastp, cur, sizeof(double) * np);
     if (d < p.stop_tolerance) {
       ++it;

@@ -136,3 +136,54 @@ def test_ctmr_full_size_registration_recovers_transform():
           f"error {err:.2f} mm (initial {init:.1f} mm)")
     assert st["levels"] == 4
     assert err < 3.0 and err < init / 5.0
+
+
+def _resampled_with_masks(pair, oracle):
+    # Both images of the 256x256x128 pair moved by a small rigid transform
+    # (zero outside the field of view, masks marking it), as the half-way
+    # images of a registration are.
+    g = oracle.geom_of(pair.moving)
+    R = phantom.euler_zxy(0.05, -0.03, 0.04)
+    t = np.array([5.3, -4.1, 2.7])
+    (xb, mb) = oracle.resample(pair.moving.data, None, g, R.ravel(), t, g, want_mask=True)
+    (xs, ms) = oracle.resample(pair.fixed.data, None, g, R.T.ravel(), -t, g, want_mask=True)
+    return xb, mb, xs, ms
+
+
+@pytest.mark.parametrize("case", ["phantom", "masked", "ctmr"])
+def test_float_search_error_within_half_the_rescore_band(oracle, case):
+    # The tiled search scores in float; blocks whose float best or gap lies
+    # within 5e-5 of a filter threshold are re-scored in double. Here the
+    # float scores (re-scoring off) stay within half that band of the
+    # oracle's, and with re-scoring on the kept sets are the oracle's.
+    from mirorr_amd.phantom import Volume
+    if case == "ctmr":
+        pair = phantom.make_ct_mr_pair(dims=(256, 256, 128), spacing=(2.0, 2.0, 2.0),
+                                       n_ellipsoids=24, seed=3)
+    else:
+        pair = phantom.make_pair(dims=(256, 256, 128), spacing=(1.0, 1.0, 1.0), n_ellipsoids=60,
+                                 seed=2, model="affine")
+    b, bm_, s, sm_ = pair.moving.data, None, pair.fixed.data, None
+    if case == "masked":
+        b, bm_, s, sm_ = _resampled_with_masks(pair, oracle)
+    g = oracle.geom_of(pair.moving)
+    ref = oracle.bm_get_offsets(b, bm_, s, sm_, g, bw=4, bg=1, nw=9, ng=3, nthreads=THREADS)
+    for rescore in (0, 1):
+        bm = _matcher(nw=9, ng=3)
+        bm.SetExactRescore(rescore)
+        bm.SetBaseImage(Volume(b, pair.moving.spacing, pair.moving.origin))
+        bm.SetSearchImage(Volume(s, pair.moving.spacing, pair.moving.origin))
+        if bm_ is not None:
+            bm.SetBaseMask(bm_)
+            bm.SetSearchMask(sm_)
+        got = bm.get_raw()
+        err = max(np.abs(got["best"].astype(np.float64) - ref["best"]).max(),
+                  np.abs(got["second"].astype(np.float64) - ref["second"]).max())
+        kg, kr = _kept(got["best"], got["second"]), _kept(ref["best"], ref["second"])
+        print(f"{case} re-scoring {rescore}: max |dNCC^2| {err:.2e}; kept gpu {kg.sum()} "
+              f"oracle {kr.sum()} differing {(kg != kr).sum()}")
+        if rescore == 0:
+            assert err < 2.5e-5
+        else:
+            np.testing.assert_array_equal(kg, kr)
+            np.testing.assert_array_equal(got["match_idx"][kr], ref["match_idx"][kr])
