@@ -343,6 +343,19 @@ mirorr_status mirorr_lts_fit(mirorr_ctx* ctx, int symmetric, int affine, const d
                              const double* y, size_t n, double* params, double* center,
                              double* fwd34, double* bwd34, int* iterations);
 
+/* mirorr_lts_fit without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, semantics and results as mirorr_lts_fit; every device
+ * buffer lives in one file-scope device array that each GPU holds until the
+ * process ends (about 66 MB), sized for at most 1048576 pairs: larger n is
+ * MIRORR_ERR_ARG. All calls on one GPU share that GPU's array, so calls that
+ * overlap on one GPU (from several contexts or threads) race and must be
+ * serialised by the caller. */
+mirorr_status mirorr_lts_fit_noalloc(mirorr_ctx* ctx, int symmetric, int affine, const double* x,
+                                     const double* y, size_t n, double* params, double* center,
+                                     double* fwd34, double* bwd34, int* iterations);
+
 /* ResampleImageFilter with LinearInterpolateImageFunction as the registration
  * uses it: out(p) = in(matrix * p + offset), 0 outside; out_mask (nullable)
  * is 255 where the point falls inside the input grid. Host arrays. */

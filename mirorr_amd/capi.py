@@ -133,6 +133,8 @@ _SIGS = {
                                   C.POINTER(RegStats)]),
     "mirorr_lts_fit": (C.c_int, [_P, C.c_int, C.c_int, _P, _P, C.c_size_t, _P, _P, _P, _P,
                                  C.POINTER(C.c_int)]),
+    "mirorr_lts_fit_noalloc": (C.c_int, [_P, C.c_int, C.c_int, _P, _P, C.c_size_t, _P, _P, _P,
+                                         _P, C.POINTER(C.c_int)]),
     "mirorr_resample": (C.c_int, [_P, _P, C.POINTER(Geom), _P, _P, C.POINTER(Geom), _P, _P]),
     "mirorr_pyramid_level": (C.c_int, [_P, _P, C.POINTER(Geom), _P, C.c_int, C.c_int, _P,
                                        C.POINTER(Geom)]),

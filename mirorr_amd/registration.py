@@ -150,9 +150,12 @@ class MirorrPyramid:
         return out[:n], oc
 
 
-def lts_fit(symmetric: bool, affine: bool, X, Y, params=None, center=None, ctx=None):
+def lts_fit(symmetric: bool, affine: bool, X, Y, params=None, center=None, ctx=None,
+            noalloc=False):
     """OptimizeTransform on given pairs (device LTS). Classic: composes into
-    (params, center); symmetric: returns the half-space transforms."""
+    (params, center); symmetric: returns the half-space transforms. With
+    noalloc, the variant that allocates no device memory
+    (mirorr_lts_fit_noalloc, at most 1048576 pairs)."""
     ctx = ctx or default_context()
     X = np.ascontiguousarray(X, np.float64)
     Y = np.ascontiguousarray(Y, np.float64)
@@ -166,9 +169,9 @@ def lts_fit(symmetric: bool, affine: bool, X, Y, params=None, center=None, ctx=N
     f = np.zeros((3, 4))
     b = np.zeros((3, 4))
     it = C.c_int()
-    capi.check(capi.lib().mirorr_lts_fit(ctx.handle, int(symmetric), int(affine), capi.ptr(X),
-                                         capi.ptr(Y), X.shape[0], capi.ptr(p), capi.ptr(c),
-                                         capi.ptr(f), capi.ptr(b), C.byref(it)), ctx.handle)
+    fit = capi.lib().mirorr_lts_fit_noalloc if noalloc else capi.lib().mirorr_lts_fit
+    capi.check(fit(ctx.handle, int(symmetric), int(affine), capi.ptr(X), capi.ptr(Y), X.shape[0],
+                   capi.ptr(p), capi.ptr(c), capi.ptr(f), capi.ptr(b), C.byref(it)), ctx.handle)
     return p[:n], c, f, b, it.value
 
 
