@@ -313,6 +313,10 @@ typedef struct mirorr_server_info {
   size_t hbm_used;        /* bytes held by the workers' contexts now         */
   size_t hbm_peak;
   int max_running;        /* most requests running at once so far            */
+  size_t reserve_peak;    /* largest sum, at an admission, of what the slots
+                             were allowed to hold (running: kept workspace
+                             plus the request's bound; idle: kept workspace);
+                             admission keeps it within hbm_budget            */
 } mirorr_server_info;
 
 mirorr_status mirorr_server_create(int device, size_t hbm_budget, int slots,
@@ -330,8 +334,10 @@ mirorr_status mirorr_server_info_get(mirorr_server* s, mirorr_server_info* out);
 /* Message of the last failed mirorr_server_create / mirorr_submit /
  * mirorr_poll call on this host thread. */
 const char* mirorr_server_last_error(void);
-/* Drops requests not yet admitted, waits for the running ones, then frees
- * the workers. */
+/* Completes the requests not yet admitted with MIRORR_ERR_STATE ("server
+ * shut down"), waits for the running ones and for threads inside
+ * mirorr_poll to return, then frees the workers. No call on the server may
+ * start once this has been called. */
 void mirorr_server_destroy(mirorr_server* s);
 
 /* OptimizeTransform on given pairs (x_i block centres, y_i matches, n x 3 mm):

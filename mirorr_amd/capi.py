@@ -96,7 +96,8 @@ class RequestResult(C.Structure):
 class ServerInfo(C.Structure):
     _fields_ = [("slots", C.c_int), ("pending", C.c_int), ("running", C.c_int),
                 ("completed", C.c_int64), ("hbm_budget", C.c_size_t), ("hbm_used", C.c_size_t),
-                ("hbm_peak", C.c_size_t), ("max_running", C.c_int)]
+                ("hbm_peak", C.c_size_t), ("max_running", C.c_int),
+                ("reserve_peak", C.c_size_t)]
 
 
 _P = C.c_void_p

@@ -229,6 +229,31 @@ void read_exact(gzFile f, void* dst, size_t n, const char* what) {
   }
 }
 
+// Reads n bytes into a vector that grows by chunks as data arrives, so a
+// header announcing more data than the file holds fails at the end of the
+// file instead of allocating the announced size first.
+std::vector<unsigned char> read_growing(gzFile f, size_t n, const char* what) {
+  constexpr size_t kChunk = size_t(64) << 20;
+  std::vector<unsigned char> out;
+  while (out.size() < n) {
+    const size_t k = std::min(n - out.size(), kChunk);
+    const size_t at = out.size();
+    out.resize(at + k);
+    read_exact(f, out.data() + at, k, what);
+  }
+  return out;
+}
+
+// Skips n bytes (the header extensions) without holding them.
+void skip_bytes(gzFile f, size_t n, const char* what) {
+  unsigned char buf[4096];
+  while (n > 0) {
+    const size_t k = std::min(n, sizeof(buf));
+    read_exact(f, buf, k, what);
+    n -= k;
+  }
+}
+
 // Reads the voxels as double (scaled), plus the geometry.
 std::vector<double> read_voxels(const char* path, mirorr_geom* geom) {
   if (!path) fail("null path");
@@ -247,12 +272,14 @@ std::vector<double> read_voxels(const char* path, mirorr_geom* geom) {
     case 64: bytes = 8; break;                      // float64
     default: fail("unsupported NIfTI datatype " + std::to_string(h.datatype));
   }
-  const long skip = (long)h.vox_offset - kHdrSize;
-  if (skip < 0) fail("NIfTI vox_offset inside the header");
-  std::vector<unsigned char> pad((size_t)skip);
-  if (skip > 0) read_exact(gz.f, pad.data(), (size_t)skip, "extensions");
-  std::vector<unsigned char> raw(n * (size_t)bytes);
-  read_exact(gz.f, raw.data(), raw.size(), "voxels");
+  // vox_offset is a float in the header: reject NaN, infinities and
+  // offsets past any real extension block before converting it.
+  if (!std::isfinite(h.vox_offset) || h.vox_offset < (double)kHdrSize ||
+      h.vox_offset > double(1u << 30))
+    fail("NIfTI vox_offset out of range");
+  const size_t skip = (size_t)h.vox_offset - kHdrSize;
+  skip_bytes(gz.f, skip, "extensions");
+  const std::vector<unsigned char> raw = read_growing(gz.f, n * (size_t)bytes, "voxels");
   std::vector<double> v(n);
   const unsigned char* r = raw.data();
   for (size_t i = 0; i < n; ++i) {

@@ -567,9 +567,19 @@ void run_registration(mirorr_ctx* ctx, const mirorr_reg_params* p, const mirorr_
       const bool fork =
           run.symmetric && !(portion < 1.0) && !ctx->prof.on() && ctx->overlap_passes;
       hipStream_t s_b = s;
+      // If anything throws between the fork and the join, the second pass's
+      // kernels may still be in flight on the second stream: wait for them
+      // before the buffers they use can be freed or rewritten.
+      struct JoinOnUnwind {
+        hipStream_t s = nullptr;
+        ~JoinOnUnwind() {
+          if (s) (void)hipStreamSynchronize(s);
+        }
+      } join_guard;
       if (fork) {
         s_b = run.second_stream();
         MIRORR_HIP(hipEventRecord(ctx->ev_fork, s));
+        join_guard.s = s_b;
         MIRORR_HIP(hipStreamWaitEvent(s_b, ctx->ev_fork, 0));
       }
       if (tiled) {
@@ -594,6 +604,7 @@ void run_registration(mirorr_ctx* ctx, const mirorr_reg_params* p, const mirorr_
       if (fork) {
         MIRORR_HIP(hipEventRecord(ctx->ev_join, s_b));
         MIRORR_HIP(hipStreamWaitEvent(s, ctx->ev_join, 0));
+        join_guard.s = nullptr;  // the first stream now orders everything after the join
       }
       // 3. LTS and transform update on the device
       run.run_lts(st, outer, npts, flags);  // leaves flags->stop read back

@@ -84,6 +84,8 @@ struct mirorr_server {
   std::mutex mu;
   std::condition_variable cv_work;  // slots wait for a request
   std::condition_variable cv_done;  // pollers wait for results
+  std::condition_variable cv_idle;  // destroy waits for the pollers to leave
+  int pollers = 0;                  // threads inside mirorr_poll
   std::deque<std::unique_ptr<Request>> pending;
   std::map<int64_t, mirorr_request_result> done;
   std::map<int64_t, bool> live;  // tickets submitted and not yet handed out
@@ -91,6 +93,7 @@ struct mirorr_server {
   int64_t next_ticket = 1;
   int64_t completed = 0;
   int max_running = 0;
+  size_t reserve_peak = 0;  // largest sum of the slots' reservations at an admission
   bool stopping = false;
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
 
@@ -129,21 +132,25 @@ struct mirorr_server {
       for (const Slot& x : slots) sum += held(x);
       return sum + r.bound <= limit;  // the chosen slot: its kept workspace plus the bound
     };
-    bool released = false;
-    auto release_idle = [&] {
-      // Release the workspaces of idle slots (on this server's device; the
-      // calling thread's device is restored).
-      if (released) return;
-      released = true;
+    // Releases the workspace of the idle slot keeping the largest one (on
+    // this server's device; the calling thread's device is restored).
+    // Returns false when no idle slot keeps a workspace.
+    auto release_one_idle = [&] {
+      Slot* big = nullptr;
+      for (Slot& x : slots)
+        if (!x.busy && !x.job && x.ctx->reg &&
+            (!big || x.ctx->budget.used.load() > big->ctx->budget.used.load()))
+          big = &x;
+      if (!big) return false;
       int prev = device;
       (void)hipGetDevice(&prev);
       (void)hipSetDevice(device);
-      for (Slot& x : slots)
-        if (!x.busy && !x.job && x.ctx->reg) {
-          std::lock_guard<std::mutex> lk(x.ctx->mu);
-          x.ctx->reg.reset();
-        }
+      {
+        std::lock_guard<std::mutex> lk(big->ctx->mu);
+        big->ctx->reg.reset();
+      }
       (void)hipSetDevice(prev);
+      return true;
     };
     std::vector<Request*> waiting;  // requests passed over in this scan
     for (auto it = pending.begin(); it != pending.end() && !stopping;) {
@@ -156,7 +163,10 @@ struct mirorr_server {
         }
       if (w < 0) return;
       Request& r = **it;
-      if (!fits(r)) release_idle();
+      // Idle workspaces are released one at a time, largest first, only
+      // until the request fits.
+      while (!fits(r) && release_one_idle()) {
+      }
       if (!fits(r)) {
         if (!running() && waiting.empty()) {
           const std::string msg = "request needs up to " + std::to_string(r.bound) +
@@ -174,13 +184,22 @@ struct mirorr_server {
       }
       for (Request* q : waiting) ++q->overtaken;
       Slot& s = slots[w];
+      // The slot keeps its workspace and each buffer only grows, so while
+      // the request runs the slot holds at most its kept bytes plus the
+      // request's bound (the sum of per-buffer maxima can exceed the larger
+      // of the two).
       s.reserve = s.ctx->budget.used.load() + r.bound;
       s.busy = true;
       s.job = std::move(*it);
       it = pending.erase(it);
       int n = 0;
-      for (const Slot& x : slots) n += x.busy ? 1 : 0;
+      size_t sum = 0;
+      for (const Slot& x : slots) {
+        n += x.busy ? 1 : 0;
+        sum += held(x);
+      }
       if (n > max_running) max_running = n;
+      if (sum > reserve_peak) reserve_peak = sum;
       cv_work.notify_all();
     }
   }
@@ -352,7 +371,17 @@ mirorr_status mirorr_poll(mirorr_server* S, int64_t ticket, double timeout_s,
     g_server_error = "unknown ticket " + std::to_string(ticket);
     return MIRORR_ERR_ARG;
   }
-  auto ready = [&] { return S->done.count(ticket) != 0; };
+  // Counted while inside, so that mirorr_server_destroy frees nothing a
+  // poller still uses; the wait also ends when another poll of the same
+  // ticket has collected it.
+  ++S->pollers;
+  struct Leave {
+    mirorr_server* S;
+    ~Leave() {
+      if (--S->pollers == 0) S->cv_idle.notify_all();
+    }
+  } leave{S};
+  auto ready = [&] { return S->done.count(ticket) != 0 || S->live.count(ticket) == 0; };
   if (!ready()) {
     if (timeout_s == 0.0) return MIRORR_PENDING;
     if (timeout_s < 0.0 || !(timeout_s < 1e6)) {  // negative, NaN or past any deadline: wait
@@ -361,9 +390,14 @@ mirorr_status mirorr_poll(mirorr_server* S, int64_t ticket, double timeout_s,
       return MIRORR_PENDING;
     }
   }
+  if (!S->done.count(ticket)) {
+    g_server_error = "ticket " + std::to_string(ticket) + " was collected by another poll";
+    return MIRORR_ERR_ARG;
+  }
   *out = S->done[ticket];
   S->done.erase(ticket);
   S->live.erase(ticket);
+  S->cv_done.notify_all();  // a concurrent poll of this ticket stops waiting
   return MIRORR_OK;
 }
 
@@ -379,20 +413,34 @@ mirorr_status mirorr_server_info_get(mirorr_server* S, mirorr_server_info* out) 
   out->hbm_used = S->budget.used.load();
   out->hbm_peak = S->budget.peak.load();
   out->max_running = S->max_running;
+  out->reserve_peak = S->reserve_peak;
   return MIRORR_OK;
 }
 
 void mirorr_server_destroy(mirorr_server* S) {
   if (!S) return;
   {
+    // Requests not yet admitted complete with MIRORR_ERR_STATE, so that
+    // their pollers return.
     std::lock_guard<std::mutex> lk(S->mu);
     S->stopping = true;
-    S->pending.clear();
+    while (!S->pending.empty()) {
+      std::unique_ptr<Request> r = std::move(S->pending.front());
+      S->pending.pop_front();
+      S->finish_unadmitted(std::move(r), MIRORR_ERR_STATE, "server shut down");
+    }
     S->cv_work.notify_all();
   }
   // Workers finish the request they were handed, then exit.
   for (Slot& s : S->slots)
     if (s.th.joinable()) s.th.join();
+  {
+    // Every live ticket now has its result; wait for the pollers inside
+    // mirorr_poll to leave before freeing what they use.
+    std::unique_lock<std::mutex> lk(S->mu);
+    S->cv_done.notify_all();
+    S->cv_idle.wait(lk, [&] { return S->pollers == 0; });
+  }
   for (Slot& s : S->slots) mirorr_ctx_destroy(s.ctx);
   delete S;
 }

@@ -142,6 +142,24 @@ def test_read_rejects_4d_and_garbage(tmp_path):
         fileio.read_nifti(str(tmp_path / "missing.nii"))
 
 
+@pytest.mark.parametrize("offset", [float("nan"), float("inf"), 3e9, 100.0])
+def test_read_rejects_bad_vox_offset(tmp_path, offset):
+    blob = bytearray(nifti_bytes(np.zeros((2, 2, 2), np.float32)))
+    struct.pack_into("<f", blob, 108, offset)
+    with pytest.raises(capi.MirorrError, match="vox_offset"):
+        fileio.read_nifti(write(tmp_path, "off.nii", bytes(blob)))
+
+
+@pytest.mark.parametrize("name", ["big.nii", "big.nii.gz"])
+def test_read_huge_header_on_a_short_file_fails_without_allocating(tmp_path, name):
+    # 32767^3 float64 voxels (280 TB) announced, 8 voxels present: the read
+    # fails at the end of the file.
+    blob = bytearray(nifti_bytes(np.zeros((2, 2, 2), np.float64), datatype=64))
+    struct.pack_into("<3h", blob, 42, 32767, 32767, 32767)
+    with pytest.raises(capi.MirorrError, match="truncated"):
+        fileio.read_nifti(write(tmp_path, name, bytes(blob)))
+
+
 # ---- NIfTI writing -------------------------------------------------------------
 
 
