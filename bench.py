@@ -10,8 +10,10 @@ a multimodal contrast remap; symmetric mode, affine model, blocks 4^3, search
 level. Prints one JSON line: NCC voxel-evals/s over the timed registrations,
 with registrations/s and p50/p99 latency beside it, the roofline of the NCC
 search kernel (HIP events on the library stream, taken on further untimed
-registrations) and a CPU baseline (the oracle's threaded block matcher on
-one full-resolution pass).
+registrations), a CPU baseline (the oracle's threaded block matcher on one
+full-resolution pass, on the host cores and on one core) and the oracle's
+own registration of the pair, against which the final transform is
+compared.
 """
 from __future__ import annotations
 
@@ -114,6 +116,45 @@ def measured_traffic():
         return float(m.group(1)) if m else None
     except OSError:
         return None
+
+
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo, as lscpu prints it)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def oracle_transform_check(pair, cfg, params, center, iterations, points, threads):
+    """The oracle's registration of the same pair with the same parameters
+    (threaded CPU restatement of the reference), and the distance between its
+    final transform and the GPU's over the field-of-view corners."""
+    from mirorr_amd import transforms
+    from oracle import oracle as orc
+    if not os.path.exists(orc._LIB_PATH):
+        orc.build()
+    affine = cfg["model"] == "affine"
+    rp = orc.reg_params(symmetric=1, affine=int(affine), nw=cfg["nhood_width"],
+                        ng=cfg["nhood_gap"], pyr_num=cfg["pyramid_levels"],
+                        iterations=cfg["iterations_per_level"], nthreads=threads)
+    t = time.perf_counter()
+    ref_p, ref_c, ref_st = orc.register(pair.fixed, pair.moving, rp)
+    dt = time.perf_counter() - t
+    M1, o1 = transforms.matrix_offset(params, center, affine)
+    M2, o2 = transforms.matrix_offset(ref_p, ref_c, affine)
+    return {
+        "transform_vs_oracle_mm": transforms.max_corner_distance(M1, o1, M2, o2, pair.fixed),
+        "rotation_vs_oracle_deg": transforms.rotation_angle_deg(M1, M2),
+        "same_iterations_and_points": bool(list(iterations) == list(ref_st["iterations"])
+                                           and list(points) == list(ref_st["points"])),
+        "oracle_seconds": dt,
+        "oracle_threads": threads,
+    }
 
 
 def cpu_baseline(pair, cfg, threads, max_evals=20e9):
@@ -423,6 +464,7 @@ def main():
     from mirorr_amd import transforms
     M, off = transforms.matrix_offset(params, center, cfg["model"] == "affine")
     err_mm = transforms.max_corner_distance(M, off, pair.matrix, pair.translation, pair.fixed)
+    iterations, points = list(reg.stats["iterations"]), list(reg.stats["points"])
 
     avg_ms = ms_fast / max(n_fast, 1)
     achieved = FLOP_PER_VOXEL_EVAL * (work_fast / max(n_fast, 1)) / (avg_ms * 1e-3) / 1e12
@@ -445,11 +487,15 @@ def main():
         "latency_ms_p50": float(np.percentile(lat_ms, 50)),
         "latency_ms_p99": float(np.percentile(lat_ms, 99)),
         "voxel_evals_per_registration": evals / args.steps,
+        "value_note": "voxel-evals by the formula count of every block's candidate windows "
+                      "x 64, including blocks whose base mask is off everywhere (not searched)",
         "transform_error_mm": err_mm,
         "upload_ms": upload_ms,
         "roofline": {
-            "bound": "mfma",
-            "compute": "fp32 vector ALU (the search uses no MFMA)",
+            "bound": "valu",
+            "compute": "fp32 vector ALU FMAs (the search uses no MFMA); packed FP32 is "
+                       "not used, so the issue-rate ceiling of the kernel's FMAs is half the "
+                       "published peak",
             "kernel": "ncc_tile_fast",
             "launch_scope": "full-resolution level",
             "achieved": achieved,
@@ -469,8 +515,16 @@ def main():
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
+        # The host cores a one-GPU job may use (16 on the MI355X boxes, whatever
+        # nproc reports), then one core on a smaller sample.
         threads = min(16, len(os.sched_getaffinity(0)))
         out["cpu_baseline"] = cpu_baseline(pair, cfg, threads)
+        out["cpu_baseline"]["cpu_model"] = cpu_model()
+        single = cpu_baseline(pair, cfg, 1, max_evals=4e9)
+        out["cpu_baseline"]["single_core"] = {k: single[k] for k in ("value", "cores", "sample")}
+        print("bench: oracle registration for the transform check", file=sys.stderr, flush=True)
+        out["oracle_check"] = oracle_transform_check(pair, cfg, params, center, iterations,
+                                                     points, threads)
     print(json.dumps(out), flush=True)
 
 

@@ -1708,77 +1708,83 @@ __device__ __forceinline__ int win_index(int w0, int v, int ex, int ey) {
 constexpr float kU = 5.9604645e-8f;          // 2^-24
 constexpr float kGamma = 64.0f * kU * 1.01f;  // error factor of a 64-term float sum
 
-// Phase 1 when every base and window voxel is on: one thread per candidate
-// row (cz, cy) scores its NW candidates from shared row loads, with
-// Sxy = Sum x~ t and Syy = Sum t^2 in float (t = y - mu, mu the window mean
-// of the statistics) and Sxx = the block's double sum (the same for every
-// candidate).
+// Phase 1 when every base and window voxel is on: one task per run of up to
+// kLeanRun candidates of a candidate row (cz, cy), scored from shared row
+// loads, with Sxy = Sum x~ t and Syy = Sum t^2 in float (t = y - mu, mu the
+// window mean of the statistics) and Sxx = the block's double sum (the same
+// for every candidate). Each candidate's sums are taken in the same order
+// whichever task holds it.
+constexpr int kLeanRun = 3;
 template <int NW>
 __device__ void lean_phase1(const RescoreArgs& ra, const BMGrid& g, const int first[3],
                             const int cnt[3], int ex, int ey, double bss, float bx, float sx,
                             RescoreLds& L, float* l1, float* l2, float* h1, float* h2) {
+  constexpr int C = kLeanRun;
+  constexpr int NR = (NW + C - 1) / C;  // tasks per candidate row
   const int nx = g.dims[0], ny = g.dims[1];
   const int rows = cnt[1] * cnt[2];
-  const int r = threadIdx.x;
-  if (r >= rows) return;
-  const int ky = r % cnt[1], kz = r / cnt[1];
-  float mu[NW], syy[NW], sxy[NW], A[NW];
-  const float2* srow = ra.stats + lin3(nx, ny, first[0], first[1] + ky, first[2] + kz);
-#pragma unroll
-  for (int cx = 0; cx < NW; ++cx) {
-    mu[cx] = cx < cnt[0] ? srow[cx].x : 0.0f;
-    syy[cx] = 0.0f;
-    sxy[cx] = 0.0f;
-    A[cx] = 0.0f;
-  }
-#pragma unroll 1
-  for (int q = 0; q < 16; ++q) {  // block row (iz, iy) = (q / 4, q % 4)
-    const float* wr = L.win + ((kz + (q >> 2)) * ey + ky + (q & 3)) * ex;
-    float w[NW + 3];
-#pragma unroll
-    for (int i = 0; i < NW + 3; ++i) w[i] = i < ex ? wr[i] : 0.0f;
-    float xr[4];
-#pragma unroll
-    for (int ix = 0; ix < 4; ++ix) xr[ix] = L.xm[4 * q + ix];
-#pragma unroll
-    for (int cx = 0; cx < NW; ++cx)
-#pragma unroll
-      for (int ix = 0; ix < 4; ++ix) {
-        const float t = w[cx + ix] - mu[cx];
-        sxy[cx] = fmaf(xr[ix], t, sxy[cx]);
-        syy[cx] = fmaf(t, t, syy[cx]);
-        A[cx] = fmaf(fabsf(xr[ix]), fabsf(t), A[cx]);
-      }
-  }
   const float isxx = (float)(1.0 / bss);
+  for (int task = threadIdx.x; task < rows * NR; task += kThreads) {
+    const int r = task / NR, c0 = (task % NR) * C;
+    if (c0 >= cnt[0]) continue;
+    const int ky = r % cnt[1], kz = r / cnt[1];
+    float mu[C], syy[C], sxy[C], A[C];
+    const float2* srow = ra.stats + lin3(nx, ny, first[0] + c0, first[1] + ky, first[2] + kz);
 #pragma unroll
-  for (int cx = 0; cx < NW; ++cx) {
-    if (cx >= cnt[0]) continue;
-    // Error bounds as in the general phase 1; with every voxel on, the
-    // exact centred values sum to zero over the block and over the window,
-    // so the error e of the float mean (|e| <= 2u |mu|) enters Sxy only
-    // through the float block sum (|Sum x~| <= |sx| + gamma bx) and Syy only
-    // as 64 e^2.
-    const float amu = fabsf(mu[cx]);
-    const float d = kGamma * A[cx] + 3.0f * kU * A[cx] + 2.0f * kU * amu * (fabsf(sx) + kGamma * bx);
-    const float dyy = kGamma * syy[cx] + 5.0f * kU * syy[cx] +
-                      256.0f * (kU * amu) * (kU * amu);
-    float lo, hi;
-    if (syy[cx] - dyy > 0.0f) {
-      const float smax = fabsf(sxy[cx]) + d, smin = fmaxf(fabsf(sxy[cx]) - d, 0.0f);
-      hi = smax * smax * isxx / (syy[cx] - dyy) * (1.0f + 16.0f * kU) + 1e-9f;
-      lo = smin * smin * isxx / (syy[cx] + dyy) * (1.0f - 16.0f * kU) - 1e-9f;
-    } else {
-      lo = -2.0f;  // the window sum may be zero: scored in double
-      hi = __builtin_huge_valf();
+    for (int cx = 0; cx < C; ++cx) {
+      mu[cx] = c0 + cx < cnt[0] ? srow[cx].x : 0.0f;
+      syy[cx] = 0.0f;
+      sxy[cx] = 0.0f;
+      A[cx] = 0.0f;
     }
-    L.iv[r * cnt[0] + cx] = make_float2(lo, hi);
-    const float nl1 = fmaxf(*l1, lo);
-    *l2 = fmaxf(fminf(*l1, lo), *l2);
-    *l1 = nl1;
-    const float nh1 = fmaxf(*h1, hi);
-    *h2 = fmaxf(fminf(*h1, hi), *h2);
-    *h1 = nh1;
+#pragma unroll 4
+    for (int q = 0; q < 16; ++q) {  // block row (iz, iy) = (q / 4, q % 4)
+      const float* wr = L.win + ((kz + (q >> 2)) * ey + ky + (q & 3)) * ex + c0;
+      float w[C + 3];
+#pragma unroll
+      for (int i = 0; i < C + 3; ++i) w[i] = c0 + i < ex ? wr[i] : 0.0f;
+      float xr[4];
+#pragma unroll
+      for (int ix = 0; ix < 4; ++ix) xr[ix] = L.xm[4 * q + ix];
+#pragma unroll
+      for (int cx = 0; cx < C; ++cx)
+#pragma unroll
+        for (int ix = 0; ix < 4; ++ix) {
+          const float t = w[cx + ix] - mu[cx];
+          sxy[cx] = fmaf(xr[ix], t, sxy[cx]);
+          syy[cx] = fmaf(t, t, syy[cx]);
+          A[cx] = fmaf(fabsf(xr[ix]), fabsf(t), A[cx]);
+        }
+    }
+#pragma unroll
+    for (int cx = 0; cx < C; ++cx) {
+      if (c0 + cx >= cnt[0]) continue;
+      // Error bounds as in the general phase 1; with every voxel on, the
+      // exact centred values sum to zero over the block and over the
+      // window, so the error e of the float mean (|e| <= 2u |mu|) enters
+      // Sxy only through the float block sum (|Sum x~| <= |sx| + gamma bx)
+      // and Syy only as 64 e^2.
+      const float amu = fabsf(mu[cx]);
+      const float d = kGamma * A[cx] + 3.0f * kU * A[cx] + 2.0f * kU * amu * (fabsf(sx) + kGamma * bx);
+      const float dyy = kGamma * syy[cx] + 5.0f * kU * syy[cx] +
+                        256.0f * (kU * amu) * (kU * amu);
+      float lo, hi;
+      if (syy[cx] - dyy > 0.0f) {
+        const float smax = fabsf(sxy[cx]) + d, smin = fmaxf(fabsf(sxy[cx]) - d, 0.0f);
+        hi = smax * smax * isxx / (syy[cx] - dyy) * (1.0f + 16.0f * kU) + 1e-9f;
+        lo = smin * smin * isxx / (syy[cx] + dyy) * (1.0f - 16.0f * kU) - 1e-9f;
+      } else {
+        lo = -2.0f;  // the window sum may be zero: scored in double
+        hi = __builtin_huge_valf();
+      }
+      L.iv[r * cnt[0] + c0 + cx] = make_float2(lo, hi);
+      const float nl1 = fmaxf(*l1, lo);
+      *l2 = fmaxf(fminf(*l1, lo), *l2);
+      *l1 = nl1;
+      const float nh1 = fmaxf(*h1, hi);
+      *h2 = fmaxf(fminf(*h1, hi), *h2);
+      *h1 = nh1;
+    }
   }
 }
 
@@ -1793,13 +1799,30 @@ __device__ void rescore_block(const RescoreArgs& ra, const BMGrid& g, int64_t b,
   const int ex = cnt[0] + 3, ey = cnt[1] + 3, ez = cnt[2] + 3;
   if (threadIdx.x == 0) L.anyoff = 0;
   __syncthreads();
-  for (int i = threadIdx.x; i < ex * ey * ez; i += kThreads) {
-    const int wx = i % ex, wy = (i / ex) % ey, wz = i / (ex * ey);
-    const int64_t o = lin3(nx, ny, first[0] + wx, first[1] + wy, first[2] + wz);
-    L.win[i] = ra.y[o];
-    const bool on = ra.sm == nullptr || ra.sm[o] != 0;
-    L.won[i] = on ? 1 : 0;
-    if (!on) L.anyoff = 1;
+  // The union of the candidate windows, kStageBatch voxels per thread with
+  // their loads in flight together.
+  constexpr int kStageBatch = 8;
+  const int nwin = ex * ey * ez;
+  for (int i0 = threadIdx.x; i0 < nwin; i0 += kStageBatch * kThreads) {
+    float yv[kStageBatch];
+    uint8_t mv[kStageBatch];
+#pragma unroll
+    for (int u = 0; u < kStageBatch; ++u) {
+      const int i = min(i0 + u * kThreads, nwin - 1);
+      const int wx = i % ex, wy = (i / ex) % ey, wz = i / (ex * ey);
+      const int64_t o = lin3(nx, ny, first[0] + wx, first[1] + wy, first[2] + wz);
+      yv[u] = ra.y[o];
+      mv[u] = ra.sm == nullptr ? (uint8_t)1 : ra.sm[o];
+    }
+#pragma unroll
+    for (int u = 0; u < kStageBatch; ++u) {
+      const int i = i0 + u * kThreads;
+      if (i >= nwin) break;
+      L.win[i] = yv[u];
+      const bool on = mv[u] != 0;
+      L.won[i] = on ? 1 : 0;
+      if (!on) L.anyoff = 1;
+    }
   }
   if (threadIdx.x < 64) {
     const int k = threadIdx.x;
