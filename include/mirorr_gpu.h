@@ -76,6 +76,9 @@ typedef struct mirorr_bm_params {
                             operation order, so their records and the kept
                             set are the reference's; 0: float scores as
                             searched (for measuring their error)          */
+  int storage;           /* MIRORR_STORAGE_F32 (default) or _F16: how
+                            mirorr_bm_set_volume keeps the images on the
+                            device (see mirorr_volume_create_storage)      */
 } mirorr_bm_params;
 
 /* Block metrics (--blockmetric nc | sd | cr; BlockMatcher's metric map,
@@ -243,6 +246,31 @@ mirorr_status mirorr_volume_create(mirorr_ctx* ctx, const float* data, const uin
                                    const mirorr_geom* geom, mirorr_volume** out);
 void mirorr_volume_destroy(mirorr_volume* v);
 
+/* Volume storage: float (the reference's float image3d uploads,
+ * itkBlockMatcherGPU.txx:808-817), or binary16 holding (v - offset) * scale
+ * with offset the nearest integer to the middle of the finite range and
+ * scale a power of two (integer intensities of up to 12 bits are exact).
+ * A registration of two binary16 volumes keeps its pyramid levels and
+ * half-way images in binary16 and widens them to float where the search
+ * stages them (NCC^2 is unchanged by the affine encoding; only the
+ * rounding of interpolated values to binary16 changes the result). It
+ * needs the NCC metric and blocks of width 4, block gap 1, nhood width 5
+ * to 13. No counterpart in the reference: the image precision is its
+ * float throughout. */
+#define MIRORR_STORAGE_F32 0
+#define MIRORR_STORAGE_F16 1
+mirorr_status mirorr_volume_create_storage(mirorr_ctx* ctx, const float* data,
+                                           const uint8_t* mask, const mirorr_geom* geom,
+                                           int storage, mirorr_volume** out);
+/* The binary16 encoding mirorr_volume_create_storage applies (host only):
+ * out[i] holds the IEEE binary16 bits of (data[i] - offset) * scale,
+ * computed in float and rounded to nearest even. */
+mirorr_status mirorr_half_encode(const float* data, size_t n, uint16_t* out, double* offset,
+                                 double* scale);
+/* A volume's storage and, for binary16, its encoding. */
+mirorr_status mirorr_volume_info(const mirorr_volume* v, int* storage, double* offset,
+                                 double* scale);
+
 /* MirorrPyramidImplement::RunMirorr (itkMirorrPyramidImplement.txx:71-158):
  * pyramid build, level loop, resample / block-match / LTS iterations.
  * init_params/init_center give the start transform (NULL: the centred
@@ -289,6 +317,7 @@ typedef struct mirorr_request {
   const double* init_params;
   const double* init_center;
   int borrow;
+  int storage;            /* MIRORR_STORAGE_F32 (0) or _F16 for both volumes  */
 } mirorr_request;
 
 typedef struct mirorr_request_result {

@@ -151,6 +151,12 @@ class BlockMatcher:
     # thresholds (on by default); off leaves the float scores as searched.
     def SetExactRescore(self, on: bool): self._set("exact_rescore", 1 if on else 0)
 
+    # Device storage of images set afterwards: "f32" (default) or "f16"
+    # (binary16 with a per-image affine encoding; NCC, 4-voxel blocks).
+    def SetStorage(self, name: str):
+        self._set("storage", capi.STORAGE_NAMES[name])
+        self._dirty = [True, True]  # re-uploaded in the new storage
+
     def SetBlockMetricType(self, name: str):
         """normalized_correlation (default), sum_of_squared_differences or
         correlation_ratio (BlockMatcher::SetBlockMetricType, itkBlockMatcher.h:143-175)."""

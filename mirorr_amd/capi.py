@@ -16,6 +16,9 @@ ROLE_BASE, ROLE_SEARCH = 0, 1
 INTERP_NN, INTERP_LINEAR, INTERP_BSPLINE, INTERP_SINC = range(4)
 METRIC_NC, METRIC_SSD, METRIC_CR = range(3)
 RESAMPLE_MIDDLE, RESAMPLE_BASIC, RESAMPLE_MOVING, RESAMPLE_FIXED = range(4)
+STORAGE_F32, STORAGE_F16 = 0, 1
+STORAGE_NAMES = {"f32": STORAGE_F32, "float32": STORAGE_F32, "f16": STORAGE_F16,
+                 "float16": STORAGE_F16}
 RESAMPLING_NAMES = {"middle": RESAMPLE_MIDDLE, "basic": RESAMPLE_BASIC,
                     "moving": RESAMPLE_MOVING, "fixed": RESAMPLE_FIXED}
 # BlockMatcher's metric names (itkBlockMatcher.h:316-323) and the CLI's short ones.
@@ -59,7 +62,7 @@ class BMParams(C.Structure):
     _fields_ = [("block_width", C.c_int), ("block_gap", C.c_int), ("nhood_width", C.c_int),
                 ("nhood_gap", C.c_int), ("portion_kept", C.c_double),
                 ("variance_select", C.c_int), ("verbosity", C.c_int),
-                ("block_metric", C.c_int), ("exact_rescore", C.c_int)]
+                ("block_metric", C.c_int), ("exact_rescore", C.c_int), ("storage", C.c_int)]
 
 
 class RegParams(C.Structure):
@@ -83,7 +86,7 @@ class Request(C.Structure):
     _fields_ = [("params", RegParams), ("fixed", C.c_void_p), ("fixed_mask", C.c_void_p),
                 ("fixed_geom", Geom), ("moving", C.c_void_p), ("moving_mask", C.c_void_p),
                 ("moving_geom", Geom), ("init_params", C.c_void_p), ("init_center", C.c_void_p),
-                ("borrow", C.c_int)]
+                ("borrow", C.c_int), ("storage", C.c_int)]
 
 
 class RequestResult(C.Structure):
@@ -129,6 +132,12 @@ _SIGS = {
     "mirorr_bm_voxel_evals": (C.c_int, [_P, C.POINTER(C.c_int64)]),
     "mirorr_reg_default_params": (None, [C.POINTER(RegParams)]),
     "mirorr_volume_create": (C.c_int, [_P, _P, _P, C.POINTER(Geom), C.POINTER(_P)]),
+    "mirorr_volume_create_storage": (C.c_int, [_P, _P, _P, C.POINTER(Geom), C.c_int,
+                                               C.POINTER(_P)]),
+    "mirorr_volume_info": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double)]),
+    "mirorr_half_encode": (C.c_int, [_P, C.c_size_t, _P, C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double)]),
     "mirorr_volume_destroy": (None, [_P]),
     "mirorr_register": (C.c_int, [_P, C.POINTER(RegParams), _P, _P, _P, _P, _P, _P,
                                   C.POINTER(RegStats)]),

@@ -85,9 +85,9 @@ __global__ __launch_bounds__(kThreads) void search_stats_kernel(
 // gives, so the per-plane minimum and maximum detect that case.
 constexpr int kStatsZ = 16;  // z positions per thread
 
-template <bool MASK>
+template <bool MASK, class E>
 __global__ __launch_bounds__(kThreads) void search_stats4_kernel(
-    const float* __restrict__ y, const uint8_t* __restrict__ sm, int nx, int ny, int nz,
+    const E* __restrict__ y, const uint8_t* __restrict__ sm, int nx, int ny, int nz,
     float2* __restrict__ out) {
   const int x = blockIdx.x * 64 + (threadIdx.x & 63);
   const int yy = blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -96,20 +96,20 @@ __global__ __launch_bounds__(kThreads) void search_stats4_kernel(
   const int zend = min(z0 + kStatsZ, nz - 3);  // exclusive end of output z
   if (z0 >= zend) return;
   const int64_t nxy = (int64_t)nx * ny;
-  const float* ycol = y + lin3(nx, ny, x, yy, 0);  // the column's (x, y); + z * nxy
+  const E* ycol = y + lin3(nx, ny, x, yy, 0);  // the column's (x, y); + z * nxy
   const uint8_t* mcol = MASK ? sm + lin3(nx, ny, x, yy, 0) : nullptr;
-  const double c0 = (double)ycol[(int64_t)z0 * nxy];
+  const double c0 = (double)(float)ycol[(int64_t)z0 * nxy];
   double ps[4], p1[4], p2[4];
   int pn[4];
   float pmin[4], pmax[4];
   auto load = [&](int slot, int z) {
-    const float* yp = ycol + (int64_t)z * nxy;
+    const E* yp = ycol + (int64_t)z * nxy;
     float v[16];
     uint32_t mw[4] = {0x01010101u, 0x01010101u, 0x01010101u, 0x01010101u};
 #pragma unroll
     for (int dy = 0; dy < 4; ++dy) {
 #pragma unroll
-      for (int dx = 0; dx < 4; ++dx) v[4 * dy + dx] = yp[dy * nx + dx];
+      for (int dx = 0; dx < 4; ++dx) v[4 * dy + dx] = (float)yp[dy * nx + dx];
       if (MASK) {
         const uint8_t* mr = mcol + (int64_t)z * nxy + dy * nx;
         mw[dy] = (uint32_t)mr[0] | ((uint32_t)mr[1] << 8) | ((uint32_t)mr[2] << 16) |
@@ -244,8 +244,9 @@ __device__ __forceinline__ int quad_sum_i(int v) {
   return v;
 }
 
+template <class E>
 __global__ __launch_bounds__(kThreads) void block_prep4_kernel(
-    const float* __restrict__ x, const uint8_t* __restrict__ bm, BMGrid g,
+    const E* __restrict__ x, const uint8_t* __restrict__ bm, BMGrid g,
     float* __restrict__ xt, float* __restrict__ sxx, float* __restrict__ sumxt,
     uint32_t* __restrict__ flags) {
   const int t = blockIdx.x * kThreads + threadIdx.x;  // nblocks * 4 < 2^31
@@ -261,11 +262,11 @@ __global__ __launch_bounds__(kThreads) void block_prep4_kernel(
   double s = 0.0;
 #pragma unroll
   for (int dy = 0; dy < 4; ++dy) {
-    const float* row = x + lin3(g.dims[0], g.dims[1], px, py + dy, pz);
+    const E* row = x + lin3(g.dims[0], g.dims[1], px, py + dy, pz);
 #pragma unroll
     for (int dx = 0; dx < 4; ++dx) {
-      v[dy * 4 + dx] = row[dx];
-      s += (double)row[dx];
+      v[dy * 4 + dx] = (float)row[dx];
+      s += (double)v[dy * 4 + dx];
     }
   }
   const double mx = quad_sum(s) * (1.0 / 64.0);
@@ -424,7 +425,7 @@ constexpr size_t masked_static_lds() {
 }
 
 struct TileArgs {
-  const float* y;
+  const void* y;  // float or half_t elements (the kernels' E)
   const uint8_t* sm;
   const uint8_t* bm;
   BMGrid g;
@@ -540,12 +541,13 @@ __device__ __forceinline__ void write_skipped(const TileArgs& a, int64_t b, int 
 //    on sets bit wy of zrow[wz], and a window voxel that is outside the image
 //    or whose search mask is off sets bit wy of zbad[wz].
 //  * otherwise: a voxel whose search mask is off sets bit wy of zrow[wz].
-template <class T, bool MASKED>
+template <class T, bool MASKED, class E>
 __device__ __forceinline__ void stage_window(const TileArgs& a, float* win, uint8_t* wm,
                                              unsigned long long* zrow, int WY, int WZ, int wx0,
                                              int wy0, int wz0, float ref,
                                              unsigned long long* zbad = nullptr) {
   const int nx = a.g.dims[0], ny = a.g.dims[1], nz = a.g.dims[2];
+  const E* Y = static_cast<const E*>(a.y);
   const int items = T::NV * WY * WZ;
   const float inv_wy = 1.0f / (float)WY;
   for (int it = threadIdx.x; it < items; it += kThreads) {
@@ -565,7 +567,7 @@ __device__ __forceinline__ void stage_window(const TileArgs& a, float* win, uint
         // interior: the four loads issue together
         inside = 0xfu;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) y[k] = a.y[rb + gx0 + k];
+        for (int k = 0; k < 4; ++k) y[k] = (float)Y[rb + gx0 + k];
         on = inside;
         if (a.sm) {
           uint8_t m[4];
@@ -584,7 +586,7 @@ __device__ __forceinline__ void stage_window(const TileArgs& a, float* win, uint
           y[k] = 0.0f;
           if (4 * v + k < T::WX && gx >= 0 && gx < nx) {
             inside |= 1u << k;
-            y[k] = a.y[rb + gx];
+            y[k] = (float)Y[rb + gx];
             if (a.sm == nullptr || a.sm[rb + gx] != 0) on |= 1u << k;
           }
         }
@@ -696,7 +698,7 @@ __device__ __forceinline__ Top2 merge_result(const MergeSlot& m) {
 }
 
 // Fast path: all base and search masks of the tile are on.
-template <class T>
+template <class T, class E>
 __global__ __launch_bounds__(kThreads, 4) void ncc_tile_fast(TileArgs a) {
   extern __shared__ float4 dyn4[];
   float* win = reinterpret_cast<float*>(dyn4);
@@ -761,7 +763,7 @@ __global__ __launch_bounds__(kThreads, 4) void ncc_tile_fast(TileArgs a) {
   // rounding is large next to the Sxy of a low-contrast block). A
   // search-mask-off voxel sends the blocks whose own window holds it to the
   // masked kernel; the other blocks stay here.
-  stage_window<T, false>(a, win, nullptr, zrow, WY, WZ, wx0, wy0, wz0, 0.0f);
+  stage_window<T, false, E>(a, win, nullptr, zrow, WY, WZ, wx0, wy0, wz0, 0.0f);
   for (int i = tid; i < T::NB * 16; i += kThreads) {
     const int bl = i / 16;
     const int gy = by0 + bl % T::TY, gz = bz0 + bl / T::TY;
@@ -933,7 +935,7 @@ __device__ __forceinline__ void load_mask(const uint8_t* wv, int ro, int v, floa
 //    carry the same centring (one voxel on gives exactly 1).
 //  * Base block partly masked (base mask b): Sxy = Sum b x~ t,
 //    Syy = Sum b t^2, Sxx = Sum b x~^2 v (5 ops per voxel).
-template <class T>
+template <class T, class E>
 __global__ __launch_bounds__(kThreads, T::MWG) void ncc_tile_masked(TileArgs a) {
   constexpr int kRows = T::NW * T::NW;  // row tasks per block
   extern __shared__ float4 dyn4[];
@@ -1022,7 +1024,7 @@ __global__ __launch_bounds__(kThreads, T::MWG) void ncc_tile_masked(TileArgs a) 
   // Staged as v * y; full rows shift their loads by the block's reference
   // as the fast kernel does, masked rows centre each voxel on the
   // candidate's own mean with one rounding.
-  stage_window<T, true>(a, wy_, wv_, zon, WY, WZ, wx0, wy0, wz0, 0.0f, zbad);
+  stage_window<T, true, E>(a, wy_, wv_, zon, WY, WZ, wx0, wy0, wz0, 0.0f, zbad);
   for (int i = tid; i < T::NB * 64; i += kThreads) {
     const int bl = i / 64, k = i % 64;
     const int gy = by0 + bl % T::TY, gz = bz0 + bl / T::TY;
@@ -1629,9 +1631,9 @@ __device__ __forceinline__ int cand_total(const BMGrid& g, int64_t b, int p[3], 
 // Images of a re-scoring (base and search, masks NULL = all on) and the
 // search image's window statistics (mean over all voxels in .x).
 struct RescoreArgs {
-  const float* x;
+  const void* x;  // float or half_t elements (the kernel's E)
   const uint8_t* bm;
-  const float* y;
+  const void* y;
   const uint8_t* sm;
   const float2* stats;
   unsigned long long* count;  // optional: += blocks re-scored
@@ -1790,10 +1792,13 @@ __device__ void lean_phase1(const RescoreArgs& ra, const BMGrid& g, const int fi
 
 // Re-scores block b (BW 4, BG 1) and rewrites its record. Called by every
 // thread of the workgroup with the same b.
+template <class E>
 __device__ void rescore_block(const RescoreArgs& ra, const BMGrid& g, int64_t b, float* best,
                               float* second, int32_t* match, RescoreLds& L) {
 #pragma clang fp contract(off)
   const int nx = g.dims[0], ny = g.dims[1];
+  const E* X = static_cast<const E*>(ra.x);
+  const E* Y = static_cast<const E*>(ra.y);
   int p[3], first[3], cnt[3];
   const int total = cand_total(g, b, p, first, cnt);
   const int ex = cnt[0] + 3, ey = cnt[1] + 3, ez = cnt[2] + 3;
@@ -1811,7 +1816,7 @@ __device__ void rescore_block(const RescoreArgs& ra, const BMGrid& g, int64_t b,
       const int i = min(i0 + u * kThreads, nwin - 1);
       const int wx = i % ex, wy = (i / ex) % ey, wz = i / (ex * ey);
       const int64_t o = lin3(nx, ny, first[0] + wx, first[1] + wy, first[2] + wz);
-      yv[u] = ra.y[o];
+      yv[u] = (float)Y[o];
       mv[u] = ra.sm == nullptr ? (uint8_t)1 : ra.sm[o];
     }
 #pragma unroll
@@ -1827,7 +1832,7 @@ __device__ void rescore_block(const RescoreArgs& ra, const BMGrid& g, int64_t b,
   if (threadIdx.x < 64) {
     const int k = threadIdx.x;
     const int64_t o = lin3(nx, ny, p[0] + (k & 3), p[1] + ((k >> 2) & 3), p[2] + (k >> 4));
-    L.base[k] = ra.x[o];
+    L.base[k] = (float)X[o];
     const bool on = ra.bm == nullptr || ra.bm[o] != 0;
     L.bon[k] = on ? 1 : 0;
     if (!on) L.anyoff = 1;
@@ -2038,6 +2043,7 @@ __device__ void rescore_block(const RescoreArgs& ra, const BMGrid& g, int64_t b,
 
 // Re-scores the listed blocks, one workgroup per block; the grid strides
 // over the list.
+template <class E>
 __global__ __launch_bounds__(kThreads) void rescore_list_kernel(BMGrid g, float* best,
                                                                 float* second, int32_t* match,
                                                                 RescoreArgs ra,
@@ -2047,7 +2053,8 @@ __global__ __launch_bounds__(kThreads) void rescore_list_kernel(BMGrid g, float*
   const int n = *count;
   if (blockIdx.x == 0 && threadIdx.x == 0 && ra.count && n > 0)
     atomicAdd(ra.count, (unsigned long long)n);
-  for (int i = blockIdx.x; i < n; i += gridDim.x) rescore_block(ra, g, list[i], best, second, match, L);
+  for (int i = blockIdx.x; i < n; i += gridDim.x)
+    rescore_block<E>(ra, g, list[i], best, second, match, L);
 }
 
 __global__ void postprocess_kernel(BMGrid g, Geom geom, const float* __restrict__ best,
@@ -2292,7 +2299,7 @@ int64_t tiles_of(const BMGrid& g) {
   return (int64_t)g.nb[0] * nty * ntz;
 }
 
-template <class T>
+template <class T, class E>
 void run_tiled(const BMGrid& g, TileArgs a, BMScratch& scr, hipStream_t s) {
   a.ntx = g.nb[0];
   a.nty = (g.nb[1] + T::TY - 1) / T::TY;
@@ -2310,7 +2317,7 @@ void run_tiled(const BMGrid& g, TileArgs a, BMScratch& scr, hipStream_t s) {
   const int WZ = (T::TZ - 1) * g.ng + T::WX;
   const size_t lds_fast = sizeof(float) * (size_t)T::PX * WYf * WZ;
   if (scr.prof) a.work = scr.prof->begin("ncc_tile_fast", kProfFast, s);
-  hipLaunchKernelGGL(ncc_tile_fast<T>, dim3((unsigned)nt), dim3(kThreads), lds_fast, s, a);
+  hipLaunchKernelGGL((ncc_tile_fast<T, E>), dim3((unsigned)nt), dim3(kThreads), lds_fast, s, a);
   MIRORR_CHECK_LAUNCH();
   if (scr.prof) scr.prof->end(kProfFast, s);
   const int WYm = (T::TY - 1) * g.ng + T::WX;
@@ -2318,10 +2325,21 @@ void run_tiled(const BMGrid& g, TileArgs a, BMScratch& scr, hipStream_t s) {
   // Few tiles (a coarse level): split each queued tile over four workgroups.
   a.msplit = nt < 2048 ? 4 : 1;
   if (scr.prof) a.work = scr.prof->begin("ncc_tile_masked", kProfMasked, s);
-  hipLaunchKernelGGL(ncc_tile_masked<T>, dim3((unsigned)(nt * a.msplit)), dim3(kThreads),
+  hipLaunchKernelGGL((ncc_tile_masked<T, E>), dim3((unsigned)(nt * a.msplit)), dim3(kThreads),
                      lds_masked, s, a);
   MIRORR_CHECK_LAUNCH();
   if (scr.prof) scr.prof->end(kProfMasked, s);
+}
+
+template <class E>
+void tiled_search_as(const BMGrid& g, const TileArgs& a, BMScratch& scr, hipStream_t s) {
+  switch (g.nw) {
+    case 5: run_tiled<Cfg5, E>(g, a, scr, s); break;
+    case 7: run_tiled<Cfg7, E>(g, a, scr, s); break;
+    case 9: run_tiled<Cfg9, E>(g, a, scr, s); break;
+    case 11: run_tiled<Cfg11, E>(g, a, scr, s); break;
+    case 13: run_tiled<Cfg13, E>(g, a, scr, s); break;
+  }
 }
 
 }  // namespace
@@ -2362,22 +2380,35 @@ int64_t tile_count(const BMGrid& g) {
   }
 }
 
-void launch_search_stats(const float* y, const uint8_t* smask, const BMGrid& g,
-                         SearchStats st, hipStream_t s) {
+namespace {
+template <class E>
+void search_stats4_as(const void* y, const uint8_t* smask, int nx, int ny, int nz, SearchStats st,
+                      hipStream_t s) {
+  dim3 grid((nx - 3 + 63) / 64, (ny - 3 + 3) / 4, (nz - 3 + kStatsZ - 1) / kStatsZ);
+  const E* ye = static_cast<const E*>(y);
+  if (smask)
+    hipLaunchKernelGGL((search_stats4_kernel<true, E>), grid, dim3(kThreads), 0, s, ye, smask, nx,
+                       ny, nz, st.ms);
+  else
+    hipLaunchKernelGGL((search_stats4_kernel<false, E>), grid, dim3(kThreads), 0, s, ye, smask,
+                       nx, ny, nz, st.ms);
+}
+}  // namespace
+
+void launch_search_stats(const void* y, const uint8_t* smask, const BMGrid& g,
+                         SearchStats st, hipStream_t s, int storage) {
   const int nx = g.dims[0], ny = g.dims[1], nz = g.dims[2];
   if (nx < g.bw || ny < g.bw || nz < g.bw) return;
   if (g.bw == 4) {
-    dim3 grid((nx - 3 + 63) / 64, (ny - 3 + 3) / 4, (nz - 3 + kStatsZ - 1) / kStatsZ);
-    if (smask)
-      hipLaunchKernelGGL(search_stats4_kernel<true>, grid, dim3(kThreads), 0, s, y, smask, nx, ny,
-                         nz, st.ms);
+    if (storage == kStoreF16)
+      search_stats4_as<half_t>(y, smask, nx, ny, nz, st, s);
     else
-      hipLaunchKernelGGL(search_stats4_kernel<false>, grid, dim3(kThreads), 0, s, y, smask, nx,
-                         ny, nz, st.ms);
+      search_stats4_as<float>(y, smask, nx, ny, nz, st, s);
   } else {
+    if (storage == kStoreF16) throw Error(kErrArg, "half-precision images need 4-voxel blocks");
     dim3 grid((nx + kThreads - 1) / kThreads, ny - g.bw + 1, nz - g.bw + 1);
-    hipLaunchKernelGGL(search_stats_kernel<0>, grid, dim3(kThreads), 0, s, y, smask, nx, ny, nz,
-                       g.bw, st.ms);
+    hipLaunchKernelGGL(search_stats_kernel<0>, grid, dim3(kThreads), 0, s,
+                       static_cast<const float*>(y), smask, nx, ny, nz, g.bw, st.ms);
   }
   MIRORR_CHECK_LAUNCH();
 }
@@ -2404,19 +2435,26 @@ bool launch_variance_select(const BMGrid& g, const BlockPrep& bp, double portion
   return true;
 }
 
-void launch_block_prep(const float* x, const uint8_t* bmask, const BMGrid& g,
-                       BlockPrep bp, hipStream_t s) {
+void launch_block_prep(const void* x, const uint8_t* bmask, const BMGrid& g,
+                       BlockPrep bp, hipStream_t s, int storage) {
   if (g.nblocks == 0) return;
   if (g.nblocks > (int64_t)INT_MAX / 2) throw Error(kErrArg, "block grid too large");
   if (g.bw == 4) {
     dim3 grid((unsigned)((g.nblocks * 4 + kThreads - 1) / kThreads));
-    hipLaunchKernelGGL(block_prep4_kernel, grid, dim3(kThreads), 0, s, x, bmask, g, bp.xt,
-                       bp.sxx, bp.sumxt, bp.flags);
+    if (storage == kStoreF16)
+      hipLaunchKernelGGL(block_prep4_kernel<half_t>, grid, dim3(kThreads), 0, s,
+                         static_cast<const half_t*>(x), bmask, g, bp.xt, bp.sxx, bp.sumxt,
+                         bp.flags);
+    else
+      hipLaunchKernelGGL(block_prep4_kernel<float>, grid, dim3(kThreads), 0, s,
+                         static_cast<const float*>(x), bmask, g, bp.xt, bp.sxx, bp.sumxt,
+                         bp.flags);
   } else {
+    if (storage == kStoreF16) throw Error(kErrArg, "half-precision images need 4-voxel blocks");
     const int waves = kThreads / 64;
     dim3 grid((unsigned)((g.nblocks + waves - 1) / waves));
-    hipLaunchKernelGGL(block_prep_kernel<0>, grid, dim3(kThreads), 0, s, x, bmask, g, bp.xt,
-                       bp.sxx, bp.sumxt, bp.flags);
+    hipLaunchKernelGGL(block_prep_kernel<0>, grid, dim3(kThreads), 0, s,
+                       static_cast<const float*>(x), bmask, g, bp.xt, bp.sxx, bp.sumxt, bp.flags);
   }
   MIRORR_CHECK_LAUNCH();
 }
@@ -2449,13 +2487,17 @@ void launch_metric_search(int metric, const float* x, const uint8_t* bmask, cons
   MIRORR_CHECK_LAUNCH();
 }
 
-void launch_ncc_search(const float* x, const uint8_t* bmask, const float* y,
+void launch_ncc_search(const void* x, const uint8_t* bmask, const void* y,
                        const uint8_t* smask, const BMGrid& g, const BlockPrep& bp,
                        const SearchStats& st, BMScratch& scr, BMResult res,
-                       hipStream_t s) {
+                       hipStream_t s, int storage) {
   if (g.nblocks == 0) return;
   if (!tiled_supported(g)) {
-    launch_ncc_generic(x, bmask, y, smask, g, res, s, bp.flags);
+    if (storage == kStoreF16)
+      throw Error(kErrArg, "half-precision images need the tiled search (blocks of width 4, "
+                           "block gap 1, nhood width 5 to 13)");
+    launch_ncc_generic(static_cast<const float*>(x), bmask, static_cast<const float*>(y), smask,
+                       g, res, s, bp.flags);
     return;
   }
   TileArgs a{};
@@ -2471,19 +2513,20 @@ void launch_ncc_search(const float* x, const uint8_t* bmask, const float* y,
   a.out_best = res.best;
   a.out_second = res.second;
   a.out_match = res.match;
-  switch (g.nw) {
-    case 5: run_tiled<Cfg5>(g, a, scr, s); break;
-    case 7: run_tiled<Cfg7>(g, a, scr, s); break;
-    case 9: run_tiled<Cfg9>(g, a, scr, s); break;
-    case 11: run_tiled<Cfg11>(g, a, scr, s); break;
-    case 13: run_tiled<Cfg13>(g, a, scr, s); break;
-  }
+  if (storage == kStoreF16)
+    tiled_search_as<half_t>(g, a, scr, s);
+  else
+    tiled_search_as<float>(g, a, scr, s);
   if (scr.rlist) {
     // Blocks listed by the search kernels are searched again in double.
     RescoreArgs ra{x, bmask, y, smask, st.ms, scr.rescored, scr.exact_records};
     const int64_t grid = std::min<int64_t>(g.nblocks, 1024);
-    hipLaunchKernelGGL(rescore_list_kernel, dim3((unsigned)grid), dim3(kThreads), 0, s, g,
-                       res.best, res.second, res.match, ra, scr.rlist, scr.rcount);
+    if (storage == kStoreF16)
+      hipLaunchKernelGGL(rescore_list_kernel<half_t>, dim3((unsigned)grid), dim3(kThreads), 0, s,
+                         g, res.best, res.second, res.match, ra, scr.rlist, scr.rcount);
+    else
+      hipLaunchKernelGGL(rescore_list_kernel<float>, dim3((unsigned)grid), dim3(kThreads), 0, s,
+                         g, res.best, res.second, res.match, ra, scr.rlist, scr.rcount);
     MIRORR_CHECK_LAUNCH();
   }
 }

@@ -72,10 +72,12 @@ struct BMScratch {
   unsigned long long* rescored = nullptr;  // optional: += blocks re-scored
 };
 
-void launch_search_stats(const float* y, const uint8_t* smask, const BMGrid& g,
-                         SearchStats st, hipStream_t s);
-void launch_block_prep(const float* x, const uint8_t* bmask, const BMGrid& g,
-                       BlockPrep bp, hipStream_t s);
+// Image pointers of the search and block preparation hold elements of
+// `storage` (kStoreF32 float, kStoreF16 half_t; half needs 4-voxel blocks).
+void launch_search_stats(const void* y, const uint8_t* smask, const BMGrid& g,
+                         SearchStats st, hipStream_t s, int storage = kStoreF32);
+void launch_block_prep(const void* x, const uint8_t* bmask, const BMGrid& g,
+                       BlockPrep bp, hipStream_t s, int storage = kStoreF32);
 // Keeps the top `portion` of the blocks by centred sum of squares (bp.sxx):
 // K = ceil(portion * nblocks) clamped to [1, nblocks], the cut-off is the
 // K-th largest value and every block at or above it is kept (ties kept);
@@ -86,11 +88,11 @@ bool launch_variance_select(const BMGrid& g, const BlockPrep& bp, double portion
                             VarSelScratch& scr, hipStream_t s);
 
 // Exhaustive NCC search. Chooses the tiled fast kernels for BW=4, BG=1 and a
-// supported NW, else the generic kernel.
-void launch_ncc_search(const float* x, const uint8_t* bmask, const float* y,
+// supported NW, else the generic kernel (float images only).
+void launch_ncc_search(const void* x, const uint8_t* bmask, const void* y,
                        const uint8_t* smask, const BMGrid& g, const BlockPrep& bp,
                        const SearchStats& st, BMScratch& scratch, BMResult res,
-                       hipStream_t s);
+                       hipStream_t s, int storage = kStoreF32);
 // Generic kernel only (any BW/BG/NW), double accumulation; used for
 // unsupported shapes and as an independent device cross-check.
 void launch_ncc_generic(const float* x, const uint8_t* bmask, const float* y,

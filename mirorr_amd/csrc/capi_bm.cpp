@@ -6,11 +6,13 @@
 
 #include "bm_kernels.hpp"
 #include "context.hpp"
+#include "storage.hpp"
 
 using namespace mirorr;
 
 struct VolumeSlot {
-  const float* data = nullptr;
+  const void* data = nullptr;     // float, or half_t when storage is kStoreF16
+  int storage = kStoreF32;
   const uint8_t* mask = nullptr;
   mirorr_geom geom{};
   uint64_t version = 0;  // bumps whenever data/mask/geometry change
@@ -54,6 +56,8 @@ static void check_params(const mirorr_bm_params* p) {
     throw Error(kErrArg, "portion_kept should be between 0 and 1.0");
   if (p->block_metric < MIRORR_METRIC_NC || p->block_metric > MIRORR_METRIC_CR)
     throw Error(kErrArg, "unknown block metric");
+  if (p->storage != MIRORR_STORAGE_F32 && p->storage != MIRORR_STORAGE_F16)
+    throw Error(kErrArg, "unknown image storage");
 }
 
 static BMGrid current_grid(const mirorr_bm* bm) {
@@ -75,6 +79,9 @@ static void require_inputs(const mirorr_bm* bm) {
 static void run_search(mirorr_bm* bm) {
   require_inputs(bm);
   const BMGrid g = current_grid(bm);
+  if (bm->slot[0].storage != bm->slot[1].storage)
+    throw Error(kErrArg, "base and search images have different storage");
+  const int storage = bm->slot[0].storage;
   const uint64_t key0 = bm->slot[0].version ^ ((uint64_t)g.bw << 40) ^ ((uint64_t)g.nw << 48) ^
                         ((uint64_t)g.ng << 56) ^ ((uint64_t)g.bg << 32) ^
                         ((uint64_t)bm->p.block_metric << 24) ^ ((uint64_t)bm->p.exact_rescore << 20);
@@ -109,7 +116,7 @@ static void run_search(mirorr_bm* bm) {
       bm->prep_flags[bi].ensure(sizeof(uint32_t) * nb, bud);
       const BlockPrep bp{bm->prep_xt[bi].as<float>(), bm->prep_sxx[bi].as<float>(),
                          bm->prep_sum[bi].as<float>(), bm->prep_flags[bi].as<uint32_t>()};
-      launch_block_prep(B.data, B.mask, g, bp, s);
+      launch_block_prep(B.data, B.mask, g, bp, s, storage);
       bm->prep_selected[bi] = false;
       if (portion < 1.0) {
         bm->vsel_st.ensure(sizeof(VarSelState), bud);
@@ -132,7 +139,7 @@ static void run_search(mirorr_bm* bm) {
     // Search-image statistics (slot SEARCH).
     if (bm->stats_ver[si] != S.version || bm->stats_bw[si] != g.bw) {
       bm->stats[si].ensure(sizeof(float2) * nvox, bud);
-      launch_search_stats(S.data, S.mask, g, SearchStats{bm->stats[si].as<float2>()}, s);
+      launch_search_stats(S.data, S.mask, g, SearchStats{bm->stats[si].as<float2>()}, s, storage);
       bm->stats_ver[si] = S.version;
       bm->stats_bw[si] = g.bw;
     }
@@ -150,9 +157,13 @@ static void run_search(mirorr_bm* bm) {
       scr.rescore_ties = 1;  // raw records of exact ties are the reference's too
     }
     launch_ncc_search(B.data, B.mask, S.data, S.mask, g, bp,
-                      SearchStats{bm->stats[si].as<float2>()}, scr, res, s);
+                      SearchStats{bm->stats[si].as<float2>()}, scr, res, s, storage);
   } else {
-    launch_metric_search(metric, B.data, B.mask, S.data, S.mask, g, res, s,
+    if (storage != kStoreF32)
+      throw Error(kErrArg, "half-precision images need the NCC metric and the tiled search "
+                           "(blocks of width 4, block gap 1, nhood width 5 to 13)");
+    launch_metric_search(metric, static_cast<const float*>(B.data), B.mask,
+                         static_cast<const float*>(S.data), S.mask, g, res, s,
                          bm->prep_selected[bi] ? bp.flags : nullptr);
   }
   bm->results_valid = true;
@@ -255,6 +266,7 @@ void mirorr_bm_default_params(mirorr_bm_params* p) {
   p->verbosity = 0;
   p->block_metric = MIRORR_METRIC_NC;
   p->exact_rescore = 1;
+  p->storage = MIRORR_STORAGE_F32;
 }
 
 mirorr_status mirorr_bm_create(mirorr_ctx* c, const mirorr_bm_params* p, mirorr_bm** out) {
@@ -310,10 +322,19 @@ mirorr_status mirorr_bm_set_volume(mirorr_bm* bm, int role, const float* data,
     check_geom(geom);
     VolumeSlot& s = bm->slot[role];
     const size_t n = (size_t)geom->dims[0] * geom->dims[1] * geom->dims[2];
-    s.own_data.ensure(n * sizeof(float), &bm->ctx->budget);
-    MIRORR_HIP(hipMemcpyAsync(s.own_data.as<float>(), data, n * sizeof(float),
-                              hipMemcpyHostToDevice, bm->ctx->stream));
-    s.data = s.own_data.as<float>();
+    s.storage = bm->p.storage;
+    s.own_data.ensure(n * storage_bytes(s.storage), &bm->ctx->budget);
+    std::vector<half_t> h;
+    if (s.storage == kStoreF16) {
+      h.resize(n);
+      encode_half(data, n, half_encoding(data, n), h.data());
+      MIRORR_HIP(hipMemcpyAsync(s.own_data.as<void>(), h.data(), n * sizeof(half_t),
+                                hipMemcpyHostToDevice, bm->ctx->stream));
+    } else {
+      MIRORR_HIP(hipMemcpyAsync(s.own_data.as<float>(), data, n * sizeof(float),
+                                hipMemcpyHostToDevice, bm->ctx->stream));
+    }
+    s.data = s.own_data.as<void>();
     if (mask) {
       s.own_mask.ensure(n, &bm->ctx->budget);
       MIRORR_HIP(hipMemcpyAsync(s.own_mask.as<uint8_t>(), mask, n, hipMemcpyHostToDevice,
@@ -340,6 +361,7 @@ mirorr_status mirorr_bm_set_volume_device(mirorr_bm* bm, int role, const float* 
     VolumeSlot& s = bm->slot[role];
     s.own_data.reset();
     s.own_mask.reset();
+    s.storage = kStoreF32;
     s.data = d_data;
     s.mask = d_mask;
     s.geom = *geom;

@@ -29,6 +29,15 @@ enum : int { kErrArg = 1, kErrPlatform = 2, kErrAlgo = 3, kErrNoMem = 4, kErrSta
 
 #define MIRORR_CHECK_LAUNCH() MIRORR_HIP(hipGetLastError())
 
+// Element type of the image buffers a search or registration reads: float,
+// or binary16 holding an affine encoding of the intensities, (v - offset) *
+// scale with a power-of-two scale (integers of up to 12 bits are exact).
+// NCC^2 does not change under an affine map of either image, so kernels
+// work on the encoded values directly.
+enum Storage : int { kStoreF32 = 0, kStoreF16 = 1 };
+typedef _Float16 half_t;
+inline size_t storage_bytes(int storage) { return storage == kStoreF16 ? 2 : 4; }
+
 // Device image geometry (index -> physical: origin + D * diag(spacing) * idx).
 struct Geom {
   int dims[3];

@@ -13,11 +13,11 @@ namespace mirorr {
 struct RegCache;  // registration workspace, defined with the driver
 
 // Upper bound of the device bytes one mirorr_register call with these
-// parameters and images charges to a context whose cache is empty, the two
-// volumes (and masks) included. Throws on parameters whose pyramid schedule
-// is empty.
+// parameters and images (volumes of `storage`) charges to a context whose
+// cache is empty, the two volumes (and masks) included. Throws on
+// parameters whose pyramid schedule is empty.
 size_t registration_bytes_bound(const mirorr_reg_params* p, const Geom& fixed, bool fixed_mask,
-                                const Geom& moving, bool moving_mask);
+                                const Geom& moving, bool moving_mask, int storage = 0);
 }
 
 struct mirorr_ctx {

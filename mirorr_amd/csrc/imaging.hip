@@ -23,11 +23,12 @@ namespace {
 
 constexpr int kThreads = 256;
 
+template <class Tin, class Tout>
 __global__ __launch_bounds__(kThreads) void resample_kernel(
-    const float* __restrict__ in, int inx, int iny, int inz, const IndexMap* __restrict__ map,
+    const Tin* __restrict__ in, int inx, int iny, int inz, const IndexMap* __restrict__ map,
     const uint8_t* __restrict__ mask, int mnx, int mny, int mnz,
     const IndexMap* __restrict__ mmap, int mask_mode, int nx, int ny, int nz,
-    float* __restrict__ out, uint8_t* __restrict__ out_mask) {
+    Tout* __restrict__ out, uint8_t* __restrict__ out_mask, float outside) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y, z = blockIdx.z;
   if (x >= nx) return;
@@ -39,7 +40,7 @@ __global__ __launch_bounds__(kThreads) void resample_kernel(
     const double cz = m.A[6] * x + m.A[7] * y + m.A[8] * z + m.b[2];
     bool inside;
     const double v = trilinear(in, inx, iny, inz, cx, cy, cz, &inside);
-    out[o] = inside ? (float)v : 0.0f;
+    out[o] = inside ? (Tout)(float)v : (Tout)outside;
   }
   if (mask_mode != kMaskNone) {
     const IndexMap& m = *mmap;
@@ -156,26 +157,42 @@ int gaussian_kernel(double variance, double max_error, int max_width, double* ou
   return n - 1;
 }
 
-void launch_resample(const float* in, const int dims_in[3], const IndexMap* d_map,
-                     const uint8_t* mask, const int dims_m[3], const IndexMap* d_mmap,
-                     int mask_mode, const int dims_out[3], float* out, uint8_t* out_mask,
-                     hipStream_t s) {
+namespace {
+template <class Tin, class Tout>
+void resample_as(const void* in, const int dims_in[3], const IndexMap* d_map,
+                 const uint8_t* mask, const int dims_m[3], const IndexMap* d_mmap, int mask_mode,
+                 const int dims_out[3], void* out, uint8_t* out_mask, hipStream_t s,
+                 float outside) {
   dim3 grid((dims_out[0] + kThreads - 1) / kThreads, dims_out[1], dims_out[2]);
   const int mnx = dims_m ? dims_m[0] : 1, mny = dims_m ? dims_m[1] : 1, mnz = dims_m ? dims_m[2] : 1;
-  hipLaunchKernelGGL(resample_kernel, grid, dim3(kThreads), 0, s, in, dims_in[0], dims_in[1],
-                     dims_in[2], d_map, mask, mnx, mny, mnz, d_mmap, mask_mode, dims_out[0],
-                     dims_out[1], dims_out[2], out, out_mask);
+  hipLaunchKernelGGL((resample_kernel<Tin, Tout>), grid, dim3(kThreads), 0, s,
+                     static_cast<const Tin*>(in), dims_in[0], dims_in[1], dims_in[2], d_map, mask,
+                     mnx, mny, mnz, d_mmap, mask_mode, dims_out[0], dims_out[1], dims_out[2],
+                     static_cast<Tout*>(out), out_mask, outside);
   MIRORR_CHECK_LAUNCH();
 }
+}  // namespace
 
-void launch_pyramid_step(const float* prev, const Geom& gprev, const int ratio[3],
-                         const Geom& gout, float* out, double* tmp, IndexMap* d_map,
-                         hipStream_t s) {
+void launch_resample(const void* in, const int dims_in[3], const IndexMap* d_map,
+                     const uint8_t* mask, const int dims_m[3], const IndexMap* d_mmap,
+                     int mask_mode, const int dims_out[3], void* out, uint8_t* out_mask,
+                     hipStream_t s, int storage, float outside) {
+  if (storage == kStoreF16)
+    resample_as<half_t, half_t>(in, dims_in, d_map, mask, dims_m, d_mmap, mask_mode, dims_out,
+                                out, out_mask, s, outside);
+  else
+    resample_as<float, float>(in, dims_in, d_map, mask, dims_m, d_mmap, mask_mode, dims_out, out,
+                              out_mask, s, outside);
+}
+
+void launch_pyramid_step(const void* prev, const Geom& gprev, const int ratio[3],
+                         const Geom& gout, void* out, double* tmp, IndexMap* d_map,
+                         hipStream_t s, int storage, float outside) {
   const int nx = gprev.dims[0], ny = gprev.dims[1], nz = gprev.dims[2];
   const int64_t n = (int64_t)nx * ny * nz;
   bool all_ones = ratio[0] == 1 && ratio[1] == 1 && ratio[2] == 1;
   if (all_ones) {  // the filter copies the data onto the same grid
-    MIRORR_HIP(hipMemcpyAsync(out, prev, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+    MIRORR_HIP(hipMemcpyAsync(out, prev, n * storage_bytes(storage), hipMemcpyDeviceToDevice, s));
     return;
   }
   // Kernel coefficients per axis, uploaded once into the tail of tmp.
@@ -192,8 +209,12 @@ void launch_pyramid_step(const float* prev, const Geom& gprev, const int ratio[3
   MIRORR_HIP(hipMemcpyAsync(dk, hk.data(), hk.size() * sizeof(double), hipMemcpyHostToDevice, s));
   dim3 grid((nx + kThreads - 1) / kThreads, ny, nz);
   // z, then y, then x; the last pass writes float into `out`'s scratch.
-  hipLaunchKernelGGL((conv_axis_kernel<float, double>), grid, dim3(kThreads), 0, s, prev, a, nx,
-                     ny, nz, 2, radius[2], dk + 128);
+  if (storage == kStoreF16)
+    hipLaunchKernelGGL((conv_axis_kernel<half_t, double>), grid, dim3(kThreads), 0, s,
+                       static_cast<const half_t*>(prev), a, nx, ny, nz, 2, radius[2], dk + 128);
+  else
+    hipLaunchKernelGGL((conv_axis_kernel<float, double>), grid, dim3(kThreads), 0, s,
+                       static_cast<const float*>(prev), a, nx, ny, nz, 2, radius[2], dk + 128);
   hipLaunchKernelGGL((conv_axis_kernel<double, double>), grid, dim3(kThreads), 0, s, a, b, nx,
                      ny, nz, 1, radius[1], dk + 64);
   float* smoothed = reinterpret_cast<float*>(a);  // a is free again
@@ -201,8 +222,12 @@ void launch_pyramid_step(const float* prev, const Geom& gprev, const int ratio[3
                      nx, ny, nz, 0, radius[0], dk);
   MIRORR_CHECK_LAUNCH();
   hipLaunchKernelGGL(identity_map_kernel, dim3(1), dim3(1), 0, s, d_map, gprev, gout);
-  launch_resample(smoothed, gprev.dims, d_map, nullptr, nullptr, nullptr, kMaskNone, gout.dims,
-                  out, nullptr, s);
+  if (storage == kStoreF16)
+    resample_as<float, half_t>(smoothed, gprev.dims, d_map, nullptr, nullptr, nullptr, kMaskNone,
+                               gout.dims, out, nullptr, s, outside);
+  else
+    resample_as<float, float>(smoothed, gprev.dims, d_map, nullptr, nullptr, nullptr, kMaskNone,
+                              gout.dims, out, nullptr, s, outside);
   // hk is reused by the next call: the copy above must finish first.
   MIRORR_HIP(hipStreamSynchronize(s));
 }

@@ -76,14 +76,15 @@ enum MaskMode : int {
 };
 
 // Resample `in` (dims_in) onto an output grid of dims_out under `map`
-// (linear interpolation, neighbours clamped to the buffer, 0 outside
-// [-0.5, n-0.5)). A mask is written under `mmap` from the mask grid
-// (dims_m; a separate grid so full-resolution masks serve every level).
-// Maps are read from device memory (a transform the LTS updates on device).
-void launch_resample(const float* in, const int dims_in[3], const IndexMap* d_map,
+// (linear interpolation, neighbours clamped to the buffer, `outside` outside
+// [-0.5, n-0.5): the encoding of 0). A mask is written under `mmap` from
+// the mask grid (dims_m; a separate grid so full-resolution masks serve
+// every level). Maps are read from device memory (a transform the LTS
+// updates on device). `in` and `out` hold elements of `storage`.
+void launch_resample(const void* in, const int dims_in[3], const IndexMap* d_map,
                      const uint8_t* mask, const int dims_m[3], const IndexMap* d_mmap,
-                     int mask_mode, const int dims_out[3], float* out, uint8_t* out_mask,
-                     hipStream_t s);
+                     int mask_mode, const int dims_out[3], void* out, uint8_t* out_mask,
+                     hipStream_t s, int storage = kStoreF32, float outside = 0.0f);
 
 // ITK GaussianOperator coefficients (centre first), normalised to unit sum.
 int gaussian_kernel(double variance, double max_error, int max_width, double* out, int cap);
@@ -91,8 +92,8 @@ int gaussian_kernel(double variance, double max_error, int max_width, double* ou
 // One recursive pyramid step: smooth `prev` per axis with variance
 // (ratio/2)^2 voxels (zero-flux Neumann edges, double intermediates) and
 // resample onto `gout`. `tmp` holds 2 * nvox(prev) doubles.
-void launch_pyramid_step(const float* prev, const Geom& gprev, const int ratio[3],
-                         const Geom& gout, float* out, double* tmp, IndexMap* d_map,
-                         hipStream_t s);
+void launch_pyramid_step(const void* prev, const Geom& gprev, const int ratio[3],
+                         const Geom& gout, void* out, double* tmp, IndexMap* d_map,
+                         hipStream_t s, int storage = kStoreF32, float outside = 0.0f);
 
 }  // namespace mirorr

@@ -21,6 +21,7 @@
 #include "imaging.hpp"
 #include "lts.hpp"
 #include "schedule.hpp"
+#include "storage.hpp"
 
 using namespace mirorr;
 
@@ -29,6 +30,9 @@ struct mirorr_volume {
   DeviceBuffer data, mask;
   Geom geom{};
   bool has_mask = false;
+  int storage = mirorr::kStoreF32;  // element type of `data`
+  double enc_offset = 0.0, enc_scale = 1.0;  // half storage: (v - offset) * scale
+  float enc_zero = 0.0f;            // the encoding of intensity 0 (half storage)
 };
 
 namespace {
@@ -113,18 +117,18 @@ struct Level {
 };
 
 // Recursive pyramid of one image (coarsest first) into `out`, whose level
-// buffers are reused when large enough.
-void build_pyramid(const float* d_in, const Geom& g, const std::vector<int>& sched, int rows,
+// buffers are reused when large enough. Levels hold elements of `storage`.
+void build_pyramid(const void* d_in, const Geom& g, const std::vector<int>& sched, int rows,
                    std::vector<Level>& out, DeviceBuffer& tmp, DeviceBuffer& map, Budget* bud,
-                   hipStream_t s) {
+                   hipStream_t s, int storage = kStoreF32, float enc_zero = 0.0f) {
   if ((int)out.size() < rows) out.resize(rows);
   map.ensure(sizeof(IndexMap), bud);
   for (int l = rows - 1; l >= 0; --l) {
     const int f[3] = {sched[3 * l], sched[3 * l + 1], sched[3 * l + 2]};
     out[l].geom = level_geom(g, f);
-    out[l].data.ensure(sizeof(float) * nvox(out[l].geom), bud);
+    out[l].data.ensure(storage_bytes(storage) * nvox(out[l].geom), bud);
     int ratio[3];
-    const float* prev;
+    const void* prev;
     const Geom* pg;
     if (l == rows - 1) {
       for (int a = 0; a < 3; ++a) ratio[a] = f[a];
@@ -132,12 +136,12 @@ void build_pyramid(const float* d_in, const Geom& g, const std::vector<int>& sch
       pg = &g;
     } else {
       for (int a = 0; a < 3; ++a) ratio[a] = f[a] / sched[3 * (l + 1) + a];
-      prev = out[l + 1].data.as<float>();
+      prev = out[l + 1].data.as<void>();
       pg = &out[l + 1].geom;
     }
     tmp.ensure(sizeof(double) * (2 * nvox(*pg) + 3 * 64), bud);
-    launch_pyramid_step(prev, *pg, ratio, out[l].geom, out[l].data.as<float>(), tmp.as<double>(),
-                        map.as<IndexMap>(), s);
+    launch_pyramid_step(prev, *pg, ratio, out[l].geom, out[l].data.as<void>(), tmp.as<double>(),
+                        map.as<IndexMap>(), s, storage, enc_zero);
   }
 }
 
@@ -217,6 +221,7 @@ struct RegRun {
   hipStream_t s;
   Budget* bud;
   bool finest = false;  // the current level is the last (full-resolution) one
+  int storage = kStoreF32;  // element type of the levels and half-way images
 
   // The context's second stream (and fork/join events), created on first use.
   hipStream_t second_stream() {
@@ -230,11 +235,11 @@ struct RegRun {
 
   // Block preparation of a base image and, with portion < 1, the variance
   // selection over its blocks. Returns whether blocks were dropped.
-  bool prepare_base(const float* base, const uint8_t* bmask, const BMGrid& g, DeviceBuffer& xt,
+  bool prepare_base(const void* base, const uint8_t* bmask, const BMGrid& g, DeviceBuffer& xt,
                     DeviceBuffer& sxx, DeviceBuffer& sum, DeviceBuffer& fl, double portion,
                     hipStream_t s) {
     const BlockPrep bp{xt.as<float>(), sxx.as<float>(), sum.as<float>(), fl.as<uint32_t>()};
-    launch_block_prep(base, bmask, g, bp, s);
+    launch_block_prep(base, bmask, g, bp, s, storage);
     if (!(portion < 1.0)) return false;
     VarSelScratch vs{w.vsel_st.as<VarSelState>(), w.vsel_hist.as<unsigned int>(),
                      w.vsel_evals.as<unsigned long long>()};
@@ -244,7 +249,7 @@ struct RegRun {
   // One BM pass base -> search over the half grid, writing LTS pairs. With
   // `selected`, the prepared flags drop blocks and the pass's voxel-evals are
   // counted on the device.
-  void match_pass(const float* base, const uint8_t* bmask, const float* search,
+  void match_pass(const void* base, const uint8_t* bmask, const void* search,
                   const uint8_t* smask, const Geom& g_base, DeviceBuffer& xt, DeviceBuffer& sxx,
                   DeviceBuffer& sum, DeviceBuffer& fl, DeviceBuffer& stats_search, int swap,
                   bool selected, LtsState* st, hipStream_t s, bool second_set) {
@@ -266,9 +271,13 @@ struct RegRun {
       scr.rescored = (second_set ? w.rescored2 : w.rescored).as<unsigned long long>();
       scr.exact_records = 0;  // the pairs need the filter's decision and the match only
       launch_ncc_search(base, bmask, search, smask, g, bp, SearchStats{stats_search.as<float2>()},
-                        scr, res, s);
+                        scr, res, s, storage);
     } else {
-      launch_metric_search(p->block_metric, base, bmask, search, smask, g, res, s,
+      if (storage != kStoreF32)
+        throw Error(kErrArg, "half-precision images need the NCC metric and the tiled search "
+                             "(blocks of width 4, block gap 1, nhood width 5 to 13)");
+      launch_metric_search(p->block_metric, static_cast<const float*>(base), bmask,
+                           static_cast<const float*>(search), smask, g, res, s,
                            selected ? fl.as<uint32_t>() : nullptr);
     }
     const int64_t off = swap ? g.nblocks : 0;
@@ -295,7 +304,7 @@ struct RegRun {
                                p->nhood_gap);
     const int64_t nb = std::max<int64_t>(g.nblocks, 1);
     const int64_t n3 = (int64_t)g.bw * g.bw * g.bw;
-    for (DeviceBuffer* b : {&w.Fh, &w.Mh}) b->ensure(sizeof(float) * n, bud);
+    for (DeviceBuffer* b : {&w.Fh, &w.Mh}) b->ensure(storage_bytes(storage) * n, bud);
     for (DeviceBuffer* b : {&w.Fmh, &w.Mmh}) b->ensure(n, bud);
     for (DeviceBuffer* b : {&w.statsF, &w.statsM}) b->ensure(sizeof(float2) * n, bud);
     for (DeviceBuffer* b : {&w.xtF, &w.xtM}) b->ensure(sizeof(float) * nb * n3, bud);
@@ -359,16 +368,17 @@ struct RegRun {
 };
 
 size_t registration_bytes_bound(const mirorr_reg_params* p, const Geom& FG, bool fmask,
-                                const Geom& MG, bool mmask) {
+                                const Geom& MG, bool mmask, int storage) {
   const size_t nF = nvox(FG), nM = nvox(MG);
+  const size_t eb = storage_bytes(storage);
   // The two uploaded volumes and masks.
-  size_t total = sizeof(float) * (nF + nM) + (fmask ? nF : 0) + (mmask ? nM : 0);
+  size_t total = eb * (nF + nM) + (fmask ? nF : 0) + (mmask ? nM : 0);
   // Pyramid levels (build_pyramid), its double scratch, sized by the finest
   // input, and index map.
   const LevelPlan plan = plan_levels(p, FG, MG);
   for (int l = 0; l < plan.nlev; ++l) {
-    total += sizeof(float) * nvox(level_geom(FG, &plan.fs[3 * l]));
-    total += sizeof(float) * nvox(level_geom(MG, &plan.ms[3 * l]));
+    total += eb * nvox(level_geom(FG, &plan.fs[3 * l]));
+    total += eb * nvox(level_geom(MG, &plan.ms[3 * l]));
   }
   total += sizeof(double) * (2 * std::max(nF, nM) + 3 * 64) + sizeof(IndexMap);
   // Level buffers (RegRun::ensure_level_buffers) at a grid no larger than
@@ -382,7 +392,7 @@ size_t registration_bytes_bound(const mirorr_reg_params* p, const Geom& FG, bool
   const size_t n3 = (size_t)g.bw * g.bw * g.bw;
   const size_t tiles = (size_t)std::max<int64_t>(tile_count(g), 1);
   const size_t np = 2 * nb;
-  total += 2 * sizeof(float) * n + 2 * n + 2 * sizeof(float2) * n;  // Fh, Mh, masks, stats
+  total += 2 * eb * n + 2 * n + 2 * sizeof(float2) * n;  // Fh, Mh, masks, stats
   total += 2 * sizeof(float) * nb * n3 + 4 * sizeof(float) * nb + 2 * sizeof(uint32_t) * nb;
   // best, second, match and the masked-tile queue, one set per pass
   total += 2 * (2 * sizeof(float) * nb + 3 * sizeof(int32_t) * nb);
@@ -411,6 +421,9 @@ void run_registration(mirorr_ctx* ctx, const mirorr_reg_params* p, const mirorr_
   RegCache& cache = reg_cache(ctx);
   RegRun run{ctx, p, cache.w, stats, p->affine ? xf::kAffine : xf::kRigid, p->symmetric != 0, s,
              bud};
+  if (fixed->storage != moving->storage)
+    throw Error(kErrArg, "the fixed and moving volumes have different storage");
+  run.storage = fixed->storage;
   Workspace& w = run.w;
   const Geom& FG = fixed->geom;
   const Geom& MG = moving->geom;
@@ -418,8 +431,10 @@ void run_registration(mirorr_ctx* ctx, const mirorr_reg_params* p, const mirorr_
   const int nlev = plan.nlev;
   std::vector<Level>& Fp = cache.F;
   std::vector<Level>& Mp = cache.M;
-  build_pyramid(fixed->data.as<float>(), FG, plan.fs, nlev, Fp, w.pyr_tmp, w.pyr_map, bud, s);
-  build_pyramid(moving->data.as<float>(), MG, plan.ms, nlev, Mp, w.pyr_tmp, w.pyr_map, bud, s);
+  build_pyramid(fixed->data.as<void>(), FG, plan.fs, nlev, Fp, w.pyr_tmp, w.pyr_map, bud, s,
+                fixed->storage, fixed->enc_zero);
+  build_pyramid(moving->data.as<void>(), MG, plan.ms, nlev, Mp, w.pyr_tmp, w.pyr_map, bud, s,
+                moving->storage, moving->enc_zero);
   const int t_max = plan.t_max;
   // Initial transform: given, or centred (itkMirorrPyramidWrapper.txx:611-623:
   // centre of the moving image, translation between the image centres), then
@@ -539,11 +554,11 @@ void run_registration(mirorr_ctx* ctx, const mirorr_reg_params* p, const mirorr_
       IndexMap mi[2] = {make_index_map(I, z, gM, o.gHM), make_index_map(I, z, MG, o.gHM)};
       w.mapI.ensure(sizeof(mi), bud);
       MIRORR_HIP(hipMemcpyAsync(w.mapI.as<IndexMap>(), mi, sizeof(mi), hipMemcpyHostToDevice, s));
-      launch_resample(Mp[level].data.as<float>(), gM.dims, w.mapI.as<IndexMap>(), mmask, MG.dims,
-                      w.mapI.as<IndexMap>() + 1, mmode, o.gHM.dims, w.Mh.as<float>(),
-                      w.Mmh.as<uint8_t>(), s);
+      launch_resample(Mp[level].data.as<void>(), gM.dims, w.mapI.as<IndexMap>(), mmask, MG.dims,
+                      w.mapI.as<IndexMap>() + 1, mmode, o.gHM.dims, w.Mh.as<void>(),
+                      w.Mmh.as<uint8_t>(), s, run.storage, moving->enc_zero);
       if (prep)
-        selM = run.prepare_base(w.Mh.as<float>(), w.Mmh.as<uint8_t>(), gh, w.xtM, w.sxxM, w.sumM,
+        selM = run.prepare_base(w.Mh.as<void>(), w.Mmh.as<uint8_t>(), gh, w.xtM, w.sxxM, w.sumM,
                                 w.flM, portion, s);
       // (the half-grid stays fixed through the level; hipMemcpy is ordered)
       MIRORR_HIP(hipStreamSynchronize(s));
@@ -551,12 +566,13 @@ void run_registration(mirorr_ctx* ctx, const mirorr_reg_params* p, const mirorr_
     int it = 0;
     for (; it < max_it; ++it) {
       // 1. resample onto the half-way grid under the current transforms
-      launch_resample(Fp[level].data.as<float>(), gF.dims, &outer->mapF, fmask, FG.dims,
-                      &outer->mmapF, fmode, o.gHF.dims, w.Fh.as<float>(), w.Fmh.as<uint8_t>(), s);
+      launch_resample(Fp[level].data.as<void>(), gF.dims, &outer->mapF, fmask, FG.dims,
+                      &outer->mmapF, fmode, o.gHF.dims, w.Fh.as<void>(), w.Fmh.as<uint8_t>(), s,
+                      run.storage, fixed->enc_zero);
       if (run.symmetric)
-        launch_resample(Mp[level].data.as<float>(), gM.dims, &outer->mapM, mmask, MG.dims,
-                        &outer->mmapM, mmode, o.gHM.dims, w.Mh.as<float>(), w.Mmh.as<uint8_t>(),
-                        s);
+        launch_resample(Mp[level].data.as<void>(), gM.dims, &outer->mapM, mmask, MG.dims,
+                        &outer->mmapM, mmode, o.gHM.dims, w.Mh.as<void>(), w.Mmh.as<uint8_t>(),
+                        s, run.storage, moving->enc_zero);
       // 2. block matching: moving half as base, then (symmetric) swapped.
       // Without variance selection (whose scratch the passes share) the two
       // symmetric passes are independent until the LTS: the second runs on
@@ -583,22 +599,22 @@ void run_registration(mirorr_ctx* ctx, const mirorr_reg_params* p, const mirorr_
         MIRORR_HIP(hipStreamWaitEvent(s_b, ctx->ev_fork, 0));
       }
       if (tiled) {
-        launch_search_stats(w.Fh.as<float>(), w.Fmh.as<uint8_t>(), gh,
-                            SearchStats{w.statsF.as<float2>()}, s);
+        launch_search_stats(w.Fh.as<void>(), w.Fmh.as<uint8_t>(), gh,
+                            SearchStats{w.statsF.as<float2>()}, s, run.storage);
         if (run.symmetric)
-          launch_search_stats(w.Mh.as<float>(), w.Mmh.as<uint8_t>(), gh,
-                              SearchStats{w.statsM.as<float2>()}, s_b);
+          launch_search_stats(w.Mh.as<void>(), w.Mmh.as<uint8_t>(), gh,
+                              SearchStats{w.statsM.as<float2>()}, s_b, run.storage);
       }
       if (run.symmetric && prep) {
-        selM = run.prepare_base(w.Mh.as<float>(), w.Mmh.as<uint8_t>(), gh, w.xtM, w.sxxM, w.sumM,
+        selM = run.prepare_base(w.Mh.as<void>(), w.Mmh.as<uint8_t>(), gh, w.xtM, w.sxxM, w.sumM,
                                 w.flM, portion, s);
-        selF = run.prepare_base(w.Fh.as<float>(), w.Fmh.as<uint8_t>(), gh, w.xtF, w.sxxF, w.sumF,
+        selF = run.prepare_base(w.Fh.as<void>(), w.Fmh.as<uint8_t>(), gh, w.xtF, w.sxxF, w.sumF,
                                 w.flF, portion, s_b);  // the second pass's base
       }
-      run.match_pass(w.Mh.as<float>(), w.Mmh.as<uint8_t>(), w.Fh.as<float>(), w.Fmh.as<uint8_t>(),
+      run.match_pass(w.Mh.as<void>(), w.Mmh.as<uint8_t>(), w.Fh.as<void>(), w.Fmh.as<uint8_t>(),
                      o.gHM, w.xtM, w.sxxM, w.sumM, w.flM, w.statsF, 0, selM, st, s, false);
       if (run.symmetric)
-        run.match_pass(w.Fh.as<float>(), w.Fmh.as<uint8_t>(), w.Mh.as<float>(),
+        run.match_pass(w.Fh.as<void>(), w.Fmh.as<uint8_t>(), w.Mh.as<void>(),
                        w.Mmh.as<uint8_t>(), o.gHF, w.xtF, w.sxxF, w.sumF, w.flF, w.statsM, 1,
                        selF, st, s_b, fork);
       if (fork) {
@@ -697,20 +713,42 @@ void mirorr_reg_default_params(mirorr_reg_params* p) {
 
 mirorr_status mirorr_volume_create(mirorr_ctx* c, const float* data, const uint8_t* mask,
                                    const mirorr_geom* geom, mirorr_volume** out) {
+  return mirorr_volume_create_storage(c, data, mask, geom, MIRORR_STORAGE_F32, out);
+}
+
+mirorr_status mirorr_volume_create_storage(mirorr_ctx* c, const float* data, const uint8_t* mask,
+                                           const mirorr_geom* geom, int storage,
+                                           mirorr_volume** out) {
   if (!c || !out) return MIRORR_ERR_ARG;
   *out = nullptr;
   mirorr_volume* v = nullptr;
   mirorr_status st = guarded(c, [&] {
     if (!data || !geom) throw Error(kErrArg, "null volume data or geometry");
+    if (storage != MIRORR_STORAGE_F32 && storage != MIRORR_STORAGE_F16)
+      throw Error(kErrArg, "unknown volume storage");
     for (int i = 0; i < 3; ++i)
       if (geom->dims[i] < 1 || !(geom->spacing[i] > 0.0)) throw Error(kErrArg, "bad geometry");
     v = new mirorr_volume();
     v->ctx = c;
     v->geom = to_geom(*geom);
+    v->storage = storage;
     const size_t n = (size_t)geom->dims[0] * geom->dims[1] * geom->dims[2];
-    v->data.ensure(n * sizeof(float), &c->budget);
-    MIRORR_HIP(hipMemcpyAsync(v->data.as<float>(), data, n * sizeof(float), hipMemcpyHostToDevice,
-                              c->stream));
+    v->data.ensure(n * storage_bytes(storage), &c->budget);
+    if (storage == MIRORR_STORAGE_F16) {
+      // Encoded on the host: half the bytes cross the bus.
+      const HalfEncoding e = half_encoding(data, n);
+      std::vector<half_t> h(n);
+      encode_half(data, n, e, h.data());
+      v->enc_offset = e.offset;
+      v->enc_scale = e.scale;
+      v->enc_zero = e.zero();
+      MIRORR_HIP(hipMemcpyAsync(v->data.as<void>(), h.data(), n * sizeof(half_t),
+                                hipMemcpyHostToDevice, c->stream));
+      MIRORR_HIP(hipStreamSynchronize(c->stream));  // h leaves scope
+    } else {
+      MIRORR_HIP(hipMemcpyAsync(v->data.as<float>(), data, n * sizeof(float),
+                                hipMemcpyHostToDevice, c->stream));
+    }
     if (mask) {
       v->mask.ensure(n, &c->budget);
       MIRORR_HIP(hipMemcpyAsync(v->mask.as<uint8_t>(), mask, n, hipMemcpyHostToDevice, c->stream));
@@ -723,6 +761,15 @@ mirorr_status mirorr_volume_create(mirorr_ctx* c, const float* data, const uint8
     return st;
   }
   *out = v;
+  return MIRORR_OK;
+}
+
+mirorr_status mirorr_volume_info(const mirorr_volume* v, int* storage, double* offset,
+                                 double* scale) {
+  if (!v) return MIRORR_ERR_ARG;
+  if (storage) *storage = v->storage;
+  if (offset) *offset = v->enc_offset;
+  if (scale) *scale = v->enc_scale;
   return MIRORR_OK;
 }
 

@@ -54,6 +54,7 @@ struct Request {
   const uint8_t* pMm = nullptr;
   bool has_init = false, has_center = false;
   double init[12] = {0}, center[3] = {0};
+  int storage = MIRORR_STORAGE_F32;
   size_t bound = 0;
   double t_submit = 0.0;
   int overtaken = 0;  // later requests admitted while this one waited for memory
@@ -212,8 +213,8 @@ struct mirorr_server {
     mirorr_ctx* c = slot.ctx;
     mirorr_volume* vf = nullptr;
     mirorr_volume* vm = nullptr;
-    mirorr_status st = mirorr_volume_create(c, r.pF, r.pFm, &r.gF, &vf);
-    if (st == MIRORR_OK) st = mirorr_volume_create(c, r.pM, r.pMm, &r.gM, &vm);
+    mirorr_status st = mirorr_volume_create_storage(c, r.pF, r.pFm, &r.gF, r.storage, &vf);
+    if (st == MIRORR_OK) st = mirorr_volume_create_storage(c, r.pM, r.pMm, &r.gM, r.storage, &vm);
     // The host copies are not needed once uploaded.
     std::vector<float>().swap(r.F);
     std::vector<float>().swap(r.M);
@@ -315,10 +316,13 @@ mirorr_status mirorr_submit(mirorr_server* S, const mirorr_request* q, int64_t* 
     r->p = q->params;
     r->gF = q->fixed_geom;
     r->gM = q->moving_geom;
+    if (q->storage != MIRORR_STORAGE_F32 && q->storage != MIRORR_STORAGE_F16)
+      throw Error(mirorr::kErrArg, "unknown volume storage");
+    r->storage = q->storage;
     const size_t nF = checked_nvox(r->gF), nM = checked_nvox(r->gM);
     r->bound = mirorr::registration_bytes_bound(&r->p, mirorr::to_geom(r->gF),
                                                 q->fixed_mask != nullptr, mirorr::to_geom(r->gM),
-                                                q->moving_mask != nullptr);
+                                                q->moving_mask != nullptr, r->storage);
     if (q->borrow) {
       r->pF = q->fixed;
       r->pM = q->moving;

@@ -25,7 +25,7 @@ from .blockmatcher import Context, _live_volumes, default_context
 class DeviceVolume:
     """An image (+ optional mask) resident in HBM (mirorr_volume)."""
 
-    def __init__(self, vol, ctx: Context | None = None, mask=None):
+    def __init__(self, vol, ctx: Context | None = None, mask=None, storage="f32"):
         self.ctx = ctx or default_context()
         data = np.ascontiguousarray(vol.data, np.float32)
         m = mask if mask is not None else getattr(vol, "mask", None)
@@ -33,11 +33,19 @@ class DeviceVolume:
             m = np.ascontiguousarray(m, np.uint8)
         g = capi.Geom.make(vol.dims, vol.spacing, vol.origin, vol.direction)
         self._h = C.c_void_p()
-        capi.check(capi.lib().mirorr_volume_create(self.ctx.handle, capi.ptr(data), capi.ptr(m),
-                                                   C.byref(g), C.byref(self._h)),
-                   self.ctx.handle)
+        st = capi.STORAGE_NAMES[storage] if isinstance(storage, str) else int(storage)
+        capi.check(capi.lib().mirorr_volume_create_storage(
+            self.ctx.handle, capi.ptr(data), capi.ptr(m), C.byref(g), st, C.byref(self._h)),
+            self.ctx.handle)
         self.dims = vol.dims
         _live_volumes.add(self)
+
+    def info(self) -> dict:
+        """Storage and, for binary16, the (offset, scale) encoding."""
+        st, off, sc = C.c_int(), C.c_double(), C.c_double()
+        capi.check(capi.lib().mirorr_volume_info(self._h, C.byref(st), C.byref(off),
+                                                 C.byref(sc)), self.ctx.handle)
+        return {"storage": st.value, "offset": off.value, "scale": sc.value}
 
     @property
     def handle(self):
@@ -77,6 +85,7 @@ class MirorrPyramid:
         self._p = p
         self._fixed = self._moving = None
         self._fixed_mask = self._moving_mask = None
+        self._storage = "f32"
         self.stats = None
         self._tuner = _ScheduleTuner(p)
 
@@ -98,6 +107,14 @@ class MirorrPyramid:
     def SetVarianceSelection(self, on: bool): self._p.variance_select = 1 if on else 0
     def SetVerbosity(self, v): self._p.verbosity = int(v)
     def SetCentreInitialisation(self, on: bool): self._p.centre_init = 1 if on else 0
+
+    def SetStorage(self, name: str):
+        """Device storage of the volumes, pyramid levels and half-way images:
+        "f32" (the reference's float) or "f16" (binary16 with a per-volume
+        affine encoding, widened to float where the search stages it)."""
+        if name not in capi.STORAGE_NAMES:
+            raise ValueError(f"storage must be one of {sorted(capi.STORAGE_NAMES)}")
+        self._storage = name
 
     def SetResamplingType(self, name: str):
         """middle (default), basic, moving or fixed (SetResamplingType,
@@ -130,8 +147,8 @@ class MirorrPyramid:
     def RunMirorr(self, init_params=None, init_center=None, fixed_dev=None, moving_dev=None):
         """Registers moving to fixed; returns (ITK parameters, centre). Device
         volumes may be passed to skip the upload (they stay resident)."""
-        fd = fixed_dev or DeviceVolume(self._fixed, self.ctx, self._fixed_mask)
-        md = moving_dev or DeviceVolume(self._moving, self.ctx, self._moving_mask)
+        fd = fixed_dev or DeviceVolume(self._fixed, self.ctx, self._fixed_mask, self._storage)
+        md = moving_dev or DeviceVolume(self._moving, self.ctx, self._moving_mask, self._storage)
         n = 12 if self._p.affine else 6
         out = np.zeros(12)
         oc = np.zeros(3)

@@ -52,12 +52,15 @@ class RegistrationServer:
         self._borrowed = {}  # ticket -> arrays the library reads until the result is polled
 
     def submit(self, fixed, moving, params: capi.RegParams, fixed_mask=None, moving_mask=None,
-               init_params=None, init_center=None, borrow: bool = False) -> int:
+               init_params=None, init_center=None, borrow: bool = False,
+               storage: str = "f32") -> int:
         """Queues one registration of `moving` to `fixed` (Volume-like objects
         with data/dims/spacing/origin/direction); returns its ticket. The
         library copies the arrays, or with `borrow` uploads from them when the
         request runs: this object then keeps them referenced until poll()
-        returns the result, and the caller must not change them meanwhile."""
+        returns the result, and the caller must not change them meanwhile.
+        `storage` "f16" keeps both volumes in binary16 (see
+        MirorrPyramid.SetStorage)."""
         q = capi.Request()
         q.params = params
         keep = []
@@ -81,6 +84,7 @@ class RegistrationServer:
         q.init_params = arr(init_params, np.float64)
         q.init_center = arr(init_center, np.float64)
         q.borrow = 1 if borrow else 0
+        q.storage = capi.STORAGE_NAMES[storage]
         t = C.c_int64()
         capi.check_server(capi.lib().mirorr_submit(self._h, C.byref(q), C.byref(t)))
         self._affine[t.value] = bool(params.affine)
