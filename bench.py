@@ -30,6 +30,11 @@ sys.path.insert(0, HERE)
 
 PEAK_FP32_TFLOPS = 157.3      # MI355X dense fp32 (vector/matrix) peak
 PEAK_HBM_GBS = 8000.0
+PEAK_LDS_GBS = 128 * 256 * 2.4  # 128 B per clock per CU, 256 CUs, 2.4 GHz
+# LDS bytes the fast search kernel reads per voxel-eval (NW 9 tile, KY 3,
+# rows of 12 floats): per lane and block z-plane, KY + 3 window rows of 48 B
+# and the 64 B block plane serve KY * NW * 16 voxel-evals.
+LDS_BYTES_PER_VOXEL_EVAL_NW9 = ((3 + 3) * 48 + 64) / (3 * 9 * 16)
 FLOP_PER_VOXEL_EVAL = 6       # two-pass NCC: mean add, centring, 2 FMAs
 ROOFLINE_REGISTRATIONS = 3    # untimed registrations whose fast-kernel launches are timed
 
@@ -461,13 +466,16 @@ def main():
     _, ms_mask, work_mask = ctx.profile("ncc_tile_masked")
     ctx.set_profiling(False)
 
-    from mirorr_amd import transforms
+    from mirorr_amd import capi, transforms
     M, off = transforms.matrix_offset(params, center, cfg["model"] == "affine")
     err_mm = transforms.max_corner_distance(M, off, pair.matrix, pair.translation, pair.fixed)
     iterations, points = list(reg.stats["iterations"]), list(reg.stats["points"])
 
     avg_ms = ms_fast / max(n_fast, 1)
     achieved = FLOP_PER_VOXEL_EVAL * (work_fast / max(n_fast, 1)) / (avg_ms * 1e-3) / 1e12
+    traffic = measured_traffic() if args.workload == "pair256_affine_symmetric" else None
+    lds_gbs = (LDS_BYTES_PER_VOXEL_EVAL_NW9 * (work_fast / max(n_fast, 1)) / (avg_ms * 1e-3) / 1e9
+               if cfg["nhood_width"] == 9 else None)
     lat_ms = np.array(lat) * 1e3
     out = {
         "metric": "NCC voxel-evals/s + registrations/s (p50/p99 latency) vs plain GPU, 1 GPU",
@@ -503,7 +511,15 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
             # the committed traffic summary covers the default workload only
-            "traffic": measured_traffic() if args.workload == "pair256_affine_symmetric" else None,
+            "traffic": traffic,
+            "hbm": None if traffic is None else {
+                "achieved": traffic / (avg_ms * 1e-3) / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": traffic / (avg_ms * 1e-3) / 1e9 / PEAK_HBM_GBS},
+            "lds": None if lds_gbs is None else {
+                "achieved": lds_gbs, "peak": PEAK_LDS_GBS, "unit": "GB/s",
+                "frac": lds_gbs / PEAK_LDS_GBS,
+                "bytes_per_voxel_eval": LDS_BYTES_PER_VOXEL_EVAL_NW9,
+                "note": "inner-loop window and block reads of the tile kernel"},
             "launches": n_fast,
             "launch_note": f"{ROOFLINE_REGISTRATIONS} untimed registrations after the timed "
                            "steps, block-matching passes one after the other",
@@ -513,6 +529,9 @@ def main():
             "masked_kernel_share_of_evals_full_resolution": work_mask / max(work_fast1 + work_mask, 1.0),
         },
         "cpu_baseline": None,
+        # the NCC search's tile shape per level grid, picked by timing each
+        # shape on the first search of that grid (the warm-up registration)
+        "tile_autotune": capi.tile_autotune_report().splitlines(),
     }
     if not args.no_cpu_baseline:
         # The host cores a one-GPU job may use (16 on the MI355X boxes, whatever

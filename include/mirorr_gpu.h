@@ -178,6 +178,20 @@ mirorr_status mirorr_bm_match_async(mirorr_bm* bm);
  * valid candidates times BW^3 (the unit of the throughput metric). */
 mirorr_status mirorr_bm_voxel_evals(mirorr_bm* bm, int64_t* evals);
 
+/* Tile shapes of the NCC search (BW 4, BG 1). Each neighbourhood width has a
+ * few shapes (blocks per workgroup, candidate rows per lane); every shape
+ * gives the same records. The first search of a (width, gap, storage,
+ * grid) in the process times each shape that fits and keeps the fastest
+ * (MIRORR_TILE_AUTOTUNE=0 in the environment keeps the first shape).
+ * No counterpart in the reference, whose work-group shape is fixed
+ * (itkBlockMatcherGPU.txx:330-338).
+ * mirorr_tile_force pins width nhood_width to its index-th shape (0 = the
+ * default; a negative index returns it to the autotuner).
+ * mirorr_tile_autotune_report writes the choices so far (one line per key,
+ * with each trial's time) and returns the report's length. */
+mirorr_status mirorr_tile_force(int nhood_width, int index);
+int mirorr_tile_autotune_report(char* buf, size_t cap);
+
 /* ---- device-resident registration ------------------------------------ */
 
 typedef struct mirorr_volume mirorr_volume;

@@ -138,6 +138,8 @@ _SIGS = {
                                      C.POINTER(C.c_double)]),
     "mirorr_half_encode": (C.c_int, [_P, C.c_size_t, _P, C.POINTER(C.c_double),
                                      C.POINTER(C.c_double)]),
+    "mirorr_tile_force": (C.c_int, [C.c_int, C.c_int]),
+    "mirorr_tile_autotune_report": (C.c_int, [C.c_char_p, C.c_size_t]),
     "mirorr_volume_destroy": (None, [_P]),
     "mirorr_register": (C.c_int, [_P, C.POINTER(RegParams), _P, _P, _P, _P, _P, _P,
                                   C.POINTER(RegStats)]),
@@ -217,3 +219,11 @@ def check_io(status: int):
 
 def ptr(a):
     return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def tile_autotune_report() -> str:
+    """The NCC search's tile-shape choices so far (one line per key)."""
+    n = lib().mirorr_tile_autotune_report(None, 0)
+    buf = C.create_string_buffer(n + 1)
+    lib().mirorr_tile_autotune_report(buf, n + 1)
+    return buf.value.decode()

@@ -28,6 +28,13 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
 
 namespace mirorr {
 namespace {
@@ -2285,12 +2292,7 @@ __global__ __launch_bounds__(kThreads) void vsel_mark(const float* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// Tile configurations per neighbourhood width (BW = 4, BG = 1).
-using Cfg5 = TileCfg<5, 5, 5, 10>;
-using Cfg7 = TileCfg<7, 4, 3, 6>;
-using Cfg9 = TileCfg<9, 3, 3, 3>;
-using Cfg11 = TileCfg<11, 4, 7, 1>;
-using Cfg13 = TileCfg<13, 5, 2, 3>;
+// Tile configurations (BW = 4, BG = 1): see kTileVariants below.
 
 template <class T>
 int64_t tiles_of(const BMGrid& g) {
@@ -2331,17 +2333,6 @@ void run_tiled(const BMGrid& g, TileArgs a, BMScratch& scr, hipStream_t s) {
   if (scr.prof) scr.prof->end(kProfMasked, s);
 }
 
-template <class E>
-void tiled_search_as(const BMGrid& g, const TileArgs& a, BMScratch& scr, hipStream_t s) {
-  switch (g.nw) {
-    case 5: run_tiled<Cfg5, E>(g, a, scr, s); break;
-    case 7: run_tiled<Cfg7, E>(g, a, scr, s); break;
-    case 9: run_tiled<Cfg9, E>(g, a, scr, s); break;
-    case 11: run_tiled<Cfg11, E>(g, a, scr, s); break;
-    case 13: run_tiled<Cfg13, E>(g, a, scr, s); break;
-  }
-}
-
 }  // namespace
 
 template <class T>
@@ -2357,27 +2348,183 @@ bool lds_fits(int ng) {
   return fast <= 65536 && masked <= 65536;
 }
 
+// ---------------------------------------------------------------------------
+// Tile shapes and their autotuner. Every shape of one neighbourhood width
+// scores every candidate with the same operations in the same order, so the
+// records do not depend on the shape; only the time does. The first search
+// of a (width, gap, storage, grid) in the process runs each shape that fits
+// its LDS once, timed with HIP events on the caller's stream, and keeps the
+// fastest for the later searches of that key. MIRORR_TILE_AUTOTUNE=0 keeps
+// the first (hand-picked) shape of each width.
+namespace {
+
+struct TileVariant {
+  int nw;
+  bool (*fits)(int ng);
+  int64_t (*tiles)(const BMGrid&);
+  void (*run_f32)(const BMGrid&, TileArgs, BMScratch&, hipStream_t);
+  void (*run_f16)(const BMGrid&, TileArgs, BMScratch&, hipStream_t);
+  const char* name;  // NW, candidate rows per lane, blocks per tile in y and z
+};
+
+#define MIRORR_TILE(NW, KY, TY, TZ)                                                       \
+  TileVariant {                                                                             \
+    NW, lds_fits<TileCfg<NW, KY, TY, TZ>>, tiles_of<TileCfg<NW, KY, TY, TZ>>,               \
+        run_tiled<TileCfg<NW, KY, TY, TZ>, float>, run_tiled<TileCfg<NW, KY, TY, TZ>, half_t>, \
+        #NW "/" #KY "/" #TY "x" #TZ                                                         \
+  }
+// The first shape of each width is the default.
+const TileVariant kTileVariants[] = {
+    MIRORR_TILE(5, 5, 5, 10),  MIRORR_TILE(5, 5, 10, 5), MIRORR_TILE(5, 5, 6, 6),
+    MIRORR_TILE(7, 4, 3, 6),   MIRORR_TILE(7, 4, 6, 3),  MIRORR_TILE(7, 4, 3, 3),
+    MIRORR_TILE(9, 3, 3, 3),   MIRORR_TILE(9, 3, 2, 3),  MIRORR_TILE(9, 5, 2, 7),
+    MIRORR_TILE(11, 4, 7, 1),  MIRORR_TILE(11, 4, 1, 7), MIRORR_TILE(11, 4, 3, 2),
+    MIRORR_TILE(13, 5, 2, 3),  MIRORR_TILE(13, 5, 3, 2), MIRORR_TILE(13, 5, 1, 6),
+};
+#undef MIRORR_TILE
+constexpr int kNumTileVariants = (int)(sizeof(kTileVariants) / sizeof(kTileVariants[0]));
+
+int default_variant(int nw) {
+  for (int i = 0; i < kNumTileVariants; ++i)
+    if (kTileVariants[i].nw == nw) return i;
+  return -1;
+}
+
+struct TuneKey {
+  int nw, ng, storage, dims[3];
+  bool operator<(const TuneKey& o) const {
+    return std::tie(nw, ng, storage, dims[0], dims[1], dims[2]) <
+           std::tie(o.nw, o.ng, o.storage, o.dims[0], o.dims[1], o.dims[2]);
+  }
+};
+struct TuneResult {
+  int variant;
+  std::vector<std::pair<int, float>> times;  // (variant, ms) of the trial
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneResult> g_tune;
+std::map<int, int> g_forced;  // width -> variant pinned by tile_force
+
+bool autotune_enabled() {
+  static const bool on = [] {
+    const char* e = std::getenv("MIRORR_TILE_AUTOTUNE");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
+void run_variant(int v, int storage, const BMGrid& g, const TileArgs& a, BMScratch& scr,
+                 hipStream_t s) {
+  if (storage == kStoreF16)
+    kTileVariants[v].run_f16(g, a, scr, s);
+  else
+    kTileVariants[v].run_f32(g, a, scr, s);
+}
+
+// Runs the search with the tuned shape for this key, tuning it first when
+// the key is new (each fitting shape once; the records of every trial are
+// the same, so the last trial's stand).
+void tiled_search(const BMGrid& g, const TileArgs& a, BMScratch& scr, hipStream_t s,
+                  int storage) {
+  const TuneKey key{g.nw, g.ng, storage, {g.dims[0], g.dims[1], g.dims[2]}};
+  int chosen = -1;  // -1: tune now
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto f = g_forced.find(g.nw);
+    auto it = g_tune.find(key);
+    if (f != g_forced.end() && kTileVariants[f->second].fits(g.ng) &&
+        kTileVariants[f->second].tiles(g) <= scr.ntiles_cap)
+      chosen = f->second;
+    else if (it != g_tune.end())
+      chosen = it->second.variant;
+    else if (!autotune_enabled() || (scr.prof && scr.prof->on()))
+      chosen = default_variant(g.nw);  // not tuned now
+  }
+  if (chosen >= 0) {
+    run_variant(chosen, storage, g, a, scr, s);
+    return;
+  }
+  TuneResult res{default_variant(g.nw), {}};
+  float best = 0.0f;
+  hipEvent_t e0, e1;
+  MIRORR_HIP(hipEventCreate(&e0));
+  MIRORR_HIP(hipEventCreate(&e1));
+  try {
+    for (int v = 0; v < kNumTileVariants; ++v) {
+      if (kTileVariants[v].nw != g.nw || !kTileVariants[v].fits(g.ng)) continue;
+      if (kTileVariants[v].tiles(g) > scr.ntiles_cap) continue;
+      MIRORR_HIP(hipEventRecord(e0, s));
+      run_variant(v, storage, g, a, scr, s);
+      MIRORR_HIP(hipEventRecord(e1, s));
+      MIRORR_HIP(hipEventSynchronize(e1));
+      float ms = 0.0f;
+      MIRORR_HIP(hipEventElapsedTime(&ms, e0, e1));
+      res.times.emplace_back(v, ms);
+      if (res.times.size() == 1 || ms < best) {
+        best = ms;
+        res.variant = v;
+      }
+    }
+  } catch (...) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    throw;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune.emplace(key, res);
+}
+
+}  // namespace
+
 bool tiled_supported(const BMGrid& g) {
   if (g.bw != 4 || g.bg != 1 || g.ng < 1) return false;
-  switch (g.nw) {
-    case 5: return lds_fits<Cfg5>(g.ng);
-    case 7: return lds_fits<Cfg7>(g.ng);
-    case 9: return lds_fits<Cfg9>(g.ng);
-    case 11: return lds_fits<Cfg11>(g.ng);
-    case 13: return lds_fits<Cfg13>(g.ng);
-    default: return false;
-  }
+  const int v = default_variant(g.nw);
+  return v >= 0 && kTileVariants[v].fits(g.ng);
 }
 
 int64_t tile_count(const BMGrid& g) {
-  switch (g.nw) {
-    case 5: return tiles_of<Cfg5>(g);
-    case 7: return tiles_of<Cfg7>(g);
-    case 9: return tiles_of<Cfg9>(g);
-    case 11: return tiles_of<Cfg11>(g);
-    case 13: return tiles_of<Cfg13>(g);
-    default: return 0;
+  int64_t n = 0;
+  for (int v = 0; v < kNumTileVariants; ++v)
+    if (kTileVariants[v].nw == g.nw) n = std::max(n, kTileVariants[v].tiles(g));
+  return n;
+}
+
+bool tile_force(int nw, int index) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  if (index < 0) {
+    g_forced.erase(nw);
+    return true;
   }
+  for (int v = 0, k = 0; v < kNumTileVariants; ++v) {
+    if (kTileVariants[v].nw != nw) continue;
+    if (k++ == index) {
+      g_forced[nw] = v;
+      return true;
+    }
+  }
+  return false;
+}
+
+std::string tile_autotune_report() {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  std::string out;
+  for (const auto& kv : g_tune) {
+    const TuneKey& k = kv.first;
+    char line[256];
+    std::snprintf(line, sizeof(line), "nw %d ng %d %s grid %dx%dx%d: %s;", k.nw, k.ng,
+                  k.storage == kStoreF16 ? "f16" : "f32", k.dims[0], k.dims[1], k.dims[2],
+                  kTileVariants[kv.second.variant].name);
+    out += line;
+    for (const auto& t : kv.second.times) {
+      std::snprintf(line, sizeof(line), " %s %.1f us", kTileVariants[t.first].name,
+                    1e3 * t.second);
+      out += line;
+    }
+    out += "\n";
+  }
+  return out;
 }
 
 namespace {
@@ -2513,10 +2660,7 @@ void launch_ncc_search(const void* x, const uint8_t* bmask, const void* y,
   a.out_best = res.best;
   a.out_second = res.second;
   a.out_match = res.match;
-  if (storage == kStoreF16)
-    tiled_search_as<half_t>(g, a, scr, s);
-  else
-    tiled_search_as<float>(g, a, scr, s);
+  tiled_search(g, a, scr, s, storage);
   if (scr.rlist) {
     // Blocks listed by the search kernels are searched again in double.
     RescoreArgs ra{x, bmask, y, smask, st.ms, scr.rescored, scr.exact_records};

@@ -1,6 +1,8 @@
 // Launchers of the block-matching kernels (implemented in bm_kernels.hip).
 #pragma once
 
+#include <string>
+
 #include "common.hpp"
 #include "profiler.hpp"
 
@@ -105,8 +107,16 @@ constexpr int kMaxMetricBW = 16;
 void launch_metric_search(int metric, const float* x, const uint8_t* bmask, const float* y,
                           const uint8_t* smask, const BMGrid& g, BMResult res, hipStream_t s,
                           const uint32_t* flags = nullptr);
+// Tiles of the largest tile shape count for this grid (queue sizing), and
+// whether the tiled kernels serve it.
 int64_t tile_count(const BMGrid& g);
 bool tiled_supported(const BMGrid& g);
+// The autotuner's choices so far, one line per (width, gap, storage, grid):
+// the chosen shape and each trial's time.
+std::string tile_autotune_report();
+// Pins the tile shape of width nw to its index-th variant (0 = the default),
+// or returns it to the autotuner (index < 0). False for an unknown index.
+bool tile_force(int nw, int index);
 
 // Score filter + mm conversion (itkBlockMatcher.txx:886-916,
 // itkAbstractBlockMatcher.h:271-287). kept[i] in {0,1}; pts are nblocks x 3.

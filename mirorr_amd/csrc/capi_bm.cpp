@@ -1,6 +1,7 @@
 // C-ABI: context and block matcher (replaces BlockMatcherGPU,
 // itkBlockMatcherGPU.txx:186-522,597-830, behind the AbstractBlockMatcher
 // contract of itkAbstractBlockMatcher.h:41-313).
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -368,6 +369,16 @@ mirorr_status mirorr_bm_set_volume_device(mirorr_bm* bm, int role, const float* 
     s.version = ++g_version;
     s.set = true;
   });
+}
+
+int mirorr_tile_autotune_report(char* buf, size_t cap) {
+  const std::string r = tile_autotune_report();
+  if (buf && cap > 0) std::snprintf(buf, cap, "%s", r.c_str());
+  return (int)r.size();
+}
+
+mirorr_status mirorr_tile_force(int nhood_width, int index) {
+  return tile_force(nhood_width, index) ? MIRORR_OK : MIRORR_ERR_ARG;
 }
 
 mirorr_status mirorr_bm_swap(mirorr_bm* bm) {
