@@ -11,9 +11,11 @@ level. Prints one JSON line: NCC voxel-evals/s over the timed registrations,
 with registrations/s and p50/p99 latency beside it, the roofline of the NCC
 search kernel (HIP events on the library stream, taken on further untimed
 registrations), a CPU baseline (the oracle's threaded block matcher on one
-full-resolution pass, on the host cores and on one core) and the oracle's
+full-resolution pass, on the host cores and on one core), the oracle's
 own registration of the pair, against which the final transform is
-compared.
+compared, and a served batch: 16 distinct pairs of the workload through the
+registration server (float32 and binary16 storage) against the plain path
+registering them one at a time.
 """
 from __future__ import annotations
 
@@ -252,11 +254,8 @@ def _served_line(cfg, args, extra):
     return out
 
 
-def run_batch(args, cfg):
-    """One step = every pair of the batch registered once, through the server
-    (all submitted at once) and, for comparison, through the plain path."""
-    from mirorr_amd.blockmatcher import default_context
-    from mirorr_amd.server import RegistrationServer
+def batch_jobs(cfg):
+    """The pairs (generator jobs) and registration parameters of a batch."""
     rng = np.random.default_rng(cfg["seed"])
     jobs, params = [], []
     for i in range(cfg["pairs"]):
@@ -270,56 +269,92 @@ def run_batch(args, cfg):
                                                      3))
         jobs.append((i, dims, sp, model, cfg[model]["ellipsoids"], cfg["seed"] * 1000 + i))
         params.append(_params(model, cfg[model]))
-    print(f"bench: generating {len(jobs)} pairs", file=sys.stderr, flush=True)
-    pairs = _pool_map(_make_batch_pair, jobs)
+    return jobs, params
+
+
+def measure_batch(pairs, params, cfg, steps, warmup, storages=("f32",)):
+    """One step = every pair registered once: through the plain path (one
+    context, one pair at a time, float32) and through the server (all
+    submitted at once) in each storage. Uploads are included on both sides."""
+    from mirorr_amd.blockmatcher import default_context
+    from mirorr_amd.server import RegistrationServer
     ctx = default_context()
     print("bench: plain path", file=sys.stderr, flush=True)
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         for pr, p in zip(pairs, params):
             _plain_run(ctx, pr, p)
     t = time.perf_counter()
     plain = []
-    for _ in range(args.steps):
+    for _ in range(steps):
         plain = [_plain_run(ctx, pr, p) for pr, p in zip(pairs, params)]
     t_plain = time.perf_counter() - t
     evals = sum(e for _, _, e in plain)
-    print(f"bench: plain {t_plain / args.steps * 1e3:.0f} ms per batch", file=sys.stderr,
-          flush=True)
-    with RegistrationServer(slots=cfg["slots"]) as srv:
-        def batch():
-            ts = [srv.submit(pr.fixed, pr.moving, p, borrow=True) for pr, p in zip(pairs, params)]
-            return [srv.poll(x) for x in ts]
-        for _ in range(args.warmup):
-            batch()
-        lat, res = [], []
-        t = time.perf_counter()
-        for k in range(args.steps):
-            res = batch()
-            lat += [r["latency"] for r in res]
-            print(f"bench: served step {k + 1}/{args.steps}", file=sys.stderr, flush=True)
-        t_served = time.perf_counter() - t
-        info = srv.info()
-    bad = [r["error"] for r in res if r["status"] != 0]
-    if bad:
-        raise RuntimeError(f"served registrations failed: {bad[:3]}")
-    same = all(np.array_equal(r["params"], q) and np.array_equal(r["center"], c)
-               for r, (q, c, _) in zip(res, plain))
-    n = cfg["pairs"] * args.steps
-    lat_ms = np.array(lat) * 1e3
+    print(f"bench: plain {t_plain / steps * 1e3:.0f} ms per batch", file=sys.stderr, flush=True)
+    n = len(pairs) * steps
+    out = {"plain_registrations_per_s": n / t_plain, "voxel_evals_per_step": evals,
+           "plain_ms_per_step": t_plain / steps * 1e3}
+    for storage in storages:
+        with RegistrationServer(slots=cfg["slots"]) as srv:
+            def batch():
+                ts = [srv.submit(pr.fixed, pr.moving, p, borrow=True, storage=storage)
+                      for pr, p in zip(pairs, params)]
+                return [srv.poll(x) for x in ts]
+            for _ in range(warmup):
+                batch()
+            lat, res = [], []
+            t = time.perf_counter()
+            for k in range(steps):
+                res = batch()
+                lat += [r["latency"] for r in res]
+                print(f"bench: served {storage} step {k + 1}/{steps}", file=sys.stderr,
+                      flush=True)
+            t_served = time.perf_counter() - t
+            info = srv.info()
+        bad = [r["error"] for r in res if r["status"] != 0]
+        if bad:
+            raise RuntimeError(f"served registrations failed: {bad[:3]}")
+        # binary16 storage rounds the images, so only float32 results are
+        # expected to equal the plain path's bit for bit
+        same = (all(np.array_equal(r["params"], q) and np.array_equal(r["center"], c)
+                    for r, (q, c, _) in zip(res, plain)) if storage == "f32" else None)
+        lat_ms = np.array(lat) * 1e3
+        out[storage] = {
+            "registrations_per_s": n / t_served,
+            "vs_plain": t_plain / t_served,
+            "ms_per_step": t_served / steps * 1e3,
+            "value": evals * steps / t_served,
+            "latency_ms_p50": float(np.percentile(lat_ms, 50)),
+            "latency_ms_p99": float(np.percentile(lat_ms, 99)),
+            "results_identical_to_plain": same,
+            "max_running": info["max_running"],
+            "hbm_peak_bytes": info["hbm_peak"],
+        }
+    return out
+
+
+def run_batch(args, cfg):
+    """One step = every pair of the batch registered once, through the server
+    (all submitted at once) and, for comparison, through the plain path."""
+    jobs, params = batch_jobs(cfg)
+    print(f"bench: generating {len(jobs)} pairs", file=sys.stderr, flush=True)
+    pairs = _pool_map(_make_batch_pair, jobs)
+    m = measure_batch(pairs, params, cfg, args.steps, args.warmup, ("f32", "f16"))
+    f32 = m["f32"]
     return _served_line(cfg, args, {
-        "value": evals * args.steps / t_served,
+        "value": f32["value"],
         "unit": "voxel-evals/s",
-        "ms_per_step": t_served / args.steps * 1e3,
-        "registrations_per_s": n / t_served,
-        "plain_registrations_per_s": n / t_plain,
-        "vs_plain": t_plain / t_served,
-        "latency_ms_p50": float(np.percentile(lat_ms, 50)),
-        "latency_ms_p99": float(np.percentile(lat_ms, 99)),
+        "ms_per_step": f32["ms_per_step"],
+        "registrations_per_s": f32["registrations_per_s"],
+        "plain_registrations_per_s": m["plain_registrations_per_s"],
+        "vs_plain": f32["vs_plain"],
+        "latency_ms_p50": f32["latency_ms_p50"],
+        "latency_ms_p99": f32["latency_ms_p99"],
         "latency_note": "all pairs of a step are submitted at once",
-        "voxel_evals_per_step": evals,
-        "results_identical_to_plain": bool(same),
-        "max_running": info["max_running"],
-        "hbm_peak_bytes": info["hbm_peak"],
+        "voxel_evals_per_step": m["voxel_evals_per_step"],
+        "results_identical_to_plain": f32["results_identical_to_plain"],
+        "max_running": f32["max_running"],
+        "hbm_peak_bytes": f32["hbm_peak_bytes"],
+        "f16_storage": m["f16"],
     })
 
 
@@ -402,6 +437,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-served", action="store_true",
+                    help="skip the served batch (16 pairs of the workload through the server)")
     ap.add_argument("--workload", default="pair256_affine_symmetric",
                     choices=sorted(WORKLOADS) + sorted(SERVED))
     ap.add_argument("--slots", type=int, default=0,
@@ -424,6 +461,14 @@ def main():
     from mirorr_amd.registration import DeviceVolume, MirorrPyramid
 
     cfg = workload(args.workload)
+    served = None
+    if not args.no_served and args.workload == "pair256_affine_symmetric":
+        # generated in worker processes before the GPU is touched
+        bcfg = dict(SERVED["batch16_pair256_affine"])
+        bjobs, bparams = batch_jobs(bcfg)
+        print(f"bench: generating {len(bjobs)} pairs for the served batch", file=sys.stderr,
+              flush=True)
+        served = (bcfg, _pool_map(_make_batch_pair, bjobs), bparams)
     pair = make_inputs(cfg)
     ctx = default_context()
     t_up = time.perf_counter()
@@ -533,6 +578,17 @@ def main():
         # shape on the first search of that grid (the warm-up registration)
         "tile_autotune": capi.tile_autotune_report().splitlines(),
     }
+    if served is not None:
+        bcfg, bpairs, bparams = served
+        m = measure_batch(bpairs, bparams, bcfg, 2, 1, ("f32", "f16"))
+        out["served"] = {
+            "workload": bcfg["workload"], "pairs": bcfg["pairs"], "slots": bcfg["slots"],
+            "note": "16 distinct 256x256x128 affine pairs submitted at once, uploads included; "
+                    "plain = one pair at a time on one context, float32",
+            "plain_registrations_per_s": m["plain_registrations_per_s"],
+            "f32": m["f32"], "f16": m["f16"],
+        }
+        del bpairs
     if not args.no_cpu_baseline:
         # The host cores a one-GPU job may use (16 on the MI355X boxes, whatever
         # nproc reports), then one core on a smaller sample.
