@@ -443,6 +443,9 @@ def main():
                     choices=sorted(WORKLOADS) + sorted(SERVED))
     ap.add_argument("--slots", type=int, default=0,
                     help="served workloads: worker slots (0: the workload's own)")
+    ap.add_argument("--iteration-graphs", type=int, choices=(0, 1), default=None,
+                    help="replay each level's outer iterations from captured HIP graphs "
+                         "(default: the library's default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported "
@@ -477,6 +480,8 @@ def main():
     upload_ms = (time.perf_counter() - t_up) * 1e3  # host -> HBM, outside the timed region
     reg = MirorrPyramid(ctx)
     configure(reg, cfg)
+    if args.iteration_graphs is not None:
+        ctx.iteration_graphs = bool(args.iteration_graphs)
     print(f"bench: warmup {args.warmup}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         reg.RunMirorr(fixed_dev=fixed, moving_dev=moving)
@@ -499,6 +504,18 @@ def main():
               f"iterations {reg.stats['iterations']}", file=sys.stderr, flush=True)
     ctx.synchronize()
     total = time.perf_counter() - t0
+    # The same steps with iteration graphs switched the other way (one
+    # untimed registration first, which captures them when they are on).
+    graphs_on = ctx.iteration_graphs
+    ctx.iteration_graphs = not graphs_on
+    reg.RunMirorr(fixed_dev=fixed, moving_dev=moving)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        reg.RunMirorr(fixed_dev=fixed, moving_dev=moving)
+    ctx.synchronize()
+    other_ms = (time.perf_counter() - t1) / args.steps * 1e3
+    ctx.iteration_graphs = graphs_on
     ctx.set_profiling(ctx.PROF_FAST | ctx.PROF_FINEST_ONLY)
     ctx.profile_reset()
     for _ in range(ROOFLINE_REGISTRATIONS):
@@ -544,6 +561,10 @@ def main():
                       "x 64, including blocks whose base mask is off everywhere (not searched)",
         "transform_error_mm": err_mm,
         "upload_ms": upload_ms,
+        "iteration_graphs": {
+            "on": graphs_on,
+            "note": "each level's outer iterations replayed from a captured HIP graph",
+            "ms_per_step_other_setting": other_ms},
         "roofline": {
             "bound": "valu",
             "compute": "fp32 vector ALU FMAs (the search uses no MFMA); packed FP32 is "

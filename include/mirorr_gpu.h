@@ -103,6 +103,18 @@ mirorr_status mirorr_ctx_synchronize(mirorr_ctx* ctx);
 /* The context's HIP stream (hipStream_t) for callers that interleave work. */
 void* mirorr_ctx_stream(mirorr_ctx* ctx);
 
+/* Iteration graphs (no counterpart in the reference, whose outer loop
+ * itkSymMirorrRegistrationMethod.txx:456-692 runs on the host): with `on`,
+ * mirorr_register captures the second outer iteration of each level (the
+ * resampling, both block-matching passes, the LTS start, its first four
+ * inner iterations and the finish) as a HIP graph and replays it for the
+ * level's later iterations; further inner iterations, when the LTS needs
+ * them, are queued as usual. Results are identical either way. Off by
+ * default, or as the environment variable MIRORR_ITERATION_GRAPH says at
+ * context creation; ignored while per-kernel timing is on. */
+mirorr_status mirorr_ctx_set_iteration_graphs(mirorr_ctx* ctx, int on);
+int mirorr_ctx_iteration_graphs(const mirorr_ctx* ctx);
+
 /* Per-kernel timing (no counterpart in the reference; the OpenCL host timed
  * its kernel with CL_QUEUE_PROFILING_ENABLE, itkBlockMatcherGPU.txx:380-391).
  * `on` selects the instrumented kernels (0 switches profiling off):

@@ -63,6 +63,16 @@ class Context:
         mask = (self.PROF_FAST | self.PROF_MASKED) if on is True else int(on)
         capi.check(capi.lib().mirorr_ctx_set_profiling(self._h, mask), self._h)
 
+    @property
+    def iteration_graphs(self) -> bool:
+        """Whether registrations replay each level's later outer iterations
+        from a captured HIP graph (results are identical either way)."""
+        return bool(capi.lib().mirorr_ctx_iteration_graphs(self._h))
+
+    @iteration_graphs.setter
+    def iteration_graphs(self, on: bool):
+        capi.check(capi.lib().mirorr_ctx_set_iteration_graphs(self._h, 1 if on else 0), self._h)
+
     def profile(self, kernel: str):
         """(launches, total ms, work units) of an instrumented kernel."""
         n = C.c_int64()

@@ -114,6 +114,8 @@ _SIGS = {
     "mirorr_ctx_synchronize": (C.c_int, [_P]),
     "mirorr_ctx_stream": (_P, [_P]),
     "mirorr_ctx_set_profiling": (C.c_int, [_P, C.c_int]),
+    "mirorr_ctx_set_iteration_graphs": (C.c_int, [_P, C.c_int]),
+    "mirorr_ctx_iteration_graphs": (C.c_int, [_P]),
     "mirorr_ctx_profile_get": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_int64),
                                          C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "mirorr_ctx_profile_reset": (C.c_int, [_P]),

@@ -2440,6 +2440,13 @@ void tiled_search(const BMGrid& g, const TileArgs& a, BMScratch& scr, hipStream_
     else if (!autotune_enabled() || (scr.prof && scr.prof->on()))
       chosen = default_variant(g.nw);  // not tuned now
   }
+  if (chosen < 0) {
+    // A stream under graph capture cannot be timed: the default shape is
+    // recorded and the key stays untuned.
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    MIRORR_HIP(hipStreamIsCapturing(s, &cs));
+    if (cs != hipStreamCaptureStatusNone) chosen = default_variant(g.nw);
+  }
   if (chosen >= 0) {
     run_variant(chosen, storage, g, a, scr, s);
     return;
@@ -2477,6 +2484,16 @@ void tiled_search(const BMGrid& g, const TileArgs& a, BMScratch& scr, hipStream_
 }
 
 }  // namespace
+
+int tile_choice(const BMGrid& g, int storage) {
+  const TuneKey key{g.nw, g.ng, storage, {g.dims[0], g.dims[1], g.dims[2]}};
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto f = g_forced.find(g.nw);
+  if (f != g_forced.end()) return 1000 + f->second;  // pinned (the cap is checked per launch)
+  auto it = g_tune.find(key);
+  if (it != g_tune.end()) return it->second.variant;
+  return autotune_enabled() ? -1 : default_variant(g.nw);
+}
 
 bool tiled_supported(const BMGrid& g) {
   if (g.bw != 4 || g.bg != 1 || g.ng < 1) return false;

@@ -110,6 +110,10 @@ void launch_metric_search(int metric, const float* x, const uint8_t* bmask, cons
 // Tiles of the largest tile shape count for this grid (queue sizing), and
 // whether the tiled kernels serve it.
 int64_t tile_count(const BMGrid& g);
+// The tile shape a search over `g` with volumes of `storage` runs with now,
+// as an opaque code that changes whenever that choice does: a pinned shape,
+// the tuned one, or -1 when the next search would tune.
+int tile_choice(const BMGrid& g, int storage);
 bool tiled_supported(const BMGrid& g);
 // The autotuner's choices so far, one line per (width, gap, storage, grid):
 // the chosen shape and each trial's time.

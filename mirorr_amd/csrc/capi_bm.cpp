@@ -1,6 +1,7 @@
 // C-ABI: context and block matcher (replaces BlockMatcherGPU,
 // itkBlockMatcherGPU.txx:186-522,597-830, behind the AbstractBlockMatcher
 // contract of itkAbstractBlockMatcher.h:41-313).
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -187,6 +188,8 @@ mirorr_status mirorr_ctx_create(int device, size_t hbm_budget, mirorr_ctx** out)
     if (device < 0 || device >= n) throw Error(kErrPlatform, "no such HIP device");
     MIRORR_HIP(hipSetDevice(device));
     MIRORR_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    const char* e = std::getenv("MIRORR_ITERATION_GRAPH");
+    if (e && e[0]) c->iteration_graphs = e[0] != '0';
   });
   if (st != MIRORR_OK) {
     delete c;
@@ -231,6 +234,15 @@ mirorr_status mirorr_ctx_set_profiling(mirorr_ctx* c, int on) {
                    &c->budget);
   });
 }
+
+mirorr_status mirorr_ctx_set_iteration_graphs(mirorr_ctx* c, int on) {
+  if (!c) return MIRORR_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->iteration_graphs = on != 0;
+  return MIRORR_OK;
+}
+
+int mirorr_ctx_iteration_graphs(const mirorr_ctx* c) { return c && c->iteration_graphs ? 1 : 0; }
 
 mirorr_status mirorr_ctx_profile_get(mirorr_ctx* c, const char* kernel, int64_t* launches,
                                      double* total_ms, double* work) {

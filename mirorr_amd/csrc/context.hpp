@@ -32,6 +32,9 @@ struct mirorr_ctx {
   // worker contexts do not: their requests already overlap, and more
   // streams than hardware queues measured slower.
   bool overlap_passes = true;
+  // Whether a registration replays each level's outer iterations after the
+  // first from a HIP graph captured at the second (mirorr_ctx_set_iteration_graphs).
+  bool iteration_graphs = false;
   mirorr::Budget budget;
   std::string last_error;
   std::mutex mu;

@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "bm_kernels.hpp"
@@ -191,6 +192,74 @@ struct Flags {
   int stop;
 };
 
+// Bytes of every host value an outer iteration's launches depend on: two
+// iterations with equal keys queue the same launches with the same
+// arguments. Fields are appended one by one (no struct padding).
+struct KeyBytes {
+  std::vector<unsigned char> b;
+  template <class T>
+  void add(const T& v) {
+    static_assert(std::is_trivially_copyable<T>::value, "plain values only");
+    const unsigned char* p = reinterpret_cast<const unsigned char*>(&v);
+    b.insert(b.end(), p, p + sizeof(T));
+  }
+  void geom(const Geom& g) {
+    for (int a = 0; a < 3; ++a) add(g.dims[a]);
+    for (int a = 0; a < 3; ++a) add(g.spacing[a]);
+    for (int a = 0; a < 3; ++a) add(g.origin[a]);
+    for (int i = 0; i < 9; ++i) add(g.dir[i]);
+  }
+  void buffers(const Workspace& w) {
+    for (const DeviceBuffer* d :
+         {&w.Fh, &w.Mh, &w.Fmh, &w.Mmh, &w.statsF, &w.statsM, &w.xtF, &w.xtM, &w.sxxF, &w.sxxM,
+          &w.sumF, &w.sumM, &w.flF, &w.flM, &w.best, &w.second, &w.match, &w.queue, &w.qbits,
+          &w.qcount, &w.best2, &w.second2, &w.match2, &w.queue2, &w.qbits2, &w.qcount2, &w.X,
+          &w.Y, &w.kept, &w.r, &w.hist1, &w.hist2, &w.cand, &w.part, &w.lts, &w.outer, &w.mapI,
+          &w.flagsD, &w.vsel_st, &w.vsel_hist, &w.vsel_evals, &w.rlist, &w.rcount, &w.rlist2,
+          &w.rcount2, &w.rescored, &w.rescored2})
+      add(d->as<void>());
+  }
+};
+
+// An instantiated graph of one outer iteration of a level, captured from the
+// launches an enqueue function makes on a stream (and on the streams it
+// forks from it), with the key of the host values it was captured under.
+struct IterationGraph {
+  hipGraphExec_t exec = nullptr;
+  std::vector<unsigned char> key;
+  int64_t evals = 0;  // voxel-evals the captured enqueue counted on the host
+  IterationGraph() = default;
+  IterationGraph(const IterationGraph&) = delete;
+  IterationGraph& operator=(const IterationGraph&) = delete;
+  ~IterationGraph() { reset(); }
+  void reset() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    exec = nullptr;
+    key.clear();
+    evals = 0;
+  }
+  template <class F>
+  void capture(hipStream_t s, F&& enqueue) {
+    reset();
+    MIRORR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipGraph_t g = nullptr;
+    try {
+      enqueue();
+    } catch (...) {
+      // leave the stream out of capture mode; the partial graph is dropped
+      if (hipStreamEndCapture(s, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    MIRORR_HIP(hipStreamEndCapture(s, &g));
+    const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      exec = nullptr;
+      throw Error(kErrPlatform, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+  }
+};
+
 }  // namespace
 
 namespace mirorr {
@@ -199,6 +268,9 @@ struct RegCache {
   Workspace w;
   std::vector<Level> F, M;  // pyramid levels of the fixed and moving image
   Flags* flags = nullptr;   // pinned host flags read once per LTS chunk
+  // Per level, the last captured outer iteration (mirorr_ctx_set_iteration_graphs);
+  // replayed while the level's key matches.
+  IterationGraph graphs[16];
   RegCache() { MIRORR_HIP(hipHostMalloc((void**)&flags, sizeof(Flags), hipHostMallocDefault)); }
   ~RegCache() {
     if (flags) (void)hipHostFree(flags);
@@ -334,8 +406,7 @@ struct RegRun {
     w.vsel_hist.ensure(sizeof(unsigned int) * kVarSelHistBins, bud);
   }
 
-  // Runs the LTS to convergence with predicated chunks; returns when done.
-  void run_lts(LtsState* st, OuterState* outer, int64_t npts, Flags* h_flags) {
+  LtsBuffers lts_buffers(int64_t npts) {
     LtsBuffers lb;
     lb.r = w.r.as<double>();
     lb.hist1 = w.hist1.as<unsigned int>();
@@ -343,27 +414,47 @@ struct RegRun {
     lb.cand = w.cand.as<unsigned long long>();
     lb.part = w.part.as<double>();
     lb.nparts_cap = lts_parts(npts);
-    lts_start(st, outer, kind, symmetric ? 1 : 0, p->lts_tolerance, lb, s);
-    // Each chunk is followed by the (predicated) finish and one read-back of
-    // {active, stop}: when the run has converged, the outer update is already
-    // done and its stop flag known.
-    w.flagsD.ensure(2 * sizeof(int), bud);
+    return lb;
+  }
+
+  // Inner iterations [it0, it1) (predicated on the LTS being active), the
+  // finish and one read-back of {active, stop}: when the run has converged,
+  // the outer update is already done and its stop flag known.
+  void enqueue_lts_chunk(LtsState* st, OuterState* outer, int64_t npts, Flags* h_flags, int it0,
+                         int it1) {
     static_assert(sizeof(Flags) == 2 * sizeof(int), "Flags is {active, stop}");
-    const int chunk = 4;
-    auto enqueue_chunk = [&](int it0, int it1) {
-      for (int j = it0; j < it1; ++j)
-        lts_enqueue_iteration(w.X.as<double>(), w.Y.as<double>(), w.kept.as<uint8_t>(), npts, st,
-                              outer, lb, j, s);
-      lts_finish(st, outer, w.flagsD.as<int>(), s);
-      MIRORR_HIP(hipMemcpyAsync(h_flags, w.flagsD.as<int>(), sizeof(Flags), hipMemcpyDeviceToHost,
-                                s));
-    };
-    for (int it = 0; it < 101; it += chunk) {
-      const int it1 = std::min(it + chunk, 101);
-      enqueue_chunk(it, it1);
+    LtsBuffers lb = lts_buffers(npts);
+    for (int j = it0; j < it1; ++j)
+      lts_enqueue_iteration(w.X.as<double>(), w.Y.as<double>(), w.kept.as<uint8_t>(), npts, st,
+                            outer, lb, j, s);
+    lts_finish(st, outer, w.flagsD.as<int>(), s);
+    MIRORR_HIP(hipMemcpyAsync(h_flags, w.flagsD.as<int>(), sizeof(Flags), hipMemcpyDeviceToHost,
+                              s));
+  }
+
+  static constexpr int kLtsChunk = 4;
+
+  // The LTS start and its first chunk, queued without a host wait.
+  void lts_begin(LtsState* st, OuterState* outer, int64_t npts, Flags* h_flags) {
+    LtsBuffers lb = lts_buffers(npts);
+    lts_start(st, outer, kind, symmetric ? 1 : 0, p->lts_tolerance, lb, s);
+    enqueue_lts_chunk(st, outer, npts, h_flags, 0, kLtsChunk);
+  }
+
+  // Waits for the first chunk's flags and runs further chunks while the LTS
+  // is active (at most 101 inner iterations).
+  void lts_complete(LtsState* st, OuterState* outer, int64_t npts, Flags* h_flags) {
+    MIRORR_HIP(hipStreamSynchronize(s));
+    for (int it = kLtsChunk; h_flags->active && it < 101; it += kLtsChunk) {
+      enqueue_lts_chunk(st, outer, npts, h_flags, it, std::min(it + kLtsChunk, 101));
       MIRORR_HIP(hipStreamSynchronize(s));
-      if (!h_flags->active) break;
     }
+  }
+
+  // Runs the LTS to convergence with predicated chunks; returns when done.
+  void run_lts(LtsState* st, OuterState* outer, int64_t npts, Flags* h_flags) {
+    lts_begin(st, outer, npts, h_flags);
+    lts_complete(st, outer, npts, h_flags);
   }
 };
 
@@ -460,6 +551,7 @@ void run_registration(mirorr_ctx* ctx, const mirorr_reg_params* p, const mirorr_
   w.lts.ensure(sizeof(LtsState), bud);
   w.outer.ensure(sizeof(OuterState), bud);
   w.mapI.ensure(sizeof(IndexMap), bud);
+  w.flagsD.ensure(2 * sizeof(int), bud);
   LtsState* st = w.lts.as<LtsState>();
   OuterState* outer = w.outer.as<OuterState>();
   Flags* flags = cache.flags;
@@ -563,8 +655,8 @@ void run_registration(mirorr_ctx* ctx, const mirorr_reg_params* p, const mirorr_
       // (the half-grid stays fixed through the level; hipMemcpy is ordered)
       MIRORR_HIP(hipStreamSynchronize(s));
     }
-    int it = 0;
-    for (; it < max_it; ++it) {
+    // One outer iteration up to the LTS's first chunk, queued on `s`.
+    auto enqueue_iteration = [&] {
       // 1. resample onto the half-way grid under the current transforms
       launch_resample(Fp[level].data.as<void>(), gF.dims, &outer->mapF, fmask, FG.dims,
                       &outer->mmapF, fmode, o.gHF.dims, w.Fh.as<void>(), w.Fmh.as<uint8_t>(), s,
@@ -622,8 +714,67 @@ void run_registration(mirorr_ctx* ctx, const mirorr_reg_params* p, const mirorr_
         MIRORR_HIP(hipStreamWaitEvent(s, ctx->ev_join, 0));
         join_guard.s = nullptr;  // the first stream now orders everything after the join
       }
-      // 3. LTS and transform update on the device
-      run.run_lts(st, outer, npts, flags);  // leaves flags->stop read back
+      // 3. LTS start and its first chunk
+      run.lts_begin(st, outer, npts, flags);
+    };
+    // Iteration graph of the level: every launch of an outer iteration up to
+    // the LTS's first chunk takes the same arguments within a level (the
+    // transforms live on the device). The graph cached for this level is
+    // replayed when its key (every host value those launches read) matches;
+    // otherwise the first iteration runs as it is (its searches tune their
+    // tile shapes outside any capture) and the second is captured. Host-side
+    // counts the captured enqueue adds (the voxel-evals) are added per replay.
+    const bool graphs = ctx->iteration_graphs && !ctx->prof.on();
+    IterationGraph& graph = cache.graphs[level];
+    auto iteration_key = [&] {
+      KeyBytes k;
+      k.buffers(w);
+      for (int v : {p->symmetric, p->affine, p->block_width, p->block_gap, p->nhood_width,
+                    p->nhood_gap, p->variance_select, p->block_metric, p->resampling})
+        k.add(v);
+      k.add(p->portion_kept);
+      k.add(p->lts_tolerance);
+      k.add(run.storage);
+      k.add(fixed->enc_zero);
+      k.add(moving->enc_zero);
+      k.add(Fp[level].data.as<void>());
+      k.add(Mp[level].data.as<void>());
+      k.add(fmask);
+      k.add(mmask);
+      for (const Geom* g : {&FG, &MG, &gF, &gM, (const Geom*)&o.gHF, (const Geom*)&o.gHM})
+        k.geom(*g);
+      k.add(npts);
+      k.add(portion);
+      k.add(tiled);
+      k.add((const void*)flags);
+      k.add(ctx->overlap_passes);
+      k.add((const void*)ctx->stream2);
+      if (tiled)
+        for (const Geom* g : {(const Geom*)&o.gHM, (const Geom*)&o.gHF})
+          k.add(tile_choice(make_grid(g->dims, p->block_width, p->block_gap, p->nhood_width,
+                                      p->nhood_gap),
+                            run.storage));
+      return std::move(k.b);
+    };
+    bool replay = graphs && graph.exec && graph.key == iteration_key();
+    int it = 0;
+    for (; it < max_it; ++it) {
+      if (replay) {
+        MIRORR_HIP(hipGraphLaunch(graph.exec, s));
+        stats->voxel_evals += graph.evals;
+      } else if (graphs && it >= 1) {
+        std::vector<unsigned char> key = iteration_key();
+        const int64_t ev0 = stats->voxel_evals;
+        graph.capture(s, enqueue_iteration);
+        graph.evals = stats->voxel_evals - ev0;
+        graph.key = std::move(key);
+        MIRORR_HIP(hipGraphLaunch(graph.exec, s));
+        replay = true;
+      } else {
+        enqueue_iteration();
+      }
+      // 3. further LTS chunks while it is active; the transform update
+      run.lts_complete(st, outer, npts, flags);  // leaves flags->stop read back
       if (p->verbosity >= 1) {  // as the reference's iteration report
         unsigned long long nk = 0;
         double d = 0.0;
