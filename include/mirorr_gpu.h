@@ -307,6 +307,28 @@ mirorr_status mirorr_register(mirorr_ctx* ctx, const mirorr_reg_params* p,
                               const double* init_params, const double* init_center,
                               double* out_params, double* out_center, mirorr_reg_stats* stats);
 
+/* The registrations of n pairs (n <= 256) with one set of parameters, run
+ * in lockstep. No counterpart in the reference, which registers one pair
+ * per process (mirorr.cxx:649). Each outer iteration queues every pair's
+ * resampling, window statistics and block preparation, then per matching
+ * pass ONE launch of each search kernel over a flat work-list of all the
+ * pairs' tiles (pairs of any sizes), and one launch per LTS phase for all
+ * pairs, with one host read of all the pairs' flags per chunk of inner
+ * iterations. Pairs whose schedules have fewer levels join at their first
+ * level (the finest levels align); a pair whose stop rule fires leaves its
+ * level while the others go on. Pair i's result and statistics equal those
+ * of mirorr_register on that pair alone. Needs the NCC metric with the
+ * tiled search (blocks of width 4, gap 1, nhood width 5 to 13) and volumes
+ * of one storage; per-kernel timing must be off. Arrays: fixed[n],
+ * moving[n]; init_params n x 12 and init_center n x 3 (or NULL: the
+ * centred initialisation of each pair); out_params n x 12 (the first 6 of
+ * each row for rigid transforms); out_center n x 3 or NULL; stats[n] or NULL. */
+mirorr_status mirorr_register_batch(mirorr_ctx* ctx, const mirorr_reg_params* p, int n,
+                                    const mirorr_volume* const* fixed,
+                                    const mirorr_volume* const* moving, const double* init_params,
+                                    const double* init_center, double* out_params,
+                                    double* out_center, mirorr_reg_stats* stats);
+
 /* ---- serving loop ---------------------------------------------------- */
 
 /* A registration server on one GPU. It has no counterpart in the reference,

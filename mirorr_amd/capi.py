@@ -145,6 +145,8 @@ _SIGS = {
     "mirorr_volume_destroy": (None, [_P]),
     "mirorr_register": (C.c_int, [_P, C.POINTER(RegParams), _P, _P, _P, _P, _P, _P,
                                   C.POINTER(RegStats)]),
+    "mirorr_register_batch": (C.c_int, [_P, C.POINTER(RegParams), C.c_int, _P, _P, _P, _P, _P,
+                                        _P, _P]),
     "mirorr_lts_fit": (C.c_int, [_P, C.c_int, C.c_int, _P, _P, C.c_size_t, _P, _P, _P, _P,
                                  C.POINTER(C.c_int)]),
     "mirorr_lts_fit_noalloc": (C.c_int, [_P, C.c_int, C.c_int, _P, _P, C.c_size_t, _P, _P, _P,

@@ -704,9 +704,10 @@ __device__ __forceinline__ Top2 merge_result(const MergeSlot& m) {
   return r;
 }
 
-// Fast path: all base and search masks of the tile are on.
+// Fast path: all base and search masks of the tile are on. Tile `tile` of
+// the search `a`, run by workgroup `wg` of its launch.
 template <class T, class E>
-__global__ __launch_bounds__(kThreads, 4) void ncc_tile_fast(TileArgs a) {
+__device__ __forceinline__ void tile_fast(const TileArgs& a, int tile, int wg) {
   extern __shared__ float4 dyn4[];
   float* win = reinterpret_cast<float*>(dyn4);
   __shared__ float4 xs4[T::NB * 16];
@@ -722,7 +723,6 @@ __global__ __launch_bounds__(kThreads, 4) void ncc_tile_fast(TileArgs a) {
   const int nx = g.dims[0], ny = g.dims[1], nz = g.dims[2];
   const int tid = threadIdx.x;
   int bx, by0, bz0;
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
   tile_origin<T>(a, tile, bx, by0, bz0);
   const int NG = g.ng;
   const int WY = T::fast_pitch((T::TY - 1) * NG + T::WX + T::YPAD);
@@ -796,7 +796,7 @@ __global__ __launch_bounds__(kThreads, 4) void ncc_tile_fast(TileArgs a) {
     __syncthreads();
   }
   if (a.work && tid == 0)
-    atomicAdd(a.work + blockIdx.x % kWorkStripe, tile_evals<T>(a, px, py0, pz0, bvalid));
+    atomicAdd(a.work + wg % kWorkStripe, tile_evals<T>(a, px, py0, pz0, bvalid));
 
   // Workgroup-uniform: every candidate window of the tile's blocks (those
   // inside the grid or not) lies inside the image.
@@ -943,7 +943,7 @@ __device__ __forceinline__ void load_mask(const uint8_t* wv, int ro, int v, floa
 //  * Base block partly masked (base mask b): Sxy = Sum b x~ t,
 //    Syy = Sum b t^2, Sxx = Sum b x~^2 v (5 ops per voxel).
 template <class T, class E>
-__global__ __launch_bounds__(kThreads, T::MWG) void ncc_tile_masked(TileArgs a) {
+__device__ __forceinline__ void tile_masked(const TileArgs& a, int wg) {
   constexpr int kRows = T::NW * T::NW;  // row tasks per block
   extern __shared__ float4 dyn4[];
   __shared__ float rbest[T::NB * kRows], rsecond[T::NB * kRows];  // per-task records
@@ -964,7 +964,7 @@ __global__ __launch_bounds__(kThreads, T::MWG) void ncc_tile_masked(TileArgs a) 
 
   // Workgroup (q, part) searches every msplit-th queued block of tile q
   // (more, shorter workgroups where the level has few tiles).
-  const int q = blockIdx.x / a.msplit, part = blockIdx.x % a.msplit;
+  const int q = wg / a.msplit, part = wg % a.msplit;
   if (q >= *a.qcount) return;
   const int tile = a.queue[q];
   const unsigned long long queued = a.qbits[q];
@@ -1053,7 +1053,7 @@ __global__ __launch_bounds__(kThreads, T::MWG) void ncc_tile_masked(TileArgs a) 
   __syncthreads();
 
   if (a.work && tid == 0)
-    atomicAdd(a.work + blockIdx.x % kWorkStripe, tile_evals<T>(a, px, py0, pz0, bvalid));
+    atomicAdd(a.work + wg % kWorkStripe, tile_evals<T>(a, px, py0, pz0, bvalid));
   // Row tasks (block k, cz, cy) of the queued blocks, numbered
   // k * NW^2 + cz * NW + cy. Each task's top-2 record goes to LDS: empty for
   // rows outside the image, -2 at every candidate for rows whose window holds
@@ -1309,6 +1309,47 @@ __global__ __launch_bounds__(kThreads, T::MWG) void ncc_tile_masked(TileArgs a) 
       write_block_result<T>(a, b, px, py0 + NG * (bl % T::TY), pz0 + NG * (bl / T::TY), acc);
     }
   }
+}
+
+template <class T, class E>
+__global__ __launch_bounds__(kThreads, 4) void ncc_tile_fast(TileArgs a) {
+  tile_fast<T, E>(a, xcd_tile(blockIdx.x, gridDim.x), blockIdx.x);
+}
+
+template <class T, class E>
+__global__ __launch_bounds__(kThreads, T::MWG) void ncc_tile_masked(TileArgs a) {
+  tile_masked<T, E>(a, blockIdx.x);
+}
+
+// The search of several pairs in one launch: workgroups are numbered over a
+// flat list of every pair's work items (tiles, or queued-tile parts), pair p
+// owning items [start[p], start[p + 1]). The pairs share the tile shape and
+// hence the LDS layout; each has its own arguments (images, grid, queues).
+__device__ __forceinline__ int pair_of(const int* __restrict__ start, int n, int item) {
+  int lo = 0, hi = n - 1;  // the last p with start[p] <= item
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= item) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <class T, class E>
+__global__ __launch_bounds__(kThreads, 4) void ncc_tile_fast_multi(
+    const TileArgs* __restrict__ args, const int* __restrict__ start, int n) {
+  const int item = xcd_tile(blockIdx.x, gridDim.x);
+  const int p = pair_of(start, n, item);
+  const TileArgs a = args[p];
+  tile_fast<T, E>(a, item - start[p], blockIdx.x);
+}
+
+template <class T, class E>
+__global__ __launch_bounds__(kThreads, T::MWG) void ncc_tile_masked_multi(
+    const TileArgs* __restrict__ args, const int* __restrict__ start, int n) {
+  const int p = pair_of(start, n, blockIdx.x);
+  const TileArgs a = args[p];
+  tile_masked<T, E>(a, blockIdx.x - start[p]);
 }
 
 __global__ void reset_counters(int32_t* c, int32_t* d) {
@@ -2064,6 +2105,39 @@ __global__ __launch_bounds__(kThreads) void rescore_list_kernel(BMGrid g, float*
     rescore_block<E>(ra, g, list[i], best, second, match, L);
 }
 
+// The re-scoring of several pairs' lists in one launch: workgroup w takes
+// entries w, w + gridDim.x, ... of every pair's list.
+struct RescoreItem {
+  BMGrid g;
+  float* best;
+  float* second;
+  int32_t* match;
+  RescoreArgs ra;
+  const int32_t* list;   // NULL: no re-scoring for this pair
+  const int32_t* count;
+};
+
+template <class E>
+__global__ __launch_bounds__(kThreads) void rescore_multi_kernel(
+    const RescoreItem* __restrict__ items, int npairs) {
+  __shared__ RescoreLds L;
+  for (int p = 0; p < npairs; ++p) {
+    const RescoreItem it = items[p];
+    if (!it.list) continue;
+    const int n = *it.count;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && it.ra.count && n > 0)
+      atomicAdd(it.ra.count, (unsigned long long)n);
+    for (int i = blockIdx.x; i < n; i += gridDim.x)
+      rescore_block<E>(it.ra, it.g, it.list[i], it.best, it.second, it.match, L);
+  }
+}
+
+// Zeroes the masked-tile and re-score counters of several pairs.
+__global__ void reset_counters_multi(int32_t* const* __restrict__ ctr, int n) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i < n && ctr[i]) *ctr[i] = 0;
+}
+
 __global__ void postprocess_kernel(BMGrid g, Geom geom, const float* __restrict__ best,
                                    const float* __restrict__ second,
                                    const int32_t* __restrict__ match, uint8_t* kept,
@@ -2333,6 +2407,102 @@ void run_tiled(const BMGrid& g, TileArgs a, BMScratch& scr, hipStream_t s) {
   if (scr.prof) scr.prof->end(kProfMasked, s);
 }
 
+// Per-pair arguments of a batched search, packed into one device buffer:
+// tile arguments, the flat item starts of the fast and masked launches,
+// the re-score items and the counters to zero.
+struct BatchLayout {
+  size_t args, fstart, mstart, items, ctr, bytes;
+  explicit BatchLayout(int n) {
+    auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
+    args = 0;
+    fstart = al(args + sizeof(TileArgs) * n);
+    mstart = al(fstart + sizeof(int) * (n + 1));
+    items = al(mstart + sizeof(int) * (n + 1));
+    ctr = al(items + sizeof(RescoreItem) * n);
+    bytes = al(ctr + sizeof(int32_t*) * 2 * n);
+  }
+};
+
+template <class T, class E>
+void run_tiled_multi(const std::vector<SearchJob>& jobs, unsigned char* dev, hipStream_t s) {
+  const int n = (int)jobs.size();
+  const BatchLayout lay(n);
+  std::vector<unsigned char> host(lay.bytes, 0);
+  TileArgs* A = reinterpret_cast<TileArgs*>(host.data() + lay.args);
+  int* fs = reinterpret_cast<int*>(host.data() + lay.fstart);
+  int* ms = reinterpret_cast<int*>(host.data() + lay.mstart);
+  RescoreItem* R = reinterpret_cast<RescoreItem*>(host.data() + lay.items);
+  int32_t** C = reinterpret_cast<int32_t**>(host.data() + lay.ctr);
+  int64_t nf = 0, nm = 0;
+  bool rescore = false;
+  for (int p = 0; p < n; ++p) {
+    const SearchJob& j = jobs[p];
+    TileArgs a{};
+    a.y = j.y;
+    a.sm = j.smask;
+    a.bm = j.bmask;
+    a.g = j.g;
+    a.xt = j.bp.xt;
+    a.sxx = j.bp.sxx;
+    a.sumxt = j.bp.sumxt;
+    a.bflags = j.bp.flags;
+    a.stats = j.st.ms;
+    a.out_best = j.res.best;
+    a.out_second = j.res.second;
+    a.out_match = j.res.match;
+    a.ntx = j.g.nb[0];
+    a.nty = (j.g.nb[1] + T::TY - 1) / T::TY;
+    a.ntz = (j.g.nb[2] + T::TZ - 1) / T::TZ;
+    const int64_t nt = (int64_t)a.ntx * a.nty * a.ntz;
+    if (nt > j.scr.ntiles_cap) throw Error(kErrState, "tile queue too small");
+    a.queue = j.scr.queue;
+    a.qbits = j.scr.qbits;
+    a.qcount = j.scr.count;
+    a.rlist = j.scr.rlist;
+    a.rcount = j.scr.rcount;
+    a.rties = j.scr.rescore_ties;
+    a.msplit = nt < 2048 ? 4 : 1;
+    A[p] = a;
+    fs[p] = (int)nf;
+    ms[p] = (int)nm;
+    nf += nt;
+    nm += nt * a.msplit;
+    R[p] = RescoreItem{j.g, j.res.best, j.res.second, j.res.match,
+                       RescoreArgs{j.x, j.bmask, j.y, j.smask, j.st.ms, j.scr.rescored,
+                                   j.scr.exact_records},
+                       j.scr.rlist, j.scr.rcount};
+    rescore = rescore || j.scr.rlist != nullptr;
+    C[2 * p] = j.scr.count;
+    C[2 * p + 1] = j.scr.rcount;
+  }
+  if (nf > INT_MAX || nm > INT_MAX) throw Error(kErrArg, "batched search too large");
+  fs[n] = (int)nf;
+  ms[n] = (int)nm;
+  // Pageable source: the copy has read it when the call returns.
+  MIRORR_HIP(hipMemcpyAsync(dev, host.data(), lay.bytes, hipMemcpyHostToDevice, s));
+  const TileArgs* dA = reinterpret_cast<const TileArgs*>(dev + lay.args);
+  const int* dfs = reinterpret_cast<const int*>(dev + lay.fstart);
+  const int* dms = reinterpret_cast<const int*>(dev + lay.mstart);
+  hipLaunchKernelGGL(reset_counters_multi, dim3((unsigned)((2 * n + 63) / 64)), dim3(64), 0, s,
+                     reinterpret_cast<int32_t* const*>(dev + lay.ctr), 2 * n);
+  const int ng = jobs[0].g.ng;
+  const int WYf = T::fast_pitch((T::TY - 1) * ng + T::WX + T::YPAD);
+  const int WZ = (T::TZ - 1) * ng + T::WX;
+  const size_t lds_fast = sizeof(float) * (size_t)T::PX * WYf * WZ;
+  if (nf > 0)
+    hipLaunchKernelGGL((ncc_tile_fast_multi<T, E>), dim3((unsigned)nf), dim3(kThreads), lds_fast,
+                       s, dA, dfs, n);
+  const int WYm = (T::TY - 1) * ng + T::WX;
+  const size_t lds_masked = (sizeof(float) + 1) * (size_t)T::PX * WYm * WZ;
+  if (nm > 0)
+    hipLaunchKernelGGL((ncc_tile_masked_multi<T, E>), dim3((unsigned)nm), dim3(kThreads),
+                       lds_masked, s, dA, dms, n);
+  if (rescore)
+    hipLaunchKernelGGL(rescore_multi_kernel<E>, dim3(1024), dim3(kThreads), 0, s,
+                       reinterpret_cast<const RescoreItem*>(dev + lay.items), n);
+  MIRORR_CHECK_LAUNCH();
+}
+
 }  // namespace
 
 template <class T>
@@ -2364,6 +2534,8 @@ struct TileVariant {
   int64_t (*tiles)(const BMGrid&);
   void (*run_f32)(const BMGrid&, TileArgs, BMScratch&, hipStream_t);
   void (*run_f16)(const BMGrid&, TileArgs, BMScratch&, hipStream_t);
+  void (*multi_f32)(const std::vector<SearchJob>&, unsigned char*, hipStream_t);
+  void (*multi_f16)(const std::vector<SearchJob>&, unsigned char*, hipStream_t);
   const char* name;  // NW, candidate rows per lane, blocks per tile in y and z
 };
 
@@ -2371,7 +2543,8 @@ struct TileVariant {
   TileVariant {                                                                             \
     NW, lds_fits<TileCfg<NW, KY, TY, TZ>>, tiles_of<TileCfg<NW, KY, TY, TZ>>,               \
         run_tiled<TileCfg<NW, KY, TY, TZ>, float>, run_tiled<TileCfg<NW, KY, TY, TZ>, half_t>, \
-        #NW "/" #KY "/" #TY "x" #TZ                                                         \
+        run_tiled_multi<TileCfg<NW, KY, TY, TZ>, float>,                                    \
+        run_tiled_multi<TileCfg<NW, KY, TY, TZ>, half_t>, #NW "/" #KY "/" #TY "x" #TZ        \
   }
 // The first shape of each width is the default.
 const TileVariant kTileVariants[] = {
@@ -2690,6 +2863,38 @@ void launch_ncc_search(const void* x, const uint8_t* bmask, const void* y,
                          g, res.best, res.second, res.match, ra, scr.rlist, scr.rcount);
     MIRORR_CHECK_LAUNCH();
   }
+}
+
+size_t search_batch_bytes(int njobs) { return BatchLayout(std::max(njobs, 1)).bytes; }
+
+void launch_ncc_search_batch(const std::vector<SearchJob>& jobs, void* dev, size_t dev_bytes,
+                             hipStream_t s, int storage) {
+  if (jobs.empty()) return;
+  if (dev_bytes < search_batch_bytes((int)jobs.size()))
+    throw Error(kErrState, "batched search argument buffer too small");
+  const BMGrid& g0 = jobs[0].g;
+  for (const SearchJob& j : jobs) {
+    if (j.g.bw != g0.bw || j.g.bg != g0.bg || j.g.nw != g0.nw || j.g.ng != g0.ng)
+      throw Error(kErrArg, "a batched search needs one block and neighbourhood geometry");
+    if (!tiled_supported(j.g)) throw Error(kErrArg, "a batched search needs the tiled search");
+    if (j.g.nblocks > (int64_t)INT_MAX / 2) throw Error(kErrArg, "block grid too large");
+  }
+  // One shape for the whole batch: the pinned one where it fits every pair,
+  // else the width's default.
+  int v = default_variant(g0.nw);
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto f = g_forced.find(g0.nw);
+    if (f != g_forced.end() && kTileVariants[f->second].fits(g0.ng)) {
+      bool ok = true;
+      for (const SearchJob& j : jobs) ok = ok && kTileVariants[f->second].tiles(j.g) <= j.scr.ntiles_cap;
+      if (ok) v = f->second;
+    }
+  }
+  if (storage == kStoreF16)
+    kTileVariants[v].multi_f16(jobs, static_cast<unsigned char*>(dev), s);
+  else
+    kTileVariants[v].multi_f32(jobs, static_cast<unsigned char*>(dev), s);
 }
 
 void launch_postprocess(const BMGrid& g, const Geom& geom, const BMResult& res,

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <string>
+#include <vector>
 
 #include "common.hpp"
 #include "profiler.hpp"
@@ -73,6 +74,30 @@ struct BMScratch {
   int exact_records = 1;
   unsigned long long* rescored = nullptr;  // optional: += blocks re-scored
 };
+
+// One pair's search in a batched launch: the arguments of launch_ncc_search.
+struct SearchJob {
+  const void* x = nullptr;
+  const uint8_t* bmask = nullptr;
+  const void* y = nullptr;
+  const uint8_t* smask = nullptr;
+  BMGrid g{};
+  BlockPrep bp;
+  SearchStats st;
+  BMScratch scr;
+  BMResult res;
+};
+
+// The NCC searches of several pairs (one block and neighbourhood geometry,
+// volumes of one storage, any image sizes) in one launch of each tiled
+// kernel over a flat list of every pair's tiles, then one re-scoring launch
+// over every pair's list. Each pair's records equal those of
+// launch_ncc_search on that pair. `dev` is device memory of at least
+// search_batch_bytes(n) bytes for the per-pair arguments (rewritten by
+// every call; calls on one buffer are ordered by the stream).
+size_t search_batch_bytes(int njobs);
+void launch_ncc_search_batch(const std::vector<SearchJob>& jobs, void* dev, size_t dev_bytes,
+                             hipStream_t s, int storage = kStoreF32);
 
 // Image pointers of the search and block preparation hold elements of
 // `storage` (kStoreF32 float, kStoreF16 half_t; half needs 4-voxel blocks).

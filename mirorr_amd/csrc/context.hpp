@@ -10,7 +10,8 @@
 #include "profiler.hpp"
 
 namespace mirorr {
-struct RegCache;  // registration workspace, defined with the driver
+struct RegCache;    // registration workspace, defined with the driver
+struct BatchCache;  // batched registrations' workspaces, likewise
 
 // Upper bound of the device bytes one mirorr_register call with these
 // parameters and images (volumes of `storage`) charges to a context whose
@@ -43,6 +44,8 @@ struct mirorr_ctx {
   // this context: grown to the largest request and reused, so a registration
   // after the first performs no device allocation. Guarded by `mu`.
   std::shared_ptr<mirorr::RegCache> reg;
+  // Per-pair caches of mirorr_register_batch, likewise grown and reused.
+  std::shared_ptr<mirorr::BatchCache> batch;
 };
 
 namespace mirorr {

@@ -34,6 +34,8 @@ namespace {
 constexpr int kT = sel::kSelT;
 constexpr int kPart = 1024;  // points per moment workgroup (4 per thread)
 
+__device__ __forceinline__ int64_t lts_parts_dev(int64_t npts) { return (npts + kPart - 1) / kPart; }
+
 using namespace xf;
 using sel::block_scan;
 
@@ -42,12 +44,11 @@ __device__ __forceinline__ unsigned long long key_of(double r) {
 }
 
 // ---- residuals + first digit ----------------------------------------------
-__global__ __launch_bounds__(kT) void residual_kernel(const double* __restrict__ X,
-                                                      const double* __restrict__ Y,
-                                                      const uint8_t* __restrict__ kept,
-                                                      int64_t n, LtsState* st,
-                                                      double* __restrict__ r,
-                                                      unsigned int* hist1, int do_hist) {
+__device__ __forceinline__ void residual_body(const double* __restrict__ X,
+                                              const double* __restrict__ Y,
+                                              const uint8_t* __restrict__ kept, int64_t n,
+                                              LtsState* st, double* __restrict__ r,
+                                              unsigned int* hist1, int do_hist) {
   if (!st->active) return;
   __shared__ unsigned int h[4096];
   if (do_hist)
@@ -86,9 +87,8 @@ __global__ __launch_bounds__(kT) void residual_kernel(const double* __restrict__
 // Second digit (bits 51..40) among residuals in the first digit's bin. Each
 // workgroup finds that bin (rank k in the first-digit histogram) itself;
 // workgroup 0 records it.
-__global__ __launch_bounds__(kT) void hist2_kernel(const double* __restrict__ r, int64_t n,
-                                                   LtsState* st, const unsigned int* hist1,
-                                                   unsigned int* hist2) {
+__device__ __forceinline__ void hist2_body(const double* __restrict__ r, int64_t n, LtsState* st,
+                                           const unsigned int* hist1, unsigned int* hist2) {
   if (!st->active) return;
   __shared__ unsigned int h[4096];
   __shared__ unsigned int s_bin;
@@ -165,9 +165,9 @@ __device__ void final_select(const unsigned long long* cand, LtsState* st) {
 // Candidates sharing the 24-bit prefix of the k-th residual. Each workgroup
 // finds the second digit's bin (rank k2 in the second histogram) itself;
 // workgroup 0 records it.
-__global__ __launch_bounds__(kT) void compact_kernel(const double* __restrict__ r, int64_t n,
-                                                     LtsState* st, const unsigned int* hist2,
-                                                     unsigned long long* __restrict__ cand) {
+__device__ __forceinline__ void compact_body(const double* __restrict__ r, int64_t n,
+                                             LtsState* st, const unsigned int* hist2,
+                                             unsigned long long* __restrict__ cand) {
   if (!st->active) return;
   __shared__ unsigned int s_bin;
   __shared__ unsigned long long s_krem;
@@ -254,10 +254,10 @@ __device__ void reduce_parts(const double* part, int stride, int nparts, double*
 constexpr int kMoments = 28;   // values per workgroup partial
 constexpr int kPartStride = 32;
 
-__global__ __launch_bounds__(kT) void moments_kernel(const double* __restrict__ X,
-                                                     const double* __restrict__ Y,
-                                                     const double* __restrict__ r, int64_t n,
-                                                     LtsState* st, double* part) {
+__device__ __forceinline__ void moments_body(const double* __restrict__ X,
+                                             const double* __restrict__ Y,
+                                             const double* __restrict__ r, int64_t n,
+                                             LtsState* st, double* part) {
   if (!st->active || n < 1) return;
   const double thr = st->fraction > 0.99999 ? DBL_MAX : st->thr;
   double v[kMoments];
@@ -553,8 +553,8 @@ __device__ void solve_classic(LtsState* st, const Moments& m) {
 // Totals of the moments pass (four waves), cleared histograms for the next
 // iteration, then the solve on lanes 0 and 1 (its register footprint stays
 // out of the streaming kernels).
-__global__ __launch_bounds__(kT) void solve_kernel(LtsState* st, const double* part, int nparts,
-                                                   unsigned int* hist1, unsigned int* hist2) {
+__device__ __forceinline__ void solve_body(LtsState* st, const double* part, int nparts,
+                                           unsigned int* hist1, unsigned int* hist2) {
   if (!st->active) return;
   __shared__ double t[kMoments];
   reduce_parts<kMoments>(part, kPartStride, nparts, t);
@@ -588,8 +588,9 @@ __global__ __launch_bounds__(kT) void solve_kernel(LtsState* st, const double* p
 }
 
 
-__global__ void init_kernel(LtsState* st, const OuterState* outer, int kind, int symmetric,
-                            double tol, unsigned int* hist1, unsigned int* hist2) {
+__device__ __forceinline__ void init_body(LtsState* st, const OuterState* outer, int kind,
+                                          int symmetric, double tol, unsigned int* hist1,
+                                          unsigned int* hist2) {
   // Runs with one workgroup; thread 0 writes the state.
   for (int i = threadIdx.x; i < 4096; i += blockDim.x) {
     hist1[i] = 0;
@@ -645,7 +646,7 @@ __device__ void prepare_maps(OuterState* o, int symmetric) {
 
 __device__ void finish_update(LtsState* st, OuterState* o);
 
-__global__ __launch_bounds__(64) void finish_kernel(LtsState* st, OuterState* o, int* flags) {
+__device__ __forceinline__ void finish_body(LtsState* st, OuterState* o, int* flags) {
   if (!st->active && !st->finished) {
     st->finished = 1;
     finish_update(st, o);
@@ -679,6 +680,87 @@ __device__ void finish_update(LtsState* st, OuterState* o) {
 __global__ __launch_bounds__(64) void prepare_maps_kernel(OuterState* o, int symmetric) { prepare_maps(o, symmetric); }
 
 __global__ void reset_count_kernel(LtsState* st) { st->n_kept = 0ull; }
+
+// One pair per launch.
+__global__ __launch_bounds__(kT) void residual_kernel(const double* __restrict__ X,
+                                                      const double* __restrict__ Y,
+                                                      const uint8_t* __restrict__ kept,
+                                                      int64_t n, LtsState* st,
+                                                      double* __restrict__ r,
+                                                      unsigned int* hist1, int do_hist) {
+  residual_body(X, Y, kept, n, st, r, hist1, do_hist);
+}
+__global__ __launch_bounds__(kT) void hist2_kernel(const double* __restrict__ r, int64_t n,
+                                                   LtsState* st, const unsigned int* hist1,
+                                                   unsigned int* hist2) {
+  hist2_body(r, n, st, hist1, hist2);
+}
+__global__ __launch_bounds__(kT) void compact_kernel(const double* __restrict__ r, int64_t n,
+                                                     LtsState* st, const unsigned int* hist2,
+                                                     unsigned long long* __restrict__ cand) {
+  compact_body(r, n, st, hist2, cand);
+}
+__global__ __launch_bounds__(kT) void moments_kernel(const double* __restrict__ X,
+                                                     const double* __restrict__ Y,
+                                                     const double* __restrict__ r, int64_t n,
+                                                     LtsState* st, double* part) {
+  moments_body(X, Y, r, n, st, part);
+}
+__global__ __launch_bounds__(kT) void solve_kernel(LtsState* st, const double* part, int nparts,
+                                                   unsigned int* hist1, unsigned int* hist2) {
+  solve_body(st, part, nparts, hist1, hist2);
+}
+__global__ void init_kernel(LtsState* st, const OuterState* outer, int kind, int symmetric,
+                            double tol, unsigned int* hist1, unsigned int* hist2) {
+  init_body(st, outer, kind, symmetric, tol, hist1, hist2);
+}
+__global__ __launch_bounds__(64) void finish_kernel(LtsState* st, OuterState* o, int* flags) {
+  finish_body(st, o, flags);
+}
+
+// Several pairs per launch: pair blockIdx.y, with the grid in x sized for
+// the largest pair (a pair's surplus workgroups find no points or parts).
+// Every result is that of the one-pair launches.
+__global__ __launch_bounds__(kT) void residual_multi(const LtsItem* __restrict__ items,
+                                                     int do_hist) {
+  const LtsItem it = items[blockIdx.y];
+  residual_body(it.X, it.Y, it.kept, it.npts, it.st, it.r, it.hist1, do_hist);
+}
+__global__ __launch_bounds__(kT) void hist2_multi(const LtsItem* __restrict__ items) {
+  const LtsItem it = items[blockIdx.y];
+  hist2_body(it.r, it.npts, it.st, it.hist1, it.hist2);
+}
+__global__ __launch_bounds__(kT) void compact_multi(const LtsItem* __restrict__ items) {
+  const LtsItem it = items[blockIdx.y];
+  compact_body(it.r, it.npts, it.st, it.hist2, it.cand);
+}
+__global__ __launch_bounds__(kT) void final_select_multi(const LtsItem* __restrict__ items) {
+  const LtsItem it = items[blockIdx.y];
+  if (!it.st->active) return;
+  final_select(it.cand, it.st);
+}
+__global__ __launch_bounds__(kT) void moments_multi(const LtsItem* __restrict__ items) {
+  const LtsItem it = items[blockIdx.y];
+  if ((int64_t)blockIdx.x >= lts_parts_dev(it.npts)) return;
+  moments_body(it.X, it.Y, it.r, it.npts, it.st, it.part);
+}
+__global__ __launch_bounds__(kT) void solve_multi(const LtsItem* __restrict__ items) {
+  const LtsItem it = items[blockIdx.y];
+  solve_body(it.st, it.part, (int)lts_parts_dev(it.npts), it.hist1, it.hist2);
+}
+__global__ void init_multi(const LtsItem* __restrict__ items, int kind, int symmetric,
+                           double tol) {
+  const LtsItem it = items[blockIdx.y];
+  init_body(it.st, it.outer, kind, symmetric, tol, it.hist1, it.hist2);
+}
+__global__ __launch_bounds__(64) void finish_multi(const LtsItem* __restrict__ items) {
+  const LtsItem it = items[blockIdx.y];
+  finish_body(it.st, it.outer, it.flags);
+}
+__global__ void reset_count_multi(const LtsItem* __restrict__ items, int n) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i < n) items[i].st->n_kept = 0ull;
+}
 
 }  // namespace
 
@@ -717,6 +799,7 @@ void lts_start(LtsState* st, const OuterState* outer, int kind, int symmetric, d
 }
 
 void lts_finish(LtsState* st, OuterState* outer, int* flags, hipStream_t s) {
+  if (!st || !outer || !flags) throw Error(kErrState, "lts_finish: null device buffer");
   hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(1), 0, s, st, outer, flags);
   MIRORR_CHECK_LAUNCH();
 }
@@ -728,6 +811,47 @@ void outer_prepare_maps(OuterState* outer, int symmetric, hipStream_t s) {
 
 void lts_reset_count(LtsState* st, hipStream_t s) {
   hipLaunchKernelGGL(reset_count_kernel, dim3(1), dim3(1), 0, s, st);
+  MIRORR_CHECK_LAUNCH();
+}
+
+void lts_start_multi(const LtsItem* items, int n, int kind, int symmetric, double tol,
+                     hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(init_multi, dim3(1, (unsigned)n), dim3(kT), 0, s, items, kind, symmetric,
+                     tol);
+  MIRORR_CHECK_LAUNCH();
+}
+
+void lts_enqueue_iteration_multi(const LtsItem* items, int n, int64_t max_npts, int iteration,
+                                 hipStream_t s) {
+  if (n <= 0) return;
+  const int64_t npts = std::max<int64_t>(max_npts, 1);
+  const int nparts = (int)lts_parts(npts);
+  const int grid = (int)std::min<int64_t>((npts + kT - 1) / kT, 2048);
+  const bool select = iteration > 0;
+  const int g2 = std::min(grid, 256);
+  const unsigned ny = (unsigned)n;
+  hipLaunchKernelGGL(residual_multi, dim3((unsigned)(select ? g2 : grid), ny), dim3(kT), 0, s,
+                     items, select ? 1 : 0);
+  if (select) {
+    hipLaunchKernelGGL(hist2_multi, dim3((unsigned)g2, ny), dim3(kT), 0, s, items);
+    hipLaunchKernelGGL(compact_multi, dim3((unsigned)g2, ny), dim3(kT), 0, s, items);
+    hipLaunchKernelGGL(final_select_multi, dim3(1, ny), dim3(kT), 0, s, items);
+  }
+  hipLaunchKernelGGL(moments_multi, dim3((unsigned)nparts, ny), dim3(kT), 0, s, items);
+  hipLaunchKernelGGL(solve_multi, dim3(1, ny), dim3(kT), 0, s, items);
+  MIRORR_CHECK_LAUNCH();
+}
+
+void lts_finish_multi(const LtsItem* items, int n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(finish_multi, dim3(1, (unsigned)n), dim3(1), 0, s, items);
+  MIRORR_CHECK_LAUNCH();
+}
+
+void lts_reset_count_multi(const LtsItem* items, int n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(reset_count_multi, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, items, n);
   MIRORR_CHECK_LAUNCH();
 }
 

@@ -75,4 +75,29 @@ void outer_prepare_maps(OuterState* outer, int symmetric, hipStream_t s);
 // Zeroes the kept-pair counter (before the block-matching passes).
 void lts_reset_count(LtsState* st, hipStream_t s);
 
+// The LTS of several pairs at once: one launch per phase for all pairs,
+// each pair with its own state, pairs and buffers (an array of LtsItem in
+// device memory). Every pair's results are those of the one-pair calls.
+struct LtsItem {
+  const double* X;
+  const double* Y;
+  const uint8_t* kept;
+  int64_t npts;          // as the one-pair calls' npts (at least 1)
+  LtsState* st;
+  OuterState* outer;
+  double* r;
+  unsigned int* hist1;
+  unsigned int* hist2;
+  unsigned long long* cand;
+  double* part;          // lts_parts(npts) partials
+  int* flags;            // {active, stop}, written by lts_finish_multi
+};
+void lts_start_multi(const LtsItem* items, int n, int kind, int symmetric, double tol,
+                     hipStream_t s);
+// max_npts: the largest npts of the items.
+void lts_enqueue_iteration_multi(const LtsItem* items, int n, int64_t max_npts, int iteration,
+                                 hipStream_t s);
+void lts_finish_multi(const LtsItem* items, int n, hipStream_t s);
+void lts_reset_count_multi(const LtsItem* items, int n, hipStream_t s);
+
 }  // namespace mirorr

@@ -283,6 +283,40 @@ RegCache& reg_cache(mirorr_ctx* c) {
   return *c->reg;
 }
 
+// Batched registrations: one registration cache per pair, the per-pair
+// arguments of the batched searches and LTS, and the pairs' {active, stop}
+// flags (device and pinned host).
+struct BatchCache {
+  std::vector<std::unique_ptr<RegCache>> regs;
+  DeviceBuffer args[2], items, flagsD;
+  int* flagsH = nullptr;
+  int cap = 0;
+  BatchCache() = default;
+  BatchCache(const BatchCache&) = delete;
+  BatchCache& operator=(const BatchCache&) = delete;
+  ~BatchCache() {
+    if (flagsH) (void)hipHostFree(flagsH);
+  }
+  void ensure(int n, Budget* bud) {
+    while ((int)regs.size() < n) regs.emplace_back(new RegCache());
+    for (DeviceBuffer& a : args) a.ensure(search_batch_bytes(n), bud);
+    items.ensure(sizeof(LtsItem) * n, bud);
+    flagsD.ensure(sizeof(int) * 2 * n, bud);
+    if (cap < n) {
+      if (flagsH) (void)hipHostFree(flagsH);
+      flagsH = nullptr;
+      cap = 0;
+      MIRORR_HIP(hipHostMalloc((void**)&flagsH, sizeof(int) * 2 * n, hipHostMallocDefault));
+      cap = n;
+    }
+  }
+};
+
+BatchCache& batch_cache(mirorr_ctx* c) {
+  if (!c->batch) c->batch = std::make_shared<BatchCache>();
+  return *c->batch;
+}
+
 struct RegRun {
   mirorr_ctx* ctx;
   const mirorr_reg_params* p;
@@ -318,6 +352,46 @@ struct RegRun {
     return launch_variance_select(g, bp, portion, vs, s);
   }
 
+  // One NCC pass base -> search over the half grid as a tiled-search job,
+  // into the first or (second_set) the second set of result buffers.
+  SearchJob ncc_job(const void* base, const uint8_t* bmask, const void* search,
+                    const uint8_t* smask, const Geom& g_base, DeviceBuffer& xt, DeviceBuffer& sxx,
+                    DeviceBuffer& sum, DeviceBuffer& fl, DeviceBuffer& stats_search,
+                    bool second_set) {
+    SearchJob j;
+    j.x = base;
+    j.bmask = bmask;
+    j.y = search;
+    j.smask = smask;
+    j.g = make_grid(g_base.dims, p->block_width, p->block_gap, p->nhood_width, p->nhood_gap);
+    j.bp = BlockPrep{xt.as<float>(), sxx.as<float>(), sum.as<float>(), fl.as<uint32_t>()};
+    j.st = SearchStats{stats_search.as<float2>()};
+    j.res = second_set
+                ? BMResult{w.best2.as<float>(), w.second2.as<float>(), w.match2.as<int32_t>()}
+                : BMResult{w.best.as<float>(), w.second.as<float>(), w.match.as<int32_t>()};
+    DeviceBuffer& qu = second_set ? w.queue2 : w.queue;
+    DeviceBuffer& qc = second_set ? w.qcount2 : w.qcount;
+    DeviceBuffer& qb = second_set ? w.qbits2 : w.qbits;
+    j.scr = BMScratch{qu.as<int32_t>(), qc.as<int32_t>(), qb.as<unsigned long long>(),
+                      tile_count(j.g),
+                      (ctx->prof.finest_only() && !finest) ? nullptr : &ctx->prof};
+    j.scr.rlist = (second_set ? w.rlist2 : w.rlist).as<int32_t>();
+    j.scr.rcount = (second_set ? w.rcount2 : w.rcount).as<int32_t>();
+    j.scr.rescored = (second_set ? w.rescored2 : w.rescored).as<unsigned long long>();
+    j.scr.exact_records = 0;  // the pairs need the filter's decision and the match only
+    return j;
+  }
+
+  // The LTS pairs of a pass's records (the second pass's after the first's).
+  void post_pass(const BMGrid& g, const Geom& g_base, const BMResult& res, int swap,
+                 bool selected, LtsState* st, hipStream_t s) {
+    const int64_t off = swap ? g.nblocks : 0;
+    launch_postprocess_pairs(g, g_base, res, swap, w.kept.as<uint8_t>() + off,
+                             w.X.as<double>() + 3 * off, w.Y.as<double>() + 3 * off,
+                             &st->n_kept, s);
+    if (!selected) stats->voxel_evals += voxel_evals(g);
+  }
+
   // One BM pass base -> search over the half grid, writing LTS pairs. With
   // `selected`, the prepared flags drop blocks and the pass's voxel-evals are
   // counted on the device.
@@ -328,35 +402,23 @@ struct RegRun {
     const BMGrid g = make_grid(g_base.dims, p->block_width, p->block_gap, p->nhood_width,
                                p->nhood_gap);
     if (g.nblocks <= 0) return;
-    BMResult res = second_set
-                       ? BMResult{w.best2.as<float>(), w.second2.as<float>(), w.match2.as<int32_t>()}
-                       : BMResult{w.best.as<float>(), w.second.as<float>(), w.match.as<int32_t>()};
     if (p->block_metric == MIRORR_METRIC_NC && tiled_supported(g)) {
-      BlockPrep bp{xt.as<float>(), sxx.as<float>(), sum.as<float>(), fl.as<uint32_t>()};
-      DeviceBuffer& qu = second_set ? w.queue2 : w.queue;
-      DeviceBuffer& qc = second_set ? w.qcount2 : w.qcount;
-      DeviceBuffer& qb = second_set ? w.qbits2 : w.qbits;
-      BMScratch scr{qu.as<int32_t>(), qc.as<int32_t>(), qb.as<unsigned long long>(), tile_count(g),
-                    (ctx->prof.finest_only() && !finest) ? nullptr : &ctx->prof};
-      scr.rlist = (second_set ? w.rlist2 : w.rlist).as<int32_t>();
-      scr.rcount = (second_set ? w.rcount2 : w.rcount).as<int32_t>();
-      scr.rescored = (second_set ? w.rescored2 : w.rescored).as<unsigned long long>();
-      scr.exact_records = 0;  // the pairs need the filter's decision and the match only
-      launch_ncc_search(base, bmask, search, smask, g, bp, SearchStats{stats_search.as<float2>()},
-                        scr, res, s, storage);
-    } else {
-      if (storage != kStoreF32)
-        throw Error(kErrArg, "half-precision images need the NCC metric and the tiled search "
-                             "(blocks of width 4, block gap 1, nhood width 5 to 13)");
-      launch_metric_search(p->block_metric, static_cast<const float*>(base), bmask,
-                           static_cast<const float*>(search), smask, g, res, s,
-                           selected ? fl.as<uint32_t>() : nullptr);
+      SearchJob j = ncc_job(base, bmask, search, smask, g_base, xt, sxx, sum, fl, stats_search,
+                            second_set);
+      launch_ncc_search(j.x, j.bmask, j.y, j.smask, j.g, j.bp, j.st, j.scr, j.res, s, storage);
+      post_pass(g, g_base, j.res, swap, selected, st, s);
+      return;
     }
-    const int64_t off = swap ? g.nblocks : 0;
-    launch_postprocess_pairs(g, g_base, res, swap, w.kept.as<uint8_t>() + off,
-                             w.X.as<double>() + 3 * off, w.Y.as<double>() + 3 * off,
-                             &st->n_kept, s);
-    if (!selected) stats->voxel_evals += voxel_evals(g);
+    if (storage != kStoreF32)
+      throw Error(kErrArg, "half-precision images need the NCC metric and the tiled search "
+                           "(blocks of width 4, block gap 1, nhood width 5 to 13)");
+    const BMResult res =
+        second_set ? BMResult{w.best2.as<float>(), w.second2.as<float>(), w.match2.as<int32_t>()}
+                   : BMResult{w.best.as<float>(), w.second.as<float>(), w.match.as<int32_t>()};
+    launch_metric_search(p->block_metric, static_cast<const float*>(base), bmask,
+                         static_cast<const float*>(search), smask, g, res, s,
+                         selected ? fl.as<uint32_t>() : nullptr);
+    post_pass(g, g_base, res, swap, selected, st, s);
   }
 
   static int64_t voxel_evals(const BMGrid& g) {
@@ -434,8 +496,10 @@ struct RegRun {
 
   static constexpr int kLtsChunk = 4;
 
-  // The LTS start and its first chunk, queued without a host wait.
+  // The LTS start and its first chunk, queued without a host wait (the
+  // flags buffer is allocated beforehand: this may run under graph capture).
   void lts_begin(LtsState* st, OuterState* outer, int64_t npts, Flags* h_flags) {
+    if (w.flagsD.bytes() < sizeof(Flags)) throw Error(kErrState, "LTS flags buffer missing");
     LtsBuffers lb = lts_buffers(npts);
     lts_start(st, outer, kind, symmetric ? 1 : 0, p->lts_tolerance, lb, s);
     enqueue_lts_chunk(st, outer, npts, h_flags, 0, kLtsChunk);
@@ -453,6 +517,7 @@ struct RegRun {
 
   // Runs the LTS to convergence with predicated chunks; returns when done.
   void run_lts(LtsState* st, OuterState* outer, int64_t npts, Flags* h_flags) {
+    w.flagsD.ensure(sizeof(Flags), bud);
     lts_begin(st, outer, npts, h_flags);
     lts_complete(st, outer, npts, h_flags);
   }
@@ -502,82 +567,138 @@ size_t registration_bytes_bound(const mirorr_reg_params* p, const Geom& FG, bool
 
 namespace {
 
-void run_registration(mirorr_ctx* ctx, const mirorr_reg_params* p, const mirorr_volume* fixed,
-                      const mirorr_volume* moving, const double* init_params,
-                      const double* init_center, double* out_params, double* out_center,
-                      mirorr_reg_stats* stats) {
-  const auto t0 = std::chrono::steady_clock::now();
-  hipStream_t s = ctx->stream;
-  Budget* bud = &ctx->budget;
-  RegCache& cache = reg_cache(ctx);
-  RegRun run{ctx, p, cache.w, stats, p->affine ? xf::kAffine : xf::kRigid, p->symmetric != 0, s,
-             bud};
-  if (fixed->storage != moving->storage)
-    throw Error(kErrArg, "the fixed and moving volumes have different storage");
-  run.storage = fixed->storage;
-  Workspace& w = run.w;
-  const Geom& FG = fixed->geom;
-  const Geom& MG = moving->geom;
-  const LevelPlan plan = plan_levels(p, FG, MG);
-  const int nlev = plan.nlev;
-  std::vector<Level>& Fp = cache.F;
-  std::vector<Level>& Mp = cache.M;
-  build_pyramid(fixed->data.as<void>(), FG, plan.fs, nlev, Fp, w.pyr_tmp, w.pyr_map, bud, s,
-                fixed->storage, fixed->enc_zero);
-  build_pyramid(moving->data.as<void>(), MG, plan.ms, nlev, Mp, w.pyr_tmp, w.pyr_map, bud, s,
-                moving->storage, moving->enc_zero);
-  const int t_max = plan.t_max;
-  // Initial transform: given, or centred (itkMirorrPyramidWrapper.txx:611-623:
-  // centre of the moving image, translation between the image centres), then
-  // composed into an identity transform with centre 0 as the command line
-  // does before RunMirorr (itkMirorrPyramidWrapper.txx:626-640).
-  xf::Xf T = xf::identity(run.kind);
-  if (init_params) {
-    if (init_center) xf::set_center(T, init_center);
-    xf::set_params(T, init_params);
-  } else if (p->centre_init) {
-    double cM[3], cF[3], t[3];
-    centre_of(MG, cM);
-    centre_of(FG, cF);
-    for (int k = 0; k < 3; ++k) t[k] = cF[k] - cM[k];
-    xf::set_center(T, cM);
-    xf::set_translation(T, t);
-    xf::set_center_to_origin(T);
-  }
+constexpr int kMaxBatchPairs = 256;
+
+void check_reg_params(const mirorr_reg_params* p) {
+  if (p->block_width < 1 || p->block_gap < 1 || p->nhood_width < 1 || p->nhood_gap < 1)
+    throw Error(kErrArg, "block-matcher parameters must be positive");
+  // SetPortionMatchesKept (itkAbstractBlockMatcher.txx:83-99): 0 means 0.5.
+  if (p->portion_kept < 0.0 || p->portion_kept > 1.0)
+    throw Error(kErrArg, "portion_kept should be between 0 and 1.0");
+  if (p->block_metric < MIRORR_METRIC_NC || p->block_metric > MIRORR_METRIC_CR)
+    throw Error(kErrArg, "unknown block metric");
+  if (p->resampling < MIRORR_RESAMPLE_MIDDLE || p->resampling > MIRORR_RESAMPLE_FIXED)
+    throw Error(kErrArg, "Unknown resampling type");
+  if (p->block_metric != MIRORR_METRIC_NC && p->block_width > kMaxMetricBW)
+    throw Error(kErrArg, "block width above 16 for this block metric");
+  if (!(p->stop_tolerance >= 0.0) || !(p->lts_tolerance >= 0.0))
+    throw Error(kErrArg, "stop tolerances must be non-negative");
+}
+
+// One registration (MirorrPyramidImplement::RunMirorr, itkMirorrPyramidImplement.txx:71-158)
+// as host state between the launches of its levels and iterations: the
+// pyramids, the transform between levels, the current level's outer state
+// and grids. The one-pair driver and the batched driver below queue its
+// launches.
+struct PairReg {
+  mirorr_ctx* ctx;
+  const mirorr_reg_params* p;
+  const mirorr_volume* fixed;
+  const mirorr_volume* moving;
+  RegCache& cache;
+  RegRun run;
+  mirorr_reg_stats* stats;
+  std::chrono::steady_clock::time_point t0;
+  LevelPlan plan;
+  xf::Xf T;
   double cur[12];
-  xf::get_params(T, cur);
-  int max_it = p->iterations != 0 ? std::abs(p->iterations) : 5;
-  const bool halve = p->iterations < 0;
-  w.lts.ensure(sizeof(LtsState), bud);
-  w.outer.ensure(sizeof(OuterState), bud);
-  w.mapI.ensure(sizeof(IndexMap), bud);
-  w.flagsD.ensure(2 * sizeof(int), bud);
-  LtsState* st = w.lts.as<LtsState>();
-  OuterState* outer = w.outer.as<OuterState>();
-  Flags* flags = cache.flags;
-  std::memset(stats, 0, sizeof(*stats));
-  w.vsel_evals.ensure(sizeof(unsigned long long), bud);
-  MIRORR_HIP(hipMemsetAsync(w.vsel_evals.as<void>(), 0, sizeof(unsigned long long), s));
-  for (DeviceBuffer* b : {&w.rescored, &w.rescored2}) {
-    b->ensure(sizeof(unsigned long long), bud);
-    MIRORR_HIP(hipMemsetAsync(b->as<void>(), 0, sizeof(unsigned long long), s));
+  int max_it = 5;
+  bool halve = false;
+  // the current level
+  OuterState o{};
+  double c0[3] = {0.0, 0.0, 0.0};
+  BMGrid gh{};
+  bool tiled = false, prep = false;
+  double portion = 1.0;
+  int64_t npts = 0;
+  const uint8_t* fmask = nullptr;
+  const uint8_t* mmask = nullptr;
+  int fmode = kMaskInside, mmode = kMaskInside;
+  bool selM = false, selF = false;
+  int level = -1;
+
+  PairReg(mirorr_ctx* c, const mirorr_reg_params* pp, const mirorr_volume* f,
+          const mirorr_volume* m, const double* init_params, const double* init_center,
+          mirorr_reg_stats* st, RegCache& rc, hipStream_t s)
+      : ctx(c), p(pp), fixed(f), moving(m), cache(rc),
+        run{c, pp, rc.w, st, pp->affine ? xf::kAffine : xf::kRigid, pp->symmetric != 0, s,
+            &c->budget},
+        stats(st), t0(std::chrono::steady_clock::now()) {
+    if (fixed->storage != moving->storage)
+      throw Error(kErrArg, "the fixed and moving volumes have different storage");
+    run.storage = fixed->storage;
+    Workspace& w = run.w;
+    Budget* bud = run.bud;
+    const Geom& FG = fixed->geom;
+    const Geom& MG = moving->geom;
+    plan = plan_levels(p, FG, MG);
+    build_pyramid(fixed->data.as<void>(), FG, plan.fs, plan.nlev, cache.F, w.pyr_tmp, w.pyr_map,
+                  bud, s, fixed->storage, fixed->enc_zero);
+    build_pyramid(moving->data.as<void>(), MG, plan.ms, plan.nlev, cache.M, w.pyr_tmp, w.pyr_map,
+                  bud, s, moving->storage, moving->enc_zero);
+    // Initial transform: given, or centred (itkMirorrPyramidWrapper.txx:611-623:
+    // centre of the moving image, translation between the image centres), then
+    // composed into an identity transform with centre 0 as the command line
+    // does before RunMirorr (itkMirorrPyramidWrapper.txx:626-640).
+    T = xf::identity(run.kind);
+    if (init_params) {
+      if (init_center) xf::set_center(T, init_center);
+      xf::set_params(T, init_params);
+    } else if (p->centre_init) {
+      double cM[3], cF[3], t[3];
+      centre_of(MG, cM);
+      centre_of(FG, cF);
+      for (int k = 0; k < 3; ++k) t[k] = cF[k] - cM[k];
+      xf::set_center(T, cM);
+      xf::set_translation(T, t);
+      xf::set_center_to_origin(T);
+    }
+    xf::get_params(T, cur);
+    max_it = p->iterations != 0 ? std::abs(p->iterations) : 5;
+    halve = p->iterations < 0;
+    w.lts.ensure(sizeof(LtsState), bud);
+    w.outer.ensure(sizeof(OuterState), bud);
+    w.mapI.ensure(sizeof(IndexMap), bud);
+    w.flagsD.ensure(2 * sizeof(int), bud);
+    std::memset(stats, 0, sizeof(*stats));
+    w.vsel_evals.ensure(sizeof(unsigned long long), bud);
+    MIRORR_HIP(hipMemsetAsync(w.vsel_evals.as<void>(), 0, sizeof(unsigned long long), s));
+    for (DeviceBuffer* b : {&w.rescored, &w.rescored2}) {
+      b->ensure(sizeof(unsigned long long), bud);
+      MIRORR_HIP(hipMemsetAsync(b->as<void>(), 0, sizeof(unsigned long long), s));
+    }
+    stats->levels = plan.t_max;
   }
-  stats->levels = t_max;
-  const int np = xf::nparams(T);
-  for (int level = 0; level < t_max && level < 16; ++level) {
-    run.finest = level + 1 == std::min(t_max, 16);
+
+  int levels() const { return std::min(plan.t_max, 16); }
+  LtsState* st() { return run.w.lts.as<LtsState>(); }
+  OuterState* outer() { return run.w.outer.as<OuterState>(); }
+
+  // Level start: the outer state, the resampling maps and the level's
+  // buffers on the device; in classic mode the moving image and mask,
+  // resampled once onto the grid.
+  void begin_level(int lv) {
+    level = lv;
+    Workspace& w = run.w;
+    Budget* bud = run.bud;
+    hipStream_t s = run.s;
+    const Geom& FG = fixed->geom;
+    const Geom& MG = moving->geom;
+    std::vector<Level>& Fp = cache.F;
+    std::vector<Level>& Mp = cache.M;
+    run.finest = level + 1 == levels();
     xf::set_params(T, cur);
     const Geom& gF = Fp[level].geom;
     const Geom& gM = Mp[level].geom;
     double cF[3], cM[3];
     centre_of(gF, cF);
     centre_of(gM, cM);
-    OuterState o{};
+    o = OuterState{};
     o.gF = gF;
     o.gM = gM;
     o.gFfull = FG;
     o.gMfull = MG;
-    const double c0[3] = {T.c[0], T.c[1], T.c[2]};
+    for (int k = 0; k < 3; ++k) c0[k] = T.c[k];
     if (run.symmetric) {
       o.gHF = level_grid(p->resampling, false, gF, gM, cF, cM, T, gF.dir);
       o.gHM = level_grid(p->resampling, false, gF, gM, cF, cM, T, gM.dir);
@@ -620,25 +741,24 @@ void run_registration(mirorr_ctx* ctx, const mirorr_reg_params* p, const mirorr_
     xf::get_params(T, o.lastp);
     for (int a = 0; a < 3; ++a) stats->level_dims[level][a] = o.gHF.dims[a];
     run.ensure_level_buffers(o.gHF);
-    MIRORR_HIP(hipMemcpyAsync(outer, &o, sizeof(o), hipMemcpyHostToDevice, s));
-    outer_prepare_maps(outer, run.symmetric ? 1 : 0, s);
+    MIRORR_HIP(hipMemcpyAsync(outer(), &o, sizeof(o), hipMemcpyHostToDevice, s));
+    outer_prepare_maps(outer(), run.symmetric ? 1 : 0, s);
     const int64_t nb = make_grid(o.gHF.dims, p->block_width, p->block_gap, p->nhood_width,
                                  p->nhood_gap).nblocks;
-    const int64_t npts = run.symmetric ? 2 * nb : nb;
-    const uint8_t* fmask = fixed->has_mask ? fixed->mask.as<uint8_t>() : nullptr;
-    const uint8_t* mmask = moving->has_mask ? moving->mask.as<uint8_t>() : nullptr;
-    const int fmode = fixed->has_mask ? kMaskLinear : kMaskInside;
-    const int mmode = moving->has_mask ? kMaskLinear : kMaskInside;
-    const BMGrid gh = make_grid(o.gHF.dims, p->block_width, p->block_gap, p->nhood_width,
-                                p->nhood_gap);
-    const bool tiled = p->block_metric == MIRORR_METRIC_NC && tiled_supported(gh);
+    npts = run.symmetric ? 2 * nb : nb;
+    fmask = fixed->has_mask ? fixed->mask.as<uint8_t>() : nullptr;
+    mmask = moving->has_mask ? moving->mask.as<uint8_t>() : nullptr;
+    fmode = fixed->has_mask ? kMaskLinear : kMaskInside;
+    mmode = moving->has_mask ? kMaskLinear : kMaskInside;
+    gh = make_grid(o.gHF.dims, p->block_width, p->block_gap, p->nhood_width, p->nhood_gap);
+    tiled = p->block_metric == MIRORR_METRIC_NC && tiled_supported(gh);
     // Portion of blocks searched: levels 0 and 1 use every block
     // (itkMirorrPyramidImplement.txx:116-119); the selection is opt-in.
-    const double portion = (p->variance_select && level >= 2)
-                               ? (p->portion_kept == 0.0 ? 0.5 : p->portion_kept)
-                               : 1.0;
-    const bool prep = tiled || portion < 1.0;
-    bool selM = false, selF = false;
+    portion = (p->variance_select && level >= 2)
+                  ? (p->portion_kept == 0.0 ? 0.5 : p->portion_kept)
+                  : 1.0;
+    prep = tiled || portion < 1.0;
+    selM = selF = false;
     if (!run.symmetric) {
       // The moving image and mask are resampled once onto the grid.
       const xf::M3 I = xf::eye3();
@@ -655,144 +775,87 @@ void run_registration(mirorr_ctx* ctx, const mirorr_reg_params* p, const mirorr_
       // (the half-grid stays fixed through the level; hipMemcpy is ordered)
       MIRORR_HIP(hipStreamSynchronize(s));
     }
-    // One outer iteration up to the LTS's first chunk, queued on `s`.
-    auto enqueue_iteration = [&] {
-      // 1. resample onto the half-way grid under the current transforms
-      launch_resample(Fp[level].data.as<void>(), gF.dims, &outer->mapF, fmask, FG.dims,
-                      &outer->mmapF, fmode, o.gHF.dims, w.Fh.as<void>(), w.Fmh.as<uint8_t>(), s,
-                      run.storage, fixed->enc_zero);
-      if (run.symmetric)
-        launch_resample(Mp[level].data.as<void>(), gM.dims, &outer->mapM, mmask, MG.dims,
-                        &outer->mmapM, mmode, o.gHM.dims, w.Mh.as<void>(), w.Mmh.as<uint8_t>(),
-                        s, run.storage, moving->enc_zero);
-      // 2. block matching: moving half as base, then (symmetric) swapped.
-      // Without variance selection (whose scratch the passes share) the two
-      // symmetric passes are independent until the LTS: the second runs on
-      // the context's second stream, forked after the resampling and joined
-      // before the LTS. While the search kernels are being timed the passes
-      // run one after the other, so that each kernel's time is its own.
-      lts_reset_count(st, s);
-      const bool fork =
-          run.symmetric && !(portion < 1.0) && !ctx->prof.on() && ctx->overlap_passes;
-      hipStream_t s_b = s;
-      // If anything throws between the fork and the join, the second pass's
-      // kernels may still be in flight on the second stream: wait for them
-      // before the buffers they use can be freed or rewritten.
-      struct JoinOnUnwind {
-        hipStream_t s = nullptr;
-        ~JoinOnUnwind() {
-          if (s) (void)hipStreamSynchronize(s);
-        }
-      } join_guard;
-      if (fork) {
-        s_b = run.second_stream();
-        MIRORR_HIP(hipEventRecord(ctx->ev_fork, s));
-        join_guard.s = s_b;
-        MIRORR_HIP(hipStreamWaitEvent(s_b, ctx->ev_fork, 0));
-      }
-      if (tiled) {
-        launch_search_stats(w.Fh.as<void>(), w.Fmh.as<uint8_t>(), gh,
-                            SearchStats{w.statsF.as<float2>()}, s, run.storage);
-        if (run.symmetric)
-          launch_search_stats(w.Mh.as<void>(), w.Mmh.as<uint8_t>(), gh,
-                              SearchStats{w.statsM.as<float2>()}, s_b, run.storage);
-      }
-      if (run.symmetric && prep) {
-        selM = run.prepare_base(w.Mh.as<void>(), w.Mmh.as<uint8_t>(), gh, w.xtM, w.sxxM, w.sumM,
-                                w.flM, portion, s);
-        selF = run.prepare_base(w.Fh.as<void>(), w.Fmh.as<uint8_t>(), gh, w.xtF, w.sxxF, w.sumF,
-                                w.flF, portion, s_b);  // the second pass's base
-      }
-      run.match_pass(w.Mh.as<void>(), w.Mmh.as<uint8_t>(), w.Fh.as<void>(), w.Fmh.as<uint8_t>(),
-                     o.gHM, w.xtM, w.sxxM, w.sumM, w.flM, w.statsF, 0, selM, st, s, false);
-      if (run.symmetric)
-        run.match_pass(w.Fh.as<void>(), w.Fmh.as<uint8_t>(), w.Mh.as<void>(),
-                       w.Mmh.as<uint8_t>(), o.gHF, w.xtF, w.sxxF, w.sumF, w.flF, w.statsM, 1,
-                       selF, st, s_b, fork);
-      if (fork) {
-        MIRORR_HIP(hipEventRecord(ctx->ev_join, s_b));
-        MIRORR_HIP(hipStreamWaitEvent(s, ctx->ev_join, 0));
-        join_guard.s = nullptr;  // the first stream now orders everything after the join
-      }
-      // 3. LTS start and its first chunk
-      run.lts_begin(st, outer, npts, flags);
-    };
-    // Iteration graph of the level: every launch of an outer iteration up to
-    // the LTS's first chunk takes the same arguments within a level (the
-    // transforms live on the device). The graph cached for this level is
-    // replayed when its key (every host value those launches read) matches;
-    // otherwise the first iteration runs as it is (its searches tune their
-    // tile shapes outside any capture) and the second is captured. Host-side
-    // counts the captured enqueue adds (the voxel-evals) are added per replay.
-    const bool graphs = ctx->iteration_graphs && !ctx->prof.on();
-    IterationGraph& graph = cache.graphs[level];
-    auto iteration_key = [&] {
-      KeyBytes k;
-      k.buffers(w);
-      for (int v : {p->symmetric, p->affine, p->block_width, p->block_gap, p->nhood_width,
-                    p->nhood_gap, p->variance_select, p->block_metric, p->resampling})
-        k.add(v);
-      k.add(p->portion_kept);
-      k.add(p->lts_tolerance);
-      k.add(run.storage);
-      k.add(fixed->enc_zero);
-      k.add(moving->enc_zero);
-      k.add(Fp[level].data.as<void>());
-      k.add(Mp[level].data.as<void>());
-      k.add(fmask);
-      k.add(mmask);
-      for (const Geom* g : {&FG, &MG, &gF, &gM, (const Geom*)&o.gHF, (const Geom*)&o.gHM})
-        k.geom(*g);
-      k.add(npts);
-      k.add(portion);
-      k.add(tiled);
-      k.add((const void*)flags);
-      k.add(ctx->overlap_passes);
-      k.add((const void*)ctx->stream2);
-      if (tiled)
-        for (const Geom* g : {(const Geom*)&o.gHM, (const Geom*)&o.gHF})
-          k.add(tile_choice(make_grid(g->dims, p->block_width, p->block_gap, p->nhood_width,
-                                      p->nhood_gap),
-                            run.storage));
-      return std::move(k.b);
-    };
-    bool replay = graphs && graph.exec && graph.key == iteration_key();
-    int it = 0;
-    for (; it < max_it; ++it) {
-      if (replay) {
-        MIRORR_HIP(hipGraphLaunch(graph.exec, s));
-        stats->voxel_evals += graph.evals;
-      } else if (graphs && it >= 1) {
-        std::vector<unsigned char> key = iteration_key();
-        const int64_t ev0 = stats->voxel_evals;
-        graph.capture(s, enqueue_iteration);
-        graph.evals = stats->voxel_evals - ev0;
-        graph.key = std::move(key);
-        MIRORR_HIP(hipGraphLaunch(graph.exec, s));
-        replay = true;
-      } else {
-        enqueue_iteration();
-      }
-      // 3. further LTS chunks while it is active; the transform update
-      run.lts_complete(st, outer, npts, flags);  // leaves flags->stop read back
-      if (p->verbosity >= 1) {  // as the reference's iteration report
-        unsigned long long nk = 0;
-        double d = 0.0;
-        MIRORR_HIP(hipMemcpyAsync(&nk, &st->n_kept, sizeof(nk), hipMemcpyDeviceToHost, s));
-        MIRORR_HIP(hipMemcpyAsync(&d, &outer->sqdiff, sizeof(d), hipMemcpyDeviceToHost, s));
-        MIRORR_HIP(hipStreamSynchronize(s));
-        std::printf("level %d iteration %d: %llu blocks, delta %.6g\n", level, it + 1, nk, d);
-        std::fflush(stdout);
-      }
-      if (flags->stop) {
-        ++it;
-        break;
-      }
+  }
+
+  // 1. resampling onto the half-way grid under the current transforms, the
+  // kept-pair counter and, for the tiled search, the window statistics of
+  // the search images (the second pass's on s_b) and the block preparation
+  // of the base images (symmetric).
+  void enqueue_front(hipStream_t s, hipStream_t s_b, bool reset_count = true) {
+    Workspace& w = run.w;
+    const Geom& FG = fixed->geom;
+    const Geom& MG = moving->geom;
+    launch_resample(cache.F[level].data.as<void>(), cache.F[level].geom.dims, &outer()->mapF,
+                    fmask, FG.dims, &outer()->mmapF, fmode, o.gHF.dims, w.Fh.as<void>(),
+                    w.Fmh.as<uint8_t>(), s, run.storage, fixed->enc_zero);
+    if (run.symmetric)
+      launch_resample(cache.M[level].data.as<void>(), cache.M[level].geom.dims, &outer()->mapM,
+                      mmask, MG.dims, &outer()->mmapM, mmode, o.gHM.dims, w.Mh.as<void>(),
+                      w.Mmh.as<uint8_t>(), s, run.storage, moving->enc_zero);
+    if (reset_count) lts_reset_count(st(), s);
+    if (s_b != s) {
+      MIRORR_HIP(hipEventRecord(ctx->ev_fork, s));
+      MIRORR_HIP(hipStreamWaitEvent(s_b, ctx->ev_fork, 0));
     }
-    stats->iterations[level] = it;
-    MIRORR_HIP(hipMemcpyAsync(&o, outer, sizeof(o), hipMemcpyDeviceToHost, s));
+    if (tiled) {
+      launch_search_stats(w.Fh.as<void>(), w.Fmh.as<uint8_t>(), gh,
+                          SearchStats{w.statsF.as<float2>()}, s, run.storage);
+      if (run.symmetric)
+        launch_search_stats(w.Mh.as<void>(), w.Mmh.as<uint8_t>(), gh,
+                            SearchStats{w.statsM.as<float2>()}, s_b, run.storage);
+    }
+    if (run.symmetric && prep) {
+      selM = run.prepare_base(w.Mh.as<void>(), w.Mmh.as<uint8_t>(), gh, w.xtM, w.sxxM, w.sumM,
+                              w.flM, portion, s);
+      selF = run.prepare_base(w.Fh.as<void>(), w.Fmh.as<uint8_t>(), gh, w.xtF, w.sxxF, w.sumF,
+                              w.flF, portion, s_b);  // the second pass's base
+    }
+  }
+
+  // 2. block matching: pass 0 takes the moving half as base, pass 1
+  // (symmetric) the fixed half.
+  void match(int pass, hipStream_t s, bool second_set) {
+    Workspace& w = run.w;
+    if (pass == 0)
+      run.match_pass(w.Mh.as<void>(), w.Mmh.as<uint8_t>(), w.Fh.as<void>(), w.Fmh.as<uint8_t>(),
+                     o.gHM, w.xtM, w.sxxM, w.sumM, w.flM, w.statsF, 0, selM, st(), s, second_set);
+    else
+      run.match_pass(w.Fh.as<void>(), w.Fmh.as<uint8_t>(), w.Mh.as<void>(), w.Mmh.as<uint8_t>(),
+                     o.gHF, w.xtF, w.sxxF, w.sumF, w.flF, w.statsM, 1, selF, st(), s, second_set);
+  }
+  // The same pass as one pair's job of a batched search, and its pairs.
+  SearchJob job(int pass) {
+    Workspace& w = run.w;
+    if (pass == 0)
+      return run.ncc_job(w.Mh.as<void>(), w.Mmh.as<uint8_t>(), w.Fh.as<void>(),
+                         w.Fmh.as<uint8_t>(), o.gHM, w.xtM, w.sxxM, w.sumM, w.flM, w.statsF, false);
+    return run.ncc_job(w.Fh.as<void>(), w.Fmh.as<uint8_t>(), w.Mh.as<void>(), w.Mmh.as<uint8_t>(),
+                       o.gHF, w.xtF, w.sxxF, w.sumF, w.flF, w.statsM, true);
+  }
+  void pairs(int pass, hipStream_t s) {
+    const SearchJob j = job(pass);
+    run.post_pass(j.g, pass == 0 ? o.gHM : o.gHF, j.res, pass, pass == 0 ? selM : selF, st(), s);
+  }
+
+  void report(int it) {
     unsigned long long nk = 0;
-    MIRORR_HIP(hipMemcpyAsync(&nk, &st->n_kept, sizeof(nk), hipMemcpyDeviceToHost, s));
+    double d = 0.0;
+    hipStream_t s = run.s;
+    MIRORR_HIP(hipMemcpyAsync(&nk, &st()->n_kept, sizeof(nk), hipMemcpyDeviceToHost, s));
+    MIRORR_HIP(hipMemcpyAsync(&d, &outer()->sqdiff, sizeof(d), hipMemcpyDeviceToHost, s));
+    MIRORR_HIP(hipStreamSynchronize(s));
+    std::printf("level %d iteration %d: %llu blocks, delta %.6g\n", level, it + 1, nk, d);
+    std::fflush(stdout);
+  }
+
+  // Level end after `it` outer iterations: the outer state read back and the
+  // transform for the next level.
+  void end_level(int it) {
+    hipStream_t s = run.s;
+    stats->iterations[level] = it;
+    MIRORR_HIP(hipMemcpyAsync(&o, outer(), sizeof(o), hipMemcpyDeviceToHost, s));
+    unsigned long long nk = 0;
+    MIRORR_HIP(hipMemcpyAsync(&nk, &st()->n_kept, sizeof(nk), hipMemcpyDeviceToHost, s));
     MIRORR_HIP(hipStreamSynchronize(s));
     stats->points[level] = (int64_t)nk;
     stats->inner_iterations += o.inner_total;
@@ -817,20 +880,263 @@ void run_registration(mirorr_ctx* ctx, const mirorr_reg_params* p, const mirorr_
     xf::get_params(T, cur);
     if (halve) max_it = std::max(1, max_it / 2);
   }
-  unsigned long long sel_evals = 0, rescored[2] = {0, 0};
-  MIRORR_HIP(hipMemcpyAsync(&sel_evals, w.vsel_evals.as<void>(), sizeof(sel_evals),
-                            hipMemcpyDeviceToHost, s));
-  MIRORR_HIP(hipMemcpyAsync(&rescored[0], w.rescored.as<void>(), sizeof(rescored[0]),
-                            hipMemcpyDeviceToHost, s));
-  MIRORR_HIP(hipMemcpyAsync(&rescored[1], w.rescored2.as<void>(), sizeof(rescored[1]),
-                            hipMemcpyDeviceToHost, s));
-  MIRORR_HIP(hipStreamSynchronize(s));
-  stats->voxel_evals += (int64_t)sel_evals;
-  stats->rescored_blocks = (int64_t)(rescored[0] + rescored[1]);
-  for (int i = 0; i < np; ++i) out_params[i] = cur[i];
-  if (out_center)
-    for (int k = 0; k < 3; ++k) out_center[k] = T.c[k];
-  stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+
+  void finish(double* out_params, double* out_center) {
+    hipStream_t s = run.s;
+    Workspace& w = run.w;
+    unsigned long long sel_evals = 0, rescored[2] = {0, 0};
+    MIRORR_HIP(hipMemcpyAsync(&sel_evals, w.vsel_evals.as<void>(), sizeof(sel_evals),
+                              hipMemcpyDeviceToHost, s));
+    MIRORR_HIP(hipMemcpyAsync(&rescored[0], w.rescored.as<void>(), sizeof(rescored[0]),
+                              hipMemcpyDeviceToHost, s));
+    MIRORR_HIP(hipMemcpyAsync(&rescored[1], w.rescored2.as<void>(), sizeof(rescored[1]),
+                              hipMemcpyDeviceToHost, s));
+    MIRORR_HIP(hipStreamSynchronize(s));
+    stats->voxel_evals += (int64_t)sel_evals;
+    stats->rescored_blocks = (int64_t)(rescored[0] + rescored[1]);
+    const int np = xf::nparams(T);
+    for (int i = 0; i < np; ++i) out_params[i] = cur[i];
+    if (out_center)
+      for (int k = 0; k < 3; ++k) out_center[k] = T.c[k];
+    stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+};
+
+void run_registration(mirorr_ctx* ctx, const mirorr_reg_params* p, const mirorr_volume* fixed,
+                      const mirorr_volume* moving, const double* init_params,
+                      const double* init_center, double* out_params, double* out_center,
+                      mirorr_reg_stats* stats) {
+  hipStream_t s = ctx->stream;
+  RegCache& cache = reg_cache(ctx);
+  PairReg R(ctx, p, fixed, moving, init_params, init_center, stats, cache, s);
+  RegRun& run = R.run;
+  Workspace& w = run.w;
+  Flags* flags = cache.flags;
+  for (int level = 0; level < R.levels(); ++level) {
+    R.begin_level(level);
+    LtsState* st = R.st();
+    OuterState* outer = R.outer();
+    // One outer iteration up to the LTS's first chunk, queued on `s`.
+    auto enqueue_iteration = [&] {
+      // Without variance selection (whose scratch the passes share) the two
+      // symmetric passes are independent until the LTS: the second runs on
+      // the context's second stream, forked after the resampling and joined
+      // before the LTS. While the search kernels are being timed the passes
+      // run one after the other, so that each kernel's time is its own.
+      const bool fork =
+          run.symmetric && !(R.portion < 1.0) && !ctx->prof.on() && ctx->overlap_passes;
+      hipStream_t s_b = fork ? run.second_stream() : s;
+      // If anything throws between the fork and the join, the second pass's
+      // kernels may still be in flight on the second stream: wait for them
+      // before the buffers they use can be freed or rewritten.
+      struct JoinOnUnwind {
+        hipStream_t s = nullptr;
+        ~JoinOnUnwind() {
+          if (s) (void)hipStreamSynchronize(s);
+        }
+      } join_guard;
+      if (fork) join_guard.s = s_b;
+      R.enqueue_front(s, s_b);
+      R.match(0, s, false);
+      if (run.symmetric) R.match(1, s_b, fork);
+      if (fork) {
+        MIRORR_HIP(hipEventRecord(ctx->ev_join, s_b));
+        MIRORR_HIP(hipStreamWaitEvent(s, ctx->ev_join, 0));
+        join_guard.s = nullptr;  // the first stream now orders everything after the join
+      }
+      // 3. LTS start and its first chunk
+      run.lts_begin(st, outer, R.npts, flags);
+    };
+    // Iteration graph of the level: every launch of an outer iteration up to
+    // the LTS's first chunk takes the same arguments within a level (the
+    // transforms live on the device). The graph cached for this level is
+    // replayed when its key (every host value those launches read) matches;
+    // otherwise the first iteration runs as it is (its searches tune their
+    // tile shapes outside any capture) and the second is captured. Host-side
+    // counts the captured enqueue adds (the voxel-evals) are added per replay.
+    const bool graphs = ctx->iteration_graphs && !ctx->prof.on();
+    IterationGraph& graph = cache.graphs[level];
+    auto iteration_key = [&] {
+      KeyBytes k;
+      k.buffers(w);
+      for (int v : {p->symmetric, p->affine, p->block_width, p->block_gap, p->nhood_width,
+                    p->nhood_gap, p->variance_select, p->block_metric, p->resampling})
+        k.add(v);
+      k.add(p->portion_kept);
+      k.add(p->lts_tolerance);
+      k.add(run.storage);
+      k.add(fixed->enc_zero);
+      k.add(moving->enc_zero);
+      k.add(cache.F[level].data.as<void>());
+      k.add(cache.M[level].data.as<void>());
+      k.add(R.fmask);
+      k.add(R.mmask);
+      for (const Geom* g : {&fixed->geom, &moving->geom, (const Geom*)&cache.F[level].geom,
+                            (const Geom*)&cache.M[level].geom, (const Geom*)&R.o.gHF,
+                            (const Geom*)&R.o.gHM})
+        k.geom(*g);
+      k.add(R.npts);
+      k.add(R.portion);
+      k.add(R.tiled);
+      k.add((const void*)flags);
+      k.add(ctx->overlap_passes);
+      k.add((const void*)ctx->stream2);
+      if (R.tiled)
+        for (const Geom* g : {(const Geom*)&R.o.gHM, (const Geom*)&R.o.gHF})
+          k.add(tile_choice(make_grid(g->dims, p->block_width, p->block_gap, p->nhood_width,
+                                      p->nhood_gap),
+                            run.storage));
+      return std::move(k.b);
+    };
+    bool replay = graphs && graph.exec && graph.key == iteration_key();
+    int it = 0;
+    for (; it < R.max_it; ++it) {
+      if (replay) {
+        MIRORR_HIP(hipGraphLaunch(graph.exec, s));
+        stats->voxel_evals += graph.evals;
+      } else if (graphs && it >= 1) {
+        std::vector<unsigned char> key = iteration_key();
+        const int64_t ev0 = stats->voxel_evals;
+        graph.capture(s, enqueue_iteration);
+        graph.evals = stats->voxel_evals - ev0;
+        graph.key = std::move(key);
+        MIRORR_HIP(hipGraphLaunch(graph.exec, s));
+        replay = true;
+      } else {
+        enqueue_iteration();
+      }
+      // 3. further LTS chunks while it is active; the transform update
+      run.lts_complete(st, outer, R.npts, flags);  // leaves flags->stop read back
+      if (p->verbosity >= 1) R.report(it);  // as the reference's iteration report
+      if (flags->stop) {
+        ++it;
+        break;
+      }
+    }
+    R.end_level(it);
+  }
+  R.finish(out_params, out_center);
+}
+
+// Registrations of several pairs in lockstep, with one set of launches per
+// step for all of them where the step is batched: per outer iteration, each
+// pair's resampling, window statistics and block preparation, then one
+// launch of each search kernel over every pair's tiles per pass (a flat
+// work-list of uneven pairs), each pair's LTS pairs, and one launch per LTS
+// phase for all pairs, with one host read of every pair's flags per chunk
+// of inner iterations. Pairs with fewer pyramid levels join at the level
+// where their schedule starts (the finest levels align); a pair whose stop
+// rule fires leaves its level's iterations while the others go on. Every
+// pair's result and statistics are those of mirorr_register on that pair
+// alone. No counterpart in the reference, which registers one pair per
+// process (mirorr.cxx:649).
+void run_registration_batch(mirorr_ctx* ctx, const mirorr_reg_params* p, int n,
+                            const mirorr_volume* const* fixed, const mirorr_volume* const* moving,
+                            const double* init_params, const double* init_center,
+                            double* out_params, double* out_center, mirorr_reg_stats* stats) {
+  if (ctx->prof.on()) throw Error(kErrArg, "batched registrations do not time their kernels");
+  if (p->block_metric != MIRORR_METRIC_NC)
+    throw Error(kErrArg, "batched registrations need the NCC metric");
+  {
+    const int dims[3] = {64, 64, 64};
+    if (!tiled_supported(make_grid(dims, p->block_width, p->block_gap, p->nhood_width,
+                                   p->nhood_gap)))
+      throw Error(kErrArg, "batched registrations need the tiled search (blocks of width 4, "
+                           "block gap 1, nhood width 5 to 13)");
+  }
+  for (int i = 0; i < n; ++i)
+    if (!fixed[i] || !moving[i]) throw Error(kErrArg, "null volume in the batch");
+  const int storage = fixed[0]->storage;
+  for (int i = 0; i < n; ++i)
+    if (fixed[i]->storage != storage || moving[i]->storage != storage)
+      throw Error(kErrArg, "the volumes of a batch have different storage");
+  hipStream_t s = ctx->stream;
+  BatchCache& bc = batch_cache(ctx);
+  bc.ensure(n, &ctx->budget);
+  std::vector<std::unique_ptr<PairReg>> R;
+  R.reserve(n);
+  for (int i = 0; i < n; ++i)
+    R.emplace_back(new PairReg(ctx, p, fixed[i], moving[i],
+                               init_params ? init_params + 12 * i : nullptr,
+                               init_center ? init_center + 3 * i : nullptr, &stats[i],
+                               *bc.regs[i], s));
+  const int kind = p->affine ? xf::kAffine : xf::kRigid;
+  const bool symmetric = p->symmetric != 0;
+  int L = 0;
+  for (const auto& r : R) L = std::max(L, r->levels());
+  std::vector<int> its(n, 0);
+  std::vector<char> going(n, 0);
+  for (int gl = 0; gl < L; ++gl) {
+    std::vector<int> A;  // pairs at a level of their schedule now
+    for (int i = 0; i < n; ++i) {
+      const int lv = gl - (L - R[i]->levels());
+      if (lv < 0) continue;
+      R[i]->begin_level(lv);
+      A.push_back(i);
+      its[i] = 0;
+      going[i] = 1;
+    }
+    for (int it = 0;; ++it) {
+      std::vector<int> I;  // pairs iterating now
+      for (int i : A)
+        if (going[i] && it < R[i]->max_it) I.push_back(i);
+      if (I.empty()) break;
+      const int nI = (int)I.size();
+      // 1. resampling, statistics and block preparation, per pair
+      for (int i : I) R[i]->enqueue_front(s, s);
+      // 2. the passes, each one launch per search kernel for every pair
+      for (int pass = 0; pass < (symmetric ? 2 : 1); ++pass) {
+        std::vector<SearchJob> jobs;
+        for (int i : I) {
+          SearchJob j = R[i]->job(pass);
+          if (j.g.nblocks > 0) jobs.push_back(j);
+        }
+        launch_ncc_search_batch(jobs, bc.args[pass].as<void>(), bc.args[pass].bytes(), s,
+                                storage);
+        for (int i : I) R[i]->pairs(pass, s);
+      }
+      // 3. the LTS of every pair, one launch per phase
+      std::vector<LtsItem> items(nI);
+      int64_t max_npts = 1;
+      for (int k = 0; k < nI; ++k) {
+        PairReg& r = *R[I[k]];
+        Workspace& w = r.run.w;
+        items[k] = LtsItem{w.X.as<double>(), w.Y.as<double>(), w.kept.as<uint8_t>(),
+                           r.npts, r.st(), r.outer(), w.r.as<double>(),
+                           w.hist1.as<unsigned int>(), w.hist2.as<unsigned int>(),
+                           w.cand.as<unsigned long long>(), w.part.as<double>(),
+                           bc.flagsD.as<int>() + 2 * k};
+        max_npts = std::max<int64_t>(max_npts, r.npts);
+      }
+      // pageable source: read when the call returns
+      MIRORR_HIP(hipMemcpyAsync(bc.items.as<void>(), items.data(), sizeof(LtsItem) * nI,
+                                hipMemcpyHostToDevice, s));
+      const LtsItem* di = bc.items.as<LtsItem>();
+      lts_start_multi(di, nI, kind, symmetric ? 1 : 0, p->lts_tolerance, s);
+      for (int c = 0; c < 101; c += 4) {
+        for (int j = c; j < std::min(c + 4, 101); ++j)
+          lts_enqueue_iteration_multi(di, nI, max_npts, j, s);
+        lts_finish_multi(di, nI, s);
+        MIRORR_HIP(hipMemcpyAsync(bc.flagsH, bc.flagsD.as<int>(), sizeof(int) * 2 * nI,
+                                  hipMemcpyDeviceToHost, s));
+        MIRORR_HIP(hipStreamSynchronize(s));
+        bool any = false;
+        for (int k = 0; k < nI; ++k) any = any || bc.flagsH[2 * k] != 0;
+        if (!any) break;
+      }
+      std::vector<char> stop(nI);
+      for (int k = 0; k < nI; ++k) stop[k] = bc.flagsH[2 * k + 1] != 0;
+      for (int k = 0; k < nI; ++k) {
+        const int i = I[k];
+        its[i] = it + 1;
+        if (p->verbosity >= 1) R[i]->report(it);
+        if (stop[k]) going[i] = 0;
+      }
+    }
+    for (int i : A) R[i]->end_level(its[i]);
+  }
+  for (int i = 0; i < n; ++i)
+    R[i]->finish(out_params + 12 * i, out_center ? out_center + 3 * i : nullptr);
 }
 
 }  // namespace
@@ -939,21 +1245,30 @@ mirorr_status mirorr_register(mirorr_ctx* c, const mirorr_reg_params* p,
   mirorr_reg_stats local{};
   std::lock_guard<std::mutex> lk(c->mu);
   return guarded(c, [&] {
-    if (p->block_width < 1 || p->block_gap < 1 || p->nhood_width < 1 || p->nhood_gap < 1)
-      throw Error(kErrArg, "block-matcher parameters must be positive");
-    // SetPortionMatchesKept (itkAbstractBlockMatcher.txx:83-99): 0 means 0.5.
-    if (p->portion_kept < 0.0 || p->portion_kept > 1.0)
-      throw Error(kErrArg, "portion_kept should be between 0 and 1.0");
-    if (p->block_metric < MIRORR_METRIC_NC || p->block_metric > MIRORR_METRIC_CR)
-      throw Error(kErrArg, "unknown block metric");
-    if (p->resampling < MIRORR_RESAMPLE_MIDDLE || p->resampling > MIRORR_RESAMPLE_FIXED)
-      throw Error(kErrArg, "Unknown resampling type");
-    if (p->block_metric != MIRORR_METRIC_NC && p->block_width > kMaxMetricBW)
-      throw Error(kErrArg, "block width above 16 for this block metric");
-    if (!(p->stop_tolerance >= 0.0) || !(p->lts_tolerance >= 0.0))
-      throw Error(kErrArg, "stop tolerances must be non-negative");
+    check_reg_params(p);
     run_registration(c, p, fixed, moving, init_params, init_center, out_params, out_center,
                      stats ? stats : &local);
+  });
+}
+
+mirorr_status mirorr_register_batch(mirorr_ctx* c, const mirorr_reg_params* p, int n,
+                                    const mirorr_volume* const* fixed,
+                                    const mirorr_volume* const* moving, const double* init_params,
+                                    const double* init_center, double* out_params,
+                                    double* out_center, mirorr_reg_stats* stats) {
+  if (!c || !p || n < 0 || (n > 0 && (!fixed || !moving || !out_params))) return MIRORR_ERR_ARG;
+  if (n == 0) return MIRORR_OK;
+  std::lock_guard<std::mutex> lk(c->mu);
+  return guarded(c, [&] {
+    check_reg_params(p);
+    if (n > kMaxBatchPairs) throw Error(kErrArg, "more than 256 pairs in a batch");
+    std::vector<mirorr_reg_stats> local;
+    if (!stats) {
+      local.resize(n);
+      stats = local.data();
+    }
+    run_registration_batch(c, p, n, fixed, moving, init_params, init_center, out_params,
+                           out_center, stats);
   });
 }
 

@@ -159,12 +159,45 @@ class MirorrPyramid:
                                               md.handle, capi.ptr(ip), capi.ptr(ic),
                                               capi.ptr(out), capi.ptr(oc), C.byref(st)),
                    self.ctx.handle)
-        self.stats = {"levels": st.levels, "iterations": list(st.iterations[: st.levels]),
-                      "points": list(st.points[: st.levels]),
-                      "inner_iterations": st.inner_iterations, "voxel_evals": st.voxel_evals,
-                      "level_dims": [tuple(st.level_dims[i]) for i in range(st.levels)],
-                      "seconds": st.seconds, "rescored_blocks": st.rescored_blocks}
+        self.stats = _stats_dict(st)
         return out[:n], oc
+
+
+def _stats_dict(st):
+    return {"levels": st.levels, "iterations": list(st.iterations[: st.levels]),
+            "points": list(st.points[: st.levels]),
+            "inner_iterations": st.inner_iterations, "voxel_evals": st.voxel_evals,
+            "level_dims": [tuple(st.level_dims[i]) for i in range(st.levels)],
+            "seconds": st.seconds, "rescored_blocks": st.rescored_blocks}
+
+
+def register_batch(reg: "MirorrPyramid", fixed_devs, moving_devs, init_params=None,
+                   init_centers=None):
+    """mirorr_register_batch: the registrations of len(fixed_devs) pairs of
+    device volumes with reg's parameters, in lockstep on reg's context.
+    Returns [(params, centre)] per pair and sets reg.batch_stats."""
+    n = len(fixed_devs)
+    if len(moving_devs) != n:
+        raise ValueError("as many moving as fixed volumes")
+    fv = (C.c_void_p * max(n, 1))(*[v.handle.value for v in fixed_devs])
+    mv = (C.c_void_p * max(n, 1))(*[v.handle.value for v in moving_devs])
+    out = np.zeros((max(n, 1), 12))
+    oc = np.zeros((max(n, 1), 3))
+    st = (capi.RegStats * max(n, 1))()
+    ip = ic = None
+    if init_params is not None:
+        ip = np.zeros((n, 12))
+        for i, q in enumerate(init_params):
+            ip[i, : len(q)] = q
+    if init_centers is not None:
+        ic = np.ascontiguousarray(init_centers, np.float64).reshape(n, 3)
+    capi.check(capi.lib().mirorr_register_batch(
+        reg.ctx.handle, C.byref(reg.params), n, C.cast(fv, C.c_void_p), C.cast(mv, C.c_void_p),
+        capi.ptr(ip), capi.ptr(ic), capi.ptr(out), capi.ptr(oc), C.cast(st, C.c_void_p)),
+        reg.ctx.handle)
+    k = 12 if reg.params.affine else 6
+    reg.batch_stats = [_stats_dict(st[i]) for i in range(n)]
+    return [(out[i, :k].copy(), oc[i].copy()) for i in range(n)]
 
 
 def lts_fit(symmetric: bool, affine: bool, X, Y, params=None, center=None, ctx=None,

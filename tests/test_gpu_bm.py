@@ -239,6 +239,40 @@ def test_searched_exact_ties_match_oracle(oracle, masked):
     _compare(got, ref)
 
 
+@pytest.mark.parametrize("masked", [False, True])
+def test_rescored_flat_neighbourhoods_match_oracle(oracle, masked):
+    # Base = search = a linear ramp (integer values, exact in float): every
+    # candidate window is the block plus a constant, so all 729 candidates of
+    # an interior block score NCC^2 = 1 exactly, in float and in double, on
+    # any subset of voxels. The exact float ties are re-scored: all 729
+    # intervals reach the cut, so the double phase scores every candidate.
+    # The records must equal the oracle's (the same sums in the same order,
+    # the first tied candidate in scan order) and the tied blocks are culled. A partial base mask sends the blocks through the
+    # masked kernel and the masked sums.
+    shape = (24, 28, 32)
+    z, y, x = np.meshgrid(*[np.arange(n, dtype=np.float64) for n in shape], indexing="ij")
+    img = 3.0 * x + 2.0 * y + z + 100.0
+    vb, vs = _vol(img), _vol(img)
+    mb = None
+    bm = _matcher(nw=9, ng=3)
+    bm.SetBaseImage(vb)
+    bm.SetSearchImage(vs)
+    if masked:
+        mb = np.full(shape, 255, np.uint8)
+        mb[:, :, 1::7] = 0
+        bm.SetBaseMask(mb)
+    got = bm.get_raw()
+    ref = _oracle_raw(oracle, vb, mb, vs, None, 4, 1, 9, 3)
+    tie = (ref["best"] == ref["second"]) & (ref["best"] > 0.5)
+    print(f"blocks {tie.size}, tied at NCC^2 = 1: {tie.sum()}")
+    assert tie.sum() > 0.9 * tie.size
+    np.testing.assert_array_equal(got["match_idx"][tie], ref["match_idx"][tie])
+    np.testing.assert_array_equal(got["best"][tie], ref["best"][tie])
+    np.testing.assert_array_equal(got["second"][tie], ref["second"][tie])
+    assert not _kept_any(got["best"][tie], got["second"][tie])
+    _compare(got, ref)
+
+
 def _kept_any(best, second):
     b = np.asarray(best, np.float64)
     s = np.asarray(second, np.float64)
