@@ -105,10 +105,12 @@ void* mirorr_ctx_stream(mirorr_ctx* ctx);
 
 /* Iteration graphs (no counterpart in the reference, whose outer loop
  * itkSymMirorrRegistrationMethod.txx:456-692 runs on the host): with `on`,
- * mirorr_register captures the second outer iteration of each level (the
+ * mirorr_register captures an outer iteration of each level (the
  * resampling, both block-matching passes, the LTS start, its first four
- * inner iterations and the finish) as a HIP graph and replays it for the
- * level's later iterations; further inner iterations, when the LTS needs
+ * inner iterations and the finish) as a HIP graph at the level's second
+ * iteration, keeps it on the context and replays it for the later
+ * iterations, also of later registrations, while every host value its
+ * launches read is unchanged; further inner iterations, when the LTS needs
  * them, are queued as usual. Results are identical either way. Off by
  * default, or as the environment variable MIRORR_ITERATION_GRAPH says at
  * context creation; ignored while per-kernel timing is on. */
@@ -394,10 +396,20 @@ typedef struct mirorr_server_info {
                              were allowed to hold (running: kept workspace
                              plus the request's bound; idle: kept workspace);
                              admission keeps it within hbm_budget            */
+  int batch_max;          /* requests a slot may take at once               */
+  int max_batch;          /* largest batch run so far                        */
 } mirorr_server_info;
 
 mirorr_status mirorr_server_create(int device, size_t hbm_budget, int slots,
                                    mirorr_server** out);
+/* Batching: a slot that takes a request also takes up to max_pairs - 1
+ * later pending requests with the same parameters, storage and kind of
+ * start that fit the HBM budget, and runs them together through
+ * mirorr_register_batch (requests that cannot be batched, e.g. another
+ * block metric, run alone). Each result equals the request's alone; a
+ * batch's requests start and end together. 1 (the default) runs every
+ * request alone. No counterpart in the reference (mirorr.cxx:649). */
+mirorr_status mirorr_server_set_batch(mirorr_server* server, int max_pairs);
 /* Queues a request; *ticket identifies it to mirorr_poll. */
 mirorr_status mirorr_submit(mirorr_server* s, const mirorr_request* req, int64_t* ticket);
 /* The result of a ticket. timeout_s = 0: returns MIRORR_PENDING at once if

@@ -100,7 +100,7 @@ class ServerInfo(C.Structure):
     _fields_ = [("slots", C.c_int), ("pending", C.c_int), ("running", C.c_int),
                 ("completed", C.c_int64), ("hbm_budget", C.c_size_t), ("hbm_used", C.c_size_t),
                 ("hbm_peak", C.c_size_t), ("max_running", C.c_int),
-                ("reserve_peak", C.c_size_t)]
+                ("reserve_peak", C.c_size_t), ("batch_max", C.c_int), ("max_batch", C.c_int)]
 
 
 _P = C.c_void_p
@@ -160,6 +160,7 @@ _SIGS = {
     "mirorr_submit": (C.c_int, [_P, C.POINTER(Request), C.POINTER(C.c_int64)]),
     "mirorr_poll": (C.c_int, [_P, C.c_int64, C.c_double, C.POINTER(RequestResult)]),
     "mirorr_server_info_get": (C.c_int, [_P, C.POINTER(ServerInfo)]),
+    "mirorr_server_set_batch": (C.c_int, [_P, C.c_int]),
     "mirorr_server_last_error": (C.c_char_p, []),
     "mirorr_server_destroy": (None, [_P]),
     # include/mirorr_io.h

@@ -777,11 +777,9 @@ struct PairReg {
     }
   }
 
-  // 1. resampling onto the half-way grid under the current transforms, the
-  // kept-pair counter and, for the tiled search, the window statistics of
-  // the search images (the second pass's on s_b) and the block preparation
-  // of the base images (symmetric).
-  void enqueue_front(hipStream_t s, hipStream_t s_b, bool reset_count = true) {
+  // 1. resampling onto the half-way grid under the current transforms and
+  // the kept-pair counter, on s.
+  void enqueue_resample(hipStream_t s) {
     Workspace& w = run.w;
     const Geom& FG = fixed->geom;
     const Geom& MG = moving->geom;
@@ -792,11 +790,14 @@ struct PairReg {
       launch_resample(cache.M[level].data.as<void>(), cache.M[level].geom.dims, &outer()->mapM,
                       mmask, MG.dims, &outer()->mmapM, mmode, o.gHM.dims, w.Mh.as<void>(),
                       w.Mmh.as<uint8_t>(), s, run.storage, moving->enc_zero);
-    if (reset_count) lts_reset_count(st(), s);
-    if (s_b != s) {
-      MIRORR_HIP(hipEventRecord(ctx->ev_fork, s));
-      MIRORR_HIP(hipStreamWaitEvent(s_b, ctx->ev_fork, 0));
-    }
+    lts_reset_count(st(), s);
+  }
+
+  // For the tiled search, the window statistics of the search images and
+  // (symmetric) the block preparation of the base images: the first pass's
+  // on s, the second pass's on s_b (which waits for the resampling).
+  void enqueue_prep(hipStream_t s, hipStream_t s_b) {
+    Workspace& w = run.w;
     if (tiled) {
       launch_search_stats(w.Fh.as<void>(), w.Fmh.as<uint8_t>(), gh,
                           SearchStats{w.statsF.as<float2>()}, s, run.storage);
@@ -936,7 +937,12 @@ void run_registration(mirorr_ctx* ctx, const mirorr_reg_params* p, const mirorr_
         }
       } join_guard;
       if (fork) join_guard.s = s_b;
-      R.enqueue_front(s, s_b);
+      R.enqueue_resample(s);
+      if (fork) {
+        MIRORR_HIP(hipEventRecord(ctx->ev_fork, s));
+        MIRORR_HIP(hipStreamWaitEvent(s_b, ctx->ev_fork, 0));
+      }
+      R.enqueue_prep(s, s_b);
       R.match(0, s, false);
       if (run.symmetric) R.match(1, s_b, fork);
       if (fork) {
@@ -1082,18 +1088,41 @@ void run_registration_batch(mirorr_ctx* ctx, const mirorr_reg_params* p, int n,
         if (going[i] && it < R[i]->max_it) I.push_back(i);
       if (I.empty()) break;
       const int nI = (int)I.size();
-      // 1. resampling, statistics and block preparation, per pair
-      for (int i : I) R[i]->enqueue_front(s, s);
+      // 1. resampling, statistics and block preparation, per pair; without
+      // variance selection (whose scratch a pair's passes share) the second
+      // pass runs on the context's second stream, as in mirorr_register
+      bool fork = symmetric && ctx->overlap_passes;
+      for (int i : I) fork = fork && !(R[i]->portion < 1.0);
+      hipStream_t s_b = fork ? R[I[0]]->run.second_stream() : s;
+      struct JoinOnUnwind {
+        hipStream_t s = nullptr;
+        ~JoinOnUnwind() {
+          if (s) (void)hipStreamSynchronize(s);
+        }
+      } join_guard;
+      for (int i : I) R[i]->enqueue_resample(s);
+      if (fork) {
+        join_guard.s = s_b;
+        MIRORR_HIP(hipEventRecord(ctx->ev_fork, s));
+        MIRORR_HIP(hipStreamWaitEvent(s_b, ctx->ev_fork, 0));
+      }
+      for (int i : I) R[i]->enqueue_prep(s, s_b);
       // 2. the passes, each one launch per search kernel for every pair
       for (int pass = 0; pass < (symmetric ? 2 : 1); ++pass) {
+        hipStream_t sp = pass == 0 ? s : s_b;
         std::vector<SearchJob> jobs;
         for (int i : I) {
           SearchJob j = R[i]->job(pass);
           if (j.g.nblocks > 0) jobs.push_back(j);
         }
-        launch_ncc_search_batch(jobs, bc.args[pass].as<void>(), bc.args[pass].bytes(), s,
+        launch_ncc_search_batch(jobs, bc.args[pass].as<void>(), bc.args[pass].bytes(), sp,
                                 storage);
-        for (int i : I) R[i]->pairs(pass, s);
+        for (int i : I) R[i]->pairs(pass, sp);
+      }
+      if (fork) {
+        MIRORR_HIP(hipEventRecord(ctx->ev_join, s_b));
+        MIRORR_HIP(hipStreamWaitEvent(s, ctx->ev_join, 0));
+        join_guard.s = nullptr;
       }
       // 3. the LTS of every pair, one launch per phase
       std::vector<LtsItem> items(nI);

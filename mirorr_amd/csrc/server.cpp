@@ -19,6 +19,14 @@
 // MIRORR_ERR_NOMEM when none is running and nothing waits before it. Requests
 // are taken in arrival order, but later requests that fit may overtake one
 // that waits for memory, at most kMaxOvertake times.
+//
+// Batching (mirorr_server_set_batch): a slot that takes a request also takes
+// the later pending requests with the same parameters (the NCC tiled search,
+// the same storage and initialisation kind) that fit, up to the batch size,
+// and runs them together through mirorr_register_batch: their iterations
+// share launches (one search launch per pass over every pair's tiles, one
+// per LTS phase). Each result equals the request's alone.
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstdio>
@@ -31,6 +39,7 @@
 #include <thread>
 #include <vector>
 
+#include "bm_kernels.hpp"
 #include "context.hpp"
 
 namespace {
@@ -63,10 +72,33 @@ struct Request {
 struct Slot {
   mirorr_ctx* ctx = nullptr;
   std::thread th;
-  std::unique_ptr<Request> job;  // handed over by the scheduler
+  std::vector<std::unique_ptr<Request>> jobs;  // handed over by the scheduler (a batch)
   bool busy = false;
-  size_t reserve = 0;  // bytes the slot may hold while its request runs
+  size_t reserve = 0;  // bytes the slot may hold while its requests run
 };
+
+// Requests mirorr_register_batch can run together: the NCC metric with the
+// tiled search, equal parameters, one storage, the same kind of start.
+bool batchable(const Request& r) {
+  const mirorr_reg_params& p = r.p;
+  if (p.block_metric != MIRORR_METRIC_NC) return false;
+  const int dims[3] = {64, 64, 64};
+  return mirorr::tiled_supported(
+      mirorr::make_grid(dims, p.block_width, p.block_gap, p.nhood_width, p.nhood_gap));
+}
+bool same_batch(const Request& a, const Request& b) {
+  const mirorr_reg_params &x = a.p, &y = b.p;
+  return x.symmetric == y.symmetric && x.affine == y.affine && x.block_width == y.block_width &&
+         x.block_gap == y.block_gap && x.nhood_width == y.nhood_width &&
+         x.nhood_gap == y.nhood_gap && x.iterations == y.iterations &&
+         x.pyr_start == y.pyr_start && x.pyr_end == y.pyr_end && x.pyr_num == y.pyr_num &&
+         x.pyr_min_size == y.pyr_min_size && x.portion_kept == y.portion_kept &&
+         x.variance_select == y.variance_select && x.verbosity == y.verbosity &&
+         x.centre_init == y.centre_init && x.stop_tolerance == y.stop_tolerance &&
+         x.lts_tolerance == y.lts_tolerance && x.block_metric == y.block_metric &&
+         x.resampling == y.resampling && a.storage == b.storage &&
+         a.has_init == b.has_init && a.has_center == b.has_center;
+}
 
 size_t checked_nvox(const mirorr_geom& g) {
   size_t n = 1;
@@ -95,6 +127,8 @@ struct mirorr_server {
   int64_t completed = 0;
   int max_running = 0;
   size_t reserve_peak = 0;  // largest sum of the slots' reservations at an admission
+  int batch_max = 1;        // requests a slot may take at once
+  int max_batch = 0;        // largest batch run so far
   bool stopping = false;
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
 
@@ -139,7 +173,7 @@ struct mirorr_server {
     auto release_one_idle = [&] {
       Slot* big = nullptr;
       for (Slot& x : slots)
-        if (!x.busy && !x.job && x.ctx->reg &&
+        if (!x.busy && x.jobs.empty() && (x.ctx->reg || x.ctx->batch) &&
             (!big || x.ctx->budget.used.load() > big->ctx->budget.used.load()))
           big = &x;
       if (!big) return false;
@@ -149,6 +183,7 @@ struct mirorr_server {
       {
         std::lock_guard<std::mutex> lk(big->ctx->mu);
         big->ctx->reg.reset();
+        big->ctx->batch.reset();
       }
       (void)hipSetDevice(prev);
       return true;
@@ -157,7 +192,7 @@ struct mirorr_server {
     for (auto it = pending.begin(); it != pending.end() && !stopping;) {
       int w = -1;
       for (size_t i = 0; i < slots.size(); ++i)
-        if (!slots[i].busy && !slots[i].job) {
+        if (!slots[i].busy && slots[i].jobs.empty()) {
           // prefer the idle slot keeping the largest workspace
           if (w < 0 || slots[i].ctx->budget.used.load() > slots[w].ctx->budget.used.load())
             w = (int)i;
@@ -191,8 +226,29 @@ struct mirorr_server {
       // of the two).
       s.reserve = s.ctx->budget.used.load() + r.bound;
       s.busy = true;
-      s.job = std::move(*it);
+      s.jobs.clear();
+      s.jobs.push_back(std::move(*it));
       it = pending.erase(it);
+      if (batch_max > 1 && batchable(*s.jobs[0])) {
+        // Later requests that run with it and fit join the batch (indices:
+        // erasing from a deque invalidates its iterators).
+        const size_t pos = (size_t)(it - pending.begin());
+        size_t sum = 0;
+        for (const Slot& x : slots) sum += held(x);
+        for (size_t k = pos; k < pending.size() && (int)s.jobs.size() < batch_max;) {
+          Request& q = *pending[k];
+          if (same_batch(*s.jobs[0], q) && (limit == 0 || sum + q.bound <= limit)) {
+            sum += q.bound;
+            s.reserve += q.bound;
+            for (Request* wq : waiting) ++wq->overtaken;
+            s.jobs.push_back(std::move(pending[k]));
+            pending.erase(pending.begin() + (std::ptrdiff_t)k);
+          } else {
+            ++k;
+          }
+        }
+        it = pending.begin() + (std::ptrdiff_t)pos;
+      }
       int n = 0;
       size_t sum = 0;
       for (const Slot& x : slots) {
@@ -230,23 +286,95 @@ struct mirorr_server {
     res.t_end = now();
   }
 
+  // Several requests through mirorr_register_batch; their results carry the
+  // batch's start and end times.
+  void run_batch(Slot& slot, int w, std::vector<std::unique_ptr<Request>>& jobs,
+                 std::vector<mirorr_request_result>& res) {
+    if (jobs.size() == 1) {
+      run(slot, w, *jobs[0], res[0]);
+      return;
+    }
+    mirorr_ctx* c = slot.ctx;
+    const int n = (int)jobs.size();
+    const double t_start = now();
+    std::vector<mirorr_volume*> vf(n, nullptr), vm(n, nullptr);
+    std::vector<int> ok;
+    for (int i = 0; i < n; ++i) {
+      Request& r = *jobs[i];
+      res[i].t_submit = r.t_submit;
+      res[i].t_start = t_start;
+      res[i].slot = w;
+      res[i].bytes_bound = r.bound;
+      mirorr_status st = mirorr_volume_create_storage(c, r.pF, r.pFm, &r.gF, r.storage, &vf[i]);
+      if (st == MIRORR_OK)
+        st = mirorr_volume_create_storage(c, r.pM, r.pMm, &r.gM, r.storage, &vm[i]);
+      std::vector<float>().swap(r.F);
+      std::vector<float>().swap(r.M);
+      std::vector<uint8_t>().swap(r.Fm);
+      std::vector<uint8_t>().swap(r.Mm);
+      res[i].status = st;
+      if (st != MIRORR_OK)
+        std::snprintf(res[i].error, sizeof(res[i].error), "%s", mirorr_last_error(c));
+      else
+        ok.push_back(i);
+    }
+    const int k = (int)ok.size();
+    if (k > 0) {
+      const Request& r0 = *jobs[ok[0]];
+      std::vector<const mirorr_volume*> F(k), M(k);
+      std::vector<double> init(r0.has_init ? 12 * k : 0), center(r0.has_center ? 3 * k : 0);
+      std::vector<double> out(12 * k), oc(3 * k);
+      std::vector<mirorr_reg_stats> st(k);
+      for (int j = 0; j < k; ++j) {
+        const Request& r = *jobs[ok[j]];
+        F[j] = vf[ok[j]];
+        M[j] = vm[ok[j]];
+        if (r0.has_init) std::memcpy(&init[12 * j], r.init, sizeof(r.init));
+        if (r0.has_center) std::memcpy(&center[3 * j], r.center, sizeof(r.center));
+      }
+      const mirorr_status bs = mirorr_register_batch(
+          c, &r0.p, k, F.data(), M.data(), r0.has_init ? init.data() : nullptr,
+          r0.has_center ? center.data() : nullptr, out.data(), oc.data(), st.data());
+      for (int j = 0; j < k; ++j) {
+        mirorr_request_result& q = res[ok[j]];
+        q.status = bs;
+        if (bs == MIRORR_OK) {
+          std::memcpy(q.params, &out[12 * j], sizeof(q.params));
+          std::memcpy(q.center, &oc[3 * j], sizeof(q.center));
+          q.stats = st[j];
+        } else {
+          std::snprintf(q.error, sizeof(q.error), "%s", mirorr_last_error(c));
+        }
+      }
+    }
+    for (int i = 0; i < n; ++i) {
+      mirorr_volume_destroy(vf[i]);
+      mirorr_volume_destroy(vm[i]);
+    }
+    const double t_end = now();
+    for (auto& q : res) q.t_end = t_end;
+  }
+
   void worker(int w) {
     (void)hipSetDevice(device);
     for (;;) {
-      std::unique_ptr<Request> job;
+      std::vector<std::unique_ptr<Request>> jobs;
       {
         std::unique_lock<std::mutex> lk(mu);
-        cv_work.wait(lk, [&] { return stopping || slots[w].job != nullptr; });
-        if (!slots[w].job) return;  // stopping with nothing to run
-        job = std::move(slots[w].job);
+        cv_work.wait(lk, [&] { return stopping || !slots[w].jobs.empty(); });
+        if (slots[w].jobs.empty()) return;  // stopping with nothing to run
+        jobs.swap(slots[w].jobs);
       }
-      mirorr_request_result res{};
-      run(slots[w], w, *job, res);
+      std::vector<mirorr_request_result> res(jobs.size());
+      run_batch(slots[w], w, jobs, res);
       std::lock_guard<std::mutex> lk(mu);
       slots[w].busy = false;
       slots[w].reserve = 0;
-      done[job->ticket] = res;
-      ++completed;
+      for (size_t i = 0; i < jobs.size(); ++i) {
+        done[jobs[i]->ticket] = res[i];
+        ++completed;
+      }
+      max_batch = std::max(max_batch, (int)jobs.size());
       schedule();
       cv_done.notify_all();
     }
@@ -418,6 +546,18 @@ mirorr_status mirorr_server_info_get(mirorr_server* S, mirorr_server_info* out) 
   out->hbm_peak = S->budget.peak.load();
   out->max_running = S->max_running;
   out->reserve_peak = S->reserve_peak;
+  out->batch_max = S->batch_max;
+  out->max_batch = S->max_batch;
+  return MIRORR_OK;
+}
+
+mirorr_status mirorr_server_set_batch(mirorr_server* S, int max_pairs) {
+  if (!S || max_pairs < 1 || max_pairs > 256) {
+    g_server_error = "batch size must be 1 to 256";
+    return MIRORR_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lk(S->mu);
+  S->batch_max = max_pairs;
   return MIRORR_OK;
 }
 

@@ -44,10 +44,14 @@ class RegistrationServer:
     """Owns a mirorr_server: `slots` workers sharing an HBM budget of
     `hbm_budget` bytes (0: the device's free memory less a reserve)."""
 
-    def __init__(self, device: int = 0, hbm_budget: int = 0, slots: int = 4):
+    def __init__(self, device: int = 0, hbm_budget: int = 0, slots: int = 4, batch: int = 1):
+        """`batch` > 1: a slot runs up to that many pending requests with the
+        same parameters together (mirorr_server_set_batch)."""
         self._h = C.c_void_p()
         capi.check_server(capi.lib().mirorr_server_create(int(device), int(hbm_budget),
                                                           int(slots), C.byref(self._h)))
+        if batch != 1:
+            capi.check_server(capi.lib().mirorr_server_set_batch(self._h, int(batch)))
         self._affine = {}
         self._borrowed = {}  # ticket -> arrays the library reads until the result is polled
 

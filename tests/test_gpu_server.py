@@ -198,3 +198,55 @@ def test_destroy_completes_the_waiting_tickets():
     assert all("shut down" in r["error"] and r["slot"] == -1 for r in shut)
     for r in ok:
         np.testing.assert_array_equal(r["params"], ref[0])
+
+
+def test_batched_slots_give_each_request_its_alone_result():
+    # Requests with equal parameters are batched on a slot (one search launch
+    # per pass over every pair's tiles); the others run alone. Three groups:
+    # rigid NW 7 (uneven sizes), affine NW 9, and an SSD request that cannot
+    # be batched. Each result equals the request's mirorr_register alone, bit
+    # for bit, and the budget check still covers what the slots hold.
+    specs = [((64, 64, 32), (3.0, 3.0, 3.0), "rigid", 7, 3, 2),
+             ((72, 56, 40), (2.5, 2.5, 3.0), "rigid", 7, 3, 2),
+             ((48, 48, 48), (3.0, 3.0, 3.0), "rigid", 7, 3, 2),
+             ((56, 64, 36), (2.0, 2.5, 3.0), "affine", 9, 3, 2),
+             ((64, 48, 40), (2.5, 3.0, 2.5), "affine", 9, 3, 2),
+             ((40, 48, 40), (3.0, 3.0, 3.0), "rigid", 7, 3, 2),
+             ((60, 52, 44), (2.5, 2.5, 2.5), "affine", 9, 3, 2)]
+    pairs = [phantom.make_pair(dims=d, spacing=sp, n_ellipsoids=16, seed=300 + k, model=m)
+             for k, (d, sp, m, _, _, _) in enumerate(specs)]
+
+    def reg(k, metric="normalized_correlation"):
+        _, _, m, nw, ng, lev = specs[k]
+        r = MirorrPyramid()
+        r.SetTransformType(m)
+        r.SetNhoodWidth(nw)
+        r.SetNhoodGap(ng)
+        r.GetPyramidScheduleTuner().SetMaxLevelNumber(lev)
+        r.SetBlockMetricType(metric)
+        return r
+
+    ssd = len(pairs) - 1
+    metrics = ["normalized_correlation"] * (len(pairs) - 1) + ["sum_of_squared_differences"]
+    alone = []
+    for k, pr in enumerate(pairs):
+        r = reg(k, metrics[k])
+        r.SetFixedImage(pr.fixed)
+        r.SetMovingImage(pr.moving)
+        alone.append(r.RunMirorr())
+    with RegistrationServer(slots=2, batch=4) as srv:
+        ts = [srv.submit(pr.fixed, pr.moving, reg(k, metrics[k]).params, borrow=k % 2 == 0)
+              for k, pr in enumerate(pairs)]
+        res = [srv.poll(t) for t in ts]
+        info = srv.info()
+    for k, (r, (p, c)) in enumerate(zip(res, alone)):
+        assert r["status"] == capi.OK, r["error"]
+        np.testing.assert_array_equal(r["params"], p, err_msg=f"request {k}")
+        np.testing.assert_array_equal(r["center"], c, err_msg=f"request {k}")
+    starts = {}
+    for r in res:
+        starts.setdefault((r["slot"], r["t_start"]), []).append(r)
+    print("batches:", sorted(len(v) for v in starts.values()), "max_batch", info["max_batch"])
+    assert info["batch_max"] == 4 and info["max_batch"] >= 2
+    assert info["reserve_peak"] <= info["hbm_budget"]
+    assert res[ssd]["status"] == capi.OK
