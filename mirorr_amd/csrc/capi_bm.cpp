@@ -212,6 +212,10 @@ void mirorr_ctx_destroy(mirorr_ctx* c) {
   }
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  for (int i = 0; i < 2; ++i) {
+    if (c->stage[i]) (void)hipHostFree(c->stage[i]);
+    if (c->stage_ev[i]) (void)hipEventDestroy(c->stage_ev[i]);
+  }
   delete c;
 }
 

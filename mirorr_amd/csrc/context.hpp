@@ -46,6 +46,10 @@ struct mirorr_ctx {
   std::shared_ptr<mirorr::RegCache> reg;
   // Per-pair caches of mirorr_register_batch, likewise grown and reused.
   std::shared_ptr<mirorr::BatchCache> batch;
+  // Pinned staging chunks of volume uploads (double-buffered), created on
+  // first use and freed with the context.
+  void* stage[2] = {nullptr, nullptr};
+  hipEvent_t stage_ev[2] = {nullptr, nullptr};
 };
 
 namespace mirorr {

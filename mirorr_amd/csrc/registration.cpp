@@ -1202,12 +1202,48 @@ mirorr_status mirorr_volume_create(mirorr_ctx* c, const float* data, const uint8
   return mirorr_volume_create_storage(c, data, mask, geom, MIRORR_STORAGE_F32, out);
 }
 
+}  // extern "C"
+
+namespace {
+
+constexpr size_t kStageChunk = size_t(8) << 20;  // bytes per pinned staging chunk
+
+// Copies `bytes` bytes to device memory `dst` through the context's two
+// pinned staging chunks: fill(offset, count, chunk) writes bytes
+// [offset, offset + count) of the stream into a chunk, and the chunk goes to
+// the device while the host fills the other one. Queued on the context's
+// stream; the caller synchronises it before the source may change.
+template <class Fill>
+void staged_upload(mirorr_ctx* c, unsigned char* dst, size_t bytes, Fill fill) {
+  for (int i = 0; i < 2; ++i) {
+    if (!c->stage[i])
+      MIRORR_HIP(hipHostMalloc(&c->stage[i], kStageChunk, hipHostMallocDefault));
+    if (!c->stage_ev[i])
+      MIRORR_HIP(hipEventCreateWithFlags(&c->stage_ev[i], hipEventDisableTiming));
+  }
+  for (size_t off = 0, k = 0; off < bytes; off += kStageChunk, ++k) {
+    const int b = (int)(k & 1);
+    const size_t cnt = std::min(kStageChunk, bytes - off);
+    // The chunk's last copy, of this upload or an earlier one, has been
+    // read (an event never recorded completes at once).
+    MIRORR_HIP(hipEventSynchronize(c->stage_ev[b]));
+    fill(off, cnt, static_cast<unsigned char*>(c->stage[b]));
+    MIRORR_HIP(hipMemcpyAsync(dst + off, c->stage[b], cnt, hipMemcpyHostToDevice, c->stream));
+    MIRORR_HIP(hipEventRecord(c->stage_ev[b], c->stream));
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
 mirorr_status mirorr_volume_create_storage(mirorr_ctx* c, const float* data, const uint8_t* mask,
                                            const mirorr_geom* geom, int storage,
                                            mirorr_volume** out) {
   if (!c || !out) return MIRORR_ERR_ARG;
   *out = nullptr;
   mirorr_volume* v = nullptr;
+  std::lock_guard<std::mutex> lk(c->mu);  // the staging chunks are the context's
   mirorr_status st = guarded(c, [&] {
     if (!data || !geom) throw Error(kErrArg, "null volume data or geometry");
     if (storage != MIRORR_STORAGE_F32 && storage != MIRORR_STORAGE_F16)
@@ -1220,24 +1256,32 @@ mirorr_status mirorr_volume_create_storage(mirorr_ctx* c, const float* data, con
     v->storage = storage;
     const size_t n = (size_t)geom->dims[0] * geom->dims[1] * geom->dims[2];
     v->data.ensure(n * storage_bytes(storage), &c->budget);
+    // Uploads go through pinned staging chunks (the host fills one while
+    // the other crosses the bus), twice the rate of a pageable copy.
+    unsigned char* dst = v->data.as<unsigned char>();
     if (storage == MIRORR_STORAGE_F16) {
-      // Encoded on the host: half the bytes cross the bus.
+      // Encoded on the host, chunk by chunk into the staging memory: half
+      // the bytes cross the bus.
       const HalfEncoding e = half_encoding(data, n);
-      std::vector<half_t> h(n);
-      encode_half(data, n, e, h.data());
       v->enc_offset = e.offset;
       v->enc_scale = e.scale;
       v->enc_zero = e.zero();
-      MIRORR_HIP(hipMemcpyAsync(v->data.as<void>(), h.data(), n * sizeof(half_t),
-                                hipMemcpyHostToDevice, c->stream));
-      MIRORR_HIP(hipStreamSynchronize(c->stream));  // h leaves scope
+      staged_upload(c, dst, n * sizeof(half_t), [&](size_t off, size_t cnt, unsigned char* h) {
+        encode_half(data + off / sizeof(half_t), cnt / sizeof(half_t), e,
+                    reinterpret_cast<half_t*>(h));
+      });
     } else {
-      MIRORR_HIP(hipMemcpyAsync(v->data.as<float>(), data, n * sizeof(float),
-                                hipMemcpyHostToDevice, c->stream));
+      const unsigned char* src = reinterpret_cast<const unsigned char*>(data);
+      staged_upload(c, dst, n * sizeof(float), [&](size_t off, size_t cnt, unsigned char* h) {
+        std::memcpy(h, src + off, cnt);
+      });
     }
     if (mask) {
       v->mask.ensure(n, &c->budget);
-      MIRORR_HIP(hipMemcpyAsync(v->mask.as<uint8_t>(), mask, n, hipMemcpyHostToDevice, c->stream));
+      staged_upload(c, v->mask.as<unsigned char>(), n,
+                    [&](size_t off, size_t cnt, unsigned char* h) {
+                      std::memcpy(h, mask + off, cnt);
+                    });
       v->has_mask = true;
     }
     MIRORR_HIP(hipStreamSynchronize(c->stream));
