@@ -272,6 +272,57 @@ def batch_jobs(cfg):
     return jobs, params
 
 
+def measure_lockstep(ctx, pairs, params, plain, steps):
+    """mirorr_register_batch over the pairs (one batch per parameter set),
+    against mirorr_register one pair at a time, both on volumes already in
+    HBM (uploads excluded on both sides)."""
+    from mirorr_amd.registration import DeviceVolume, MirorrPyramid, register_batch
+    F = [DeviceVolume(pr.fixed, ctx) for pr in pairs]
+    M = [DeviceVolume(pr.moving, ctx) for pr in pairs]
+    regs = []
+    for p in params:
+        r = MirorrPyramid(ctx)
+        r.SetParameters(p)
+        regs.append(r)
+    groups = {}
+    for i, p in enumerate(params):
+        key = (p.affine, p.nhood_width, p.nhood_gap, p.pyr_num, p.iterations, p.variance_select,
+               p.portion_kept)
+        groups.setdefault(key, []).append(i)
+
+    def one_by_one():
+        return [r.RunMirorr(fixed_dev=f, moving_dev=m) for r, f, m in zip(regs, F, M)]
+
+    def lockstep():
+        res = [None] * len(pairs)
+        for g in groups.values():
+            for i, x in zip(g, register_batch(regs[g[0]], [F[i] for i in g], [M[i] for i in g])):
+                res[i] = x
+        return res
+
+    one_by_one()
+    lockstep()
+    ctx.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        one_by_one()
+    t_one = time.perf_counter() - t
+    t = time.perf_counter()
+    for _ in range(steps):
+        res = lockstep()
+    t_lock = time.perf_counter() - t
+    same = all(np.array_equal(a[0], q) and np.array_equal(a[1], c)
+               for a, (q, c, _) in zip(res, plain))
+    for v in F + M:
+        v.close()
+    n = len(pairs) * steps
+    return {"registrations_per_s": n / t_lock, "one_by_one_registrations_per_s": n / t_one,
+            "vs_one_by_one": t_one / t_lock, "batches": len(groups),
+            "results_identical_to_plain": same,
+            "note": "mirorr_register_batch per parameter set against mirorr_register one pair "
+                    "at a time, volumes already in HBM"}
+
+
 def measure_batch(pairs, params, cfg, steps, warmup, storages=("f32",)):
     """One step = every pair registered once: through the plain path (one
     context, one pair at a time, float32) and through the server (all
@@ -293,6 +344,7 @@ def measure_batch(pairs, params, cfg, steps, warmup, storages=("f32",)):
     n = len(pairs) * steps
     out = {"plain_registrations_per_s": n / t_plain, "voxel_evals_per_step": evals,
            "plain_ms_per_step": t_plain / steps * 1e3}
+    out["lockstep"] = measure_lockstep(ctx, pairs, params, plain, steps)
     for storage in storages:
         with RegistrationServer(slots=cfg["slots"]) as srv:
             def batch():
@@ -355,6 +407,7 @@ def run_batch(args, cfg):
         "max_running": f32["max_running"],
         "hbm_peak_bytes": f32["hbm_peak_bytes"],
         "f16_storage": m["f16"],
+        "lockstep": m["lockstep"],
     })
 
 
@@ -618,7 +671,7 @@ def main():
             "note": "16 distinct 256x256x128 affine pairs submitted at once, uploads included; "
                     "plain = one pair at a time on one context, float32",
             "plain_registrations_per_s": m["plain_registrations_per_s"],
-            "f32": m["f32"], "f16": m["f16"],
+            "f32": m["f32"], "f16": m["f16"], "lockstep": m["lockstep"],
         }
         del bpairs
     if not args.no_cpu_baseline:

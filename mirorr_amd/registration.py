@@ -89,6 +89,10 @@ class MirorrPyramid:
         self.stats = None
         self._tuner = _ScheduleTuner(p)
 
+    def SetParameters(self, params: capi.RegParams):
+        """Takes a whole mirorr_reg_params (a copy)."""
+        C.memmove(C.byref(self._p), C.byref(params), C.sizeof(capi.RegParams))
+
     # registration type / transform (itkMirorrPyramidImplement.h:104-113)
     def SetRegistrationTypeClassic(self): self._p.symmetric = 0
     def SetRegistrationTypeSymmetrical(self): self._p.symmetric = 1
