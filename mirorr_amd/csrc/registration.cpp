@@ -468,6 +468,12 @@ struct RegRun {
     w.vsel_hist.ensure(sizeof(unsigned int) * kVarSelHistBins, bud);
   }
 
+  LtsItem lts_item(LtsState* st, OuterState* outer, int64_t npts, int* flags) {
+    return LtsItem{w.X.as<double>(), w.Y.as<double>(), w.kept.as<uint8_t>(), npts, st, outer,
+                   w.r.as<double>(), w.hist1.as<unsigned int>(), w.hist2.as<unsigned int>(),
+                   w.cand.as<unsigned long long>(), w.part.as<double>(), flags};
+  }
+
   LtsBuffers lts_buffers(int64_t npts) {
     LtsBuffers lb;
     lb.r = w.r.as<double>();
@@ -1129,12 +1135,7 @@ void run_registration_batch(mirorr_ctx* ctx, const mirorr_reg_params* p, int n,
       int64_t max_npts = 1;
       for (int k = 0; k < nI; ++k) {
         PairReg& r = *R[I[k]];
-        Workspace& w = r.run.w;
-        items[k] = LtsItem{w.X.as<double>(), w.Y.as<double>(), w.kept.as<uint8_t>(),
-                           r.npts, r.st(), r.outer(), w.r.as<double>(),
-                           w.hist1.as<unsigned int>(), w.hist2.as<unsigned int>(),
-                           w.cand.as<unsigned long long>(), w.part.as<double>(),
-                           bc.flagsD.as<int>() + 2 * k};
+        items[k] = r.run.lts_item(r.st(), r.outer(), r.npts, bc.flagsD.as<int>() + 2 * k);
         max_npts = std::max<int64_t>(max_npts, r.npts);
       }
       // pageable source: read when the call returns

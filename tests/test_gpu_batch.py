@@ -25,7 +25,7 @@ def _pairs(specs):
 
 
 def _reg(ctx, symmetric=True, affine=False, nw=9, levels=3, iterations=5, storage="f32",
-         stop=None):
+         stop=None, vsel=False, resampling=None):
     from mirorr_amd.registration import MirorrPyramid
     reg = MirorrPyramid(ctx)
     if not symmetric:
@@ -38,6 +38,10 @@ def _reg(ctx, symmetric=True, affine=False, nw=9, levels=3, iterations=5, storag
     reg.SetStorage(storage)
     if stop is not None:
         reg.SetStopTolerances(*stop)
+    if vsel:
+        reg.SetVarianceSelection(True)
+    if resampling:
+        reg.SetResamplingType(resampling)
     return reg
 
 
@@ -54,7 +58,7 @@ def _volumes(pairs, ctx, storage, masks):
     return F, M
 
 
-def _check(pairs, *, masks=False, storage="f32", **kw):
+def _check(pairs, *, masks=False, storage="f32", init=None, **kw):
     from mirorr_amd.blockmatcher import Context
     from mirorr_amd.registration import register_batch
     ctx_a, ctx_b = Context(0), Context(0)
@@ -62,12 +66,13 @@ def _check(pairs, *, masks=False, storage="f32", **kw):
     Fb, Mb = _volumes(pairs, ctx_b, storage, masks)
     alone = []
     ra = _reg(ctx_a, storage=storage, **kw)
-    for f, m in zip(Fa, Ma):
-        p, c = ra.RunMirorr(fixed_dev=f, moving_dev=m)
+    for k, (f, m) in enumerate(zip(Fa, Ma)):
+        ip, ic = (None, None) if init is None else (init[0][k], init[1][k])
+        p, c = ra.RunMirorr(init_params=ip, init_center=ic, fixed_dev=f, moving_dev=m)
         alone.append((np.array(p), np.array(c), dict(ra.stats)))
     rb = _reg(ctx_b, storage=storage, **kw)
     for rep in range(2):  # the second run reuses the batch's buffers
-        got = register_batch(rb, Fb, Mb)
+        got = register_batch(rb, Fb, Mb, *(init or (None, None)))
         for k, ((p, c), st) in enumerate(zip(got, rb.batch_stats)):
             pa, ca, sa = alone[k]
             np.testing.assert_array_equal(p, pa, err_msg=f"pair {k} run {rep}")
@@ -113,6 +118,21 @@ def test_stop_rules_and_halved_iterations_in_a_batch():
     # the reference's stop rules at their defaults (pairs stop at different
     # iterations) and negative iterations (halved per level)
     _check(_pairs(UNEVEN), iterations=-8, masks=True)
+
+
+def test_start_transforms_variance_selection_and_resampling_modes_in_a_batch():
+    pairs = _pairs(UNEVEN[:3])
+    rng = np.random.default_rng(5)
+    init_p = [np.concatenate([rng.uniform(-0.03, 0.03, 3), rng.uniform(-2, 2, 3)])
+              for _ in pairs]
+    init_c = [rng.uniform(-5, 5, 3) for _ in pairs]
+    _check(pairs, init=(init_p, init_c))
+    # variance selection acts from a pair's third level on
+    big = _pairs([((128, 128, 64), 1.5, 9, "rigid"), ((144, 112, 96), 1.2, 10, "rigid")])
+    alone = _check(big, vsel=True, masks=True)
+    assert any(a[2]["levels"] >= 3 for a in alone)
+    _check(pairs, symmetric=False, resampling="fixed")
+    _check(pairs, resampling="basic", affine=False)
 
 
 def test_single_pair_and_empty_batches():
