@@ -78,7 +78,7 @@ SERVED = {
     # the rest at 256x256x128, drawn from a pool of distinct pairs.
     "serve_poisson_ctmr": dict(
         workload="serve_poisson_ctmr", rates_per_s=[0.5, 1.0, 2.0, 4.0],
-        requests_per_rate=12, large_fraction=0.3, large_dims=[512, 512, 256],
+        requests_per_rate=100, large_fraction=0.3, large_dims=[512, 512, 256],
         small_dims=[256, 256, 128], pool_large=2, pool_small=3, slots=4, seed=11,
         nhood_width=13, nhood_gap=3, pyramid_levels=4, ellipsoids=24),
 }
@@ -434,11 +434,13 @@ def run_serve(args, cfg):
     def serve(srv, arrivals, reqs):
         t0 = time.perf_counter()
         ts = []
-        for a, pr in zip(arrivals, reqs):
+        for k, (a, pr) in enumerate(zip(arrivals, reqs)):
             d = t0 + a - time.perf_counter()
             if d > 0:
                 time.sleep(d)
             ts.append(srv.submit(pr.fixed, pr.moving, p, borrow=True))
+            if (k + 1) % 10 == 0:
+                print(f"bench: submitted {k + 1}/{len(reqs)}", file=sys.stderr, flush=True)
         res = [srv.poll(x) for x in ts]
         bad = [r["error"] for r in res if r["status"] != 0]
         if bad:
@@ -496,6 +498,8 @@ def main():
                     choices=sorted(WORKLOADS) + sorted(SERVED))
     ap.add_argument("--slots", type=int, default=0,
                     help="served workloads: worker slots (0: the workload's own)")
+    ap.add_argument("--requests-per-rate", type=int, default=0,
+                    help="serve_poisson_ctmr: requests per arrival rate (0: the workload's own)")
     ap.add_argument("--iteration-graphs", type=int, choices=(0, 1), default=None,
                     help="replay each level's outer iterations from captured HIP graphs "
                          "(default: the library's default)")
@@ -509,6 +513,8 @@ def main():
         cfg = dict(SERVED[args.workload])
         if args.slots > 0:
             cfg["slots"] = args.slots
+        if args.requests_per_rate > 0 and "requests_per_rate" in cfg:
+            cfg["requests_per_rate"] = args.requests_per_rate
         run = run_batch if args.workload.startswith("batch") else run_serve
         print(json.dumps(run(args, cfg)), flush=True)
         return
