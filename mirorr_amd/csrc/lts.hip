@@ -653,6 +653,7 @@ __device__ __forceinline__ void finish_body(LtsState* st, OuterState* o, int* fl
   }
   flags[0] = st->active;
   flags[1] = o->stop;
+  flags[2] = st->iter + 1;  // inner iterations run so far
 }
 
 __device__ void finish_update(LtsState* st, OuterState* o) {

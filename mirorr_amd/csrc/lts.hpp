@@ -68,7 +68,8 @@ void lts_enqueue_iteration(const double* X, const double* Y, const uint8_t* kept
 // resampling maps of the next iteration.
 // Queued after each chunk of inner iterations: once the LTS is inactive it
 // composes the fit into the outer state (exactly once per run). Every launch
-// then writes flags[0] = active and flags[1] = the outer stop flag.
+// then writes flags[0] = active, flags[1] = the outer stop flag and
+// flags[2] = the inner iterations run so far (flags holds 3 ints).
 void lts_finish(LtsState* st, OuterState* outer, int* flags, hipStream_t s);
 // Computes the resampling maps of the current outer state (level start).
 void outer_prepare_maps(OuterState* outer, int symmetric, hipStream_t s);
@@ -90,7 +91,7 @@ struct LtsItem {
   unsigned int* hist2;
   unsigned long long* cand;
   double* part;          // lts_parts(npts) partials
-  int* flags;            // {active, stop}, written by lts_finish_multi
+  int* flags;            // {active, stop, iterations}, written by lts_finish_multi
 };
 void lts_start_multi(const LtsItem* items, int n, int kind, int symmetric, double tol,
                      hipStream_t s);

@@ -42,7 +42,7 @@ constexpr size_t kOffHist2 = kOffHist1 + al(sizeof(unsigned int) * kHistBins);
 constexpr size_t kOffLts = kOffHist2 + al(sizeof(unsigned int) * kHistBins);
 constexpr size_t kOffOuter = kOffLts + al(sizeof(LtsState));
 constexpr size_t kOffFlags = kOffOuter + al(sizeof(OuterState));
-constexpr size_t kScratchBytes = kOffFlags + al(2 * sizeof(int));
+constexpr size_t kScratchBytes = kOffFlags + al(3 * sizeof(int));  // lts_finish writes 3
 
 // The file-scope scratch array (about 66 MB). In host code its name is a
 // host-side stand-in, never a device address: only scratch_base() below
@@ -142,8 +142,8 @@ extern "C" mirorr_status mirorr_lts_fit_noalloc(mirorr_ctx* c, int symmetric, in
     mirorr_reg_default_params(&rp);
     lts_start(st, outer, kind, symmetric ? 1 : 0, rp.lts_tolerance, lb, s);
     // Chunks of 4 predicated inner iterations, each followed by the finish
-    // and one read-back of {active, stop} into host stack memory.
-    int hflags[2] = {1, 0};
+    // and one read-back of its flags into host stack memory.
+    int hflags[3] = {1, 0, 0};
     const int chunk = 4;
     for (int it = 0; it < 101; it += chunk) {
       const int it1 = std::min(it + chunk, 101);

@@ -5,8 +5,8 @@
 // (itkSymMirorrRegistrationMethod.txx:238-720) StartOptimization in
 // middle-space resampling mode. All per-voxel and per-block work runs on the
 // GPU and the transforms of the iteration live in device memory (OuterState,
-// LtsState); the host only reads two flags per outer iteration and one per
-// chunk of LTS inner iterations. Level set-up (grids, centring, half-space
+// LtsState); the host only reads a few flags per chunk of LTS inner
+// iterations (the last chunk's carry the outer stop flag). Level set-up (grids, centring, half-space
 // square roots) is FP64 host code on 4x4 matrices.
 #include <algorithm>
 #include <chrono>
@@ -180,16 +180,20 @@ struct Workspace {
   DeviceBuffer best2, second2, match2, queue2, qbits2, qcount2;  // the second pass's
   DeviceBuffer X, Y, kept;                    // LTS pairs
   DeviceBuffer r, hist1, hist2, cand, part;
-  DeviceBuffer lts, outer, mapI, flagsD;  // flagsD: {active, stop} for one read-back
+  DeviceBuffer lts, outer, mapI, flagsD;  // flagsD: Flags for one read-back
   DeviceBuffer pyr_tmp, pyr_map;
   DeviceBuffer vsel_st, vsel_hist, vsel_evals;  // block-variance selection
   DeviceBuffer rlist, rcount, rlist2, rcount2;  // blocks listed for the exact re-scoring
   DeviceBuffer rescored, rescored2;             // blocks re-scored in double, per pass
 };
 
+// What lts_finish writes for the host: the LTS's active flag, the outer
+// stop flag and the inner iterations run so far.
 struct Flags {
   int active;
   int stop;
+  int iters;
+  int pad;
 };
 
 // Bytes of every host value an outer iteration's launches depend on: two
@@ -284,12 +288,12 @@ RegCache& reg_cache(mirorr_ctx* c) {
 }
 
 // Batched registrations: one registration cache per pair, the per-pair
-// arguments of the batched searches and LTS, and the pairs' {active, stop}
-// flags (device and pinned host).
+// arguments of the batched searches and LTS, and the pairs' Flags (device and
+// pinned host).
 struct BatchCache {
   std::vector<std::unique_ptr<RegCache>> regs;
   DeviceBuffer args[2], items, flagsD;
-  int* flagsH = nullptr;
+  Flags* flagsH = nullptr;
   int cap = 0;
   BatchCache() = default;
   BatchCache(const BatchCache&) = delete;
@@ -301,12 +305,12 @@ struct BatchCache {
     while ((int)regs.size() < n) regs.emplace_back(new RegCache());
     for (DeviceBuffer& a : args) a.ensure(search_batch_bytes(n), bud);
     items.ensure(sizeof(LtsItem) * n, bud);
-    flagsD.ensure(sizeof(int) * 2 * n, bud);
+    flagsD.ensure(sizeof(Flags) * n, bud);
     if (cap < n) {
       if (flagsH) (void)hipHostFree(flagsH);
       flagsH = nullptr;
       cap = 0;
-      MIRORR_HIP(hipHostMalloc((void**)&flagsH, sizeof(int) * 2 * n, hipHostMallocDefault));
+      MIRORR_HIP(hipHostMalloc((void**)&flagsH, sizeof(Flags) * n, hipHostMallocDefault));
       cap = n;
     }
   }
@@ -486,11 +490,11 @@ struct RegRun {
   }
 
   // Inner iterations [it0, it1) (predicated on the LTS being active), the
-  // finish and one read-back of {active, stop}: when the run has converged,
-  // the outer update is already done and its stop flag known.
+  // finish and one read-back of its flags: when the run has converged, the
+  // outer update is already done and its stop flag known.
   void enqueue_lts_chunk(LtsState* st, OuterState* outer, int64_t npts, Flags* h_flags, int it0,
                          int it1) {
-    static_assert(sizeof(Flags) == 2 * sizeof(int), "Flags is {active, stop}");
+    static_assert(sizeof(Flags) == 4 * sizeof(int), "lts_finish writes the first 3 ints");
     LtsBuffers lb = lts_buffers(npts);
     for (int j = it0; j < it1; ++j)
       lts_enqueue_iteration(w.X.as<double>(), w.Y.as<double>(), w.kept.as<uint8_t>(), npts, st,
@@ -500,32 +504,50 @@ struct RegRun {
                               s));
   }
 
-  static constexpr int kLtsChunk = 4;
+  // Chunk sizes of the predicated LTS: the first chunk (`first`, given to
+  // lts_begin and the same to lts_complete), then one of kLtsNext, then
+  // kLtsChunk each. An iteration queued after the LTS has converged is a
+  // no-op that still costs its launches; a chunk too short costs a host
+  // round trip. lts_first is the first-chunk size of this pair's next LTS.
+  static constexpr int kLtsChunk = 4, kLtsNext = 2, kLtsFirstMax = 8;
+  int lts_first = kLtsChunk;
+
+  // The inner iterations of the LTS that has just completed, as a first-chunk
+  // size for the next one: successive LTS runs of a registration take
+  // similar counts, since their inputs change little between iterations.
+  static int first_chunk_after(const Flags& f) {
+    return std::min(std::max(f.iters, 1), kLtsFirstMax);
+  }
 
   // The LTS start and its first chunk, queued without a host wait (the
   // flags buffer is allocated beforehand: this may run under graph capture).
-  void lts_begin(LtsState* st, OuterState* outer, int64_t npts, Flags* h_flags) {
+  void lts_begin(LtsState* st, OuterState* outer, int64_t npts, Flags* h_flags, int first) {
     if (w.flagsD.bytes() < sizeof(Flags)) throw Error(kErrState, "LTS flags buffer missing");
+    if (first < 1 || first > 101) throw Error(kErrArg, "LTS first chunk out of range");
     LtsBuffers lb = lts_buffers(npts);
     lts_start(st, outer, kind, symmetric ? 1 : 0, p->lts_tolerance, lb, s);
-    enqueue_lts_chunk(st, outer, npts, h_flags, 0, kLtsChunk);
+    enqueue_lts_chunk(st, outer, npts, h_flags, 0, first);
   }
 
   // Waits for the first chunk's flags and runs further chunks while the LTS
   // is active (at most 101 inner iterations).
-  void lts_complete(LtsState* st, OuterState* outer, int64_t npts, Flags* h_flags) {
+  void lts_complete(LtsState* st, OuterState* outer, int64_t npts, Flags* h_flags, int first) {
     MIRORR_HIP(hipStreamSynchronize(s));
-    for (int it = kLtsChunk; h_flags->active && it < 101; it += kLtsChunk) {
-      enqueue_lts_chunk(st, outer, npts, h_flags, it, std::min(it + kLtsChunk, 101));
+    int it = first, step = kLtsNext;
+    while (h_flags->active && it < 101) {
+      const int it1 = std::min(it + step, 101);
+      enqueue_lts_chunk(st, outer, npts, h_flags, it, it1);
       MIRORR_HIP(hipStreamSynchronize(s));
+      it = it1;
+      step = kLtsChunk;
     }
   }
 
   // Runs the LTS to convergence with predicated chunks; returns when done.
   void run_lts(LtsState* st, OuterState* outer, int64_t npts, Flags* h_flags) {
     w.flagsD.ensure(sizeof(Flags), bud);
-    lts_begin(st, outer, npts, h_flags);
-    lts_complete(st, outer, npts, h_flags);
+    lts_begin(st, outer, npts, h_flags, kLtsChunk);
+    lts_complete(st, outer, npts, h_flags, kLtsChunk);
   }
 };
 
@@ -565,7 +587,7 @@ size_t registration_bytes_bound(const mirorr_reg_params* p, const Geom& FG, bool
   total += 2 * sizeof(unsigned int) * 4096 + sizeof(VarSelState) +
            sizeof(unsigned int) * kVarSelHistBins + 3 * sizeof(unsigned long long);
   total += 2 * (sizeof(int32_t) * nb + sizeof(int32_t));  // re-scoring lists, one per pass
-  total += sizeof(LtsState) + sizeof(OuterState) + 2 * sizeof(IndexMap) + 2 * sizeof(int);
+  total += sizeof(LtsState) + sizeof(OuterState) + 2 * sizeof(IndexMap) + sizeof(Flags);
   return total;
 }
 
@@ -665,7 +687,7 @@ struct PairReg {
     w.lts.ensure(sizeof(LtsState), bud);
     w.outer.ensure(sizeof(OuterState), bud);
     w.mapI.ensure(sizeof(IndexMap), bud);
-    w.flagsD.ensure(2 * sizeof(int), bud);
+    w.flagsD.ensure(sizeof(Flags), bud);
     std::memset(stats, 0, sizeof(*stats));
     w.vsel_evals.ensure(sizeof(unsigned long long), bud);
     MIRORR_HIP(hipMemsetAsync(w.vsel_evals.as<void>(), 0, sizeof(unsigned long long), s));
@@ -957,7 +979,7 @@ void run_registration(mirorr_ctx* ctx, const mirorr_reg_params* p, const mirorr_
         join_guard.s = nullptr;  // the first stream now orders everything after the join
       }
       // 3. LTS start and its first chunk
-      run.lts_begin(st, outer, R.npts, flags);
+      run.lts_begin(st, outer, R.npts, flags, run.lts_first);
     };
     // Iteration graph of the level: every launch of an outer iteration up to
     // the LTS's first chunk takes the same arguments within a level (the
@@ -1018,7 +1040,11 @@ void run_registration(mirorr_ctx* ctx, const mirorr_reg_params* p, const mirorr_
         enqueue_iteration();
       }
       // 3. further LTS chunks while it is active; the transform update
-      run.lts_complete(st, outer, R.npts, flags);  // leaves flags->stop read back
+      // (a graph replayed here may come from an earlier registration: with
+      // graphs on, every first chunk is kLtsChunk, as captured)
+      run.lts_complete(st, outer, R.npts, flags, run.lts_first);  // leaves flags->stop read back
+      // otherwise the next LTS's first chunk is sized by this one's count
+      if (!graphs) run.lts_first = RegRun::first_chunk_after(*flags);
       if (p->verbosity >= 1) R.report(it);  // as the reference's iteration report
       if (flags->stop) {
         ++it;
@@ -1135,7 +1161,7 @@ void run_registration_batch(mirorr_ctx* ctx, const mirorr_reg_params* p, int n,
       int64_t max_npts = 1;
       for (int k = 0; k < nI; ++k) {
         PairReg& r = *R[I[k]];
-        items[k] = r.run.lts_item(r.st(), r.outer(), r.npts, bc.flagsD.as<int>() + 2 * k);
+        items[k] = r.run.lts_item(r.st(), r.outer(), r.npts, &bc.flagsD.as<Flags>()[k].active);
         max_npts = std::max<int64_t>(max_npts, r.npts);
       }
       // pageable source: read when the call returns
@@ -1143,19 +1169,28 @@ void run_registration_batch(mirorr_ctx* ctx, const mirorr_reg_params* p, int n,
                                 hipMemcpyHostToDevice, s));
       const LtsItem* di = bc.items.as<LtsItem>();
       lts_start_multi(di, nI, kind, symmetric ? 1 : 0, p->lts_tolerance, s);
-      for (int c = 0; c < 101; c += 4) {
-        for (int j = c; j < std::min(c + 4, 101); ++j)
-          lts_enqueue_iteration_multi(di, nI, max_npts, j, s);
+      // chunks as in the single registration; the first is the largest of
+      // the pairs' own first-chunk sizes
+      int step = 1;
+      for (int k = 0; k < nI; ++k) step = std::max(step, R[I[k]]->run.lts_first);
+      for (int c = 0, chunk = 0; c < 101; ++chunk) {
+        const int c1 = std::min(c + step, 101);
+        for (int j = c; j < c1; ++j) lts_enqueue_iteration_multi(di, nI, max_npts, j, s);
         lts_finish_multi(di, nI, s);
-        MIRORR_HIP(hipMemcpyAsync(bc.flagsH, bc.flagsD.as<int>(), sizeof(int) * 2 * nI,
+        MIRORR_HIP(hipMemcpyAsync(bc.flagsH, bc.flagsD.as<Flags>(), sizeof(Flags) * nI,
                                   hipMemcpyDeviceToHost, s));
         MIRORR_HIP(hipStreamSynchronize(s));
         bool any = false;
-        for (int k = 0; k < nI; ++k) any = any || bc.flagsH[2 * k] != 0;
+        for (int k = 0; k < nI; ++k) any = any || bc.flagsH[k].active != 0;
         if (!any) break;
+        c = c1;
+        step = chunk == 0 ? RegRun::kLtsNext : RegRun::kLtsChunk;
       }
       std::vector<char> stop(nI);
-      for (int k = 0; k < nI; ++k) stop[k] = bc.flagsH[2 * k + 1] != 0;
+      for (int k = 0; k < nI; ++k) {
+        stop[k] = bc.flagsH[k].stop != 0;
+        R[I[k]]->run.lts_first = RegRun::first_chunk_after(bc.flagsH[k]);
+      }
       for (int k = 0; k < nI; ++k) {
         const int i = I[k];
         its[i] = it + 1;
