@@ -214,8 +214,8 @@ __device__ void block_reduce(double (&v)[N], double* out) {
 }
 
 // Sums per-workgroup partials in a fixed order (deterministic). Thread t
-// handles value t % 32 of the partials t / 32, t / 32 + 8, ... (independent
-// loads), then the eight groups are added in order.
+// handles value t % 32 of the partials t / 32, t / 32 + 8, ... (eight loads
+// in flight at a time), then the eight groups are added in order.
 template <int N>
 __device__ void reduce_parts(const double* part, int stride, int nparts, double* out) {
   static_assert(N <= 32 && kT == 256, "32 values x 8 groups");
@@ -224,15 +224,12 @@ __device__ void reduce_parts(const double* part, int stride, int nparts, double*
   double s = 0.0;
   if (j < N) {
     int p = gi;
-    for (; p + 24 < nparts; p += 32) {  // four independent loads per step
-      const double a0 = part[(int64_t)stride * p + j];
-      const double a1 = part[(int64_t)stride * (p + 8) + j];
-      const double a2 = part[(int64_t)stride * (p + 16) + j];
-      const double a3 = part[(int64_t)stride * (p + 24) + j];
-      s += a0;
-      s += a1;
-      s += a2;
-      s += a3;
+    for (; p + 56 < nparts; p += 64) {  // eight independent loads per step
+      double a[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) a[u] = part[(int64_t)stride * (p + 8 * u) + j];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s += a[u];
     }
     for (; p < nparts; p += 8) s += part[(int64_t)stride * p + j];
   }
@@ -315,9 +312,10 @@ struct Moments {
 
 __device__ void unpack(const double* s1, const double* s2, Moments& m) {
   m.n = s1[0];
+  const double in = 1.0 / m.n;  // the solve runs on one lane: one division per divisor
   for (int k = 0; k < 3; ++k) {
-    m.mx[k] = s1[1 + k] / m.n;
-    m.my[k] = s1[4 + k] / m.n;
+    m.mx[k] = s1[1 + k] * in;
+    m.my[k] = s1[4 + k] * in;
   }
   int q = 0;
   for (int a = 0; a < 3; ++a)
@@ -358,9 +356,10 @@ __device__ Quat rigid_quat(const Moments& m) {
 
 // CovarianceOfPoints(a, b) of centred sets: sum a b^T / n.
 __device__ M3 cov(const double s[3][3], double n, bool transpose_) {
+  const double in = 1.0 / n;
   M3 r;
   for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j) r.a[i][j] = (transpose_ ? s[j][i] : s[i][j]) / n;
+    for (int j = 0; j < 3; ++j) r.a[i][j] = (transpose_ ? s[j][i] : s[i][j]) * in;
   return r;
 }
 
@@ -566,15 +565,15 @@ __device__ __forceinline__ void solve_body(LtsState* st, const double* part, int
   if (lane >= 2 || (lane == 1 && !st->symmetric)) return;
   // Centred second moments in the layout unpack() reads: xx (6), yy (6), xy (9).
   double s2[21];
-  const double cnt = t[0];
+  const double cnt = t[0], icnt = 1.0 / cnt;
   int q = 0;
   for (int a = 0; a < 3; ++a)
     for (int b = a; b < 3; ++b, ++q) {
-      s2[q] = t[7 + q] - t[1 + a] * t[1 + b] / cnt;
-      s2[6 + q] = t[13 + q] - t[4 + a] * t[4 + b] / cnt;
+      s2[q] = t[7 + q] - t[1 + a] * t[1 + b] * icnt;
+      s2[6 + q] = t[13 + q] - t[4 + a] * t[4 + b] * icnt;
     }
   for (int a = 0; a < 3; ++a)
-    for (int b = 0; b < 3; ++b) s2[12 + 3 * a + b] = t[19 + 3 * a + b] - t[1 + a] * t[4 + b] / cnt;
+    for (int b = 0; b < 3; ++b) s2[12 + 3 * a + b] = t[19 + 3 * a + b] - t[1 + a] * t[4 + b] * icnt;
   Moments m;
   unpack(t, s2, m);
   if (st->symmetric)

@@ -69,17 +69,17 @@ XF_HD double det(const M3& m) {
 }
 XF_HD M3 inverse(const M3& m) {
   const auto& a = m.a;
-  const double d = det(m);
+  const double id = 1.0 / det(m);  // one division, nine products
   M3 r;
-  r.a[0][0] = (a[1][1] * a[2][2] - a[1][2] * a[2][1]) / d;
-  r.a[0][1] = (a[0][2] * a[2][1] - a[0][1] * a[2][2]) / d;
-  r.a[0][2] = (a[0][1] * a[1][2] - a[0][2] * a[1][1]) / d;
-  r.a[1][0] = (a[1][2] * a[2][0] - a[1][0] * a[2][2]) / d;
-  r.a[1][1] = (a[0][0] * a[2][2] - a[0][2] * a[2][0]) / d;
-  r.a[1][2] = (a[0][2] * a[1][0] - a[0][0] * a[1][2]) / d;
-  r.a[2][0] = (a[1][0] * a[2][1] - a[1][1] * a[2][0]) / d;
-  r.a[2][1] = (a[0][1] * a[2][0] - a[0][0] * a[2][1]) / d;
-  r.a[2][2] = (a[0][0] * a[1][1] - a[0][1] * a[1][0]) / d;
+  r.a[0][0] = (a[1][1] * a[2][2] - a[1][2] * a[2][1]) * id;
+  r.a[0][1] = (a[0][2] * a[2][1] - a[0][1] * a[2][2]) * id;
+  r.a[0][2] = (a[0][1] * a[1][2] - a[0][2] * a[1][1]) * id;
+  r.a[1][0] = (a[1][2] * a[2][0] - a[1][0] * a[2][2]) * id;
+  r.a[1][1] = (a[0][0] * a[2][2] - a[0][2] * a[2][0]) * id;
+  r.a[1][2] = (a[0][2] * a[1][0] - a[0][0] * a[1][2]) * id;
+  r.a[2][0] = (a[1][0] * a[2][1] - a[1][1] * a[2][0]) * id;
+  r.a[2][1] = (a[0][1] * a[2][0] - a[0][0] * a[2][1]) * id;
+  r.a[2][2] = (a[0][0] * a[1][1] - a[0][1] * a[1][0]) * id;
   return r;
 }
 // Closed-form inverse by 2x2 minors (Laplace expansion along the first two
