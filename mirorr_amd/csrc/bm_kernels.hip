@@ -41,6 +41,10 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kMaxWindow = 64;  // window planes (and rows) a tile may span
+// Masked row tasks are grouped by their skip pattern: 4 bits of live planes,
+// 4 bits of live rows and one bit for a partly masked base block. The
+// bucket counters are 16-bit halves of 32-bit words (at most 64 x 169 tasks).
+constexpr int kPatBuckets = 512;
 
 __device__ __forceinline__ int64_t lin3(int nx, int ny, int x, int y, int z) {
   return (int64_t)x + (int64_t)nx * ((int64_t)y + (int64_t)ny * z);
@@ -428,7 +432,7 @@ struct TileCfg {
 template <class T>
 constexpr size_t masked_static_lds() {
   return T::NB * (16 * 16 + 8 + 4 + 4 + 8) + T::NB * T::NW * T::NW * (4 + 4 + 4 + 2) +
-         2 * 8 * kMaxWindow + 256;
+         2 * 8 * kMaxWindow + 2 * kPatBuckets + 256;
 }
 
 struct TileArgs {
@@ -949,6 +953,7 @@ __device__ __forceinline__ void tile_masked(const TileArgs& a, int wg) {
   __shared__ float rbest[T::NB * kRows], rsecond[T::NB * kRows];  // per-task records
   __shared__ int rscan[T::NB * kRows];
   __shared__ uint16_t live_list[T::NB * kRows];  // tasks that need arithmetic
+  __shared__ unsigned int pat_cnt[kPatBuckets / 2];  // counts, then next list slots
   __shared__ int s_nlive;
   __shared__ float4 xh4[T::NB * 16];  // base-masked centred blocks
   __shared__ unsigned long long bbits[T::NB];  // base mask bit per voxel
@@ -999,6 +1004,7 @@ __device__ __forceinline__ void tile_masked(const TileArgs& a, int wg) {
     zon[i] = 0ull;
     zbad[i] = 0ull;
   }
+  for (int i = tid; i < kPatBuckets / 2; i += kThreads) pat_cnt[i] = 0u;
   if (tid == 0) {
     s_fullbase = 0ull;
     s_nlive = 0;
@@ -1072,6 +1078,7 @@ __device__ __forceinline__ void tile_masked(const TileArgs& a, int wg) {
   for (int base = 0; base < ntasks; base += kThreads) {  // workgroup-uniform
     const int i = base + tid;
     bool live = false, full = false;
+    unsigned int pm = 0u, rm = 0u;  // planes and rows of the task's window with an on voxel
     if (i < ntasks) {
       const int k = i / kRows, r = i % kRows;
       const int bl = todo_list[k];
@@ -1083,10 +1090,13 @@ __device__ __forceinline__ void tile_masked(const TileArgs& a, int wg) {
       top2_init(tr);
       if (czg >= max(pz - T::H, 0) && czg <= min(pz - T::H + T::NW - 1, nz - T::BW) &&
           cyg >= max(py - T::H, 0) && cyg <= min(py - T::H + T::NW - 1, ny - T::BW)) {
-        unsigned long long rows_on = 0ull;
 #pragma unroll
-        for (int iz = 0; iz < 4; ++iz) rows_on |= zon[blz * NG + cz + iz];
-        if (((rows_on >> (bly * NG + cy)) & 0xfull) != 0ull) {
+        for (int iz = 0; iz < 4; ++iz) {
+          const unsigned int rp = (unsigned int)(zon[blz * NG + cz + iz] >> (bly * NG + cy)) & 0xfu;
+          rm |= rp;
+          pm |= (rp != 0u ? 1u : 0u) << iz;
+        }
+        if (rm != 0u) {
           live = true;
           if (((s_fullbase >> bl) & 1ull) && bbits[bl] == ~0ull) {
             unsigned long long rows_bad = 0ull;
@@ -1108,6 +1118,26 @@ __device__ __forceinline__ void tile_masked(const TileArgs& a, int wg) {
         rsecond[i] = tr.second;
         rscan[i] = tr.scan;
       }
+      if (live && !full) {
+        // A window plane or row without an on voxel, or a base plane or row
+        // without one, adds exact zeros to every sum: the task skips it. Until
+        // the task is computed its record's scan slot holds INT_MIN + pattern
+        // (records hold scans >= 0 or INT_MAX; full tasks hold 0 until then).
+        const unsigned long long bits = bbits[bl];
+        unsigned int bpm = 0u, brm = 0u;
+#pragma unroll
+        for (int iz = 0; iz < 4; ++iz) {
+          const unsigned int pl = (unsigned int)(bits >> (16 * iz)) & 0xffffu;
+          bpm |= (pl != 0u ? 1u : 0u) << iz;
+#pragma unroll
+          for (int iy = 0; iy < 4; ++iy) brm |= (((pl >> (4 * iy)) & 0xfu) != 0u ? 1u : 0u) << iy;
+        }
+        const unsigned int pat = (pm & bpm) | ((rm & brm) << 4) | (bits == ~0ull ? 0u : 256u);
+        rscan[i] = (int)(0x80000000u | pat);
+        atomicAdd(&pat_cnt[pat >> 1], 1u << (16 * (pat & 1u)));
+      } else if (full) {
+        rscan[i] = 0;
+      }
     }
     const unsigned long long balf = __ballot(full);
     const unsigned long long balm = __ballot(live && !full);
@@ -1120,11 +1150,48 @@ __device__ __forceinline__ void tile_masked(const TileArgs& a, int wg) {
     mbase = __shfl(mbase, 0, 64);
     const unsigned long long below = (1ull << (tid & 63)) - 1ull;
     if (full) live_list[fbase + __popcll(balf & below)] = (uint16_t)i;
-    else if (live) live_list[T::NB * kRows - 1 - (mbase + __popcll(balm & below))] = (uint16_t)i;
+    (void)mbase;
   }
   __syncthreads();
   const int nfull = s_nfull, nmasked = s_nlive;
   const int nlive = nfull + nmasked;
+  // The masked tasks go to the back of the list grouped by pattern (counting
+  // sort: fully-on bases first), so that the lanes of a wave mostly skip the
+  // same planes and rows. Their order does not change any record.
+  if (nmasked > 0) {
+    if (tid < 64) {  // exclusive prefix of the bucket counts, 8 buckets a lane
+      constexpr int kPer = kPatBuckets / 64;
+      unsigned int c[kPer], s = 0u;
+#pragma unroll
+      for (int u = 0; u < kPer; ++u) {
+        c[u] = (pat_cnt[(tid * kPer + u) >> 1] >> (16 * (u & 1))) & 0xffffu;
+        s += c[u];
+      }
+      unsigned int inc = s;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const unsigned int t = __shfl_up(inc, o, 64);
+        if (tid >= o) inc += t;
+      }
+      unsigned int at = (unsigned int)(T::NB * kRows - nmasked) + (inc - s);
+#pragma unroll
+      for (int u = 0; u < kPer; u += 2) {
+        pat_cnt[(tid * kPer + u) >> 1] = at | ((at + c[u]) << 16);
+        at += c[u] + c[u + 1];
+      }
+    }
+    __syncthreads();
+    for (int i = tid; i < ntasks; i += kThreads) {
+      const int m = rscan[i];
+      if (m < 0) {
+        const unsigned int pat = (unsigned int)m & 0x1ffu;
+        const unsigned int sh = 16 * (pat & 1u);
+        const unsigned int old = atomicAdd(&pat_cnt[pat >> 1], 1u << sh);
+        live_list[(old >> sh) & 0xffffu] = (uint16_t)i;
+      }
+    }
+    __syncthreads();
+  }
 #pragma unroll 1
   for (int j = tid; j < nlive; j += kThreads) {
     const bool full = j < nfull;
@@ -1190,14 +1257,17 @@ __device__ __forceinline__ void tile_masked(const TileArgs& a, int wg) {
       dmu[cx] = srow[cxg].x;
     }
     const unsigned long long bits = bbits[bl];
+    const unsigned int pat = (unsigned int)rscan[i] & 0x1ffu;  // set above
     float sxy[T::NW], sxx[T::NW], s3[T::NW];
 #pragma unroll
     for (int cx = 0; cx < T::NW; ++cx) sxy[cx] = sxx[cx] = s3[cx] = 0.0f;
     if (bits == ~0ull) {
 #pragma unroll 1
       for (int iz = 0; iz < 4; ++iz) {
+        if (((pat >> iz) & 1u) == 0u) continue;
 #pragma unroll
         for (int iy = 0; iy < 4; ++iy) {
+          if (((pat >> (4 + iy)) & 1u) == 0u) continue;
           const int ro = rowoff + (iz * WY + iy) * T::PX;
           float yr[T::PX], vr[T::PX];
           const float4* yp = reinterpret_cast<const float4*>(wy_ + ro);
@@ -1233,8 +1303,10 @@ __device__ __forceinline__ void tile_masked(const TileArgs& a, int wg) {
       // s3 = Sum b t^2 (Syy over both masks)
 #pragma unroll 1
       for (int iz = 0; iz < 4; ++iz) {
+        if (((pat >> iz) & 1u) == 0u) continue;
 #pragma unroll
         for (int iy = 0; iy < 4; ++iy) {
+          if (((pat >> (4 + iy)) & 1u) == 0u) continue;
           const int ro = rowoff + (iz * WY + iy) * T::PX;
           float yr[T::PX], vr[T::PX];
           const float4* yp = reinterpret_cast<const float4*>(wy_ + ro);
