@@ -9,7 +9,9 @@
 // would write back L2 for every workgroup):
 //  1. residual kernel: r_i = |F(x_i) - B(y_i)|^2 (classic |F(x_i) - y_i|^2),
 //     +inf for culled blocks; the first radix digit (bits 63..52 of the
-//     non-negative double) is histogrammed in LDS and merged globally;
+//     non-negative double) is histogrammed in LDS and merged globally (not
+//     launched in a run's first inner iteration, which keeps every kept
+//     pair: the moments read the kept flags then);
 //  2. exact k-th smallest residual (std::nth_element in the reference): the
 //     bin of rank k, a second 12-bit digit inside it, a compaction of the
 //     candidates sharing the 24-bit prefix and a one-workgroup finish by rank
@@ -251,10 +253,15 @@ __device__ void reduce_parts(const double* part, int stride, int nparts, double*
 constexpr int kMoments = 28;   // values per workgroup partial
 constexpr int kPartStride = 32;
 
+// first: the run's first inner iteration, which keeps every kept pair (the
+// reference keeps the pairs with r <= DBL_MAX, i.e. the finite residuals,
+// which are the kept pairs'): the kept flags stand for the residuals, whose
+// kernel is not launched then.
 __device__ __forceinline__ void moments_body(const double* __restrict__ X,
                                              const double* __restrict__ Y,
+                                             const uint8_t* __restrict__ kept,
                                              const double* __restrict__ r, int64_t n,
-                                             LtsState* st, double* part) {
+                                             LtsState* st, double* part, int first) {
   if (!st->active || n < 1) return;
   const double thr = st->fraction > 0.99999 ? DBL_MAX : st->thr;
   double v[kMoments];
@@ -269,7 +276,7 @@ __device__ __forceinline__ void moments_body(const double* __restrict__ X,
 #pragma unroll
   for (int u = 0; u < kPer; ++u) {
     const int64_t i = min(base + threadIdx.x + (int64_t)u * kT, n - 1);
-    in[u] = base + threadIdx.x + (int64_t)u * kT < n && r[i] <= thr;
+    in[u] = base + threadIdx.x + (int64_t)u * kT < n && (first ? kept[i] != 0 : r[i] <= thr);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       xs[u][k] = X[3 * i + k];
@@ -702,9 +709,10 @@ __global__ __launch_bounds__(kT) void compact_kernel(const double* __restrict__ 
 }
 __global__ __launch_bounds__(kT) void moments_kernel(const double* __restrict__ X,
                                                      const double* __restrict__ Y,
+                                                     const uint8_t* __restrict__ kept,
                                                      const double* __restrict__ r, int64_t n,
-                                                     LtsState* st, double* part) {
-  moments_body(X, Y, r, n, st, part);
+                                                     LtsState* st, double* part, int first) {
+  moments_body(X, Y, kept, r, n, st, part, first);
 }
 __global__ __launch_bounds__(kT) void solve_kernel(LtsState* st, const double* part, int nparts,
                                                    unsigned int* hist1, unsigned int* hist2) {
@@ -739,10 +747,11 @@ __global__ __launch_bounds__(kT) void final_select_multi(const LtsItem* __restri
   if (!it.st->active) return;
   final_select(it.cand, it.st);
 }
-__global__ __launch_bounds__(kT) void moments_multi(const LtsItem* __restrict__ items) {
+__global__ __launch_bounds__(kT) void moments_multi(const LtsItem* __restrict__ items,
+                                                    int first) {
   const LtsItem it = items[blockIdx.y];
   if ((int64_t)blockIdx.x >= lts_parts_dev(it.npts)) return;
-  moments_body(it.X, it.Y, it.r, it.npts, it.st, it.part);
+  moments_body(it.X, it.Y, it.kept, it.r, it.npts, it.st, it.part, first);
 }
 __global__ __launch_bounds__(kT) void solve_multi(const LtsItem* __restrict__ items) {
   const LtsItem it = items[blockIdx.y];
@@ -777,15 +786,16 @@ void lts_enqueue_iteration(const double* X, const double* Y, const uint8_t* kept
   // With the histogram, at most 256 workgroups: the residuals crowd into a
   // few bins, and each workgroup merges its bins with global atomics.
   const int g2 = std::min(grid, 256);
-  hipLaunchKernelGGL(residual_kernel, dim3(select ? g2 : grid), dim3(kT), 0, s, X, Y, kept, npts,
-                     st, b.r, b.hist1, select ? 1 : 0);
   if (select) {
+    hipLaunchKernelGGL(residual_kernel, dim3(g2), dim3(kT), 0, s, X, Y, kept, npts, st, b.r,
+                       b.hist1, 1);
     // At most 256 workgroups: each re-reads a 16 KB histogram to find its bin.
     hipLaunchKernelGGL(hist2_kernel, dim3(g2), dim3(kT), 0, s, b.r, npts, st, b.hist1, b.hist2);
     hipLaunchKernelGGL(compact_kernel, dim3(g2), dim3(kT), 0, s, b.r, npts, st, b.hist2, b.cand);
     hipLaunchKernelGGL(final_select_kernel, dim3(1), dim3(kT), 0, s, b.cand, st);
   }
-  hipLaunchKernelGGL(moments_kernel, dim3(nparts), dim3(kT), 0, s, X, Y, b.r, npts, st, b.part);
+  hipLaunchKernelGGL(moments_kernel, dim3(nparts), dim3(kT), 0, s, X, Y, kept, b.r, npts, st,
+                     b.part, select ? 0 : 1);
   hipLaunchKernelGGL(solve_kernel, dim3(1), dim3(kT), 0, s, st, b.part, nparts, b.hist1,
                      b.hist2);
   MIRORR_CHECK_LAUNCH();
@@ -831,14 +841,14 @@ void lts_enqueue_iteration_multi(const LtsItem* items, int n, int64_t max_npts, 
   const bool select = iteration > 0;
   const int g2 = std::min(grid, 256);
   const unsigned ny = (unsigned)n;
-  hipLaunchKernelGGL(residual_multi, dim3((unsigned)(select ? g2 : grid), ny), dim3(kT), 0, s,
-                     items, select ? 1 : 0);
   if (select) {
+    hipLaunchKernelGGL(residual_multi, dim3((unsigned)g2, ny), dim3(kT), 0, s, items, 1);
     hipLaunchKernelGGL(hist2_multi, dim3((unsigned)g2, ny), dim3(kT), 0, s, items);
     hipLaunchKernelGGL(compact_multi, dim3((unsigned)g2, ny), dim3(kT), 0, s, items);
     hipLaunchKernelGGL(final_select_multi, dim3(1, ny), dim3(kT), 0, s, items);
   }
-  hipLaunchKernelGGL(moments_multi, dim3((unsigned)nparts, ny), dim3(kT), 0, s, items);
+  hipLaunchKernelGGL(moments_multi, dim3((unsigned)nparts, ny), dim3(kT), 0, s, items,
+                     select ? 0 : 1);
   hipLaunchKernelGGL(solve_multi, dim3(1, ny), dim3(kT), 0, s, items);
   MIRORR_CHECK_LAUNCH();
 }
