@@ -396,11 +396,18 @@ __device__ M3 top3(const M4& m) {
 // Symmetric update (itkSymMirorrRegistrationMethod.txx:866-1379).
 // Run by lanes 0 and 1 of one wave on identical inputs: they compute the
 // same values, except that the two Denman-Beavers roots run one per lane;
-// lane 0 writes the state.
-__device__ void solve_symmetric(LtsState* st, const Moments& m, int lane) {
+// lane 0 writes the state. The state's parameters and delta are read once,
+// at the start (no reload after the fits are stored); *dlast and *dnew get
+// the previous and the new parameter change.
+__device__ void solve_symmetric(LtsState* st, const Moments& m, int lane, double* dlast,
+                                double* dnew) {
   const int kind = st->kind, np = kind == kRigid ? 6 : 12;
-  double neu[12], neuF[12], neuB[12];
-  for (int i = 0; i < np; ++i) neu[i] = neuF[i] = neuB[i] = st->last[i];
+  const double delta0 = st->delta;
+  double neu[12], neuF[12], neuB[12], last0[12];
+  for (int i = 0; i < np; ++i) {
+    last0[i] = st->last[i];
+    neu[i] = neuF[i] = neuB[i] = last0[i];
+  }
   double tr[3];
   for (int k = 0; k < 3; ++k) tr[k] = m.my[k] - m.mx[k];
   Xf fF = st->F, fB = st->B;
@@ -515,17 +522,23 @@ __device__ void solve_symmetric(LtsState* st, const Moments& m, int lane) {
   if (lane != 0) return;
   st->F = sF;
   st->B = sB;
-  st->last_delta = st->delta;
-  st->delta = delta_of(neu, st->last, np);
+  *dlast = delta0;
+  *dnew = delta_of(neu, last0, np);
+  st->last_delta = *dlast;
+  st->delta = *dnew;
   for (int i = 0; i < np; ++i) st->last[i] = neu[i];
 }
 
 // Classic update (itkMirorrRegistrationMethod.txx:841-999); last_parameters
 // keeps the identity as the reference never updates it.
-__device__ void solve_classic(LtsState* st, const Moments& m) {
+__device__ void solve_classic(LtsState* st, const Moments& m, double* dlast, double* dnew) {
   const int kind = st->kind, np = kind == kRigid ? 6 : 12;
-  double neu[12];
-  for (int i = 0; i < np; ++i) neu[i] = st->last[i];
+  const double delta0 = st->delta;
+  double neu[12], last0[12];
+  for (int i = 0; i < np; ++i) {
+    last0[i] = st->last[i];
+    neu[i] = last0[i];
+  }
   double a[3], b[3], ra[3];
   for (int k = 0; k < 3; ++k) {
     a[k] = m.my[k] - st->rc[k];
@@ -549,8 +562,10 @@ __device__ void solve_classic(LtsState* st, const Moments& m) {
       neu[3 + k] = a[k] - ra[k];
     }
   }
-  st->last_delta = st->delta;
-  st->delta = delta_of(neu, st->last, np);
+  *dlast = delta0;
+  *dnew = delta_of(neu, last0, np);
+  st->last_delta = *dlast;
+  st->delta = *dnew;
   for (int k = 0; k < 3; ++k) st->rc[k] = m.mx[k];
   set_center(st->F, st->rc);
   set_params(st->F, neu);
@@ -562,6 +577,8 @@ __device__ void solve_classic(LtsState* st, const Moments& m) {
 __device__ __forceinline__ void solve_body(LtsState* st, const double* part, int nparts,
                                            unsigned int* hist1, unsigned int* hist2) {
   if (!st->active) return;
+  const int iter0 = st->iter;
+  const double tol = st->tol;
   __shared__ double t[kMoments];
   reduce_parts<kMoments>(part, kPartStride, nparts, t);
   for (int i = threadIdx.x; i < 4096; i += blockDim.x) {
@@ -583,14 +600,16 @@ __device__ __forceinline__ void solve_body(LtsState* st, const double* part, int
     for (int b = 0; b < 3; ++b) s2[12 + 3 * a + b] = t[19 + 3 * a + b] - t[1 + a] * t[4 + b] * icnt;
   Moments m;
   unpack(t, s2, m);
+  double dlast = 0.0, dnew = 0.0;
   if (st->symmetric)
-    solve_symmetric(st, m, lane);
+    solve_symmetric(st, m, lane, &dlast, &dnew);
   else
-    solve_classic(st, m);
+    solve_classic(st, m, &dlast, &dnew);
   if (lane != 0) return;
-  st->iter += 1;  // the loop head of the reference
+  const int iter = iter0 + 1;  // the loop head of the reference
+  st->iter = iter;
   st->fraction = 0.5;
-  st->active = (st->delta > st->tol && st->iter < 100 && fabs(st->delta - st->last_delta) > st->tol) ? 1 : 0;
+  st->active = (dnew > tol && iter < 100 && fabs(dnew - dlast) > tol) ? 1 : 0;
 }
 
 
