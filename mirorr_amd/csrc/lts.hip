@@ -654,19 +654,38 @@ __device__ void centered(const Xf& t12, const Xf& shift, Xf& out) {
   compose(out, shift, false);
 }
 
-__device__ void prepare_maps(OuterState* o, int symmetric) {
+// The resampling maps of the outer transforms T (classic) or t12F / t12B
+// (symmetric), given in registers. The grids are read once and the maps
+// stored at the end: the state lives in global memory, where a store
+// between two reads of it would make the second wait for the first.
+__device__ void prepare_maps_of(OuterState* o, int symmetric, const Xf& T, const Xf& t12F,
+                                const Xf& t12B) {
+  const Geom gF = o->gF, gFfull = o->gFfull, gHF = o->gHF;
   if (symmetric) {
+    const Geom gM = o->gM, gMfull = o->gMfull, gHM = o->gHM;
+    const Xf fixT = o->fixT, movT = o->movT;
     Xf cF, cM;
-    centered(o->t12F, o->fixT, cF);
-    centered(o->t12B, o->movT, cM);
-    o->mapF = make_index_map(cF.M, cF.off, o->gF, o->gHF);
-    o->mmapF = make_index_map(cF.M, cF.off, o->gFfull, o->gHF);
-    o->mapM = make_index_map(cM.M, cM.off, o->gM, o->gHM);
-    o->mmapM = make_index_map(cM.M, cM.off, o->gMfull, o->gHM);
+    centered(t12F, fixT, cF);
+    centered(t12B, movT, cM);
+    const IndexMap a = make_index_map(cF.M, cF.off, gF, gHF);
+    const IndexMap b = make_index_map(cF.M, cF.off, gFfull, gHF);
+    const IndexMap c = make_index_map(cM.M, cM.off, gM, gHM);
+    const IndexMap d = make_index_map(cM.M, cM.off, gMfull, gHM);
+    o->mapF = a;
+    o->mmapF = b;
+    o->mapM = c;
+    o->mmapM = d;
   } else {
-    o->mapF = make_index_map(o->T.M, o->T.off, o->gF, o->gHF);
-    o->mmapF = make_index_map(o->T.M, o->T.off, o->gFfull, o->gHF);
+    const IndexMap a = make_index_map(T.M, T.off, gF, gHF);
+    const IndexMap b = make_index_map(T.M, T.off, gFfull, gHF);
+    o->mapF = a;
+    o->mmapF = b;
   }
+}
+
+__device__ void prepare_maps(OuterState* o, int symmetric) {
+  const Xf T = o->T, t12F = o->t12F, t12B = o->t12B;
+  prepare_maps_of(o, symmetric, T, t12F, t12B);
 }
 
 __device__ void finish_update(LtsState* st, OuterState* o);
@@ -682,25 +701,39 @@ __device__ __forceinline__ void finish_body(LtsState* st, OuterState* o, int* fl
 }
 
 __device__ void finish_update(LtsState* st, OuterState* o) {
-  const int kind = st->kind;
-  if (st->symmetric) {
-    compose(o->t12F, st->F, true);
-    compose(o->t12B, st->B, true);
-    const Xf ib = inverse(o->t12B);
-    o->T = identity(kind);
-    compose(o->T, ib, true);
-    compose(o->T, o->t12F, false);
+  // The transforms are composed in registers and stored once (see
+  // prepare_maps_of).
+  const int kind = st->kind, symmetric = st->symmetric;
+  Xf T = o->T, t12F = o->t12F, t12B = o->t12B;
+  if (symmetric) {
+    const Xf F = st->F, B = st->B;
+    compose(t12F, F, true);
+    compose(t12B, B, true);
+    const Xf ib = inverse(t12B);
+    T = identity(kind);
+    compose(T, ib, true);
+    compose(T, t12F, false);
   } else {
-    compose(o->T, st->F, true);
+    const Xf F = st->F;
+    compose(T, F, true);
   }
-  double cur[12];
-  get_params(o->T, cur);
+  double cur[12], lastp[12];
+  get_params(T, cur);
   const int np = kind == kRigid ? 6 : 12;
-  o->sqdiff = delta_of(o->lastp, cur, np);
+  for (int i = 0; i < np; ++i) lastp[i] = o->lastp[i];
+  const double sq = delta_of(lastp, cur, np);
+  const int inner = o->inner_total + st->iter + 1;
+  const int stop = sq < o->stop_tol ? 1 : 0;
+  if (symmetric) {
+    o->t12F = t12F;
+    o->t12B = t12B;
+  }
+  o->T = T;
+  o->sqdiff = sq;
   for (int i = 0; i < np; ++i) o->lastp[i] = cur[i];
-  o->stop = o->sqdiff < o->stop_tol ? 1 : 0;
-  o->inner_total += st->iter + 1;
-  prepare_maps(o, st->symmetric);
+  o->stop = stop;
+  o->inner_total = inner;
+  prepare_maps_of(o, symmetric, T, t12F, t12B);
 }
 
 __global__ __launch_bounds__(64) void prepare_maps_kernel(OuterState* o, int symmetric) { prepare_maps(o, symmetric); }
