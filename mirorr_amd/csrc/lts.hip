@@ -655,32 +655,22 @@ __device__ void centered(const Xf& t12, const Xf& shift, Xf& out) {
 }
 
 // The resampling maps of the outer transforms T (classic) or t12F / t12B
-// (symmetric), given in registers. The grids are read once and the maps
-// stored at the end: the state lives in global memory, where a store
-// between two reads of it would make the second wait for the first.
+// (symmetric), given in registers (the same in every lane). Lane l of the
+// first wave computes one map, on the fixed (l < 2) or moving (l >= 2) side,
+// at the level's grid (l even) or the full-resolution one (l odd): the four
+// maps of the symmetric method take the time of one.
 __device__ void prepare_maps_of(OuterState* o, int symmetric, const Xf& T, const Xf& t12F,
                                 const Xf& t12B) {
-  const Geom gF = o->gF, gFfull = o->gFfull, gHF = o->gHF;
-  if (symmetric) {
-    const Geom gM = o->gM, gMfull = o->gMfull, gHM = o->gHM;
-    const Xf fixT = o->fixT, movT = o->movT;
-    Xf cF, cM;
-    centered(t12F, fixT, cF);
-    centered(t12B, movT, cM);
-    const IndexMap a = make_index_map(cF.M, cF.off, gF, gHF);
-    const IndexMap b = make_index_map(cF.M, cF.off, gFfull, gHF);
-    const IndexMap c = make_index_map(cM.M, cM.off, gM, gHM);
-    const IndexMap d = make_index_map(cM.M, cM.off, gMfull, gHM);
-    o->mapF = a;
-    o->mmapF = b;
-    o->mapM = c;
-    o->mmapM = d;
-  } else {
-    const IndexMap a = make_index_map(T.M, T.off, gF, gHF);
-    const IndexMap b = make_index_map(T.M, T.off, gFfull, gHF);
-    o->mapF = a;
-    o->mmapF = b;
-  }
+  const int l = threadIdx.x;
+  if (l >= (symmetric ? 4 : 2)) return;
+  const bool side = (l >> 1) != 0, full = (l & 1) != 0;
+  const Geom gin = side ? (full ? o->gMfull : o->gM) : (full ? o->gFfull : o->gF);
+  const Geom gout = side ? o->gHM : o->gHF;
+  Xf c = T;
+  if (symmetric) centered(side ? t12B : t12F, side ? o->movT : o->fixT, c);
+  const IndexMap map = make_index_map(c.M, c.off, gin, gout);
+  IndexMap* dst = side ? (full ? &o->mmapM : &o->mapM) : (full ? &o->mmapF : &o->mapF);
+  *dst = map;
 }
 
 __device__ void prepare_maps(OuterState* o, int symmetric) {
@@ -690,11 +680,14 @@ __device__ void prepare_maps(OuterState* o, int symmetric) {
 
 __device__ void finish_update(LtsState* st, OuterState* o);
 
+// Run by the lanes of one wave (at least 4): each computes the update, lane
+// 0 stores the state and the flags, lanes 0..3 the maps.
 __device__ __forceinline__ void finish_body(LtsState* st, OuterState* o, int* flags) {
   if (!st->active && !st->finished) {
-    st->finished = 1;
+    if (threadIdx.x == 0) st->finished = 1;
     finish_update(st, o);
   }
+  if (threadIdx.x != 0) return;
   flags[0] = st->active;
   flags[1] = o->stop;
   flags[2] = st->iter + 1;  // inner iterations run so far
@@ -724,15 +717,17 @@ __device__ void finish_update(LtsState* st, OuterState* o) {
   const double sq = delta_of(lastp, cur, np);
   const int inner = o->inner_total + st->iter + 1;
   const int stop = sq < o->stop_tol ? 1 : 0;
-  if (symmetric) {
-    o->t12F = t12F;
-    o->t12B = t12B;
+  if (threadIdx.x == 0) {
+    if (symmetric) {
+      o->t12F = t12F;
+      o->t12B = t12B;
+    }
+    o->T = T;
+    o->sqdiff = sq;
+    for (int i = 0; i < np; ++i) o->lastp[i] = cur[i];
+    o->stop = stop;
+    o->inner_total = inner;
   }
-  o->T = T;
-  o->sqdiff = sq;
-  for (int i = 0; i < np; ++i) o->lastp[i] = cur[i];
-  o->stop = stop;
-  o->inner_total = inner;
   prepare_maps_of(o, symmetric, T, t12F, t12B);
 }
 
@@ -862,12 +857,12 @@ void lts_start(LtsState* st, const OuterState* outer, int kind, int symmetric, d
 
 void lts_finish(LtsState* st, OuterState* outer, int* flags, hipStream_t s) {
   if (!st || !outer || !flags) throw Error(kErrState, "lts_finish: null device buffer");
-  hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(1), 0, s, st, outer, flags);
+  hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(4), 0, s, st, outer, flags);
   MIRORR_CHECK_LAUNCH();
 }
 
 void outer_prepare_maps(OuterState* outer, int symmetric, hipStream_t s) {
-  hipLaunchKernelGGL(prepare_maps_kernel, dim3(1), dim3(1), 0, s, outer, symmetric);
+  hipLaunchKernelGGL(prepare_maps_kernel, dim3(1), dim3(4), 0, s, outer, symmetric);
   MIRORR_CHECK_LAUNCH();
 }
 
@@ -907,7 +902,7 @@ void lts_enqueue_iteration_multi(const LtsItem* items, int n, int64_t max_npts, 
 
 void lts_finish_multi(const LtsItem* items, int n, hipStream_t s) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(finish_multi, dim3(1, (unsigned)n), dim3(1), 0, s, items);
+  hipLaunchKernelGGL(finish_multi, dim3(1, (unsigned)n), dim3(4), 0, s, items);
   MIRORR_CHECK_LAUNCH();
 }
 
