@@ -34,9 +34,19 @@ namespace mirorr {
 namespace {
 
 constexpr int kT = sel::kSelT;
-constexpr int kPart = 1024;  // points per moment workgroup (4 per thread)
+// Points per moment workgroup: kPart (4 per thread), or twice that (8 per
+// thread, half the partials for the solve to total) for runs of more than
+// kBigRun pairs; fewer, larger workgroups are slower on the smaller runs.
+constexpr int kPart = 1024;
+constexpr int64_t kBigRun = 262144;
+__host__ __device__ __forceinline__ int part_points(int64_t npts) {
+  return npts > kBigRun ? 2 * kPart : kPart;
+}
 
-__device__ __forceinline__ int64_t lts_parts_dev(int64_t npts) { return (npts + kPart - 1) / kPart; }
+__device__ __forceinline__ int64_t lts_parts_dev(int64_t npts) {
+  const int p = part_points(npts);
+  return (npts + p - 1) / p;
+}
 
 using namespace xf;
 using sel::block_scan;
@@ -257,6 +267,7 @@ constexpr int kPartStride = 32;
 // reference keeps the pairs with r <= DBL_MAX, i.e. the finite residuals,
 // which are the kept pairs'): the kept flags stand for the residuals, whose
 // kernel is not launched then.
+template <int PER>  // points per thread
 __device__ __forceinline__ void moments_body(const double* __restrict__ X,
                                              const double* __restrict__ Y,
                                              const uint8_t* __restrict__ kept,
@@ -267,10 +278,10 @@ __device__ __forceinline__ void moments_body(const double* __restrict__ X,
   double v[kMoments];
 #pragma unroll
   for (int j = 0; j < kMoments; ++j) v[j] = 0.0;
-  const int64_t base = (int64_t)blockIdx.x * kPart;
+  const int64_t base = (int64_t)blockIdx.x * (PER * kT);
   // The thread's points are loaded together (one round trip), then summed
   // in order.
-  constexpr int kPer = kPart / kT;
+  constexpr int kPer = PER;
   double xs[kPer][3], ys[kPer][3];
   bool in[kPer];
 #pragma unroll
@@ -754,12 +765,13 @@ __global__ __launch_bounds__(kT) void compact_kernel(const double* __restrict__ 
                                                      unsigned long long* __restrict__ cand) {
   compact_body(r, n, st, hist2, cand);
 }
+template <int PER>
 __global__ __launch_bounds__(kT) void moments_kernel(const double* __restrict__ X,
                                                      const double* __restrict__ Y,
                                                      const uint8_t* __restrict__ kept,
                                                      const double* __restrict__ r, int64_t n,
                                                      LtsState* st, double* part, int first) {
-  moments_body(X, Y, kept, r, n, st, part, first);
+  moments_body<PER>(X, Y, kept, r, n, st, part, first);
 }
 __global__ __launch_bounds__(kT) void solve_kernel(LtsState* st, const double* part, int nparts,
                                                    unsigned int* hist1, unsigned int* hist2) {
@@ -794,11 +806,14 @@ __global__ __launch_bounds__(kT) void final_select_multi(const LtsItem* __restri
   if (!it.st->active) return;
   final_select(it.cand, it.st);
 }
+// PER 8 takes the pairs of more than kBigRun points, PER 4 the others.
+template <int PER>
 __global__ __launch_bounds__(kT) void moments_multi(const LtsItem* __restrict__ items,
                                                     int first) {
   const LtsItem it = items[blockIdx.y];
+  if ((part_points(it.npts) == 2 * kPart) != (PER == 8)) return;
   if ((int64_t)blockIdx.x >= lts_parts_dev(it.npts)) return;
-  moments_body(it.X, it.Y, it.kept, it.r, it.npts, it.st, it.part, first);
+  moments_body<PER>(it.X, it.Y, it.kept, it.r, it.npts, it.st, it.part, first);
 }
 __global__ __launch_bounds__(kT) void solve_multi(const LtsItem* __restrict__ items) {
   const LtsItem it = items[blockIdx.y];
@@ -820,7 +835,10 @@ __global__ void reset_count_multi(const LtsItem* __restrict__ items, int n) {
 
 }  // namespace
 
-int64_t lts_parts(int64_t npts) { return (npts + kPart - 1) / kPart; }
+int64_t lts_parts(int64_t npts) {
+  const int p = part_points(npts);
+  return (npts + p - 1) / p;
+}
 
 void lts_enqueue_iteration(const double* X, const double* Y, const uint8_t* kept, int64_t npts,
                            LtsState* st, OuterState* outer, LtsBuffers& b, int iteration,
@@ -841,8 +859,12 @@ void lts_enqueue_iteration(const double* X, const double* Y, const uint8_t* kept
     hipLaunchKernelGGL(compact_kernel, dim3(g2), dim3(kT), 0, s, b.r, npts, st, b.hist2, b.cand);
     hipLaunchKernelGGL(final_select_kernel, dim3(1), dim3(kT), 0, s, b.cand, st);
   }
-  hipLaunchKernelGGL(moments_kernel, dim3(nparts), dim3(kT), 0, s, X, Y, kept, b.r, npts, st,
-                     b.part, select ? 0 : 1);
+  if (part_points(npts) == 2 * kPart)
+    hipLaunchKernelGGL(moments_kernel<8>, dim3(nparts), dim3(kT), 0, s, X, Y, kept, b.r, npts, st,
+                       b.part, select ? 0 : 1);
+  else
+    hipLaunchKernelGGL(moments_kernel<4>, dim3(nparts), dim3(kT), 0, s, X, Y, kept, b.r, npts, st,
+                       b.part, select ? 0 : 1);
   hipLaunchKernelGGL(solve_kernel, dim3(1), dim3(kT), 0, s, st, b.part, nparts, b.hist1,
                      b.hist2);
   MIRORR_CHECK_LAUNCH();
@@ -879,8 +901,8 @@ void lts_start_multi(const LtsItem* items, int n, int kind, int symmetric, doubl
   MIRORR_CHECK_LAUNCH();
 }
 
-void lts_enqueue_iteration_multi(const LtsItem* items, int n, int64_t max_npts, int iteration,
-                                 hipStream_t s) {
+void lts_enqueue_iteration_multi(const LtsItem* items, int n, int64_t max_npts,
+                                 int64_t min_npts, int iteration, hipStream_t s) {
   if (n <= 0) return;
   const int64_t npts = std::max<int64_t>(max_npts, 1);
   const int nparts = (int)lts_parts(npts);
@@ -894,8 +916,15 @@ void lts_enqueue_iteration_multi(const LtsItem* items, int n, int64_t max_npts, 
     hipLaunchKernelGGL(compact_multi, dim3((unsigned)g2, ny), dim3(kT), 0, s, items);
     hipLaunchKernelGGL(final_select_multi, dim3(1, ny), dim3(kT), 0, s, items);
   }
-  hipLaunchKernelGGL(moments_multi, dim3((unsigned)nparts, ny), dim3(kT), 0, s, items,
-                     select ? 0 : 1);
+  // nparts: the most partials of any pair (part_points grows with npts, so
+  // the largest pair of each kind has the most of its kind)
+  if (part_points(npts) == 2 * kPart)
+    hipLaunchKernelGGL(moments_multi<8>, dim3((unsigned)nparts, ny), dim3(kT), 0, s, items,
+                       select ? 0 : 1);
+  if (part_points(std::max<int64_t>(min_npts, 1)) == kPart)
+    hipLaunchKernelGGL(moments_multi<4>,
+                       dim3((unsigned)lts_parts(std::min<int64_t>(npts, kBigRun)), ny), dim3(kT),
+                       0, s, items, select ? 0 : 1);
   hipLaunchKernelGGL(solve_multi, dim3(1, ny), dim3(kT), 0, s, items);
   MIRORR_CHECK_LAUNCH();
 }

@@ -95,9 +95,9 @@ struct LtsItem {
 };
 void lts_start_multi(const LtsItem* items, int n, int kind, int symmetric, double tol,
                      hipStream_t s);
-// max_npts: the largest npts of the items.
-void lts_enqueue_iteration_multi(const LtsItem* items, int n, int64_t max_npts, int iteration,
-                                 hipStream_t s);
+// max_npts, min_npts: the largest and the smallest npts of the items.
+void lts_enqueue_iteration_multi(const LtsItem* items, int n, int64_t max_npts,
+                                 int64_t min_npts, int iteration, hipStream_t s);
 void lts_finish_multi(const LtsItem* items, int n, hipStream_t s);
 void lts_reset_count_multi(const LtsItem* items, int n, hipStream_t s);
 

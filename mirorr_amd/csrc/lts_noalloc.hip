@@ -25,7 +25,7 @@ using namespace mirorr;
 namespace {
 
 constexpr size_t kNoAllocMaxPoints = size_t(1) << 20;
-constexpr size_t kPartPoints = 1024;  // points per moment partial (lts.hip)
+constexpr size_t kPartPoints = 1024;  // fewest points per moment partial (lts.hip)
 constexpr size_t kPartDoubles = 32;   // doubles per moment partial (lts.hip)
 constexpr size_t kHistBins = 4096;
 

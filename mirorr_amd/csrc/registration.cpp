@@ -1158,11 +1158,12 @@ void run_registration_batch(mirorr_ctx* ctx, const mirorr_reg_params* p, int n,
       }
       // 3. the LTS of every pair, one launch per phase
       std::vector<LtsItem> items(nI);
-      int64_t max_npts = 1;
+      int64_t max_npts = 1, min_npts = INT64_MAX;
       for (int k = 0; k < nI; ++k) {
         PairReg& r = *R[I[k]];
         items[k] = r.run.lts_item(r.st(), r.outer(), r.npts, &bc.flagsD.as<Flags>()[k].active);
         max_npts = std::max<int64_t>(max_npts, r.npts);
+        min_npts = std::min<int64_t>(min_npts, r.npts);
       }
       // pageable source: read when the call returns
       MIRORR_HIP(hipMemcpyAsync(bc.items.as<void>(), items.data(), sizeof(LtsItem) * nI,
@@ -1175,7 +1176,8 @@ void run_registration_batch(mirorr_ctx* ctx, const mirorr_reg_params* p, int n,
       for (int k = 0; k < nI; ++k) step = std::max(step, R[I[k]]->run.lts_first);
       for (int c = 0, chunk = 0; c < 101; ++chunk) {
         const int c1 = std::min(c + step, 101);
-        for (int j = c; j < c1; ++j) lts_enqueue_iteration_multi(di, nI, max_npts, j, s);
+        for (int j = c; j < c1; ++j)
+          lts_enqueue_iteration_multi(di, nI, max_npts, min_npts, j, s);
         lts_finish_multi(di, nI, s);
         MIRORR_HIP(hipMemcpyAsync(bc.flagsH, bc.flagsD.as<Flags>(), sizeof(Flags) * nI,
                                   hipMemcpyDeviceToHost, s));
