@@ -210,3 +210,55 @@ def test_overlapped_passes_match_serial_passes(affine):
     np.testing.assert_array_equal(out["overlapped"][0], out["serial"][0])
     np.testing.assert_array_equal(out["overlapped"][1], out["serial"][1])
     assert out["overlapped"][2] == out["serial"][2]
+
+
+@pytest.mark.parametrize("symmetric,affine", [(1, 1), (0, 0)])
+def test_long_lts_runs_match_the_oracle_and_fixed_chunks(oracle, symmetric, affine):
+    """A tight LTS threshold (1e-9) makes the inner runs long (up to the
+    reference's 100 iterations), so the host queues each run's first chunk
+    sized by the previous run's inner iterations and then chunks of 2 and 4.
+    The registration must match the oracle's (same outer iterations and
+    points, transform within 1e-5 mm) and equal, bit for bit, the same
+    registration with iteration graphs on, where every first chunk is 4."""
+    from mirorr_amd.blockmatcher import Context
+    from mirorr_amd.registration import DeviceVolume, MirorrPyramid
+    pair = phantom.make_pair(dims=(64, 64, 32), spacing=(3.0, 3.0, 3.0), n_ellipsoids=20,
+                             seed=12, model="affine" if affine else "rigid")
+    rp = oracle.reg_params(symmetric=symmetric, affine=affine, nw=7, ng=3, pyr_num=2,
+                           nthreads=8, lts_tolerance=1e-9)
+    ref_p, ref_c, ref_st = oracle.register(pair.fixed, pair.moving, rp)
+    runs = []
+    for graphs in (False, True):
+        ctx = Context(0)
+        ctx.iteration_graphs = graphs
+        reg = MirorrPyramid(ctx)
+        if not symmetric:
+            reg.SetRegistrationTypeClassic()
+        reg.SetTransformType("affine" if affine else "rigid")
+        reg.SetNhoodWidth(7)
+        reg.SetNhoodGap(3)
+        reg.GetPyramidScheduleTuner().SetMaxLevelNumber(2)
+        reg.SetStopTolerances(1e-3, 1e-9)
+        f = DeviceVolume(pair.fixed, ctx)
+        m = DeviceVolume(pair.moving, ctx)
+        p, c = reg.RunMirorr(fixed_dev=f, moving_dev=m)
+        runs.append((np.array(p), np.array(c), dict(reg.stats)))
+        f.close()
+        m.close()
+        ctx.close()
+    (pa, ca, sa), (pb, cb, sb) = runs
+    print("oracle", ref_st["iterations"], ref_st["points"], ref_st["inner_iterations"])
+    print("gpu   ", sa["iterations"], sa["points"], sa["inner_iterations"])
+    # long runs: several chunks per run on average
+    assert sa["inner_iterations"] > 6 * sum(sa["iterations"][:sa["levels"]])
+    np.testing.assert_array_equal(pa, pb)
+    np.testing.assert_array_equal(ca, cb)
+    assert sa["inner_iterations"] == sb["inner_iterations"]
+    levels = ref_st["levels"]
+    assert list(sa["iterations"][:levels]) == list(ref_st["iterations"][:levels])
+    assert list(sa["points"][:levels]) == list(ref_st["points"][:levels])
+    M1, o1 = transforms.matrix_offset(pa, ca, affine)
+    M2, o2 = transforms.matrix_offset(ref_p, ref_c, affine)
+    dmm = transforms.max_corner_distance(M1, o1, M2, o2, pair.fixed)
+    print(f"corner distance {dmm:.3e} mm")
+    assert dmm < 1e-5
