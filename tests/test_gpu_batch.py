@@ -172,3 +172,12 @@ def test_batch_rejects_what_it_cannot_batch():
     for v in (f, m, m16):
         v.close()
     ctx.close()
+
+
+def test_batch_mixing_large_and_small_lts_runs():
+    # The 256x256x128 pair's full-resolution LTS run has more than 262,144
+    # pairs (eight per moment thread), the other pair's runs fewer (four per
+    # thread): one iteration launches both moment kernels, each for its own
+    # pairs, and each pair still equals its own registration.
+    _check(_pairs([((256, 256, 128), 1.0, 21, "affine"), ((96, 80, 48), 1.5, 22, "affine")]),
+           affine=True, iterations=2)
